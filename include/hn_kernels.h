@@ -1,0 +1,258 @@
+/*
+ * hn_kernels.h - C ABI of libhn_kernels.so, the MI355X (gfx950) kernel library
+ * behind the HyperNova compute-backend plugin.
+ *
+ * The reference solver reaches its GPU through two layers:
+ *   - the C++ plugin interface IComputeBackend
+ *     (core/execution/gpu_backend.hpp:18-51), and
+ *   - a dynamically loaded C-ABI driver table (core/execution/cuda_driver.hpp:41-93,
+ *     bound by dlopen/dlsym in core/execution/cuda_driver.cpp:24-48,112-145).
+ * This header is the replacement for the second layer.  Every entry point is
+ * extern "C", takes plain pointers and sizes, and returns an int status
+ * (HN_OK == 0).  On failure hn_last_error() returns a message, mirroring the
+ * reference's "last CUresult" slot (cuda_driver.cpp:65-73,356-358).
+ *
+ * Pointer conventions
+ *   - Functions whose names end in _dev take DEVICE pointers and an optional
+ *     stream (hipStream_t passed as void*, NULL = the library stream).  They
+ *     enqueue work and return without synchronising.
+ *   - All other functions take HOST pointers, are synchronous, and behave like
+ *     the corresponding IComputeBackend method.
+ *
+ * Threading: every call re-establishes the device chosen by hn_init(), so any
+ * host thread may call.  Calls that use the same matrix handle or the library
+ * stream must be serialised by the caller (the reference serialises all calls
+ * with one mutex, gpu_backend.cpp:160,205).
+ */
+#ifndef HN_KERNELS_H
+#define HN_KERNELS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define HN_OK 0
+#define HN_ERR_INVALID 1      /* bad argument (null pointer, size mismatch)   */
+#define HN_ERR_HIP 2          /* a HIP runtime call failed                    */
+#define HN_ERR_NOMEM 3        /* device allocation failed                     */
+#define HN_ERR_UNSUPPORTED 4  /* shape outside what the kernels support       */
+#define HN_ERR_NOT_INIT 5     /* hn_init() has not succeeded                  */
+#define HN_ERR_NO_DEVICE 6    /* no usable gfx950 device                      */
+
+/* ---- precision selectors for _dev entry points -------------------------- */
+#define HN_F64 1
+#define HN_F32 2
+
+/* ---- lifecycle ---------------------------------------------------------- */
+typedef struct hn_device_info {
+  char name[256];          /* marketing name of the device                    */
+  char arch[64];           /* gcnArchName, e.g. "gfx950:sramecc+:xnack-"       */
+  uint64_t total_memory;   /* bytes of device memory                          */
+  int compute_units;
+  int device;              /* ordinal used                                    */
+} hn_device_info;
+
+/* Select the device (-1 = the one with most memory, as the reference does in
+ * CudaDriverBackend::initialize, gpu_backend.cpp:159-192), create the library
+ * stream.  Replaces IComputeBackend::initialize / DriverApi::load+init_context
+ * (gpu_backend.hpp:25, cuda_driver.cpp:112-145,204-236).  Idempotent. */
+int hn_init(int device, hn_device_info* info);
+/* Releases the stream, workspaces and every live matrix handle.
+ * Replaces IComputeBackend::finalize (gpu_backend.hpp:26). */
+int hn_finalize(void);
+/* Copies the message of the last failure on this thread into buf.
+ * Replaces DriverApi::last_result/result_string (cuda_driver.hpp:41-93). */
+int hn_last_error(char* buf, size_t len);
+/* Replaces IComputeBackend::device_memory/free_memory (gpu_backend.hpp:45-46). */
+int hn_mem_info(size_t* free_bytes, size_t* total_bytes);
+/* Replaces was_gpu_kernel_executed/gpu_kernel_executions/reset_execution_stats
+ * (gpu_backend.hpp:48-50): number of hot-path kernel launches so far. */
+uint64_t hn_kernel_executions(void);
+void hn_reset_execution_stats(void);
+
+/* ---- device-resident CSR matrices --------------------------------------- */
+typedef struct hn_csr hn_csr;
+
+/* Upload a CSR matrix (64-bit indices as in the reference's std::size_t
+ * vectors, gpu_backend.hpp:28-32) once; the handle keeps it resident.
+ * Indices are narrowed to int32 on the device (nnz, nrows, ncols < 2^31,
+ * otherwise HN_ERR_UNSUPPORTED).  flags: HN_F64 and/or HN_F32 copies of the
+ * values to keep.  Replaces the pointer-keyed residency cache
+ * CudaDriverBackend::resident_matrix/upload_matrix (gpu_backend.cpp:287-306,
+ * 360-393) with an explicit handle. */
+int hn_csr_upload(const double* values, const uint64_t* col_indices,
+                  const uint64_t* row_ptr, uint64_t nrows, uint64_t ncols,
+                  uint64_t nnz, int flags, hn_csr** out);
+/* Build the transpose on the device by a stable counting sort: entries of
+ * each output row appear in increasing source-row order, exactly the layout
+ * of the counting scatter in build_normal_equations
+ * (core/lp/interior_point.cpp:153-173).  Duplicates and explicit zeros are
+ * kept (the scatter keeps them; SparseMatrix::transpose would drop zero sums,
+ * core/numerical/sparse_matrix.cpp:84-96,250-254). */
+int hn_csr_transpose(const hn_csr* a, hn_csr** out);
+int hn_csr_free(hn_csr* a);
+int hn_csr_dims(const hn_csr* a, uint64_t* nrows, uint64_t* ncols, uint64_t* nnz);
+/* Copy the device CSR back (row_ptr: nrows+1, col/val: nnz entries). */
+int hn_csr_download(const hn_csr* a, double* values, uint64_t* col_indices,
+                    uint64_t* row_ptr);
+/* Merge-path tiling of SpMV: items (row ends + nonzeros) per workgroup is
+ * 256*items_per_thread.  Valid items_per_thread: 4, 8, 16. */
+int hn_csr_set_tiling(hn_csr* a, int items_per_thread);
+int hn_csr_get_tiling(const hn_csr* a, int* items_per_thread);
+/* Time every valid tiling on this matrix (iters launches each, FP64 and FP32
+ * separately when both copies exist) and keep the fastest.  best_us (may be
+ * NULL) receives the best FP64 time per product in microseconds. */
+int hn_csr_autotune(hn_csr* a, int iters, double* best_us);
+
+/* ---- drop-in host-buffer operations (IComputeBackend semantics) -------- */
+/* y = A*x with host buffers; x has ncols entries, y receives nrows entries.
+ * Replaces IComputeBackend::spmv (gpu_backend.hpp:28-32,
+ * CudaDriverBackend::spmv gpu_backend.cpp:200-215). */
+int hn_spmv(const hn_csr* a, const double* x, double* y);
+/* C = A*B, B row-major ncols x k, C row-major nrows x k.
+ * Replaces IComputeBackend::spmm (gpu_backend.hpp:34-39). */
+int hn_spmm(const hn_csr* a, const double* B, double* C, uint64_t k);
+/* Replace IComputeBackend::dot/axpy/norm_inf (gpu_backend.hpp:41-43). */
+int hn_dot(const double* a, const double* b, uint64_t n, double* out);
+int hn_axpy(double alpha, const double* x, double* y, uint64_t n);
+int hn_norm_inf(const double* v, uint64_t n, double* out);
+
+/* ---- device-resident operations ----------------------------------------- */
+/* y = A*x on device buffers in the given precision (HN_F64: double x/y,
+ * HN_F32: float x/y with float accumulation). */
+int hn_spmv_dev(const hn_csr* a, int prec, const void* x, void* y, void* stream);
+/* C = A*B, device buffers, row-major, FP64. */
+int hn_spmm_dev(const hn_csr* a, const double* B, double* C, uint64_t k, void* stream);
+/* out[0] = a.b, out[0] = max|v|, y += alpha*x on device buffers (FP64);
+ * out is a device pointer. */
+int hn_dot_dev(const double* a, const double* b, uint64_t n, double* out, void* stream);
+int hn_norm_inf_dev(const double* v, uint64_t n, double* out, void* stream);
+int hn_axpy_dev(double alpha, const double* x, double* y, uint64_t n, void* stream);
+
+/* ---- the interior-point solver ------------------------------------------ */
+/* Constraint senses in the order of ConstraintSense (core/model/problem.hpp:24-28). */
+#define HN_LE 0
+#define HN_GE 1
+#define HN_EQ 2
+
+/* Result status in the order of ProblemStatus (core/model/problem.hpp:35-45). */
+#define HN_STATUS_UNKNOWN 0
+#define HN_STATUS_OPTIMAL 1
+#define HN_STATUS_INFEASIBLE 2
+#define HN_STATUS_UNBOUNDED 3
+#define HN_STATUS_SUBOPTIMAL 4
+#define HN_STATUS_TIME_LIMIT 5
+#define HN_STATUS_ITER_LIMIT 6
+#define HN_STATUS_NUMERICAL_ERROR 7
+#define HN_STATUS_INTERRUPTED 8
+
+/* An LP in the shape of the reference's model::Problem
+ * (core/model/problem.hpp:79-170): CSR constraint rows, one sense and rhs per
+ * row, bounds and objective per variable.  Rows are normalised the way
+ * ProblemBuilder::build does (columns sorted, duplicates summed, exact zeros
+ * dropped, problem.hpp:265-281). */
+typedef struct hn_lp {
+  int64_t m, n;
+  const int64_t* row_ptr;  /* m+1 */
+  const int64_t* col;      /* nnz */
+  const double* val;       /* nnz */
+  const int8_t* sense;     /* m: HN_LE / HN_GE / HN_EQ */
+  const double* rhs;       /* m */
+  const double* lb;        /* n (-INFINITY = none) */
+  const double* ub;        /* n (+INFINITY = none) */
+  const double* obj;       /* n */
+  int maximize;
+  double obj_offset;
+} hn_lp;
+
+/* Normal-equation solver of each Newton step. */
+#define HN_NE_AUTO 0   /* dense below dense_max_m rows, else PCG             */
+#define HN_NE_DENSE 1  /* dense inverse of M D M' + FP64 iterative refinement */
+#define HN_NE_PCG 2    /* Jacobi-preconditioned CG on M D M' (matrix-free)   */
+/* Arithmetic of the normal-equation solve. */
+#define HN_PREC_F64 1   /* plain FP64 everywhere                              */
+#define HN_PREC_MIXED 2 /* FP32 bulk, FP64 residuals/refinement, FP64 below mu_switch */
+
+typedef int (*hn_interrupt_fn)(void* user);
+
+/* InteriorPointOptions (core/lp/interior_point.hpp:16-27) plus the GPU
+ * choices.  Crossover (the CPU simplex polish, :23-24) is not part of this
+ * library: the caller runs it on the returned point if it wants one. */
+typedef struct hn_ipm_opts {
+  int max_iterations;          /* 100 */
+  double time_limit_seconds;   /* 0 = none */
+  double convergence_tol;      /* 1e-8 */
+  double complementarity_tol;  /* 1e-8 */
+  double regularization;       /* 1e-8 */
+  hn_interrupt_fn interrupt_callback; /* may be NULL */
+  void* interrupt_user;
+  int solver;                  /* HN_NE_* */
+  int precision;               /* HN_PREC_* */
+  double mu_switch;            /* mixed: FP64 normal equations once mu < mu_switch (1e-4) */
+  int64_t dense_max_m;         /* HN_NE_AUTO threshold (40000) */
+  int pcg_max_iter;            /* per solve (2000) */
+  const double* x0;            /* optional warm start of the scaled standard form: */
+  const double* y0;            /* x0, s0 of length sf_n, y0 of length sf_m        */
+  const double* s0;            /* (all three or none)                              */
+  int verbose;                 /* nonzero: per-iteration trace on stderr          */
+} hn_ipm_opts;
+
+typedef struct hn_ipm_result {
+  int status;                  /* HN_STATUS_* */
+  int64_t iterations;
+  double objective;            /* original space, obj_offset included */
+  double solve_ms;             /* wall time of hn_ipm_solve */
+  /* caller buffers (each may be NULL): primal has n entries; x, s hold the
+   * scaled standard form (capacity cap_n), y capacity cap_m */
+  double* primal;
+  double* x;
+  double* y;
+  double* s;
+  int64_t cap_m, cap_n;
+  int64_t sf_m, sf_n;          /* out: standard-form dimensions */
+  /* the reference's stopping quantities on the scaled form (interior_point.cpp:525-530) */
+  double rp_norm, rd_norm, mu;
+  /* original-space relative KKT metrics */
+  double rel_primal;           /* ||Ax-b||_inf / (1 + ||b||_inf) over the scaled rows */
+  double rel_dual;             /* ||M'y + s - c||_inf / (1 + ||c||_inf) */
+  double rel_gap;              /* |c'x - b'y| / (1 + |c'x|) */
+  int64_t ne_solves;           /* normal-equation solves */
+  int64_t refine_sweeps;       /* FP64 refinement sweeps */
+  int64_t cg_iterations;       /* PCG iterations */
+  int64_t fp64_from_iter;      /* first iteration solved in FP64 (-1: never) */
+  int warm_started;            /* 1 when x0/y0/s0 were used */
+} hn_ipm_result;
+
+int hn_ipm_default_opts(hn_ipm_opts* o);
+/* InteriorPointSolver::solve (core/lp/interior_point.cpp:402-886) with the
+ * Newton steps computed on the GPU.  Same standard form, start point, step
+ * rule, sigma, termination and status mapping as the reference; the normal
+ * equations are solved by the method in opts->solver. */
+int hn_ipm_solve(const hn_lp* lp, const hn_ipm_opts* opts, hn_ipm_result* res);
+
+/* ---- harness helpers (device buffers, copies, events) ------------------- */
+int hn_dev_alloc(size_t bytes, void** ptr);
+int hn_dev_free(void* ptr);
+int hn_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int hn_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int hn_stream_sync(void* stream);
+/* Plain device-to-device copy kernel (16-byte vector loads/stores); bytes
+ * must be a multiple of 16.  Used as the known-size reference when scaling
+ * hardware byte counters. */
+int hn_copy_dev(void* dst, const void* src, size_t bytes, void* stream);
+/* Elapsed milliseconds between two events recorded on a stream. */
+typedef struct hn_event hn_event;
+int hn_event_create(hn_event** ev);
+int hn_event_destroy(hn_event* ev);
+int hn_event_record(hn_event* ev, void* stream);
+int hn_event_elapsed_ms(const hn_event* start, const hn_event* stop, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HN_KERNELS_H */

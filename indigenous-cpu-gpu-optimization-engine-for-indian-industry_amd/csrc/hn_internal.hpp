@@ -1,0 +1,138 @@
+// Internal declarations shared by the translation units of libhn_kernels.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hn_kernels.h"
+
+namespace hn {
+
+// ---- error reporting ------------------------------------------------------
+// The message of the last failure is kept per host thread; every entry point
+// returns a status code and never throws across the C ABI.
+void set_error(const std::string& msg);
+const std::string& get_error();
+
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define HN_HIP(call)                                                        \
+  do {                                                                      \
+    hipError_t hn_e_ = (call);                                              \
+    if (hn_e_ != hipSuccess) return ::hn::hip_fail(hn_e_, #call, __FILE__, __LINE__); \
+  } while (0)
+
+#define HN_CHECK(cond, code, msg)                                           \
+  do {                                                                      \
+    if (!(cond)) return ::hn::fail((code), (msg));                          \
+  } while (0)
+
+#define HN_TRY(expr)                                                        \
+  do {                                                                      \
+    int hn_s_ = (expr);                                                     \
+    if (hn_s_ != HN_OK) return hn_s_;                                       \
+  } while (0)
+
+// ---- library state ----------------------------------------------------------
+struct Context {
+  bool initialized = false;
+  int device = -1;
+  hipStream_t stream = nullptr;
+  hn_device_info info{};
+  std::atomic<uint64_t> launches{0};
+};
+Context& ctx();
+
+// Makes the library device current on the calling thread; fails if hn_init
+// has not succeeded.
+int enter();
+inline hipStream_t pick_stream(void* s) {
+  return s ? static_cast<hipStream_t>(s) : ctx().stream;
+}
+inline void count_launch(uint64_t n = 1) { ctx().launches.fetch_add(n, std::memory_order_relaxed); }
+
+// Checks the launch that was just enqueued.
+int check_launch(const char* what);
+
+// Grow-only device scratch owned by one object (never shrinks until freed).
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int reserve(size_t want);
+  void release();
+  ~DevBuf() { release(); }
+  template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Library-wide scratch for the host-buffer entry points (serialised by mtx).
+struct HostOpScratch {
+  std::mutex mtx;
+  DevBuf a, b, c, red;
+};
+HostOpScratch& host_scratch();
+
+// ---- device CSR -------------------------------------------------------------
+constexpr int kSpmvBlock = 256;
+
+struct Tiling {
+  int ipt = 0;            // items (row ends + nonzeros) per thread
+  int ntiles = 0;
+  DevBuf coord;           // int2[ntiles + 1]: (row, nnz) where each tile starts
+  DevBuf carry_row;       // int[ntiles]
+  DevBuf carry_val64;     // double[ntiles]
+  DevBuf carry_val32;     // float[ntiles]
+};
+
+}  // namespace hn
+
+struct hn_csr {
+  int64_t nrows = 0, ncols = 0, nnz = 0;
+  int flags = 0;
+  hn::DevBuf row_ptr;     // int32[nrows + 1]
+  hn::DevBuf col;         // int32[nnz]
+  hn::DevBuf val64;       // double[nnz]   (if flags & HN_F64)
+  hn::DevBuf val32;       // float[nnz]    (if flags & HN_F32)
+  hn::Tiling tiling;      // current merge-path tiling
+  hn::DevBuf spmm_scratch;
+};
+
+namespace hn {
+
+// Builds (or rebuilds) the merge-path tiling for the given items per thread.
+int build_tiling(hn_csr* a, int ipt, hipStream_t s);
+// Allocates an empty CSR of the given shape (values per flags).
+int csr_alloc(hn_csr* a, int64_t nrows, int64_t ncols, int64_t nnz, int flags);
+
+// y = A x on device buffers (FP64 or FP32) with the handle's current tiling.
+int spmv_launch(const hn_csr* a, int prec, const void* x, void* y, hipStream_t s);
+
+// Sorts keys[ptr[s]..ptr[s+1]) ascending for every segment s (duplicates
+// allowed); synchronises the stream once.
+struct SortScratch {
+  DevBuf lists, counters, tmp;
+};
+int segmented_sort(const int* ptr, int nseg, int* keys, int64_t total, SortScratch& sc, hipStream_t s);
+// out[0..n] = exclusive scan of in[0..n-1], out[n] = total (int range).
+int exclusive_scan(const int* in, int64_t n, int* out, DevBuf& tmp, hipStream_t s);
+
+// Two-pass deterministic FP64 reductions into a device scalar.
+int dot_launch(const double* a, const double* b, int64_t n, double* out, hipStream_t s);
+int norm_inf_launch(const double* v, int64_t n, double* out, hipStream_t s);
+int axpy_launch(double alpha, const double* x, double* y, int64_t n, hipStream_t s);
+int fill_launch(double* p, double v, int64_t n, hipStream_t s);
+
+inline unsigned grid_for(int64_t n, int block, int64_t cap = 1 << 20) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return static_cast<unsigned>(g);
+}
+
+}  // namespace hn

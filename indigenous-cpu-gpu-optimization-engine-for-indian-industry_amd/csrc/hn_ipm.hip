@@ -1,0 +1,887 @@
+// Mehrotra predictor-corrector IPM with device-resident iterates.
+//
+// Reference: InteriorPointSolver::solve (core/lp/interior_point.cpp:402-886).
+// The host keeps the reference's control flow - start point, convergence
+// test, sigma rule, step rule, guards, relaxed post-loop test and status
+// mapping - and reads back only the scalars it decides on.  Every vector
+// operation runs on the GPU:
+//   residuals  rp = M x - b, rd = M'lam + z - c        (SpMV on M and on the
+//              pre-built M', then fused elementwise kernels)
+//   scalings   D = x/z clamped to [0, 1e10], D rd
+//   Newton     rhs = -rp - M(D rd) + M x; N dlam = rhs (normal equations,
+//              hn_normal.hip) with FP64 iterative refinement
+//   direction  dx = D (M'dlam + rd) - x [+ zs], dz = -(M'dlam + rd)
+//   ratio test fused min-reductions of -v/d over d < 0
+//   update     x += ap dx, z += ad dz, lam += ad dlam with the guards.
+// Elementwise expressions are evaluated in the reference's order without
+// FMA contraction, so they round like the CPU code; reductions use a fixed
+// two-pass tree (deterministic, different summation order).
+#include "hn_device.hpp"
+#include "hn_internal.hpp"
+#include "hn_ipm_internal.hpp"
+#include "hn_stdform.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#pragma clang fp contract(off)
+
+namespace hn {
+
+namespace {
+
+enum Slot {
+  S_XZ = 0,    // sum x z
+  S_RPN,       // ||rp||_inf
+  S_RDN,       // ||rd||_inf
+  S_AP_A,      // predictor primal step
+  S_AD_A,      // predictor dual step
+  S_MU_AFF,    // sum (x + ap dx)(z + ad dz)
+  S_AP,        // corrector primal step
+  S_AD,        // corrector dual step
+  S_REG,       // dynamic regularisation of N
+  S_IRR,       // ||rhs - N dlam||_inf
+  S_IRB,       // ||rhs||_inf
+  S_CTX, S_BTY, S_DRN,  // c'x, b'y, ||M'y + s - c||_inf (final metrics)
+  S_PCG_RZ, S_PCG_PQ, S_PCG_RN, S_PCG_RZ_OLD,
+  S_COUNT
+};
+enum Flag { F_DIR = 0, F_UPD, F_COUNT };
+
+constexpr int kNT = 256;
+constexpr int kMaxParts = 512;
+
+int parts_for(int64_t n) {
+  return (int)std::min<int64_t>(kMaxParts, std::max<int64_t>(1, (n + kNT * 8 - 1) / (kNT * 8)));
+}
+
+// ---- elementwise ---------------------------------------------------------
+__global__ void k_sub(const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ o, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) o[i] = a[i] - b[i];
+}
+
+__global__ void k_rd(const double* __restrict__ t, const double* __restrict__ z, const double* __restrict__ c,
+                     double* __restrict__ rd, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) rd[i] = t[i] + z[i] - c[i];  // interior_point.cpp:515
+}
+
+__global__ void k_scaling(const double* __restrict__ x, const double* __restrict__ z, const double* __restrict__ rd,
+                          double* __restrict__ D, double* __restrict__ Drd, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = x[i] / z[i];  // interior_point.cpp:532-539
+  if (!(d >= 0.0) || !isfinite(d) || d > 1e10) d = 1e10;
+  D[i] = d;
+  Drd[i] = d * rd[i];      // :584
+}
+
+__global__ void k_rhs(const double* __restrict__ rp, const double* __restrict__ mdrd, const double* __restrict__ mx,
+                      double* __restrict__ rhs, int m) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) rhs[i] = -rp[i] - mdrd[i] + mx[i];  // :598-600
+}
+
+__global__ void k_direction(const double* __restrict__ D, const double* __restrict__ t,
+                            const double* __restrict__ rd, const double* __restrict__ x,
+                            const double* __restrict__ zs, double* __restrict__ dx, double* __restrict__ dz, int n,
+                            int* __restrict__ flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double s = t[i] + rd[i];
+  double a = D[i] * s - x[i];     // :610 / :668
+  if (zs) a = a + zs[i];
+  const double b = -s;            // :614 / :672
+  dx[i] = a;
+  dz[i] = b;
+  if (!isfinite(a) || !isfinite(b)) flags[F_DIR] = 1;
+}
+
+__global__ void k_finite(const double* __restrict__ v, int n, int* __restrict__ flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && !isfinite(v[i])) flags[F_DIR] = 1;
+}
+
+__global__ void k_corrector(const double* __restrict__ dx, const double* __restrict__ dz,
+                            const double* __restrict__ z, double sigma_mu, double* __restrict__ zs, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double zinv = 1.0 / z[i];  // :645-650
+  zs[i] = (sigma_mu - dx[i] * dz[i]) * zinv;
+}
+
+__global__ void k_update(double* __restrict__ x, double* __restrict__ z, const double* __restrict__ dx,
+                         const double* __restrict__ dz, const double* __restrict__ S, int n,
+                         int* __restrict__ flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double xi = x[i] + S[S_AP] * dx[i];  // :690-693
+  const double zi = z[i] + S[S_AD] * dz[i];
+  x[i] = xi;
+  z[i] = zi;
+  if (xi < 0.0 || zi < 0.0 || !isfinite(xi) || !isfinite(zi)) flags[F_UPD] = 1;  // :698-708
+}
+
+__global__ void k_update_lam(double* __restrict__ lam, const double* __restrict__ dlam,
+                             const double* __restrict__ S, int m) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) lam[i] = lam[i] + S[S_AD] * dlam[i];  // :694-696
+}
+
+__global__ void k_fill(double* __restrict__ p, double v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---- reductions (fixed two-pass trees) -------------------------------------
+// sum x z over n; max |rp| over m; max |rd| over n
+__global__ __launch_bounds__(kNT) void k_red_start(const double* __restrict__ x, const double* __restrict__ z,
+                                                   const double* __restrict__ rp, const double* __restrict__ rd,
+                                                   int n, int m, double* __restrict__ part) {
+  __shared__ double red[kNT / kWave];
+  double s = 0.0, a = 0.0, b = 0.0;
+  for (int i = blockIdx.x * kNT + threadIdx.x; i < n; i += gridDim.x * kNT) {
+    s += x[i] * z[i];
+    b = nan_ignoring_max(b, fabs(rd[i]));
+  }
+  for (int i = blockIdx.x * kNT + threadIdx.x; i < m; i += gridDim.x * kNT) a = nan_ignoring_max(a, fabs(rp[i]));
+  s = block_sum<double, kNT>(s, red);
+  a = block_max<double, kNT>(a, red);
+  b = block_max<double, kNT>(b, red);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s;
+    part[kMaxParts + blockIdx.x] = a;
+    part[2 * kMaxParts + blockIdx.x] = b;
+  }
+}
+
+__global__ __launch_bounds__(kNT) void k_fin_start(const double* __restrict__ part, int np, double* __restrict__ S) {
+  __shared__ double red[kNT / kWave];
+  double s = 0.0, a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < np; i += kNT) {
+    s += part[i];
+    a = nan_ignoring_max(a, part[kMaxParts + i]);
+    b = nan_ignoring_max(b, part[2 * kMaxParts + i]);
+  }
+  s = block_sum<double, kNT>(s, red);
+  a = block_max<double, kNT>(a, red);
+  b = block_max<double, kNT>(b, red);
+  if (threadIdx.x == 0) {
+    S[S_XZ] = s;
+    S[S_RPN] = a;
+    S[S_RDN] = b;
+  }
+}
+
+// ratio test of (x, dx) and (z, dz): min over d < 0 of -v/d
+__global__ __launch_bounds__(kNT) void k_red_step(const double* __restrict__ x, const double* __restrict__ dx,
+                                                  const double* __restrict__ z, const double* __restrict__ dz, int n,
+                                                  double* __restrict__ part) {
+  __shared__ double red[kNT / kWave];
+  double a = 1.0, b = 1.0;
+  for (int i = blockIdx.x * kNT + threadIdx.x; i < n; i += gridDim.x * kNT) {
+    const double d1 = dx[i], d2 = dz[i];
+    if (d1 < 0.0) {
+      const double q = -x[i] / d1;
+      a = (q < a) ? q : a;
+    }
+    if (d2 < 0.0) {
+      const double q = -z[i] / d2;
+      b = (q < b) ? q : b;
+    }
+  }
+  a = block_min<double, kNT>(a, red);
+  b = block_min<double, kNT>(b, red);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[kMaxParts + blockIdx.x] = b;
+  }
+}
+
+__global__ __launch_bounds__(kNT) void k_fin_step(const double* __restrict__ part, int np, double* __restrict__ S,
+                                                  int slot_p, int slot_d) {
+  __shared__ double red[kNT / kWave];
+  double a = 1.0, b = 1.0;
+  for (int i = threadIdx.x; i < np; i += kNT) {
+    a = (part[i] < a) ? part[i] : a;
+    b = (part[kMaxParts + i] < b) ? part[kMaxParts + i] : b;
+  }
+  a = block_min<double, kNT>(a, red);
+  b = block_min<double, kNT>(b, red);
+  if (threadIdx.x == 0) {
+    const double sa = 0.99 * a, sb = 0.99 * b;  // interior_point.cpp:141
+    S[slot_p] = (sa < 1.0) ? sa : 1.0;
+    S[slot_d] = (sb < 1.0) ? sb : 1.0;
+  }
+}
+
+// sum (x + ap dx)(z + ad dz) with the predictor steps (:632-636)
+__global__ __launch_bounds__(kNT) void k_red_muaff(const double* __restrict__ x, const double* __restrict__ dx,
+                                                   const double* __restrict__ z, const double* __restrict__ dz, int n,
+                                                   const double* __restrict__ S, double* __restrict__ part) {
+  __shared__ double red[kNT / kWave];
+  const double ap = S[S_AP_A], ad = S[S_AD_A];
+  double s = 0.0;
+  for (int i = blockIdx.x * kNT + threadIdx.x; i < n; i += gridDim.x * kNT) s += (x[i] + ap * dx[i]) * (z[i] + ad * dz[i]);
+  s = block_sum<double, kNT>(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kNT) void k_fin_sum(const double* __restrict__ part, int np, double* __restrict__ S,
+                                                 int slot) {
+  __shared__ double red[kNT / kWave];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < np; i += kNT) s += part[i];
+  s = block_sum<double, kNT>(s, red);
+  if (threadIdx.x == 0) S[slot] = s;
+}
+
+// refinement residual: r = rhs - Nd, ||r||_inf and ||rhs||_inf
+__global__ __launch_bounds__(kNT) void k_red_resid(const double* __restrict__ rhs, const double* __restrict__ nd,
+                                                   double* __restrict__ r, int m, double* __restrict__ part) {
+  __shared__ double red[kNT / kWave];
+  double a = 0.0, b = 0.0;
+  for (int i = blockIdx.x * kNT + threadIdx.x; i < m; i += gridDim.x * kNT) {
+    const double ri = rhs[i] - nd[i];
+    r[i] = ri;
+    a = nan_ignoring_max(a, fabs(ri));
+    b = nan_ignoring_max(b, fabs(rhs[i]));
+  }
+  a = block_max<double, kNT>(a, red);
+  b = block_max<double, kNT>(b, red);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[kMaxParts + blockIdx.x] = b;
+  }
+}
+
+__global__ __launch_bounds__(kNT) void k_fin_max2(const double* __restrict__ part, int np, double* __restrict__ S,
+                                                  int slot_a, int slot_b) {
+  __shared__ double red[kNT / kWave];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < np; i += kNT) {
+    a = nan_ignoring_max(a, part[i]);
+    b = nan_ignoring_max(b, part[kMaxParts + i]);
+  }
+  a = block_max<double, kNT>(a, red);
+  b = block_max<double, kNT>(b, red);
+  if (threadIdx.x == 0) {
+    S[slot_a] = a;
+    S[slot_b] = b;
+  }
+}
+
+// final metrics: c'x, b'y (over different lengths) and ||M'y + s - c||_inf
+__global__ __launch_bounds__(kNT) void k_red_final(const double* __restrict__ c, const double* __restrict__ x,
+                                                   const double* __restrict__ b, const double* __restrict__ y,
+                                                   const double* __restrict__ rd, int n, int m,
+                                                   double* __restrict__ part) {
+  __shared__ double red[kNT / kWave];
+  double s1 = 0.0, s2 = 0.0, a = 0.0;
+  for (int i = blockIdx.x * kNT + threadIdx.x; i < n; i += gridDim.x * kNT) {
+    s1 += c[i] * x[i];
+    a = nan_ignoring_max(a, fabs(rd[i]));
+  }
+  for (int i = blockIdx.x * kNT + threadIdx.x; i < m; i += gridDim.x * kNT) s2 += b[i] * y[i];
+  s1 = block_sum<double, kNT>(s1, red);
+  s2 = block_sum<double, kNT>(s2, red);
+  a = block_max<double, kNT>(a, red);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s1;
+    part[kMaxParts + blockIdx.x] = s2;
+    part[2 * kMaxParts + blockIdx.x] = a;
+  }
+}
+
+__global__ __launch_bounds__(kNT) void k_fin_final(const double* __restrict__ part, int np, double* __restrict__ S) {
+  __shared__ double red[kNT / kWave];
+  double s1 = 0.0, s2 = 0.0, a = 0.0;
+  for (int i = threadIdx.x; i < np; i += kNT) {
+    s1 += part[i];
+    s2 += part[kMaxParts + i];
+    a = nan_ignoring_max(a, part[2 * kMaxParts + i]);
+  }
+  s1 = block_sum<double, kNT>(s1, red);
+  s2 = block_sum<double, kNT>(s2, red);
+  a = block_max<double, kNT>(a, red);
+  if (threadIdx.x == 0) {
+    S[S_CTX] = s1;
+    S[S_BTY] = s2;
+    S[S_DRN] = a;
+  }
+}
+
+// ---- PCG pieces (Jacobi, matrix-free N = M D M' + r I) -----------------------
+__global__ void k_mul(const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ o, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) o[i] = a[i] * b[i];
+}
+
+// q += r p; partial p'q
+__global__ __launch_bounds__(kNT) void k_pcg_pq(double* __restrict__ q, const double* __restrict__ p,
+                                                const double* __restrict__ S, int m, double* __restrict__ part) {
+  __shared__ double red[kNT / kWave];
+  const double reg = S[S_REG];
+  double s = 0.0;
+  for (int i = blockIdx.x * kNT + threadIdx.x; i < m; i += gridDim.x * kNT) {
+    const double qi = q[i] + reg * p[i];
+    q[i] = qi;
+    s += p[i] * qi;
+  }
+  s = block_sum<double, kNT>(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// x += a p; r -= a q; z = r / diag; partials r'z and max |r|
+__global__ __launch_bounds__(kNT) void k_pcg_step(double* __restrict__ xk, double* __restrict__ r,
+                                                  double* __restrict__ zk, const double* __restrict__ p,
+                                                  const double* __restrict__ q, const double* __restrict__ dinv,
+                                                  const double* __restrict__ S, int m, double* __restrict__ part) {
+  __shared__ double red[kNT / kWave];
+  const double alpha = S[S_PCG_RZ] / S[S_PCG_PQ];
+  double s = 0.0, a = 0.0;
+  for (int i = blockIdx.x * kNT + threadIdx.x; i < m; i += gridDim.x * kNT) {
+    xk[i] = xk[i] + alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    const double zi = ri * dinv[i];
+    zk[i] = zi;
+    s += ri * zi;
+    a = nan_ignoring_max(a, fabs(ri));
+  }
+  s = block_sum<double, kNT>(s, red);
+  a = block_max<double, kNT>(a, red);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s;
+    part[kMaxParts + blockIdx.x] = a;
+  }
+}
+
+__global__ __launch_bounds__(kNT) void k_pcg_fin(const double* __restrict__ part, int np, double* __restrict__ S) {
+  __shared__ double red[kNT / kWave];
+  double s = 0.0, a = 0.0;
+  for (int i = threadIdx.x; i < np; i += kNT) {
+    s += part[i];
+    a = nan_ignoring_max(a, part[kMaxParts + i]);
+  }
+  s = block_sum<double, kNT>(s, red);
+  a = block_max<double, kNT>(a, red);
+  if (threadIdx.x == 0) {
+    S[S_PCG_RZ_OLD] = S[S_PCG_RZ];
+    S[S_PCG_RZ] = s;
+    S[S_PCG_RN] = a;
+  }
+}
+
+__global__ void k_pcg_dir(double* __restrict__ p, const double* __restrict__ zk, const double* __restrict__ S, int m) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const double beta = S[S_PCG_RZ] / S[S_PCG_RZ_OLD];
+  p[i] = zk[i] + beta * p[i];
+}
+
+// r = rhs (x0 = 0), z = r/diag, p = z, dinv = 1/(diag + reg); partials r'z, max|r|
+__global__ __launch_bounds__(kNT) void k_pcg_init(const double* __restrict__ rhs, const double* __restrict__ diag,
+                                                  const double* __restrict__ S, double* __restrict__ xk,
+                                                  double* __restrict__ r, double* __restrict__ zk,
+                                                  double* __restrict__ p, double* __restrict__ dinv, int m,
+                                                  double* __restrict__ part) {
+  __shared__ double red[kNT / kWave];
+  const double reg = S[S_REG];
+  double s = 0.0, a = 0.0;
+  for (int i = blockIdx.x * kNT + threadIdx.x; i < m; i += gridDim.x * kNT) {
+    const double di = 1.0 / (diag[i] + reg);
+    dinv[i] = di;
+    xk[i] = 0.0;
+    const double ri = rhs[i];
+    r[i] = ri;
+    const double zi = ri * di;
+    zk[i] = zi;
+    p[i] = zi;
+    s += ri * zi;
+    a = nan_ignoring_max(a, fabs(ri));
+  }
+  s = block_sum<double, kNT>(s, red);
+  a = block_max<double, kNT>(a, red);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s;
+    part[kMaxParts + blockIdx.x] = a;
+  }
+}
+
+// r = max(reg, 1e-12 max diag) for the matrix-free operator
+__global__ __launch_bounds__(kNT) void k_pcg_reg(const double* __restrict__ diag, int m, double reg,
+                                                 double* __restrict__ S) {
+  __shared__ double red[kNT / kWave];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < m; i += kNT) a = nan_ignoring_max(a, diag[i]);
+  a = block_max<double, kNT>(a, red);
+  if (threadIdx.x == 0) S[S_REG] = fmax(reg, 1e-12 * a);
+}
+
+struct Timer {
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  double s() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+// ---------------------------------------------------------------------------
+struct Solver {
+  const hn_ipm_opts& o;
+  hn_ipm_result& res;
+  hipStream_t s;
+  StandardForm f;
+  int m = 0, n = 0;
+  hn_csr* M = nullptr;
+  hn_csr* Mt = nullptr;
+  // device vectors
+  DevBuf b, c, x, z, lam, rp, rd, D, Drd, tn, tm, mdrd, rhs, rhs2, dlam, dx, dz, zs, mz, ir_r, ir_nd, ir_e;
+  DevBuf pcg_r, pcg_z, pcg_p, pcg_q, pcg_dinv, pcg_diag, pcg_tn;
+  DevBuf S, flags, part;
+  double* hS = nullptr;  // pinned mirror of S
+  int* hF = nullptr;
+  NormalEq ne;
+  DenseNE dense;
+  bool use_dense = true;
+  bool fp64_solve = true;  // current normal-equation precision (dense)
+  double reg_now = 0.0;
+  bool dense_ready = false;
+
+  Solver(const hn_ipm_opts& oo, hn_ipm_result& rr, hipStream_t ss) : o(oo), res(rr), s(ss) {}
+  ~Solver() {
+    delete M;
+    delete Mt;
+    if (hS) (void)hipHostFree(hS);
+    if (hF) (void)hipHostFree(hF);
+  }
+
+  dim3 g(int64_t cnt) const { return dim3(grid_for(cnt, 256)); }
+
+  int pull() {  // scalars and flags to the host
+    HN_HIP(hipMemcpyAsync(hS, S.p, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, s));
+    HN_HIP(hipMemcpyAsync(hF, flags.p, sizeof(int) * F_COUNT, hipMemcpyDeviceToHost, s));
+    HN_HIP(hipStreamSynchronize(s));
+    return HN_OK;
+  }
+
+  int setup() {
+    m = (int)f.m;
+    n = (int)f.n;
+    HN_CHECK(f.m < INT_MAX / 2 && f.n < INT_MAX / 2, HN_ERR_UNSUPPORTED, "hn_ipm_solve: LP too large");
+    std::vector<uint64_t> rp64(f.rp.begin(), f.rp.end()), ci64(f.ci.begin(), f.ci.end());
+    const int flags32 = HN_F64 | (o.precision == HN_PREC_MIXED ? HN_F32 : 0);
+    HN_TRY(hn_csr_upload(f.v.data(), ci64.data(), rp64.data(), f.m, f.n, f.v.size(), flags32, &M));
+    HN_TRY(hn_csr_transpose(M, &Mt));
+    const size_t bm = sizeof(double) * std::max(m, 1), bn = sizeof(double) * std::max(n, 1);
+    for (DevBuf* d : {&b, &lam, &rp, &tm, &mdrd, &rhs, &rhs2, &dlam, &mz, &ir_r, &ir_nd, &ir_e}) HN_TRY(d->reserve(bm));
+    for (DevBuf* d : {&c, &x, &z, &rd, &D, &Drd, &tn, &dx, &dz, &zs}) HN_TRY(d->reserve(bn));
+    HN_TRY(S.reserve(sizeof(double) * S_COUNT));
+    HN_TRY(flags.reserve(sizeof(int) * F_COUNT));
+    HN_TRY(part.reserve(sizeof(double) * 3 * kMaxParts));
+    HN_HIP(hipHostMalloc((void**)&hS, sizeof(double) * S_COUNT, hipHostMallocDefault));
+    HN_HIP(hipHostMalloc((void**)&hF, sizeof(int) * F_COUNT, hipHostMallocDefault));
+    HN_HIP(hipMemsetAsync(S.p, 0, sizeof(double) * S_COUNT, s));
+    HN_HIP(hipMemsetAsync(flags.p, 0, sizeof(int) * F_COUNT, s));
+    HN_HIP(hipMemcpyAsync(b.p, f.rhs.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+    HN_HIP(hipMemcpyAsync(c.p, f.c.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    use_dense = o.solver == HN_NE_DENSE || (o.solver == HN_NE_AUTO && f.m <= o.dense_max_m);
+    if (use_dense) {
+      HN_TRY(ne_symbolic(M, Mt, ne, s));
+    } else {
+      for (DevBuf* d : {&pcg_r, &pcg_z, &pcg_p, &pcg_q, &pcg_dinv, &pcg_diag}) HN_TRY(d->reserve(bm));
+      HN_TRY(pcg_tn.reserve(bn));
+    }
+    fp64_solve = o.precision != HN_PREC_MIXED;
+    HN_HIP(hipStreamSynchronize(s));
+    return HN_OK;
+  }
+
+  int spmv(const hn_csr* A, const double* xin, double* yout) { return spmv_launch(A, HN_F64, xin, yout, s); }
+
+  // Factor (dense) or prepare (PCG) the normal equations for the current D.
+  int prepare_ne() {
+    if (use_dense) {
+      HN_TRY(ne_numeric(M, Mt, D.as<double>(), o.regularization, ne, S.as<double>() + S_REG, s));
+      HN_TRY(dense_factor(ne, dense, !fp64_solve, s));
+      dense_ready = true;
+    } else {
+      HN_TRY(ne_jacobi_diag(M, D.as<double>(), pcg_diag.as<double>(), s));
+      hipLaunchKernelGGL(k_pcg_reg, dim3(1), dim3(kNT), 0, s, pcg_diag.as<double>(), m, o.regularization,
+                         S.as<double>());
+      HN_TRY(check_launch("k_pcg_reg"));
+    }
+    return HN_OK;
+  }
+
+  // N d = v by the matrix-free operator (FP64).
+  int apply_N(const double* v, double* out) {
+    if (use_dense) return spmv(ne.N, v, out);
+    HN_TRY(spmv(Mt, v, pcg_tn.as<double>()));
+    hipLaunchKernelGGL(k_mul, g(n), dim3(256), 0, s, pcg_tn.as<double>(), D.as<double>(), pcg_tn.as<double>(), n);
+    HN_TRY(spmv(M, pcg_tn.as<double>(), out));
+    return HN_OK;
+  }
+
+  // Dense: dlam = N^{-1} rhs then refinement against N (refinement.cpp:7-36):
+  // up to 5 corrections, stop when ||rhs - N dlam||_inf <= 1e-10 ||rhs||_inf.
+  int solve_dense(const double* r_in, double* out, bool* converged) {
+    HN_TRY(dense_apply(dense, r_in, out, false, s));
+    ++res.ne_solves;
+    const int np = parts_for(m);
+    *converged = false;
+    for (int it = 0; it <= 5; ++it) {
+      HN_TRY(apply_N(out, ir_nd.as<double>()));
+      hipLaunchKernelGGL(k_red_resid, dim3(np), dim3(kNT), 0, s, r_in, ir_nd.as<double>(), ir_r.as<double>(), m,
+                         part.as<double>());
+      hipLaunchKernelGGL(k_fin_max2, dim3(1), dim3(kNT), 0, s, part.as<double>(), np, S.as<double>(), S_IRR, S_IRB);
+      HN_TRY(pull());
+      const double nb = hS[S_IRB];
+      if (nb == 0.0 || nb != nb) {
+        *converged = true;
+        return HN_OK;
+      }
+      if (hS[S_IRR] / nb <= 1e-10) {
+        *converged = true;
+        return HN_OK;
+      }
+      if (it == 5) break;
+      HN_TRY(dense_apply(dense, ir_r.as<double>(), out, true, s));
+      ++res.refine_sweeps;
+    }
+    return HN_OK;
+  }
+
+  // PCG on the matrix-free N with a Jacobi preconditioner; the relative
+  // tolerance follows mu (tighter as the iterates approach the optimum).
+  int solve_pcg(const double* r_in, double* out, double mu) {
+    const int np = parts_for(m);
+    hipLaunchKernelGGL(k_pcg_init, dim3(np), dim3(kNT), 0, s, r_in, pcg_diag.as<double>(), S.as<double>(), out,
+                       pcg_r.as<double>(), pcg_z.as<double>(), pcg_p.as<double>(), pcg_dinv.as<double>(), m,
+                       part.as<double>());
+    hipLaunchKernelGGL(k_pcg_fin, dim3(1), dim3(kNT), 0, s, part.as<double>(), np, S.as<double>());
+    HN_TRY(pull());
+    const double r0 = hS[S_PCG_RN];
+    const double rel = std::max(1e-11, std::min(1e-3, 0.1 * mu));
+    if (r0 == 0.0) return HN_OK;
+    ++res.ne_solves;
+    for (int it = 0; it < o.pcg_max_iter; ++it) {
+      HN_TRY(apply_N(pcg_p.as<double>(), pcg_q.as<double>()));
+      hipLaunchKernelGGL(k_pcg_pq, dim3(np), dim3(kNT), 0, s, pcg_q.as<double>(), pcg_p.as<double>(), S.as<double>(),
+                         m, part.as<double>());
+      hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(kNT), 0, s, part.as<double>(), np, S.as<double>(), (int)S_PCG_PQ);
+      hipLaunchKernelGGL(k_pcg_step, dim3(np), dim3(kNT), 0, s, out, pcg_r.as<double>(), pcg_z.as<double>(),
+                         pcg_p.as<double>(), pcg_q.as<double>(), pcg_dinv.as<double>(), S.as<double>(), m,
+                         part.as<double>());
+      hipLaunchKernelGGL(k_pcg_fin, dim3(1), dim3(kNT), 0, s, part.as<double>(), np, S.as<double>());
+      hipLaunchKernelGGL(k_pcg_dir, g(m), dim3(256), 0, s, pcg_p.as<double>(), pcg_z.as<double>(), S.as<double>(), m);
+      ++res.cg_iterations;
+      if ((it & 7) == 7) {
+        HN_TRY(pull());
+        if (!(hS[S_PCG_RN] > rel * r0)) break;
+      }
+    }
+    return check_launch("pcg");
+  }
+
+  int solve(const double* r_in, double* out, double mu, bool* escalated) {
+    *escalated = false;
+    if (!use_dense) return solve_pcg(r_in, out, mu);
+    bool ok = false;
+    HN_TRY(solve_dense(r_in, out, &ok));
+    if (!ok && !fp64_solve) {  // FP32 inverse could not be refined: switch to FP64 for good
+      fp64_solve = true;
+      *escalated = true;
+      HN_TRY(dense_factor(ne, dense, false, s));
+      HN_TRY(solve_dense(r_in, out, &ok));
+    }
+    return HN_OK;
+  }
+};
+
+}  // namespace
+
+int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
+  Timer clock;
+  res.status = HN_STATUS_UNKNOWN;
+  res.iterations = 0;
+  res.objective = 0.0;
+  res.sf_m = res.sf_n = 0;
+  res.rp_norm = res.rd_norm = res.mu = 0.0;
+  res.rel_primal = res.rel_dual = res.rel_gap = 0.0;
+  res.ne_solves = res.refine_sweeps = res.cg_iterations = 0;
+  res.fp64_from_iter = -1;
+  res.warm_started = 0;
+  auto finish = [&]() { res.solve_ms = 1e3 * clock.s(); };
+
+  if (lp.n == 0) {  // interior_point.cpp:406-410
+    res.status = HN_STATUS_OPTIMAL;
+    res.objective = lp.obj_offset;
+    finish();
+    return HN_OK;
+  }
+  HN_TRY(enter());
+  hipStream_t s = ctx().stream;
+  Solver sv(o, res, s);
+  HN_TRY(build_standard_form(lp, sv.f));
+  StandardForm& f = sv.f;
+  res.sf_m = f.m;
+  res.sf_n = f.n;
+  std::vector<double> primal(lp.n, 0.0);
+  auto copy_primal = [&]() {
+    if (res.primal) std::memcpy(res.primal, primal.data(), sizeof(double) * lp.n);
+  };
+  if (f.n == 0) {  // :416-421
+    res.status = HN_STATUS_OPTIMAL;
+    res.objective = f.offset;
+    to_original_primal(f, nullptr, primal.data());
+    copy_primal();
+    finish();
+    return HN_OK;
+  }
+  if (f.m == 0) {  // :423-441
+    bool unbounded = false;
+    for (int64_t j = 0; j < f.n; ++j)
+      if (f.c[j] < -1e-9) {
+        unbounded = true;
+        break;
+      }
+    res.status = unbounded ? HN_STATUS_UNBOUNDED : HN_STATUS_OPTIMAL;
+    std::vector<double> y(f.n, 0.0);
+    to_original_primal(f, y.data(), primal.data());
+    double obj = lp.obj_offset;
+    for (int64_t j = 0; j < lp.n; ++j) obj += lp.obj[j] * primal[j];
+    res.objective = obj;
+    copy_primal();
+    finish();
+    return HN_OK;
+  }
+  HN_TRY(sv.setup());
+  const int m = sv.m, n = sv.n;
+  double* S = sv.S.as<double>();
+  int* F = sv.flags.as<int>();
+  double* hS = sv.hS;
+
+  // start point (:443-453) or the caller's warm start
+  {
+    std::vector<double> x0(n), z0(n), l0(m, 0.0);
+    if (o.x0 && o.y0 && o.s0) {
+      std::memcpy(x0.data(), o.x0, sizeof(double) * n);
+      std::memcpy(z0.data(), o.s0, sizeof(double) * n);
+      std::memcpy(l0.data(), o.y0, sizeof(double) * m);
+      res.warm_started = 1;
+    } else {
+      double b_norm = 1.0, c_norm = 1.0;
+      for (double v : f.rhs) b_norm = std::max(b_norm, std::abs(v));
+      for (double v : f.c) c_norm = std::max(c_norm, std::abs(v));
+      for (int j = 0; j < n; ++j) {
+        x0[j] = b_norm + 1.0;
+        z0[j] = std::max(c_norm + 1.0, std::abs(f.c[j]));
+      }
+    }
+    HN_HIP(hipMemcpyAsync(sv.x.p, x0.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    HN_HIP(hipMemcpyAsync(sv.z.p, z0.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    HN_HIP(hipMemcpyAsync(sv.lam.p, l0.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+    HN_HIP(hipStreamSynchronize(s));
+  }
+
+  const dim3 B(256);
+  const int np_n = parts_for(std::max(n, m));
+  bool done = false, time_up = false, interrupted = false, numerical_error = false;
+  auto deadline = [&]() { return o.time_limit_seconds > 0.0 && clock.s() >= o.time_limit_seconds; };
+  // residuals, mu and their norms on the device; host copies in hS
+  auto residuals = [&]() -> int {
+    HN_TRY(sv.spmv(sv.M, sv.x.as<double>(), sv.tm.as<double>()));
+    hipLaunchKernelGGL(k_sub, sv.g(m), B, 0, s, sv.tm.as<double>(), sv.b.as<double>(), sv.rp.as<double>(), m);
+    HN_TRY(sv.spmv(sv.Mt, sv.lam.as<double>(), sv.tn.as<double>()));
+    hipLaunchKernelGGL(k_rd, sv.g(n), B, 0, s, sv.tn.as<double>(), sv.z.as<double>(), sv.c.as<double>(),
+                       sv.rd.as<double>(), n);
+    hipLaunchKernelGGL(k_red_start, dim3(np_n), dim3(kNT), 0, s, sv.x.as<double>(), sv.z.as<double>(),
+                       sv.rp.as<double>(), sv.rd.as<double>(), n, m, sv.part.as<double>());
+    hipLaunchKernelGGL(k_fin_start, dim3(1), dim3(kNT), 0, s, sv.part.as<double>(), np_n, S);
+    HN_TRY(check_launch("residuals"));
+    return sv.pull();
+  };
+
+  int64_t iter = 0;
+  double mu = 0.0;
+  for (iter = 0; iter < o.max_iterations; ++iter) {
+    if (o.interrupt_callback && o.interrupt_callback(o.interrupt_user)) {  // :490-493
+      interrupted = true;
+      break;
+    }
+    if (deadline()) {  // :494-501
+      time_up = true;
+      break;
+    }
+    HN_TRY(residuals());
+    mu = hS[S_XZ] / (double)n;
+    if (hS[S_RPN] < o.convergence_tol && hS[S_RDN] < o.convergence_tol && mu < o.complementarity_tol) {
+      done = true;  // :525-530
+      break;
+    }
+    if (!sv.fp64_solve && mu < o.mu_switch) {
+      sv.fp64_solve = true;
+      if (res.fp64_from_iter < 0) res.fp64_from_iter = iter;
+    }
+    hipLaunchKernelGGL(k_scaling, sv.g(n), B, 0, s, sv.x.as<double>(), sv.z.as<double>(), sv.rd.as<double>(),
+                       sv.D.as<double>(), sv.Drd.as<double>(), n);
+    HN_TRY(sv.prepare_ne());
+    if (deadline()) {
+      time_up = true;
+      break;
+    }
+    // predictor (:584-615)
+    HN_TRY(sv.spmv(sv.M, sv.Drd.as<double>(), sv.mdrd.as<double>()));
+    hipLaunchKernelGGL(k_rhs, sv.g(m), B, 0, s, sv.rp.as<double>(), sv.mdrd.as<double>(), sv.tm.as<double>(),
+                       sv.rhs.as<double>(), m);
+    bool esc = false;
+    HN_TRY(sv.solve(sv.rhs.as<double>(), sv.dlam.as<double>(), mu, &esc));
+    if (esc && res.fp64_from_iter < 0) res.fp64_from_iter = iter;
+    HN_TRY(sv.spmv(sv.Mt, sv.dlam.as<double>(), sv.tn.as<double>()));
+    hipLaunchKernelGGL(k_direction, sv.g(n), B, 0, s, sv.D.as<double>(), sv.tn.as<double>(), sv.rd.as<double>(),
+                       sv.x.as<double>(), (const double*)nullptr, sv.dx.as<double>(), sv.dz.as<double>(), n, F);
+    hipLaunchKernelGGL(k_finite, sv.g(m), B, 0, s, sv.dlam.as<double>(), m, F);
+    // ratio test and mu_aff (:629-643)
+    hipLaunchKernelGGL(k_red_step, dim3(np_n), dim3(kNT), 0, s, sv.x.as<double>(), sv.dx.as<double>(),
+                       sv.z.as<double>(), sv.dz.as<double>(), n, sv.part.as<double>());
+    hipLaunchKernelGGL(k_fin_step, dim3(1), dim3(kNT), 0, s, sv.part.as<double>(), np_n, S, (int)S_AP_A, (int)S_AD_A);
+    hipLaunchKernelGGL(k_red_muaff, dim3(np_n), dim3(kNT), 0, s, sv.x.as<double>(), sv.dx.as<double>(),
+                       sv.z.as<double>(), sv.dz.as<double>(), n, S, sv.part.as<double>());
+    hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(kNT), 0, s, sv.part.as<double>(), np_n, S, (int)S_MU_AFF);
+    HN_TRY(check_launch("predictor"));
+    HN_TRY(sv.pull());
+    if (sv.hF[F_DIR]) {
+      numerical_error = true;
+      break;
+    }
+    const double mu_aff = hS[S_MU_AFF] / (double)n;
+    double sigma;
+    if (!std::isfinite(mu_aff) || mu_aff < 0.0 || !(mu > 0.0))
+      sigma = 0.5;
+    else
+      sigma = std::clamp(std::pow(mu_aff / std::max(mu, 1e-300), 3.0), 0.1, 0.9);
+    // corrector (:645-685)
+    hipLaunchKernelGGL(k_corrector, sv.g(n), B, 0, s, sv.dx.as<double>(), sv.dz.as<double>(), sv.z.as<double>(),
+                       sigma * mu, sv.zs.as<double>(), n);
+    HN_TRY(sv.spmv(sv.M, sv.zs.as<double>(), sv.mz.as<double>()));
+    hipLaunchKernelGGL(k_sub, sv.g(m), B, 0, s, sv.rhs.as<double>(), sv.mz.as<double>(), sv.rhs2.as<double>(), m);
+    HN_TRY(sv.solve(sv.rhs2.as<double>(), sv.dlam.as<double>(), mu, &esc));
+    if (esc && res.fp64_from_iter < 0) res.fp64_from_iter = iter;
+    HN_TRY(sv.spmv(sv.Mt, sv.dlam.as<double>(), sv.tn.as<double>()));
+    hipLaunchKernelGGL(k_direction, sv.g(n), B, 0, s, sv.D.as<double>(), sv.tn.as<double>(), sv.rd.as<double>(),
+                       sv.x.as<double>(), sv.zs.as<double>(), sv.dx.as<double>(), sv.dz.as<double>(), n, F);
+    hipLaunchKernelGGL(k_finite, sv.g(m), B, 0, s, sv.dlam.as<double>(), m, F);
+    hipLaunchKernelGGL(k_red_step, dim3(np_n), dim3(kNT), 0, s, sv.x.as<double>(), sv.dx.as<double>(),
+                       sv.z.as<double>(), sv.dz.as<double>(), n, sv.part.as<double>());
+    hipLaunchKernelGGL(k_fin_step, dim3(1), dim3(kNT), 0, s, sv.part.as<double>(), np_n, S, (int)S_AP, (int)S_AD);
+    HN_TRY(check_launch("corrector"));
+    HN_TRY(sv.pull());
+    if (sv.hF[F_DIR]) {
+      numerical_error = true;
+      break;
+    }
+    // update (:687-708)
+    hipLaunchKernelGGL(k_update, sv.g(n), B, 0, s, sv.x.as<double>(), sv.z.as<double>(), sv.dx.as<double>(),
+                       sv.dz.as<double>(), S, n, F);
+    hipLaunchKernelGGL(k_update_lam, sv.g(m), B, 0, s, sv.lam.as<double>(), sv.dlam.as<double>(), S, m);
+    HN_TRY(check_launch("update"));
+    HN_TRY(sv.pull());
+    if (sv.hF[F_UPD]) {
+      numerical_error = true;
+      break;
+    }
+    if (o.verbose)
+      std::fprintf(stderr, "IPM it=%lld mu=%g rp=%g rd=%g ap=%g ad=%g%s\n", (long long)iter, mu, hS[S_RPN],
+                   hS[S_RDN], hS[S_AP], hS[S_AD], sv.fp64_solve ? "" : " [fp32 NE]");
+    res.iterations = iter + 1;
+  }
+
+  if (!done && !interrupted) {  // :728-753
+    HN_TRY(residuals());
+    mu = hS[S_XZ] / (double)n;
+    if (hS[S_RPN] < 1e-4 && hS[S_RDN] < 1e-4 && mu < 1e-4) done = true;
+  } else if (interrupted) {
+    HN_TRY(residuals());
+    mu = hS[S_XZ] / (double)n;
+  }
+  std::vector<double> hx(n), hz(n), hl(m);
+  HN_HIP(hipMemcpyAsync(hx.data(), sv.x.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  HN_HIP(hipMemcpyAsync(hz.data(), sv.z.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  HN_HIP(hipMemcpyAsync(hl.data(), sv.lam.p, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+  // final metrics on the scaled form
+  hipLaunchKernelGGL(k_red_final, dim3(np_n), dim3(kNT), 0, s, sv.c.as<double>(), sv.x.as<double>(),
+                     sv.b.as<double>(), sv.lam.as<double>(), sv.rd.as<double>(), n, m, sv.part.as<double>());
+  hipLaunchKernelGGL(k_fin_final, dim3(1), dim3(kNT), 0, s, sv.part.as<double>(), np_n, S);
+  HN_TRY(sv.pull());
+  const double rpn = hS[S_RPN];
+  to_original_primal(f, hx.data(), primal.data());  // :757
+  double obj = 0.0;                                  // :759-769
+  bool obj_ok = std::isfinite(lp.obj_offset);
+  for (int64_t j = 0; j < lp.n && obj_ok; ++j) {
+    const double term = lp.obj[j] * primal[j];
+    if (!std::isfinite(term)) obj_ok = false;
+    obj += term;
+  }
+  if (obj_ok) obj += lp.obj_offset;
+  if (!obj_ok) numerical_error = true;
+  res.objective = obj_ok ? obj : 0.0;
+  const bool feasible_incumbent = rpn <= 1e-9;       // :773-780
+  res.status = numerical_error ? HN_STATUS_NUMERICAL_ERROR
+               : interrupted   ? HN_STATUS_INTERRUPTED
+               : done          ? HN_STATUS_OPTIMAL
+               : feasible_incumbent ? HN_STATUS_SUBOPTIMAL
+               : time_up       ? HN_STATUS_TIME_LIMIT
+                               : HN_STATUS_ITER_LIMIT;
+  res.rp_norm = rpn;
+  res.rd_norm = hS[S_RDN];
+  res.mu = mu;
+  double bn = 0.0, cn = 0.0;
+  for (double v : f.rhs) bn = std::max(bn, std::abs(v));
+  for (double v : f.c) cn = std::max(cn, std::abs(v));
+  res.rel_primal = rpn / (1.0 + bn);
+  res.rel_dual = hS[S_DRN] / (1.0 + cn);
+  res.rel_gap = std::abs(hS[S_CTX] - hS[S_BTY]) / (1.0 + std::abs(hS[S_CTX]));
+  if (res.x && res.cap_n >= n) std::memcpy(res.x, hx.data(), sizeof(double) * n);
+  if (res.s && res.cap_n >= n) std::memcpy(res.s, hz.data(), sizeof(double) * n);
+  if (res.y && res.cap_m >= m) std::memcpy(res.y, hl.data(), sizeof(double) * m);
+  copy_primal();
+  finish();
+  return HN_OK;
+}
+
+}  // namespace hn
+
+extern "C" int hn_ipm_default_opts(hn_ipm_opts* o) {
+  if (!o) return HN_ERR_INVALID;
+  std::memset(o, 0, sizeof(*o));
+  o->max_iterations = 100;
+  o->time_limit_seconds = 0.0;
+  o->convergence_tol = 1e-8;
+  o->complementarity_tol = 1e-8;
+  o->regularization = 1e-8;
+  o->solver = HN_NE_AUTO;
+  o->precision = HN_PREC_F64;
+  o->mu_switch = 1e-4;
+  o->dense_max_m = 40000;
+  o->pcg_max_iter = 2000;
+  return HN_OK;
+}
+
+extern "C" int hn_ipm_solve(const hn_lp* lp, const hn_ipm_opts* opts, hn_ipm_result* res) {
+  HN_CHECK(lp && res, HN_ERR_INVALID, "hn_ipm_solve: null argument");
+  hn_ipm_opts o;
+  if (opts)
+    o = *opts;
+  else
+    hn_ipm_default_opts(&o);
+  HN_CHECK(o.precision == HN_PREC_F64 || o.precision == HN_PREC_MIXED, HN_ERR_INVALID, "hn_ipm_solve: bad precision");
+  HN_CHECK(o.solver >= HN_NE_AUTO && o.solver <= HN_NE_PCG, HN_ERR_INVALID, "hn_ipm_solve: bad solver");
+  HN_CHECK(o.max_iterations >= 0, HN_ERR_INVALID, "hn_ipm_solve: negative max_iterations");
+  return hn::ipm_solve_impl(*lp, o, *res);
+}

@@ -1,0 +1,250 @@
+// Host-side LP preparation for the device IPM.  See hn_stdform.hpp.
+//
+// The three steps follow the reference exactly because the device iterates
+// on their output and must reach the same point:
+//   - rows normalised like ProblemBuilder::build (problem.hpp:265-281);
+//   - a row whose sorted entries, sense and rhs are bit-identical to an
+//     earlier row is dropped (remove_duplicate_rows, interior_point.cpp:82-132);
+//   - bound shifts, free splits, LE/GE slacks, upper-bound rows, row scale
+//     1/max|a_ij| and column scale 1/max|m_ij| (interior_point.cpp:202-373).
+#include "hn_stdform.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <unordered_map>
+
+#include "hn_internal.hpp"
+
+namespace hn {
+
+namespace {
+
+constexpr double kFeasTol = 1e-9;  // ToleranceConfig::industrial_defaults (tolerance.hpp:60-75)
+
+struct Trip {
+  int64_t r, c;
+  double v;
+};
+
+// CSR from triplets: sort by (row, col, insertion), sum duplicates, drop
+// exact-zero sums (SparseMatrix::from_triplets/sum_duplicates,
+// sparse_matrix.cpp:8-53,230-262).
+void csr_from_triplets(int64_t rows, std::vector<Trip>& t, std::vector<int64_t>& rp,
+                       std::vector<int64_t>& ci, std::vector<double>& v) {
+  std::stable_sort(t.begin(), t.end(), [](const Trip& a, const Trip& b) {
+    return a.r != b.r ? a.r < b.r : a.c < b.c;
+  });
+  rp.assign(rows + 1, 0);
+  ci.clear();
+  v.clear();
+  ci.reserve(t.size());
+  v.reserve(t.size());
+  size_t k = 0;
+  for (int64_t i = 0; i < rows; ++i) {
+    rp[i] = (int64_t)ci.size();
+    while (k < t.size() && t[k].r == i) {
+      const int64_t c = t[k].c;
+      double s = t[k].v;
+      ++k;
+      while (k < t.size() && t[k].r == i && t[k].c == c) s += t[k++].v;
+      if (s != 0.0) {
+        ci.push_back(c);
+        v.push_back(s);
+      }
+    }
+  }
+  rp[rows] = (int64_t)ci.size();
+}
+
+uint64_t bits(double d) {
+  uint64_t u;
+  std::memcpy(&u, &d, sizeof u);
+  return u;
+}
+
+}  // namespace
+
+int build_standard_form(const hn_lp& lp, StandardForm& f) {
+  const int64_t m0 = lp.m, n = lp.n;
+  if (m0 < 0 || n < 0) return fail(HN_ERR_INVALID, "hn_ipm_solve: negative dimensions");
+  if ((m0 > 0 && (!lp.row_ptr || !lp.sense || !lp.rhs)) || (n > 0 && (!lp.lb || !lp.ub || !lp.obj)))
+    return fail(HN_ERR_INVALID, "hn_ipm_solve: null problem array");
+  const int64_t nz0 = m0 > 0 ? lp.row_ptr[m0] : 0;
+  if (m0 > 0 && lp.row_ptr[0] != 0) return fail(HN_ERR_INVALID, "hn_ipm_solve: row_ptr[0] must be 0");
+  if (nz0 > 0 && (!lp.col || !lp.val)) return fail(HN_ERR_INVALID, "hn_ipm_solve: null col/val");
+  for (int64_t i = 0; i < m0; ++i) {
+    if (lp.row_ptr[i + 1] < lp.row_ptr[i]) return fail(HN_ERR_INVALID, "hn_ipm_solve: row_ptr decreasing");
+    if (lp.sense[i] < HN_LE || lp.sense[i] > HN_EQ) return fail(HN_ERR_INVALID, "hn_ipm_solve: bad sense");
+  }
+  for (int64_t k = 0; k < nz0; ++k)
+    if (lp.col[k] < 0 || lp.col[k] >= n) return fail(HN_ERR_INVALID, "hn_ipm_solve: column out of range");
+
+  // 1. normalise rows
+  std::vector<Trip> t;
+  t.reserve(nz0);
+  for (int64_t i = 0; i < m0; ++i)
+    for (int64_t k = lp.row_ptr[i]; k < lp.row_ptr[i + 1]; ++k) t.push_back({i, lp.col[k], lp.val[k]});
+  std::vector<int64_t> arp, aci;
+  std::vector<double> av;
+  csr_from_triplets(m0, t, arp, aci, av);
+
+  // 2. duplicate rows (first occurrence kept)
+  std::vector<char> keep(m0, 1);
+  {
+    std::vector<std::pair<uint64_t, int64_t>> h(m0);
+    for (int64_t i = 0; i < m0; ++i) {
+      uint64_t x = 1469598103934665603ULL;
+      auto mix = [&](uint64_t y) {
+        x ^= y;
+        x *= 1099511628211ULL;
+        x ^= x >> 29;
+      };
+      for (int64_t k = arp[i]; k < arp[i + 1]; ++k) {
+        mix((uint64_t)aci[k]);
+        mix(bits(av[k]));
+      }
+      mix((uint64_t)lp.sense[i] + 7);
+      mix(bits(lp.rhs[i]));
+      h[i] = {x, i};
+    }
+    std::sort(h.begin(), h.end());
+    auto same = [&](int64_t a, int64_t b) {
+      if (lp.sense[a] != lp.sense[b] || bits(lp.rhs[a]) != bits(lp.rhs[b])) return false;
+      if (arp[a + 1] - arp[a] != arp[b + 1] - arp[b]) return false;
+      for (int64_t u = 0; u < arp[a + 1] - arp[a]; ++u)
+        if (aci[arp[a] + u] != aci[arp[b] + u] || bits(av[arp[a] + u]) != bits(av[arp[b] + u])) return false;
+      return true;
+    };
+    for (int64_t g = 0; g < m0;) {
+      int64_t e = g;
+      while (e < m0 && h[e].first == h[g].first) ++e;
+      for (int64_t a = g + 1; a < e; ++a)
+        for (int64_t b = g; b < a; ++b)
+          if (keep[h[b].second] && same(h[a].second, h[b].second)) {
+            keep[h[a].second] = 0;
+            break;
+          }
+      g = e;
+    }
+  }
+
+  // 3. standard form
+  std::vector<double> cbase(n);
+  for (int64_t j = 0; j < n; ++j) cbase[j] = lp.maximize ? -lp.obj[j] : lp.obj[j];
+  f = StandardForm();
+  f.vars.assign(n, VarMap());
+  int64_t nslots = 0, ntle = 0, m_ub = 0;
+  for (int64_t j = 0; j < n; ++j) {
+    const double lb = lp.lb[j], ub = lp.ub[j];
+    VarMap& vm = f.vars[j];
+    if (std::isfinite(lb) && std::isfinite(ub) && std::abs(ub - lb) <= kFeasTol) {
+      vm.fixed = true;
+      vm.fixed_value = lb;
+      f.offset += cbase[j] * lb;
+    } else if (std::isfinite(lb)) {
+      vm.nslots = 1;
+      vm.slot[0] = nslots++;
+      vm.sign[0] = 1.0;
+      vm.shift[0] = lb;
+      f.offset += cbase[j] * lb;
+      if (std::isfinite(ub)) ++m_ub;
+    } else if (std::isfinite(ub)) {
+      vm.nslots = 1;
+      vm.slot[0] = nslots++;
+      vm.sign[0] = -1.0;
+      vm.shift[0] = ub;
+      f.offset += cbase[j] * ub;
+    } else {
+      vm.nslots = 2;
+      vm.slot[0] = nslots++;
+      vm.sign[0] = 1.0;
+      vm.shift[0] = 0.0;
+      vm.slot[1] = nslots++;
+      vm.sign[1] = -1.0;
+      vm.shift[1] = 0.0;
+    }
+  }
+  for (int64_t i = 0; i < m0; ++i)
+    if (keep[i]) {
+      f.kept_rows.push_back(i);
+      if (lp.sense[i] == HN_LE || lp.sense[i] == HN_GE) ++ntle;
+    }
+  const int64_t norig = (int64_t)f.kept_rows.size();
+  const int64_t m = norig + m_ub, nc = nslots + ntle + m_ub;
+  f.m = m;
+  f.n = nc;
+  f.rhs.assign(m, 0.0);
+  f.c.assign(nc, 0.0);
+  for (int64_t j = 0; j < n; ++j)
+    for (int s = 0; s < f.vars[j].nslots; ++s) f.c[f.vars[j].slot[s]] = f.vars[j].sign[s] > 0 ? cbase[j] : -cbase[j];
+  t.clear();
+  int64_t next_slack = nslots;
+  for (int64_t r = 0; r < norig; ++r) {
+    const int64_t i = f.kept_rows[r];
+    double rowmax = 0.0;
+    for (int64_t k = arp[i]; k < arp[i + 1]; ++k) rowmax = std::max(rowmax, std::abs(av[k]));
+    const double gamma = rowmax > 0.0 ? 1.0 / rowmax : 1.0;
+    const bool flip = lp.sense[i] == HN_GE;
+    double const_i = 0.0;
+    for (int64_t k = arp[i]; k < arp[i + 1]; ++k) {
+      const VarMap& vm = f.vars[aci[k]];
+      const double a = gamma * av[k];
+      if (vm.fixed) {
+        const_i += a * vm.fixed_value;
+      } else {
+        for (int s = 0; s < vm.nslots; ++s) {
+          t.push_back({r, vm.slot[s], flip ? -a * vm.sign[s] : a * vm.sign[s]});
+          const_i += a * vm.shift[s];
+        }
+      }
+    }
+    if (lp.sense[i] == HN_LE || lp.sense[i] == HN_GE) t.push_back({r, next_slack++, gamma});
+    f.rhs[r] = flip ? (-gamma * lp.rhs[i] + const_i) : (gamma * lp.rhs[i] - const_i);
+  }
+  int64_t next_ub = norig;
+  for (int64_t j = 0; j < n; ++j) {
+    const double lb = lp.lb[j], ub = lp.ub[j];
+    if (std::isfinite(lb) && std::isfinite(ub) && std::abs(ub - lb) > kFeasTol) {
+      t.push_back({next_ub, f.vars[j].slot[0], 1.0});
+      t.push_back({next_ub, next_slack++, 1.0});
+      f.rhs[next_ub] = ub - lb;
+      ++next_ub;
+    }
+  }
+  std::vector<int64_t> mrp, mci;
+  std::vector<double> mv;
+  csr_from_triplets(m, t, mrp, mci, mv);
+  f.col_scale.assign(nc, 1.0);
+  std::vector<double> colmax(nc, 0.0);
+  for (size_t k = 0; k < mci.size(); ++k) colmax[mci[k]] = std::max(colmax[mci[k]], std::abs(mv[k]));
+  for (size_t k = 0; k < mci.size(); ++k)
+    if (colmax[mci[k]] > 0.0) f.col_scale[mci[k]] = 1.0 / colmax[mci[k]];
+  t.clear();
+  for (int64_t i = 0; i < m; ++i)
+    for (int64_t k = mrp[i]; k < mrp[i + 1]; ++k) t.push_back({i, mci[k], mv[k] * f.col_scale[mci[k]]});
+  csr_from_triplets(m, t, f.rp, f.ci, f.v);
+  for (int64_t j = 0; j < nc; ++j) f.c[j] *= f.col_scale[j];
+  return HN_OK;
+}
+
+void to_original_primal(const StandardForm& f, const double* x, double* primal) {
+  for (size_t j = 0; j < f.vars.size(); ++j) {
+    const VarMap& vm = f.vars[j];
+    if (vm.fixed) {
+      primal[j] = vm.fixed_value;
+      continue;
+    }
+    double val = 0.0;
+    for (int s = 0; s < vm.nslots; ++s) {
+      const int64_t gi = vm.slot[s];
+      if (gi < 0 || gi >= f.n) continue;
+      val += vm.sign[s] * f.col_scale[gi] * x[gi];
+      if (s == 0) val += vm.shift[s];
+    }
+    primal[j] = val;
+  }
+}
+
+}  // namespace hn

@@ -1,0 +1,577 @@
+"""Python host side of the MI355X LP hot path (ctypes over ``libhn_kernels.so``).
+
+The classes mirror the reference's plugin interface so that code written
+against ``hypernova::execution::IComputeBackend`` (``core/execution/gpu_backend.hpp:18-51``)
+reads the same here:
+
+* :class:`HipBackend` - ``type/name/is_available/initialize/finalize/spmv/spmm/
+  dot/axpy/norm_inf/device_memory/free_memory/was_gpu_kernel_executed/
+  gpu_kernel_executions/reset_execution_stats``.  Like ``CudaDriverBackend``
+  (``gpu_backend.cpp:12-14``) every failure after initialisation raises
+  ``RuntimeError``; nothing falls back to the CPU.
+* :class:`CsrMatrix` - an explicit device-resident matrix handle (the
+  reference keys residency by the host ``values.data()`` pointer,
+  ``gpu_backend.cpp:287-306``; an explicit handle cannot go stale).
+* :class:`DeviceArray` - a device buffer for the ``_dev`` entry points.
+
+The library is loaded from ``lib/`` next to this file; if it is missing the
+import raises ``ImportError`` (build it with ``python build.py``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "lib")
+KERNEL_LIB = os.path.join(LIBDIR, "libhn_kernels.so")
+GEN_LIB = os.path.join(LIBDIR, "libhn_gen.so")
+
+HN_OK = 0
+HN_F64 = 1
+HN_F32 = 2
+
+_u64 = C.c_uint64
+_i64 = C.c_int64
+_dp = C.POINTER(C.c_double)
+_u64p = C.POINTER(C.c_uint64)
+_vp = C.c_void_p
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 256), ("arch", C.c_char * 64), ("total_memory", C.c_uint64),
+                ("compute_units", C.c_int), ("device", C.c_int)]
+
+
+def _load_kernels() -> C.CDLL:
+    if not os.path.exists(KERNEL_LIB):
+        raise ImportError(f"{KERNEL_LIB} is not built; run build.py (there is no CPU fallback)")
+    lib = C.CDLL(KERNEL_LIB)
+    sig = {
+        "hn_init": (C.c_int, [C.c_int, C.POINTER(DeviceInfo)]),
+        "hn_finalize": (C.c_int, []),
+        "hn_last_error": (C.c_int, [C.c_char_p, C.c_size_t]),
+        "hn_mem_info": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+        "hn_kernel_executions": (C.c_uint64, []),
+        "hn_reset_execution_stats": (None, []),
+        "hn_csr_upload": (C.c_int, [_dp, _u64p, _u64p, _u64, _u64, _u64, C.c_int, C.POINTER(_vp)]),
+        "hn_csr_transpose": (C.c_int, [_vp, C.POINTER(_vp)]),
+        "hn_csr_free": (C.c_int, [_vp]),
+        "hn_csr_dims": (C.c_int, [_vp, _u64p, _u64p, _u64p]),
+        "hn_csr_download": (C.c_int, [_vp, _dp, _u64p, _u64p]),
+        "hn_csr_set_tiling": (C.c_int, [_vp, C.c_int]),
+        "hn_csr_get_tiling": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+        "hn_csr_autotune": (C.c_int, [_vp, C.c_int, _dp]),
+        "hn_spmv": (C.c_int, [_vp, _dp, _dp]),
+        "hn_spmm": (C.c_int, [_vp, _dp, _dp, _u64]),
+        "hn_dot": (C.c_int, [_dp, _dp, _u64, _dp]),
+        "hn_axpy": (C.c_int, [C.c_double, _dp, _dp, _u64]),
+        "hn_norm_inf": (C.c_int, [_dp, _u64, _dp]),
+        "hn_spmv_dev": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp]),
+        "hn_spmm_dev": (C.c_int, [_vp, _vp, _vp, _u64, _vp]),
+        "hn_dot_dev": (C.c_int, [_vp, _vp, _u64, _vp, _vp]),
+        "hn_norm_inf_dev": (C.c_int, [_vp, _u64, _vp, _vp]),
+        "hn_axpy_dev": (C.c_int, [C.c_double, _vp, _vp, _u64, _vp]),
+        "hn_dev_alloc": (C.c_int, [C.c_size_t, C.POINTER(_vp)]),
+        "hn_dev_free": (C.c_int, [_vp]),
+        "hn_memcpy_h2d": (C.c_int, [_vp, _vp, C.c_size_t]),
+        "hn_memcpy_d2h": (C.c_int, [_vp, _vp, C.c_size_t]),
+        "hn_stream_sync": (C.c_int, [_vp]),
+        "hn_copy_dev": (C.c_int, [_vp, _vp, C.c_size_t, _vp]),
+        "hn_event_create": (C.c_int, [C.POINTER(_vp)]),
+        "hn_event_destroy": (C.c_int, [_vp]),
+        "hn_event_record": (C.c_int, [_vp, _vp]),
+        "hn_event_elapsed_ms": (C.c_int, [_vp, _vp, C.POINTER(C.c_float)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+LIB = _load_kernels()
+
+
+class HnError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[hn status {code}] {msg}")
+        self.code = code
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(1024)
+    LIB.hn_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(status: int) -> None:
+    if status != HN_OK:
+        raise HnError(status, last_error())
+
+
+def _dptr(a: np.ndarray) -> _dp:
+    return a.ctypes.data_as(_dp)
+
+
+def _u64ptr(a: np.ndarray) -> _u64p:
+    return a.ctypes.data_as(_u64p)
+
+
+_device_info: Optional[DeviceInfo] = None
+
+
+def init(device: int = -1) -> DeviceInfo:
+    """Selects the device and creates the library stream (idempotent)."""
+    global _device_info
+    info = DeviceInfo()
+    check(LIB.hn_init(device, C.byref(info)))
+    _device_info = info
+    return info
+
+
+def finalize() -> None:
+    check(LIB.hn_finalize())
+
+
+def mem_info() -> tuple[int, int]:
+    f, t = C.c_size_t(), C.c_size_t()
+    check(LIB.hn_mem_info(C.byref(f), C.byref(t)))
+    return f.value, t.value
+
+
+class DeviceArray:
+    """A device buffer holding a 1-D numpy dtype (float64, float32, int32...)."""
+
+    def __init__(self, n: int, dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        self.nbytes = max(self.n * self.dtype.itemsize, 1)
+        p = _vp()
+        check(LIB.hn_dev_alloc(self.nbytes, C.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        if a.size:
+            check(LIB.hn_memcpy_h2d(d.ptr, a.ctypes.data_as(_vp), a.nbytes))
+        return d
+
+    def to_numpy(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=self.dtype)
+        if self.n:
+            check(LIB.hn_memcpy_d2h(out.ctypes.data_as(_vp), self.ptr, out.nbytes))
+        return out
+
+    def copy_from(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        if a.size:
+            check(LIB.hn_memcpy_h2d(self.ptr, a.ctypes.data_as(_vp), a.nbytes))
+
+    def free(self) -> None:
+        if self.ptr:
+            LIB.hn_dev_free(self.ptr)
+            self.ptr = _vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class CsrMatrix:
+    """Device-resident CSR matrix (explicit handle)."""
+
+    def __init__(self, handle: _vp, owner: bool = True):
+        self.h = handle
+        self._owner = owner
+        r, c, z = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check(LIB.hn_csr_dims(self.h, C.byref(r), C.byref(c), C.byref(z)))
+        self.nrows, self.ncols, self.nnz = r.value, c.value, z.value
+
+    @classmethod
+    def upload(cls, values: np.ndarray, col_indices: np.ndarray, row_ptr: np.ndarray,
+               ncols: int, flags: int = HN_F64) -> "CsrMatrix":
+        values = np.ascontiguousarray(values, dtype=np.float64)
+        col = np.ascontiguousarray(col_indices, dtype=np.uint64)
+        rp = np.ascontiguousarray(row_ptr, dtype=np.uint64)
+        h = _vp()
+        check(LIB.hn_csr_upload(_dptr(values), _u64ptr(col), _u64ptr(rp), rp.size - 1, ncols,
+                                values.size, flags, C.byref(h)))
+        return cls(h)
+
+    def transpose(self) -> "CsrMatrix":
+        h = _vp()
+        check(LIB.hn_csr_transpose(self.h, C.byref(h)))
+        return CsrMatrix(h)
+
+    def download(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        v = np.empty(self.nnz, np.float64)
+        c = np.empty(self.nnz, np.uint64)
+        r = np.empty(self.nrows + 1, np.uint64)
+        check(LIB.hn_csr_download(self.h, _dptr(v), _u64ptr(c), _u64ptr(r)))
+        return v, c, r
+
+    def set_tiling(self, items_per_thread: int) -> None:
+        check(LIB.hn_csr_set_tiling(self.h, items_per_thread))
+
+    def tiling(self) -> int:
+        v = C.c_int()
+        check(LIB.hn_csr_get_tiling(self.h, C.byref(v)))
+        return v.value
+
+    def autotune(self, iters: int = 20) -> float:
+        us = C.c_double()
+        check(LIB.hn_csr_autotune(self.h, iters, C.byref(us)))
+        return us.value
+
+    def spmv(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.size != self.ncols:
+            raise ValueError("x has the wrong length")
+        y = np.empty(self.nrows, np.float64)
+        check(LIB.hn_spmv(self.h, _dptr(x), _dptr(y)))
+        return y
+
+    def spmm(self, B: np.ndarray, k: int) -> np.ndarray:
+        B = np.ascontiguousarray(B, dtype=np.float64).ravel()
+        if B.size != self.ncols * k:
+            raise ValueError("B has the wrong size")
+        Cm = np.empty(self.nrows * k, np.float64)
+        check(LIB.hn_spmm(self.h, _dptr(B), _dptr(Cm), k))
+        return Cm
+
+    def spmv_dev(self, x: DeviceArray, y: DeviceArray, prec: int = HN_F64, stream=None) -> None:
+        check(LIB.hn_spmv_dev(self.h, prec, x.ptr, y.ptr, stream))
+
+    def free(self) -> None:
+        if self._owner and self.h:
+            LIB.hn_csr_free(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        p = _vp()
+        check(LIB.hn_event_create(C.byref(p)))
+        self.p = p
+
+    def record(self, stream=None) -> None:
+        check(LIB.hn_event_record(self.p, stream))
+
+    def elapsed_ms(self, stop: "Event") -> float:
+        ms = C.c_float()
+        check(LIB.hn_event_elapsed_ms(self.p, stop.p, C.byref(ms)))
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            LIB.hn_event_destroy(self.p)
+        except Exception:
+            pass
+
+
+def stream_sync(stream=None) -> None:
+    check(LIB.hn_stream_sync(stream))
+
+
+# ---------------------------------------------------------------------------
+# IComputeBackend mirror
+# ---------------------------------------------------------------------------
+class ComputeBackendType(enum.IntEnum):
+    """Same enumerators, same order as gpu_backend.hpp:10-16."""
+    CPU = 0
+    CUDA = 1
+    HIP = 2
+    SYCL = 3
+    AUTO = 4
+
+
+class HipBackend:
+    """``IComputeBackend`` on MI355X (the slot ``ComputeBackendType::HIP``
+    that the reference declares but maps to CPU, gpu_backend.cpp:926-933).
+
+    ``spmv(values, col_indices, row_ptr, x, y)`` keeps the matrix resident.
+    The key is the identity of the three arrays plus their shape, and a
+    change of any element's bytes is caught by comparing a checksum of the
+    host arrays, so a different matrix at the same addresses is re-uploaded
+    (the reference's pointer-only key would return stale results).
+    """
+
+    def __init__(self):
+        self._available = False
+        self._resident: dict = {}
+        self._name = "HIP"
+
+    def type(self) -> ComputeBackendType:
+        return ComputeBackendType.HIP
+
+    def name(self) -> str:
+        return self._name
+
+    def is_available(self) -> bool:
+        return self._available
+
+    def initialize(self) -> None:
+        info = init(-1)
+        self._available = True
+        self._name = f"HIP({info.name.decode(errors='replace')})"
+
+    def finalize(self) -> None:
+        for m in self._resident.values():
+            m[1].free()
+        self._resident.clear()
+
+    def _require(self, op: str) -> None:
+        if not self._available:
+            raise RuntimeError(f"HIP backend not available for {op}: {last_error()}")
+
+    def _matrix(self, values, col_indices, row_ptr, ncols: int) -> CsrMatrix:
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        c = np.ascontiguousarray(col_indices, dtype=np.uint64)
+        r = np.ascontiguousarray(row_ptr, dtype=np.uint64)
+        key = (id(values), id(col_indices), id(row_ptr), v.size, r.size, ncols)
+        digest = hash((v.tobytes(), c.tobytes(), r.tobytes()))
+        hit = self._resident.get(key)
+        if hit is not None and hit[0] == digest:
+            return hit[1]
+        if hit is not None:
+            hit[1].free()
+        m = CsrMatrix.upload(v, c, r, ncols)
+        self._resident[key] = (digest, m)
+        return m
+
+    def spmv(self, values, col_indices, row_ptr, x, y: Optional[list] = None) -> np.ndarray:
+        self._require("spmv")
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        nrows = max(len(row_ptr) - 1, 0)
+        if len(values) == 0 or len(row_ptr) < 2:
+            out = np.zeros(nrows)
+        else:
+            out = self._matrix(values, col_indices, row_ptr, x.size).spmv(x)
+        if y is not None:
+            y[:] = list(out)
+        return out
+
+    def spmm(self, values, col_indices, row_ptr, B, ncols_B: int, C_out: Optional[list] = None) -> np.ndarray:
+        self._require("spmm")
+        B = np.ascontiguousarray(B, dtype=np.float64).ravel()
+        nrows = max(len(row_ptr) - 1, 0)
+        if len(values) == 0 or len(row_ptr) < 2 or ncols_B == 0:
+            out = np.zeros(nrows * ncols_B)
+        else:
+            ncols = B.size // ncols_B
+            out = self._matrix(values, col_indices, row_ptr, ncols).spmm(B, ncols_B)
+        if C_out is not None:
+            C_out[:] = list(out)
+        return out
+
+    def dot(self, a, b) -> float:
+        self._require("dot")
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        n = min(a.size, b.size)
+        out = C.c_double()
+        check(LIB.hn_dot(_dptr(a), _dptr(b), n, C.byref(out)))
+        return out.value
+
+    def axpy(self, alpha: float, x, y: np.ndarray) -> np.ndarray:
+        self._require("axpy")
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if not (isinstance(y, np.ndarray) and y.dtype == np.float64 and y.flags.c_contiguous):
+            raise TypeError("y must be a contiguous float64 numpy array (updated in place)")
+        n = min(x.size, y.size)
+        check(LIB.hn_axpy(float(alpha), _dptr(x), _dptr(y), n))
+        return y
+
+    def norm_inf(self, v) -> float:
+        self._require("norm_inf")
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = C.c_double()
+        check(LIB.hn_norm_inf(_dptr(v), v.size, C.byref(out)))
+        return out.value
+
+    def device_memory(self) -> int:
+        return mem_info()[1] if self._available else 0
+
+    def free_memory(self) -> int:
+        return mem_info()[0] if self._available else 0
+
+    def was_gpu_kernel_executed(self) -> bool:
+        return LIB.hn_kernel_executions() > 0
+
+    def gpu_kernel_executions(self) -> int:
+        return int(LIB.hn_kernel_executions())
+
+    def reset_execution_stats(self) -> None:
+        LIB.hn_reset_execution_stats()
+
+
+def gpu_visible() -> bool:
+    """True when a gfx950 device can be initialised."""
+    info = DeviceInfo()
+    return LIB.hn_init(-1, C.byref(info)) == HN_OK
+
+
+# ---------------------------------------------------------------------------
+# Interior-point solver (hn_ipm_solve)
+# ---------------------------------------------------------------------------
+HN_NE_AUTO, HN_NE_DENSE, HN_NE_PCG = 0, 1, 2
+HN_PREC_F64, HN_PREC_MIXED = 1, 2
+STATUS = ["UNKNOWN", "OPTIMAL", "INFEASIBLE", "UNBOUNDED", "SUBOPTIMAL", "TIME_LIMIT",
+          "ITER_LIMIT", "NUMERICAL_ERROR", "INTERRUPTED"]
+
+_i64p = C.POINTER(C.c_int64)
+_i8p = C.POINTER(C.c_int8)
+INTERRUPT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)
+
+
+class _HnLp(C.Structure):
+    _fields_ = [("m", C.c_int64), ("n", C.c_int64), ("row_ptr", _i64p), ("col", _i64p), ("val", _dp),
+                ("sense", _i8p), ("rhs", _dp), ("lb", _dp), ("ub", _dp), ("obj", _dp), ("maximize", C.c_int),
+                ("obj_offset", C.c_double)]
+
+
+class _HnOpts(C.Structure):
+    _fields_ = [("max_iterations", C.c_int), ("time_limit_seconds", C.c_double),
+                ("convergence_tol", C.c_double), ("complementarity_tol", C.c_double),
+                ("regularization", C.c_double), ("interrupt_callback", INTERRUPT_FN),
+                ("interrupt_user", C.c_void_p), ("solver", C.c_int), ("precision", C.c_int),
+                ("mu_switch", C.c_double), ("dense_max_m", C.c_int64), ("pcg_max_iter", C.c_int),
+                ("x0", _dp), ("y0", _dp), ("s0", _dp), ("verbose", C.c_int)]
+
+
+class _HnResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_int64), ("objective", C.c_double),
+                ("solve_ms", C.c_double), ("primal", _dp), ("x", _dp), ("y", _dp), ("s", _dp),
+                ("cap_m", C.c_int64), ("cap_n", C.c_int64), ("sf_m", C.c_int64), ("sf_n", C.c_int64),
+                ("rp_norm", C.c_double), ("rd_norm", C.c_double), ("mu", C.c_double),
+                ("rel_primal", C.c_double), ("rel_dual", C.c_double), ("rel_gap", C.c_double),
+                ("ne_solves", C.c_int64), ("refine_sweeps", C.c_int64), ("cg_iterations", C.c_int64),
+                ("fp64_from_iter", C.c_int64), ("warm_started", C.c_int)]
+
+
+LIB.hn_ipm_default_opts.argtypes = [C.POINTER(_HnOpts)]
+LIB.hn_ipm_default_opts.restype = C.c_int
+LIB.hn_ipm_solve.argtypes = [C.POINTER(_HnLp), C.POINTER(_HnOpts), C.POINTER(_HnResult)]
+LIB.hn_ipm_solve.restype = C.c_int
+
+
+class InteriorPointOptions:
+    """Mirror of InteriorPointOptions (core/lp/interior_point.hpp:16-27) plus
+    the GPU choices of hn_ipm_opts.  ``crossover`` must stay False: the
+    simplex polish is the caller's (CPU) business."""
+
+    def __init__(self, **kw):
+        self.max_iterations = 100
+        self.time_limit_seconds = 0.0
+        self.convergence_tol = 1e-8
+        self.complementarity_tol = 1e-8
+        self.regularization = 1e-8
+        self.crossover = False
+        self.interrupt_callback = None
+        self.solver = HN_NE_AUTO
+        self.precision = HN_PREC_F64
+        self.mu_switch = 1e-4
+        self.dense_max_m = 40000
+        self.pcg_max_iter = 2000
+        self.warm_start = None          # (x0, y0, s0) of the scaled standard form
+        self.verbose = False
+        for k, v in kw.items():
+            if not hasattr(self, k):
+                raise TypeError(f"unknown option {k}")
+            setattr(self, k, v)
+
+
+class InteriorPointResult:
+    def __init__(self, r: _HnResult, primal, x, y, s):
+        self.status = STATUS[r.status]
+        self.objective_value = r.objective
+        self.iterations = int(r.iterations)
+        self.solve_time_ms = r.solve_ms
+        self.primal = primal
+        self.x, self.y, self.s = x[:r.sf_n].copy(), y[:r.sf_m].copy(), s[:r.sf_n].copy()
+        self.rp_norm, self.rd_norm, self.mu = r.rp_norm, r.rd_norm, r.mu
+        self.rel_primal, self.rel_dual, self.rel_gap = r.rel_primal, r.rel_dual, r.rel_gap
+        self.ne_solves, self.refine_sweeps = int(r.ne_solves), int(r.refine_sweeps)
+        self.cg_iterations, self.fp64_from_iter = int(r.cg_iterations), int(r.fp64_from_iter)
+        self.warm_started = bool(r.warm_started)
+
+    def __repr__(self):
+        return (f"InteriorPointResult({self.status}, obj={self.objective_value!r}, it={self.iterations}, "
+                f"{self.solve_time_ms:.1f} ms)")
+
+
+class InteriorPointSolver:
+    """Mirror of hypernova::lp::InteriorPointSolver (interior_point.hpp:40-54)
+    whose solve() runs hn_ipm_solve on the GPU."""
+
+    def __init__(self, options: Optional[InteriorPointOptions] = None):
+        self._opts = options or InteriorPointOptions()
+
+    def options(self) -> InteriorPointOptions:
+        return self._opts
+
+    def set_options(self, o: InteriorPointOptions) -> None:
+        self._opts = o
+
+    def solve(self, lp) -> InteriorPointResult:
+        o = self._opts
+        if o.crossover:
+            raise ValueError("crossover is not part of the GPU path; set crossover=False")
+        keep = dict(rp=np.ascontiguousarray(lp.row_ptr, np.int64), ci=np.ascontiguousarray(lp.col, np.int64),
+                    v=np.ascontiguousarray(lp.val, np.float64), sense=np.ascontiguousarray(lp.sense, np.int8),
+                    rhs=np.ascontiguousarray(lp.rhs, np.float64), lb=np.ascontiguousarray(lp.lb, np.float64),
+                    ub=np.ascontiguousarray(lp.ub, np.float64), obj=np.ascontiguousarray(lp.obj, np.float64))
+        P = _HnLp(lp.m, lp.n, keep["rp"].ctypes.data_as(_i64p), keep["ci"].ctypes.data_as(_i64p),
+                  _dptr(keep["v"]), keep["sense"].ctypes.data_as(_i8p), _dptr(keep["rhs"]), _dptr(keep["lb"]),
+                  _dptr(keep["ub"]), _dptr(keep["obj"]), int(lp.maximize), float(getattr(lp, "obj_offset", 0.0)))
+        opts = _HnOpts()
+        check(LIB.hn_ipm_default_opts(C.byref(opts)))
+        opts.max_iterations = o.max_iterations
+        opts.time_limit_seconds = o.time_limit_seconds
+        opts.convergence_tol = o.convergence_tol
+        opts.complementarity_tol = o.complementarity_tol
+        opts.regularization = o.regularization
+        cb = None
+        if o.interrupt_callback is not None:
+            user_cb = o.interrupt_callback
+            cb = INTERRUPT_FN(lambda _u: 1 if user_cb() else 0)
+            opts.interrupt_callback = cb
+        opts.solver = o.solver
+        opts.precision = o.precision
+        opts.mu_switch = o.mu_switch
+        opts.dense_max_m = o.dense_max_m
+        opts.pcg_max_iter = o.pcg_max_iter
+        opts.verbose = int(o.verbose)
+        if o.warm_start is not None:
+            ws = [np.ascontiguousarray(a, np.float64) for a in o.warm_start]
+            keep["ws"] = ws
+            opts.x0, opts.y0, opts.s0 = (_dptr(a) for a in ws)
+        cap_m = lp.m + lp.n + 8
+        cap_n = 2 * lp.n + lp.m + 8
+        primal = np.zeros(max(lp.n, 1))
+        x, s, y = np.zeros(cap_n), np.zeros(cap_n), np.zeros(cap_m)
+        r = _HnResult()
+        r.primal, r.x, r.y, r.s = _dptr(primal), _dptr(x), _dptr(y), _dptr(s)
+        r.cap_m, r.cap_n = cap_m, cap_n
+        check(LIB.hn_ipm_solve(C.byref(P), C.byref(opts), C.byref(r)))
+        return InteriorPointResult(r, primal[:lp.n].copy(), x, y, s)
+
+
+def ipm_solve(lp, **kw) -> InteriorPointResult:
+    return InteriorPointSolver(InteriorPointOptions(**kw)).solve(lp)

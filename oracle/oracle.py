@@ -1,0 +1,336 @@
+"""ctypes access to the checkers (TEST INFRASTRUCTURE ONLY).
+
+* ``liboracle.so`` - the C restatement of the reference IPM path (hn_oracle.c).
+* ``_ref/libhnref.so`` - the reference's own sources compiled by this
+  directory's Makefile, driven through ref_driver.cpp.
+
+Only tests/, ``__graft_entry__.smoke()`` and the cpu_baseline leg of
+bench.py import this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libhnref.so")
+
+_i64 = C.c_int64
+_i64p = C.POINTER(C.c_int64)
+_u64p = C.POINTER(C.c_uint64)
+_dp = C.POINTER(C.c_double)
+_i8p = C.POINTER(C.c_int8)
+
+STATUS = ["UNKNOWN", "OPTIMAL", "INFEASIBLE", "UNBOUNDED", "SUBOPTIMAL", "TIME_LIMIT",
+          "ITER_LIMIT", "NUMERICAL_ERROR", "INTERRUPTED"]
+
+
+def build() -> None:
+    """Compiles liboracle.so and, where the reference sources exist, _ref/."""
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+
+
+class _Problem(C.Structure):
+    _fields_ = [("m", _i64), ("n", _i64), ("rp", _i64p), ("ci", _i64p), ("v", _dp),
+                ("sense", _i8p), ("rhs", _dp), ("lb", _dp), ("ub", _dp), ("obj", _dp),
+                ("maximize", C.c_int), ("obj_offset", C.c_double)]
+
+
+class _Options(C.Structure):
+    _fields_ = [("max_iterations", C.c_int), ("time_limit_seconds", C.c_double),
+                ("convergence_tol", C.c_double), ("complementarity_tol", C.c_double),
+                ("regularization", C.c_double), ("interrupt", C.c_int)]
+
+
+class _Result(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", _i64), ("objective", C.c_double),
+                ("primal", _dp), ("sf_m", _i64), ("sf_n", _i64), ("x", _dp), ("lam", _dp),
+                ("z", _dp), ("cap_m", _i64), ("cap_n", _i64), ("rp_norm", C.c_double),
+                ("rd_norm", C.c_double), ("mu", C.c_double), ("trace", _dp), ("trace_cap", _i64)]
+
+
+def _load_oracle():
+    if not os.path.exists(ORACLE_SO):
+        build()
+    lib = C.CDLL(ORACLE_SO)
+    lib.or_spmv.argtypes = [_i64, _i64p, _i64p, _dp, _dp, _dp]
+    lib.or_spmv_t.argtypes = [_i64, _i64, _i64p, _i64p, _dp, _dp, _dp]
+    lib.or_spmm.argtypes = [_i64, _i64p, _i64p, _dp, _dp, _dp, _i64]
+    lib.or_csr_transpose.argtypes = [_i64, _i64, _i64p, _i64p, _dp, _i64p, _i64p, _dp]
+    lib.or_dot.argtypes = [_i64, _dp, _dp]
+    lib.or_dot.restype = C.c_double
+    lib.or_axpy.argtypes = [_i64, C.c_double, _dp, _dp]
+    lib.or_norm_inf.argtypes = [_i64, _dp]
+    lib.or_norm_inf.restype = C.c_double
+    lib.or_max_step.argtypes = [_i64, _dp, _dp]
+    lib.or_max_step.restype = C.c_double
+    lib.or_adat_pattern.argtypes = [_i64, _i64, _i64p, _i64p, _dp, _i64p, _i64p]
+    lib.or_adat_pattern.restype = _i64
+    lib.or_adat_dense.argtypes = [_i64, _i64, _i64p, _i64p, _dp, _dp, C.c_double, _dp]
+    lib.or_adat_dense.restype = C.c_double
+    lib.or_default_options.argtypes = [C.POINTER(_Options)]
+    lib.or_ipm_solve.argtypes = [C.POINTER(_Problem), C.POINTER(_Options), C.POINTER(_Result)]
+    lib.or_ipm_solve.restype = C.c_int
+    lib.or_standard_form.argtypes = [C.POINTER(_Problem), _i64p, _i64p, _i64p, _i64p, _i64p, _dp, _dp,
+                                     _dp, _dp, _dp]
+    lib.or_standard_form.restype = C.c_int
+    return lib
+
+
+OR = _load_oracle()
+
+
+def _ref_lib():
+    if not os.path.exists(REF_SO):
+        return None
+    lib = C.CDLL(REF_SO)
+    lib.ref_ipm_solve.argtypes = [_i64, _i64, _i64p, _i64p, _dp, _i8p, _dp, _dp, _dp, _dp, C.c_int,
+                                  C.c_double, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int), _i64p,
+                                  _dp, _dp, _dp, _i64, _i64p, _dp]
+    lib.ref_cpu_spmv.argtypes = [_i64, _i64, _u64p, _u64p, _dp, _i64, _dp, _dp, C.c_int]
+    lib.ref_cpu_spmv_time.argtypes = [_i64, _i64, _u64p, _u64p, _dp, _i64, _dp, C.c_int]
+    lib.ref_cpu_spmv_time.restype = C.c_double
+    lib.ref_cpu_spmv_t_time.argtypes = [_i64, _i64, _i64, _u64p, _u64p, _dp, _dp, C.c_int]
+    lib.ref_cpu_spmv_t_time.restype = C.c_double
+    lib.ref_transpose.argtypes = [_i64, _i64, _i64p, _i64p, _dp, _i64p, _i64p, _dp]
+    lib.ref_transpose.restype = _i64
+    return lib
+
+
+REF = _ref_lib()
+
+
+def have_reference() -> bool:
+    return REF is not None
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i64a(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+# ---- kernels --------------------------------------------------------------
+def spmv(rp, ci, v, x) -> np.ndarray:
+    rp, ci, v, x = _i64a(rp), _i64a(ci), _f64(v), _f64(x)
+    y = np.empty(rp.size - 1)
+    OR.or_spmv(rp.size - 1, _p(rp, _i64p), _p(ci, _i64p), _p(v, _dp), _p(x, _dp), _p(y, _dp))
+    return y
+
+
+def spmv_t(rp, ci, v, x, ncols: int) -> np.ndarray:
+    rp, ci, v, x = _i64a(rp), _i64a(ci), _f64(v), _f64(x)
+    out = np.empty(ncols)
+    OR.or_spmv_t(rp.size - 1, ncols, _p(rp, _i64p), _p(ci, _i64p), _p(v, _dp), _p(x, _dp), _p(out, _dp))
+    return out
+
+
+def spmm(rp, ci, v, B, k: int) -> np.ndarray:
+    rp, ci, v, B = _i64a(rp), _i64a(ci), _f64(v), _f64(B).ravel()
+    out = np.empty((rp.size - 1) * k)
+    OR.or_spmm(rp.size - 1, _p(rp, _i64p), _p(ci, _i64p), _p(v, _dp), _p(B, _dp), _p(out, _dp), k)
+    return out
+
+
+def csr_transpose(rp, ci, v, ncols: int):
+    rp, ci, v = _i64a(rp), _i64a(ci), _f64(v)
+    orp = np.empty(ncols + 1, np.int64)
+    oci = np.empty(ci.size, np.int64)
+    ov = np.empty(ci.size)
+    OR.or_csr_transpose(rp.size - 1, ncols, _p(rp, _i64p), _p(ci, _i64p), _p(v, _dp), _p(orp, _i64p),
+                        _p(oci, _i64p), _p(ov, _dp))
+    return orp, oci, ov
+
+
+def dot(a, b) -> float:
+    a, b = _f64(a), _f64(b)
+    return OR.or_dot(min(a.size, b.size), _p(a, _dp), _p(b, _dp))
+
+
+def norm_inf(v) -> float:
+    v = _f64(v)
+    return OR.or_norm_inf(v.size, _p(v, _dp))
+
+
+def max_step(v, d) -> float:
+    v, d = _f64(v), _f64(d)
+    return OR.or_max_step(v.size, _p(v, _dp), _p(d, _dp))
+
+
+def adat_pattern(rp, ci, m: int, n: int, D=None):
+    rp, ci = _i64a(rp), _i64a(ci)
+    orp = np.empty(m + 1, np.int64)
+    Dp = _p(_f64(D), _dp) if D is not None else None
+    nnz = OR.or_adat_pattern(m, n, _p(rp, _i64p), _p(ci, _i64p), Dp, _p(orp, _i64p), None)
+    oci = np.empty(nnz, np.int64)
+    OR.or_adat_pattern(m, n, _p(rp, _i64p), _p(ci, _i64p), Dp, _p(orp, _i64p), _p(oci, _i64p))
+    return orp, oci
+
+
+def adat_dense(rp, ci, v, D, m: int, n: int, reg: float = 1e-8):
+    rp, ci, v, D = _i64a(rp), _i64a(ci), _f64(v), _f64(D)
+    N = np.empty(m * m)
+    dyn = OR.or_adat_dense(m, n, _p(rp, _i64p), _p(ci, _i64p), _p(v, _dp), _p(D, _dp), reg, _p(N, _dp))
+    return N.reshape(m, m), dyn
+
+
+# ---- IPM ------------------------------------------------------------------
+@dataclass
+class IpmResult:
+    status: str
+    iterations: int
+    objective: float
+    primal: np.ndarray
+    x: Optional[np.ndarray] = None
+    lam: Optional[np.ndarray] = None
+    z: Optional[np.ndarray] = None
+    rp_norm: float = 0.0
+    rd_norm: float = 0.0
+    mu: float = 0.0
+    trace: Optional[np.ndarray] = None
+    solve_ms: float = 0.0
+
+
+def _problem(lp):
+    keep = dict(rp=_i64a(lp.row_ptr), ci=_i64a(lp.col), v=_f64(lp.val), sense=np.ascontiguousarray(lp.sense, np.int8),
+                rhs=_f64(lp.rhs), lb=_f64(lp.lb), ub=_f64(lp.ub), obj=_f64(lp.obj))
+    p = _Problem(lp.m, lp.n, _p(keep["rp"], _i64p), _p(keep["ci"], _i64p), _p(keep["v"], _dp),
+                 _p(keep["sense"], _i8p), _p(keep["rhs"], _dp), _p(keep["lb"], _dp), _p(keep["ub"], _dp),
+                 _p(keep["obj"], _dp), int(lp.maximize), float(getattr(lp, "obj_offset", 0.0)))
+    return p, keep
+
+
+def standard_form(lp):
+    """The scaled standard form (M, rhs, c, col_scale, offset) the IPM iterates on."""
+    p, keep = _problem(lp)
+    sm, sn, sz = _i64(), _i64(), _i64()
+    OR.or_standard_form(C.byref(p), C.byref(sm), C.byref(sn), C.byref(sz), None, None, None, None, None, None,
+                        None)
+    rp = np.empty(sm.value + 1, np.int64)
+    ci = np.empty(sz.value, np.int64)
+    v = np.empty(sz.value)
+    rhs = np.empty(sm.value)
+    c = np.empty(sn.value)
+    cs = np.empty(sn.value)
+    off = C.c_double()
+    OR.or_standard_form(C.byref(p), C.byref(sm), C.byref(sn), C.byref(sz), _p(rp, _i64p), _p(ci, _i64p),
+                        _p(v, _dp), _p(rhs, _dp), _p(c, _dp), _p(cs, _dp), C.byref(off))
+    return dict(m=sm.value, n=sn.value, rp=rp, ci=ci, v=v, rhs=rhs, c=c, col_scale=cs, offset=off.value)
+
+
+def ipm_solve(lp, max_iterations: int = 100, time_limit: float = 0.0, interrupt: bool = False,
+              trace_cap: int = 200) -> IpmResult:
+    """The restated reference IPM (crossover off)."""
+    p, keep = _problem(lp)
+    o = _Options()
+    OR.or_default_options(C.byref(o))
+    o.max_iterations = max_iterations
+    o.time_limit_seconds = time_limit
+    o.interrupt = int(interrupt)
+    cap_m = lp.m + lp.n + 8
+    cap_n = 2 * lp.n + lp.m + 8
+    primal = np.zeros(max(lp.n, 1))
+    x = np.zeros(cap_n)
+    z = np.zeros(cap_n)
+    lam = np.zeros(cap_m)
+    trace = np.zeros(5 * trace_cap)
+    r = _Result()
+    r.primal = _p(primal, _dp)
+    r.x, r.z, r.lam = _p(x, _dp), _p(z, _dp), _p(lam, _dp)
+    r.cap_m, r.cap_n = cap_m, cap_n
+    r.trace = _p(trace, _dp)
+    r.trace_cap = trace_cap
+    if OR.or_ipm_solve(C.byref(p), C.byref(o), C.byref(r)) != 0:
+        raise ValueError("oracle rejected the problem")
+    it = int(r.iterations)
+    return IpmResult(status=STATUS[r.status], iterations=it, objective=r.objective, primal=primal[:lp.n].copy(),
+                     x=x[:r.sf_n].copy(), lam=lam[:r.sf_m].copy(), z=z[:r.sf_n].copy(), rp_norm=r.rp_norm,
+                     rd_norm=r.rd_norm, mu=r.mu, trace=trace[:5 * min(it, trace_cap)].reshape(-1, 5).copy())
+
+
+def ref_ipm_solve(lp, max_iterations: int = 100, time_limit: float = 0.0, interrupt: bool = False,
+                  trace_cap: int = 200) -> IpmResult:
+    """The reference's own InteriorPointSolver (crossover off), from _ref/."""
+    if REF is None:
+        raise RuntimeError("reference build not present")
+    keep = dict(rp=_i64a(lp.row_ptr), ci=_i64a(lp.col), v=_f64(lp.val), sense=np.ascontiguousarray(lp.sense, np.int8),
+                rhs=_f64(lp.rhs), lb=_f64(lp.lb), ub=_f64(lp.ub), obj=_f64(lp.obj))
+    st = C.c_int()
+    it = _i64()
+    obj = C.c_double()
+    primal = np.zeros(max(lp.n, 1))
+    trace = np.zeros(5 * trace_cap)
+    tl = _i64()
+    ms = C.c_double()
+    REF.ref_ipm_solve(lp.m, lp.n, _p(keep["rp"], _i64p), _p(keep["ci"], _i64p), _p(keep["v"], _dp),
+                      _p(keep["sense"], _i8p), _p(keep["rhs"], _dp), _p(keep["lb"], _dp), _p(keep["ub"], _dp),
+                      _p(keep["obj"], _dp), int(lp.maximize), float(getattr(lp, "obj_offset", 0.0)),
+                      max_iterations, time_limit, int(interrupt), C.byref(st), C.byref(it), C.byref(obj),
+                      _p(primal, _dp), _p(trace, _dp), trace_cap, C.byref(tl), C.byref(ms))
+    return IpmResult(status=STATUS[st.value], iterations=int(it.value), objective=obj.value,
+                     primal=primal[:lp.n].copy(), trace=trace[:5 * tl.value].reshape(-1, 5).copy(),
+                     solve_ms=ms.value)
+
+
+def ref_cpu_spmv(rp, ci, v, x, repeats: int = 1) -> np.ndarray:
+    if REF is None:
+        raise RuntimeError("reference build not present")
+    rp = np.ascontiguousarray(rp, np.uint64)
+    ci = np.ascontiguousarray(ci, np.uint64)
+    v, x = _f64(v), _f64(x)
+    y = np.empty(rp.size - 1)
+    REF.ref_cpu_spmv(rp.size - 1, v.size, _p(rp, _u64p), _p(ci, _u64p), _p(v, _dp), x.size, _p(x, _dp),
+                     _p(y, _dp), repeats)
+    return y
+
+
+def ref_cpu_spmv_seconds(rp, ci, v, x, repeats: int) -> float:
+    """Wall seconds of `repeats` reference CPU spmv calls (1 thread)."""
+    rp = np.ascontiguousarray(rp, np.uint64)
+    ci = np.ascontiguousarray(ci, np.uint64)
+    v, x = _f64(v), _f64(x)
+    return REF.ref_cpu_spmv_time(rp.size - 1, v.size, _p(rp, _u64p), _p(ci, _u64p), _p(v, _dp), x.size,
+                                 _p(x, _dp), repeats)
+
+
+def ref_cpu_spmv_t_seconds(rp, ci, v, w, ncols: int, repeats: int) -> float:
+    """Wall seconds of `repeats` reference transpose_multiply calls (1 thread)."""
+    rp = np.ascontiguousarray(rp, np.uint64)
+    ci = np.ascontiguousarray(ci, np.uint64)
+    v, w = _f64(v), _f64(w)
+    return REF.ref_cpu_spmv_t_time(rp.size - 1, ncols, v.size, _p(rp, _u64p), _p(ci, _u64p), _p(v, _dp),
+                                   _p(w, _dp), repeats)
+
+
+def port_spmv_seconds(rp, ci, v, x, repeats: int) -> float:
+    """Wall seconds of `repeats` restated CPU spmv calls (1 thread)."""
+    import time
+    rp, ci, v, x = _i64a(rp), _i64a(ci), _f64(v), _f64(x)
+    y = np.empty(rp.size - 1)
+    t = time.perf_counter()
+    for _ in range(repeats):
+        OR.or_spmv(rp.size - 1, _p(rp, _i64p), _p(ci, _i64p), _p(v, _dp), _p(x, _dp), _p(y, _dp))
+    return time.perf_counter() - t
+
+
+def ref_transpose(rp, ci, v, ncols: int):
+    rp, ci, v = _i64a(rp), _i64a(ci), _f64(v)
+    nnz = REF.ref_transpose(rp.size - 1, ncols, _p(rp, _i64p), _p(ci, _i64p), _p(v, _dp), None, None, None)
+    orp = np.empty(ncols + 1, np.int64)
+    oci = np.empty(nnz, np.int64)
+    ov = np.empty(nnz)
+    REF.ref_transpose(rp.size - 1, ncols, _p(rp, _i64p), _p(ci, _i64p), _p(v, _dp), _p(orp, _i64p),
+                      _p(oci, _i64p), _p(ov, _dp))
+    return orp, oci, ov

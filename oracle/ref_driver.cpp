@@ -1,0 +1,162 @@
+// ref_driver.cpp - C entry points around the reference's own IPM, CPU backend
+// and sparse containers, compiled together with the reference sources into
+// oracle/_ref/libhnref.so (see oracle/Makefile).  TEST INFRASTRUCTURE: used
+// only to pin the restated oracle and as the reference CPU baseline.
+//
+// Only the reference's public API is used: ProblemBuilder
+// (core/model/problem.hpp:199-292), InteriorPointSolver
+// (core/lp/interior_point.hpp:16-54), ComputeBackendFactory
+// (core/execution/gpu_backend.hpp:53-58) and SparseMatrix
+// (core/numerical/sparse_matrix.hpp:30-89).
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <iostream>
+#include <limits>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "execution/gpu_backend.hpp"
+#include "lp/interior_point.hpp"
+#include "model/problem.hpp"
+#include "numerical/sparse_matrix.hpp"
+#include "numerical/tolerance.hpp"
+
+using namespace hypernova;
+
+extern "C" {
+
+// Solves with InteriorPointSolver (crossover off).  primal (n) may be NULL.
+// trace receives up to trace_cap rows of (mu, rp, rd, ap, ad) parsed from the
+// solver's own HYPERNOVA_IPM_DBG lines; *trace_len is the number of rows.
+int ref_ipm_solve(int64_t m, int64_t n, const int64_t* rp, const int64_t* ci, const double* v,
+                  const int8_t* sense, const double* rhs, const double* lb, const double* ub,
+                  const double* obj, int maximize, double obj_offset, int max_iterations,
+                  double time_limit, int interrupt, int* status, int64_t* iterations,
+                  double* objective, double* primal, double* trace, int64_t trace_cap,
+                  int64_t* trace_len, double* solve_ms) {
+  model::ProblemBuilder b("ref");
+  for (int64_t j = 0; j < n; ++j) b.add_variable(lb[j], ub[j], model::VarType::CONTINUOUS);
+  for (int64_t i = 0; i < m; ++i) {
+    std::vector<std::pair<std::size_t, double>> coeffs;
+    for (int64_t k = rp[i]; k < rp[i + 1]; ++k) coeffs.push_back({(std::size_t)ci[k], v[k]});
+    b.add_constraint(coeffs, static_cast<model::ConstraintSense>(sense[i]), rhs[i]);
+  }
+  std::vector<std::pair<std::size_t, double>> o;
+  for (int64_t j = 0; j < n; ++j) o.push_back({(std::size_t)j, obj[j]});
+  b.set_objective(o, maximize ? model::ObjectiveSense::MAXIMIZE : model::ObjectiveSense::MINIMIZE);
+  model::Problem prob = b.build();
+  prob.obj_offset = obj_offset;
+
+  lp::InteriorPointOptions opts;
+  opts.max_iterations = max_iterations;
+  opts.time_limit_seconds = time_limit;
+  opts.crossover = false;
+  if (interrupt) opts.interrupt_callback = []() { return true; };
+  lp::InteriorPointSolver solver(numerical::ToleranceConfig::industrial_defaults(), opts);
+
+  std::ostringstream captured;
+  std::streambuf* old = nullptr;
+  if (trace) {
+    setenv("HYPERNOVA_IPM_DBG", "1", 1);
+    old = std::cerr.rdbuf(captured.rdbuf());
+  }
+  lp::InteriorPointResult res = solver.solve(prob);
+  if (trace) {
+    std::cerr.rdbuf(old);
+    unsetenv("HYPERNOVA_IPM_DBG");
+    std::istringstream in(captured.str());
+    std::string line;
+    int64_t rows = 0;
+    while (std::getline(in, line) && rows < trace_cap) {
+      long it;
+      double mu, r1, r2, ap, ad;
+      if (std::sscanf(line.c_str(), "IPM it=%ld mu=%lf rp=%lf rd=%lf ap=%lf ad=%lf", &it, &mu, &r1, &r2,
+                      &ap, &ad) == 6) {
+        double* t = trace + 5 * rows;
+        t[0] = mu;
+        t[1] = r1;
+        t[2] = r2;
+        t[3] = ap;
+        t[4] = ad;
+        ++rows;
+      }
+    }
+    if (trace_len) *trace_len = rows;
+  }
+  *status = static_cast<int>(res.status);
+  *iterations = (int64_t)res.iterations;
+  *objective = res.objective_value;
+  if (solve_ms) *solve_ms = res.solve_time_ms;
+  if (primal)
+    for (int64_t j = 0; j < n && j < (int64_t)res.primal.size(); ++j) primal[j] = res.primal[j];
+  return 0;
+}
+
+// y = A x through ComputeBackendFactory::create_backend(CPU)->spmv.
+int ref_cpu_spmv(int64_t nrows, int64_t nnz, const uint64_t* rp, const uint64_t* ci, const double* v,
+                 int64_t ncols, const double* x, double* y, int repeats) {
+  auto be = execution::ComputeBackendFactory::create_backend(execution::ComputeBackendType::CPU);
+  std::vector<double> vals(v, v + nnz);
+  std::vector<std::size_t> cols(ci, ci + nnz), rows(rp, rp + nrows + 1);
+  std::vector<double> xs(x, x + ncols), ys;
+  for (int r = 0; r < (repeats > 0 ? repeats : 1); ++r) be->spmv(vals, cols, rows, xs, ys);
+  std::memcpy(y, ys.data(), sizeof(double) * ys.size());
+  return 0;
+}
+
+// Same, timing only the spmv calls (the std::vector copies are made once).
+double ref_cpu_spmv_time(int64_t nrows, int64_t nnz, const uint64_t* rp, const uint64_t* ci,
+                         const double* v, int64_t ncols, const double* x, int repeats) {
+  auto be = execution::ComputeBackendFactory::create_backend(execution::ComputeBackendType::CPU);
+  std::vector<double> vals(v, v + nnz);
+  std::vector<std::size_t> cols(ci, ci + nnz), rows(rp, rp + nrows + 1);
+  std::vector<double> xs(x, x + ncols), ys;
+  be->spmv(vals, cols, rows, xs, ys);
+  struct timespec t0, t1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  for (int r = 0; r < repeats; ++r) be->spmv(vals, cols, rows, xs, ys);
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}
+
+// Wall seconds of `repeats` SparseMatrix::transpose_multiply calls
+// (sparse_matrix.cpp:161-184, the row-wise scatter the IPM's matvec_t also
+// uses), on a SparseMatrix filled directly from the CSR arrays.
+double ref_cpu_spmv_t_time(int64_t nrows, int64_t ncols, int64_t nnz, const uint64_t* rp, const uint64_t* ci,
+                           const double* v, const double* w, int repeats) {
+  numerical::SparseMatrix A(nrows, ncols);
+  A.mutable_values().assign(v, v + nnz);
+  A.mutable_col_indices().assign(ci, ci + nnz);
+  A.mutable_row_ptr().assign(rp, rp + nrows + 1);
+  std::vector<double> ws(w, w + nrows);
+  std::vector<double> out = A.transpose_multiply(ws);
+  struct timespec t0, t1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  for (int r = 0; r < repeats; ++r) out = A.transpose_multiply(ws);
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}
+
+// SparseMatrix::from_triplets(rows, cols, triplets).transposed(); returns nnz,
+// and writes the CSR of the transpose when the output pointers are not NULL.
+int64_t ref_transpose(int64_t nrows, int64_t ncols, const int64_t* rp, const int64_t* ci,
+                      const double* v, int64_t* out_rp, int64_t* out_ci, double* out_v) {
+  std::vector<numerical::Triplet> t;
+  for (int64_t i = 0; i < nrows; ++i)
+    for (int64_t k = rp[i]; k < rp[i + 1]; ++k) t.emplace_back((std::size_t)i, (std::size_t)ci[k], v[k]);
+  numerical::SparseMatrix A = numerical::SparseMatrix::from_triplets(nrows, ncols, t);
+  numerical::SparseMatrix At = A.transposed();
+  if (out_rp)
+    for (int64_t i = 0; i <= ncols; ++i) out_rp[i] = (int64_t)At.row_ptr()[i];
+  if (out_ci)
+    for (std::size_t k = 0; k < At.nnz(); ++k) out_ci[k] = (int64_t)At.col_indices()[k];
+  if (out_v)
+    for (std::size_t k = 0; k < At.nnz(); ++k) out_v[k] = At.values()[k];
+  return (int64_t)At.nnz();
+}
+
+}  // extern "C"

@@ -1,0 +1,101 @@
+"""LPs of the reference's own IPM tests (tests/unit/test_lp.cpp and
+tests/integration/test_integration.cpp), built in the reference's Problem
+shape (CSR rows, senses, bounds, objective)."""
+import numpy as np
+
+import lpgen
+
+INF = np.inf
+LE, GE, EQ = lpgen.LE, lpgen.GE, lpgen.EQ
+
+
+def from_rows(nvars, rows, senses, rhs, lb, ub, obj, maximize=False):
+    rp, ci, v = [0], [], []
+    for r in rows:
+        for j, a in r:
+            ci.append(j)
+            v.append(a)
+        rp.append(len(ci))
+    return lpgen.LP(m=len(rows), n=nvars, row_ptr=np.array(rp, np.int64), col=np.array(ci, np.int64),
+                    val=np.array(v, float), rhs=np.array(rhs, float), sense=np.array(senses, np.int8),
+                    lb=np.array(lb, float), ub=np.array(ub, float), obj=np.array(obj, float), maximize=maximize)
+
+
+def simple_lp():
+    """test_lp.cpp:244-259 (also :287-400): max x1 + 2 x2, x1 + x2 <= 10,
+    x1 + 0 x2 <= 5, 0 <= x <= 10.  Optimum 20 at (0, 10)."""
+    return from_rows(2, [[(0, 1.0), (1, 1.0)], [(0, 1.0), (1, 0.0)]], [LE, LE], [10.0, 5.0], [0, 0], [10, 10],
+                     [1.0, 2.0], maximize=True)
+
+
+def larger_lp():
+    """test_lp.cpp:261-285: 50 vars in [0,100], 20 identical rows
+    sum_{j%5==0} x_j <= 10, max sum x."""
+    rows = [[(j, 1.0) for j in range(0, 50, 5)] for _ in range(20)]
+    return from_rows(50, rows, [LE] * 20, [10.0] * 20, [0] * 50, [100] * 50, [1.0] * 50, maximize=True)
+
+
+def nonfinite_guard_lp():
+    """test_lp.cpp:402-434: badly scaled coefficients."""
+    n = 12
+    rows = [[(j, 1e6 * (j + 1)) for j in range(n)] for _ in range(6)]
+    return from_rows(n, rows, [LE] * 6, [1e7 * (i + 1) for i in range(6)], [0] * n, [1e8] * n,
+                     [1e11 * (i + 1) for i in range(n)], maximize=True)
+
+
+def time_limit_lp():
+    """test_lp.cpp:604-634."""
+    rows = [[(j, float((i + 1) * (j + 1))) for j in range(20)] for i in range(10)]
+    return from_rows(20, rows, [LE] * 10, [100.0 * (i + 1) for i in range(10)], [0] * 20, [1e4] * 20,
+                     [float(i + 1) for i in range(20)], maximize=True)
+
+
+def integration_lp():
+    """test_integration.cpp:56-71: max 3x1 + 5x2, x1 + x2 <= 10, 2x1 + x2 <= 14,
+    x in [0,10].  Optimum 50 (x2 = 10)."""
+    return from_rows(2, [[(0, 1.0), (1, 1.0)], [(0, 2.0), (1, 1.0)]], [LE, LE], [10.0, 14.0], [0, 0], [10, 10],
+                     [3.0, 5.0], maximize=True)
+
+
+def gpu_parity_lp():
+    """test_gpu_backend.cpp:349-385: max 5x0+4x1+3x2, 2x0+x1+x2<=80, x0+2x1+x2<=90, x in [0,50]."""
+    return from_rows(3, [[(0, 2.0), (1, 1.0), (2, 1.0)], [(0, 1.0), (1, 2.0), (2, 1.0)]], [LE, LE], [80.0, 90.0],
+                     [0, 0, 0], [50, 50, 50], [5.0, 4.0, 3.0], maximize=True)
+
+
+def auto_mode_lp():
+    """test_gpu_backend.cpp:305-321: max 2x0 + 3x1, x0 + x1 <= 15, x in [0,10]. Optimum 40."""
+    return from_rows(2, [[(0, 1.0), (1, 1.0)]], [LE], [15.0], [0, 0], [10, 10], [2.0, 3.0], maximize=True)
+
+
+def forced_mode_lp():
+    """test_gpu_backend.cpp:323-339: max 3x0 + 4x1, x0 + 2x1 <= 12, x in [0,10]. Optimum 34."""
+    return from_rows(2, [[(0, 1.0), (1, 2.0)]], [LE], [12.0], [0, 0], [10, 10], [3.0, 4.0], maximize=True)
+
+
+def mixed_bounds_lp():
+    """Covers every standard-form branch (interior_point.cpp:220-255,316-341):
+    a fixed variable, lb-only, ub-only, free, boxed; LE, GE and EQ rows; a
+    duplicated row (removed by remove_duplicate_rows :82-132)."""
+    rows = [
+        [(0, 1.0), (1, 2.0), (2, -1.0), (3, 1.0)],
+        [(1, 1.0), (3, 1.0), (4, 3.0)],
+        [(0, 2.0), (2, 1.0), (4, 1.0)],
+        [(1, 1.0), (3, 1.0), (4, 3.0)],
+        [(2, 1.0), (3, -1.0)],
+    ]
+    return from_rows(5, rows, [LE, GE, EQ, GE, LE], [8.0, 2.0, 5.0, 2.0, 4.0],
+                     [1.0, 0.0, -INF, -INF, 0.0], [1.0, INF, 3.0, INF, 4.0], [1.0, -1.0, 2.0, 0.5, -1.0])
+
+
+REFERENCE_CASES = {
+    "simple_lp": simple_lp,
+    "larger_lp": larger_lp,
+    "nonfinite_guard": nonfinite_guard_lp,
+    "time_limit_lp": time_limit_lp,
+    "integration_lp": integration_lp,
+    "gpu_parity_lp": gpu_parity_lp,
+    "auto_mode_lp": auto_mode_lp,
+    "forced_mode_lp": forced_mode_lp,
+    "mixed_bounds_lp": mixed_bounds_lp,
+}

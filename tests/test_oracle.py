@@ -1,0 +1,210 @@
+"""Pins the oracle (oracle/hn_oracle.c) before it is trusted as the checker.
+
+Each check is chosen so that a plausible restatement mistake fails it:
+  * golden vectors of the reference's own tests (backend_cases.npz: inputs
+    regenerated with their std::mt19937 recipe, outputs of the reference's
+    CPU backend) - bit-exact for the sequential SpMV;
+  * the reference's own IPM (ipm_reference.json, produced by oracle/_ref on
+    the LPs of its IPM tests and seeded random LPs);
+  * closed forms and brute force (dense products, the ratio test by hand,
+    transposes, the A D A' pattern from a dense product);
+  * an independent optimum (scipy's HiGHS) for the IPM objective.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import lp_cases
+import lpgen
+import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CASES = ["cpu_spmv", "cpu_spmm", "auto_spmv", "cuda_scalar", "cuda_warp", "cuda_spmm", "residency"]
+
+
+def _case(g, name):
+    nrows, ncols = (int(v) for v in g[f"{name}_shape"])
+    return nrows, ncols, g[f"{name}_rp"].astype(np.int64), g[f"{name}_ci"].astype(np.int64), g[f"{name}_v"], \
+        g[f"{name}_x"], g[f"{name}_y"]
+
+
+def dense(rp, ci, v, ncols):
+    A = np.zeros((rp.size - 1, ncols))
+    for i in range(rp.size - 1):
+        for k in range(rp[i], rp[i + 1]):
+            A[i, ci[k]] += v[k]
+    return A
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_spmv_matches_reference_cpu_backend_bitwise(golden_backend, name):
+    nrows, ncols, rp, ci, v, x, y_ref = _case(golden_backend, name)
+    # same sequential per-row order as CPUBackend::spmv: identical bits
+    assert np.array_equal(O.spmv(rp, ci, v, x), y_ref)
+    # and the value itself against a dense product (catches a wrong index)
+    assert np.allclose(dense(rp, ci, v, ncols) @ x, y_ref, rtol=1e-12, atol=1e-12)
+
+
+def test_spmm_brute_force(golden_backend):
+    for name in ("cpu_spmm", "cuda_spmm"):
+        nrows, ncols, rp, ci, v, x, _ = _case(golden_backend, name)
+        B = golden_backend[f"{name}_B"]
+        k = int(golden_backend[f"{name}_k"][0])
+        got = O.spmm(rp, ci, v, B, k).reshape(nrows, k)
+        assert np.allclose(got, dense(rp, ci, v, ncols) @ B.reshape(ncols, k), rtol=1e-12, atol=1e-12)
+
+
+def test_transpose_matches_reference_and_closed_form(golden_backend):
+    for name in CASES:
+        nrows, ncols, rp, ci, v, x, _ = _case(golden_backend, name)
+        orp, oci, ov = O.csr_transpose(rp, ci, v, ncols)
+        if O.have_reference():  # no zeros/duplicates: SparseMatrix::transposed() agrees
+            rrp, rci, rv = O.ref_transpose(rp, ci, v, ncols)
+            assert np.array_equal(orp, rrp) and np.array_equal(oci, rci) and np.array_equal(ov, rv)
+        assert np.array_equal(dense(orp, oci, ov, nrows), dense(rp, ci, v, ncols).T)
+        # rows of the transpose list source rows in increasing order (stable)
+        for j in range(ncols):
+            seg = oci[orp[j]:orp[j + 1]]
+            assert np.all(np.diff(seg) > 0)
+        drp, dci, dv = O.csr_transpose(orp, oci, ov, nrows)
+        assert np.array_equal(drp, rp) and np.array_equal(dci, ci) and np.array_equal(dv, v)
+        w = np.linspace(-1, 1, nrows)
+        assert np.allclose(O.spmv_t(rp, ci, v, w, ncols), dense(rp, ci, v, ncols).T @ w, rtol=1e-12, atol=1e-12)
+
+
+def test_transpose_keeps_zeros_and_duplicates():
+    # the counting scatter keeps explicit zeros and duplicate entries in order
+    rp = np.array([0, 3, 4])
+    ci = np.array([1, 1, 0, 1])
+    v = np.array([0.0, 2.0, 3.0, 4.0])
+    orp, oci, ov = O.csr_transpose(rp, ci, v, 2)
+    assert orp.tolist() == [0, 1, 4] and oci.tolist() == [0, 0, 0, 1] and ov.tolist() == [3.0, 0.0, 2.0, 4.0]
+
+
+def test_vector_semantics():
+    a = np.array([1.0, -3.0, np.nan, 2.0])
+    assert O.norm_inf(a) == 3.0                        # NaN never wins (std::max(best, v))
+    assert O.norm_inf(np.array([np.nan])) == 0.0
+    assert O.dot([1, 2, 3], [4, 5]) == 14.0            # runs to min(len)
+    # ratio test: 0.99 * min(-v/d) over d < 0, capped at 1 (and 0.99 when unblocked)
+    assert O.max_step([1.0, 2.0], [1.0, 1.0]) == 0.99
+    assert O.max_step([1.0, 2.0], [-4.0, -1.0]) == 0.99 * 0.25
+    assert O.max_step([1.0], [-0.5]) == 0.99
+    v = np.array([3.0, 1.0, 5.0])
+    d = np.array([-1.0, -2.0, 0.0])
+    assert O.max_step(v, d) == 0.99 * 0.5
+
+
+def test_adat_pattern_and_values_brute_force():
+    rng = np.random.default_rng(3)
+    m, n = 30, 70
+    lp = lpgen.random_lp(11, m, n, 3)
+    M = dense(lp.row_ptr, lp.col, lp.val, n)
+    D = rng.uniform(0.1, 2.0, n)
+    D[rng.choice(n, 10, replace=False)] = 0.0          # skipped columns (D <= 0)
+    N, dyn = O.adat_dense(lp.row_ptr, lp.col, lp.val, D, m, n, 1e-8)
+    ref = M @ np.diag(D) @ M.T
+    assert dyn == max(1e-8, 1e-12 * np.max(np.diag(ref)))
+    assert np.allclose(N, ref + dyn * np.eye(m), rtol=1e-12, atol=1e-14)
+    orp, oci = O.adat_pattern(lp.row_ptr, lp.col, m, n, D)
+    mask = ((np.abs(M) > 0).astype(float) @ np.diag((D > 0).astype(float)) @ (np.abs(M) > 0).T.astype(float)) > 0
+    mask |= np.eye(m, dtype=bool)
+    got = np.zeros((m, m), bool)
+    for i in range(m):
+        got[i, oci[orp[i]:orp[i + 1]]] = True
+        assert np.all(np.diff(oci[orp[i]:orp[i + 1]]) > 0)
+    assert np.array_equal(got, mask)
+
+
+# ---- the IPM -------------------------------------------------------------------
+REF_IPM = json.load(open(os.path.join(GOLDEN, "ipm_reference.json")))
+
+
+def _lp(name):
+    if name in lp_cases.REFERENCE_CASES:
+        return lp_cases.REFERENCE_CASES[name]()
+    if name.startswith("simple_lp"):
+        return lp_cases.simple_lp()
+    if name.startswith("random_60x150_s"):
+        return lpgen.random_lp(int(name.rsplit("s", 1)[1]), 60, 150)
+    if name == "random_250x625_s7":
+        return lpgen.random_lp(7, 250, 625)
+    if name == "transport_8x12_s0":
+        return lpgen.transport_lp(0, 8, 12)
+    if name == "powerlaw_120x300_s3":
+        return lpgen.powerlaw_lp(3, 120, 300, 2.179, 2, 40)
+    raise KeyError(name)
+
+
+# Well-conditioned cases: the reference and the oracle take the same steps, so
+# status and iteration count agree exactly and the objective to rounding.
+TIGHT = ["simple_lp", "integration_lp", "gpu_parity_lp", "auto_mode_lp", "forced_mode_lp", "simple_lp_iter1",
+         "simple_lp_interrupt", "random_60x150_s1", "random_60x150_s2", "random_60x150_s3", "random_60x150_s4",
+         "random_60x150_s5", "random_250x625_s7", "powerlaw_120x300_s3"]
+# Degenerate / badly scaled cases: the factorisation order (minimum degree in
+# the reference, natural in the oracle) changes rounding, which these LPs
+# amplify; the status still agrees.
+LOOSE = ["larger_lp", "nonfinite_guard", "time_limit_lp", "mixed_bounds_lp", "transport_8x12_s0"]
+
+
+@pytest.mark.parametrize("name", TIGHT)
+def test_ipm_matches_reference_tight(name):
+    ref = REF_IPM[name]
+    r = O.ipm_solve(_lp(name), **ref["options"])
+    assert r.status == ref["status"]
+    assert r.iterations == ref["iterations"]
+    assert abs(r.objective - ref["objective"]) <= 1e-10 * max(1.0, abs(ref["objective"]))
+    assert np.allclose(r.primal, ref["primal"], rtol=1e-6, atol=1e-6)
+    # the reference prints its trace with 6 significant digits
+    t = np.array(ref["trace"])
+    if t.size:
+        assert np.allclose(r.trace[:, [0, 3, 4]], t[:, [0, 3, 4]], rtol=2e-5, atol=0)
+
+
+@pytest.mark.parametrize("name", LOOSE)
+def test_ipm_matches_reference_status(name):
+    ref = REF_IPM[name]
+    r = O.ipm_solve(_lp(name), **ref["options"])
+    assert r.status == ref["status"]
+    assert np.all(np.isfinite(r.primal)) and np.isfinite(r.objective)
+    if ref["status"] == "OPTIMAL":
+        assert abs(r.objective - ref["objective"]) <= 1e-6 * max(1.0, abs(ref["objective"]))
+
+
+def test_ipm_reference_known_answers():
+    # test_lp.cpp:244-259 / :347-377 (objective 20, x=(0,10)), integration 50,
+    # test_gpu_backend.cpp objectives 40 and 34
+    assert abs(O.ipm_solve(lp_cases.simple_lp()).objective - 20.0) < 1e-6
+    assert abs(O.ipm_solve(lp_cases.integration_lp()).objective - 50.0) < 1e-2
+    assert abs(O.ipm_solve(lp_cases.auto_mode_lp()).objective - 40.0) < 1e-6
+    assert abs(O.ipm_solve(lp_cases.forced_mode_lp()).objective - 34.0) < 1e-6
+    r = O.ipm_solve(lp_cases.simple_lp(), max_iterations=1)       # :306-325
+    assert r.status == "ITER_LIMIT" and r.primal.size == 2 and r.objective != 0.0
+    assert O.ipm_solve(lp_cases.simple_lp(), interrupt=True).status == "INTERRUPTED"  # :287-304
+    r = O.ipm_solve(lp_cases.time_limit_lp(), time_limit=1e-9)     # :604-634
+    assert np.isfinite(r.objective) and np.all(np.isfinite(r.primal))
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_ipm_objective_against_highs(seed):
+    scipy_opt = pytest.importorskip("scipy.optimize")
+    lp = lpgen.random_lp(seed, 80, 200)
+    A = dense(lp.row_ptr, lp.col, lp.val, lp.n)
+    res = scipy_opt.linprog(lp.obj, A_eq=A, b_eq=lp.rhs, bounds=[(0, None)] * lp.n, method="highs")
+    r = O.ipm_solve(lp)
+    assert r.status == "OPTIMAL"
+    assert abs(r.objective - res.fun) <= 1e-7 * max(1.0, abs(res.fun))
+    # the generator's certificate (x0, y0, s0) bounds the optimum from both sides
+    assert lp.obj @ lp.x0 >= r.objective - 1e-9 and lp.rhs @ lp.y0 <= r.objective + 1e-9
+
+
+def test_standard_form_invariants():
+    lp = lp_cases.mixed_bounds_lp()
+    sf = O.standard_form(lp)
+    # duplicate row 3 removed; one UB row for the boxed variable 4; variable 0 fixed
+    assert sf["m"] == 4 + 1
+    M = dense(sf["rp"], sf["ci"], sf["v"], sf["n"])
+    assert np.allclose(np.max(np.abs(M), axis=0)[np.any(M != 0, axis=0)], 1.0)   # column max 1 after scaling
+    assert abs(sf["offset"] - (1.0 * 1.0 + (-1.0) * 0.0 + 2.0 * 3.0 + (-1.0) * 0.0)) < 1e-15
