@@ -96,8 +96,22 @@ def build_gen() -> str:
     return so
 
 
+def build_backend() -> list[str]:
+    """libhn_backend.so (the C++ HipBackend plugin) and its test program."""
+    os.makedirs(LIB, exist_ok=True)
+    so = os.path.join(LIB, "libhn_backend.so")
+    _run([CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-I" + CSRC, os.path.join(CSRC, "hip_backend.cpp"),
+          "-o", so + ".tmp", "-ldl"])
+    os.replace(so + ".tmp", so)
+    exe = os.path.join(LIB, "test_hip_backend")
+    _run([CXX, "-O2", "-std=c++17", "-I" + CSRC, os.path.join(ROOT, "tests", "cpp", "test_hip_backend.cpp"),
+          os.path.join(CSRC, "hip_backend.cpp"), "-o", exe + ".tmp", "-ldl"])
+    os.replace(exe + ".tmp", exe)
+    return [so, exe]
+
+
 def build_all(force: bool = True) -> list[str]:
-    return [build_kernels(force=force), build_gen()]
+    return [build_kernels(force=force), build_gen(), *build_backend()]
 
 
 if __name__ == "__main__":

@@ -507,8 +507,8 @@ struct Solver {
   // Factor (dense) or prepare (PCG) the normal equations for the current D.
   int prepare_ne() {
     if (use_dense) {
-      HN_TRY(ne_numeric(M, Mt, D.as<double>(), o.regularization, ne, S.as<double>() + S_REG, s));
-      HN_TRY(dense_factor(ne, dense, !fp64_solve, s));
+      HN_TRY(ne_numeric(M, Mt, D.as<double>(), o.regularization, ne, S.as<double>() + S_REG, &dense, !fp64_solve, s));
+      HN_TRY(dense_factor(dense, s));
       dense_ready = true;
     } else {
       HN_TRY(ne_jacobi_diag(M, D.as<double>(), pcg_diag.as<double>(), s));
@@ -597,7 +597,8 @@ struct Solver {
     if (!ok && !fp64_solve) {  // FP32 inverse could not be refined: switch to FP64 for good
       fp64_solve = true;
       *escalated = true;
-      HN_TRY(dense_factor(ne, dense, false, s));
+      HN_TRY(ne_numeric(M, Mt, D.as<double>(), o.regularization, ne, S.as<double>() + S_REG, &dense, false, s));
+      HN_TRY(dense_factor(dense, s));
       HN_TRY(solve_dense(r_in, out, &ok));
     }
     return HN_OK;

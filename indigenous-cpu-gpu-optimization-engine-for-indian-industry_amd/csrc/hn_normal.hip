@@ -143,27 +143,62 @@ int ne_symbolic(const hn_csr* M, const hn_csr* Mt, NormalEq& ne, hipStream_t s) 
 // ---------------------------------------------------------------------------
 // numeric values
 // ---------------------------------------------------------------------------
-__global__ void k_ne_numeric(const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ v,
-                             const int* __restrict__ trp, const int* __restrict__ tci, const double* __restrict__ tv,
-                             const double* __restrict__ D, int m, const int* __restrict__ nrp,
-                             const int* __restrict__ ncol, double* __restrict__ nval) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per row i.  Lane l owns pattern entries b+l, b+l+64, ... of the
+// row; every product m_ij d_j m_rj (columns j of row i in order, rows r of
+// column j in order) is broadcast across the wave and added by the lane that
+// owns column r.  Each entry is therefore summed in the same order as a
+// sequential loop, without atomics.  The row is also written into the dense
+// matrix when one is given (entries outside the pattern must be zero).
+template <typename T>
+__global__ __launch_bounds__(256) void k_ne_numeric(
+    const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ v,
+    const int* __restrict__ trp, const int* __restrict__ tci, const double* __restrict__ tv,
+    const double* __restrict__ D, int m, const int* __restrict__ nrp, const int* __restrict__ ncol,
+    double* __restrict__ nval, T* __restrict__ dense) {
+  constexpr int CH = 8;  // rows of up to 512 pattern entries in registers; longer rows loop
+  const int i = (blockIdx.x * 256 + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
   if (i >= m) return;
   const int b = nrp[i], e = nrp[i + 1];
-  for (int k = b; k < e; ++k) nval[k] = 0.0;
-  for (int k = rp[i]; k < rp[i + 1]; ++k) {
-    const int j = ci[k];
-    const double d = D[j];
-    if (d <= 0.0) continue;  // interior_point.cpp:178
-    const double a = v[k];
-    for (int q = trp[j]; q < trp[j + 1]; ++q) {
-      const int r = tci[q];
-      int lo = b, hi = e;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (ncol[mid] < r) lo = mid + 1; else hi = mid;
+  for (int base = b; base < e; base += CH * kWave) {
+    int mycol[CH];
+    double acc[CH];
+#pragma unroll
+    for (int t = 0; t < CH; ++t) {
+      const int p = base + t * kWave + lane;
+      mycol[t] = (p < e) ? ncol[p] : -1;
+      acc[t] = 0.0;
+    }
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+      const int j = ci[k];
+      const double d = D[j];
+      if (d <= 0.0) continue;  // interior_point.cpp:178
+      const double a = v[k];
+      const int q0 = trp[j], q1 = trp[j + 1];
+      for (int qb = q0; qb < q1; qb += kWave) {
+        int r = -1;
+        double val = 0.0;
+        if (qb + lane < q1) {
+          r = tci[qb + lane];
+          val = (a * tv[qb + lane]) * d;
+        }
+        const int cnt = min(kWave, q1 - qb);
+        for (int l = 0; l < cnt; ++l) {
+          const int rr = __shfl(r, l, kWave);
+          const double vv = __shfl(val, l, kWave);
+#pragma unroll
+          for (int t = 0; t < CH; ++t)
+            if (mycol[t] == rr) acc[t] += vv;
+        }
       }
-      nval[lo] += (a * tv[q]) * d;
+    }
+#pragma unroll
+    for (int t = 0; t < CH; ++t) {
+      const int p = base + t * kWave + lane;
+      if (p < e) {
+        nval[p] = acc[t];
+        if (dense) dense[(long long)i * m + mycol[t]] = (T)acc[t];
+      }
     }
   }
 }
@@ -182,38 +217,23 @@ __global__ __launch_bounds__(256) void k_ne_diag_max(const double* __restrict__ 
 
 __global__ __launch_bounds__(256) void k_ne_add_reg(const double* __restrict__ part, int np, double reg,
                                                     double* __restrict__ nval, const int* __restrict__ diag_pos,
-                                                    int m, double* __restrict__ reg_out) {
+                                                    int m, double* __restrict__ reg_out, double* __restrict__ d64,
+                                                    float* __restrict__ d32) {
   // every block recomputes the (tiny) final maximum so no extra launch is needed
   __shared__ double red[4];
   double mx = 0.0;
   for (int i = threadIdx.x; i < np; i += 256) mx = nan_ignoring_max(mx, part[i]);
   mx = block_max<double, 256>(mx, red);
   const double r = fmax(reg, 1e-12 * mx);
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256) nval[diag_pos[i]] += r;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256) {
+    const double dv = nval[diag_pos[i]] + r;
+    nval[diag_pos[i]] = dv;
+    if (d64) d64[(long long)i * m + i] = dv;
+    if (d32) d32[(long long)i * m + i] = (float)dv;
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0 && reg_out) *reg_out = r;
 }
 
-int ne_numeric(const hn_csr* M, const hn_csr* Mt, const double* D, double reg, NormalEq& ne, double* reg_out,
-               hipStream_t s) {
-  const int m = (int)M->nrows;
-  hn_csr* N = ne.N;
-  hipLaunchKernelGGL(k_ne_numeric, dim3(grid_for(m, 128)), dim3(128), 0, s, M->row_ptr.as<int>(), M->col.as<int>(),
-                     M->val64.as<double>(), Mt->row_ptr.as<int>(), Mt->col.as<int>(), Mt->val64.as<double>(), D, m,
-                     N->row_ptr.as<int>(), N->col.as<int>(), N->val64.as<double>());
-  HN_TRY(check_launch("k_ne_numeric"));
-  const int nb = (int)std::min<int64_t>(256, std::max<int64_t>(1, (m + 255) / 256));
-  HN_TRY(ne.part.reserve(sizeof(double) * 256));
-  hipLaunchKernelGGL(k_ne_diag_max, dim3(nb), dim3(256), 0, s, N->val64.as<double>(), ne.diag_pos.as<int>(), m,
-                     ne.part.as<double>());
-  hipLaunchKernelGGL(k_ne_add_reg, dim3(nb), dim3(256), 0, s, ne.part.as<double>(), nb, reg, N->val64.as<double>(),
-                     ne.diag_pos.as<int>(), m, reg_out);
-  count_launch(3);
-  return check_launch("k_ne_add_reg");
-}
-
-// ---------------------------------------------------------------------------
-// dense inverse by blocked symmetric sweeps
-// ---------------------------------------------------------------------------
 template <typename T>
 __global__ void k_dense_fill(T* __restrict__ A, long long total) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
@@ -221,89 +241,166 @@ __global__ void k_dense_fill(T* __restrict__ A, long long total) {
     A[i] = T(0);
 }
 
-template <typename T>
-__global__ void k_dense_scatter(const int* __restrict__ nrp, const int* __restrict__ ncol,
-                                const double* __restrict__ nval, int m, T* __restrict__ A) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  for (int k = nrp[i]; k < nrp[i + 1]; ++k) A[(long long)i * m + ncol[k]] = (T)nval[k];
+int ne_numeric(const hn_csr* M, const hn_csr* Mt, const double* D, double reg, NormalEq& ne, double* reg_out,
+               DenseNE* dense, bool f32, hipStream_t s) {
+  const int m = (int)M->nrows;
+  hn_csr* N = ne.N;
+  const dim3 gw((unsigned)(((long long)m * kWave + 255) / 256));
+  double* d64 = nullptr;
+  float* d32 = nullptr;
+  if (dense) {
+    HN_TRY(dense_reserve(*dense, m, f32));
+    dense->f32 = f32;
+    const long long tot = (long long)m * m;
+    if (f32) {
+      d32 = dense->A.as<float>();
+      hipLaunchKernelGGL((k_dense_fill<float>), dim3(grid_for(tot, 256, 16384)), dim3(256), 0, s, d32, tot);
+    } else {
+      d64 = dense->A.as<double>();
+      hipLaunchKernelGGL((k_dense_fill<double>), dim3(grid_for(tot, 256, 16384)), dim3(256), 0, s, d64, tot);
+    }
+  }
+  if (f32 && dense)
+    hipLaunchKernelGGL((k_ne_numeric<float>), gw, dim3(256), 0, s, M->row_ptr.as<int>(), M->col.as<int>(),
+                       M->val64.as<double>(), Mt->row_ptr.as<int>(), Mt->col.as<int>(), Mt->val64.as<double>(), D, m,
+                       N->row_ptr.as<int>(), N->col.as<int>(), N->val64.as<double>(), d32);
+  else
+    hipLaunchKernelGGL((k_ne_numeric<double>), gw, dim3(256), 0, s, M->row_ptr.as<int>(), M->col.as<int>(),
+                       M->val64.as<double>(), Mt->row_ptr.as<int>(), Mt->col.as<int>(), Mt->val64.as<double>(), D, m,
+                       N->row_ptr.as<int>(), N->col.as<int>(), N->val64.as<double>(), d64);
+  HN_TRY(check_launch("k_ne_numeric"));
+  const int nb = (int)std::min<int64_t>(256, std::max<int64_t>(1, (m + 255) / 256));
+  HN_TRY(ne.part.reserve(sizeof(double) * 256));
+  hipLaunchKernelGGL(k_ne_diag_max, dim3(nb), dim3(256), 0, s, N->val64.as<double>(), ne.diag_pos.as<int>(), m,
+                     ne.part.as<double>());
+  hipLaunchKernelGGL(k_ne_add_reg, dim3(nb), dim3(256), 0, s, ne.part.as<double>(), nb, reg, N->val64.as<double>(),
+                     ne.diag_pos.as<int>(), m, reg_out, d64, d32);
+  count_launch(dense ? 4 : 3);
+  return check_launch("k_ne_add_reg");
 }
 
+// ---------------------------------------------------------------------------
+// dense inverse by blocked symmetric sweeps
+// ---------------------------------------------------------------------------
 constexpr int kGjNB = 64;
 constexpr int kGjTile = 64;
 
-// Inverts the pivot block A[k0:k0+b, k0:k0+b] by b scalar sweeps in LDS.
-// Pivots <= tol are replaced by tol (the reference's LDL' clamp); the count
-// of clamped pivots is added to *clamped.  Writes Pinv (NB x NB, row-major).
+// Symmetric sweep inverse of a 64 x 64 block held in registers by a
+// 256-thread workgroup: thread (ti, tj) = (tid / 16, tid % 16) owns rows
+// 4ti..4ti+3 and columns 4tj..4tj+3.  Step k publishes row k and column k in
+// double-buffered LDS (one barrier per step), then every thread updates its
+// 16 entries:  a_kk <- -1/p,  a_kj <- a_kj/p,  a_ik <- a_ik/p,
+// a_ij <- a_ij - a_ik a_kj / p.  Pivots p <= tol (and NaN-free non-positive
+// ones) are replaced by tol, the reference's LDL' clamp; steps k >= b leave
+// the block untouched.  Returns the number of clamped pivots (thread 0).
+template <typename T>
+__device__ int sweep_block64(T (&P)[4][4], int b, double tol, T (*rk)[64], T (*ck)[64]) {
+  const int tid = threadIdx.x;
+  const int ti = tid >> 4, tj = tid & 15;
+  int nclamp = 0;
+  for (int k = 0; k < b; ++k) {
+    const int par = k & 1;
+    const int kk = k & 3;
+    if (ti == (k >> 2)) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) rk[par][4 * tj + c] = P[kk][c];
+    }
+    if (tj == (k >> 2)) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a) ck[par][4 * ti + a] = P[a][kk];
+    }
+    __syncthreads();
+    T piv = rk[par][k];
+    if (piv <= T(tol)) {
+      piv = T(tol);
+      ++nclamp;
+    }
+    const T rinv = T(1) / piv;
+    T rkj[4], cik[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) rkj[c] = rk[par][4 * tj + c] * rinv;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) cik[a] = ck[par][4 * ti + a];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int i = 4 * ti + a;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int j = 4 * tj + c;
+        T v;
+        if (i == k && j == k)
+          v = -rinv;
+        else if (i == k)
+          v = rkj[c];
+        else if (j == k)
+          v = cik[a] * rinv;
+        else
+          v = P[a][c] - cik[a] * rkj[c];
+        P[a][c] = v;
+      }
+    }
+  }
+  return nclamp;
+}
+
+// Inverts the pivot block A[k0:k0+b, k0:k0+b] (identity beyond b) and writes
+// Pinv (64 x 64, row-major); clamped pivots are added to *clamped.
 template <typename T>
 __global__ __launch_bounds__(256) void k_gj_pivot(const T* __restrict__ A, int m, int k0, int b,
                                                   T* __restrict__ Pinv, double tol, int* __restrict__ clamped) {
-  constexpr int NB = kGjNB;
-  __shared__ T P[NB][NB + 1];
+  __shared__ T rk[2][64];
+  __shared__ T ck[2][64];
   const int tid = threadIdx.x;
-  for (int e = tid; e < NB * NB; e += 256) {
-    const int i = e / NB, j = e % NB;
-    P[i][j] = (i < b && j < b) ? A[(long long)(k0 + i) * m + (k0 + j)] : T(i == j ? 1 : 0);
-  }
-  __syncthreads();
-  constexpr int PER = NB * NB / 256;
-  T nv[PER];
-  for (int k = 0; k < b; ++k) {
-    T piv = P[k][k];
-    if (piv <= T(tol)) {
-      piv = T(tol);
-      if (tid == 0) atomicAdd(clamped, 1);
-    }
-    const T rinv = T(1) / piv;
+  const int ti = tid >> 4, tj = tid & 15;
+  T P[4][4];
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int e = tid + u * 256;
-      const int i = e / NB, j = e % NB;
-      T val;
-      if (i == k && j == k)
-        val = -rinv;
-      else if (i == k)
-        val = P[k][j] * rinv;
-      else if (j == k)
-        val = P[i][k] * rinv;
-      else
-        val = P[i][j] - P[i][k] * (P[k][j] * rinv);
-      nv[u] = val;
-    }
-    __syncthreads();
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int e = tid + u * 256;
-      P[e / NB][e % NB] = nv[u];
+    for (int c = 0; c < 4; ++c) {
+      const int i = 4 * ti + a, j = 4 * tj + c;
+      P[a][c] = (i < b && j < b) ? A[(long long)(k0 + i) * m + k0 + j] : T(i == j ? 1 : 0);
     }
-    __syncthreads();
-  }
-  for (int e = tid; e < NB * NB; e += 256) Pinv[e] = -P[e / NB][e % NB];
+  const int nclamp = sweep_block64<T>(P, b, tol, rk, ck);
+  if (tid == 0 && nclamp) atomicAdd(clamped, nclamp);
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) Pinv[(4 * ti + a) * 64 + 4 * tj + c] = -P[a][c];
 }
 
-// Vt[k][i] = A[i][k0+k]; Wt[c][i] = sum_k Vt[k][i] Pinv[k][c]  (k-major, m columns)
+// Vt[k][i] = A[i][k0+k]; Wt[c][i] = sum_k Vt[k][i] Pinv[k][c]  (k-major, m
+// columns).  64 rows per workgroup: the rows' pivot columns and Pinv are
+// staged in LDS, each thread produces 16 outputs.
 template <typename T>
 __global__ __launch_bounds__(256) void k_gj_panel(const T* __restrict__ A, int m, int k0, int b,
                                                   const T* __restrict__ Pinv, T* __restrict__ Vt,
                                                   T* __restrict__ Wt) {
-  constexpr int NB = kGjNB;
-  __shared__ T Ps[NB][NB];
-  for (int e = threadIdx.x; e < NB * NB; e += 256) Ps[e / NB][e % NB] = Pinv[e];
+  constexpr int NB = kGjNB, TL = 64;
+  __shared__ T Ps[NB][NB + 1];
+  __shared__ T Vs[TL][NB + 1];
+  const int i0 = blockIdx.x * TL;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < NB * NB; e += 256) Ps[e / NB][e % NB] = Pinv[e];
+  for (int e = tid; e < TL * NB; e += 256) {
+    const int r = e / NB, k = e % NB;
+    const int i = i0 + r;
+    const T val = (i < m && k < b) ? A[(long long)i * m + k0 + k] : T(0);
+    Vs[r][k] = val;
+  }
   __syncthreads();
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= m) return;
-  T v[NB];
-  const T* row = A + (long long)i * m + k0;
-#pragma unroll
-  for (int k = 0; k < NB; ++k) v[k] = (k < b) ? row[k] : T(0);
-#pragma unroll
-  for (int k = 0; k < NB; ++k)
-    if (k < b) Vt[(long long)k * m + i] = v[k];
-  for (int c = 0; c < b; ++c) {
+  for (int e = tid; e < TL * NB; e += 256) {
+    const int k = e / TL, r = e % TL;
+    if (i0 + r < m && k < b) Vt[(long long)k * m + i0 + r] = Vs[r][k];
+  }
+  // outputs: column c = tid / 4 (0..63), rows r = (tid % 4) + 4 t
+  const int c = tid >> 2;
+#pragma unroll 4
+  for (int t = 0; t < TL / 4; ++t) {
+    const int r = (tid & 3) + 4 * t;
     T acc = 0;
 #pragma unroll
-    for (int k = 0; k < NB; ++k) acc += v[k] * Ps[k][c];
-    Wt[(long long)c * m + i] = acc;
+    for (int k = 0; k < NB; ++k) acc += Vs[r][k] * Ps[k][c];
+    if (i0 + r < m && c < b) Wt[(long long)c * m + i0 + r] = acc;
   }
 }
 
@@ -375,7 +472,7 @@ static int dense_invert_typed(DenseNE& d, int m, double tol, hipStream_t s) {
   for (int k0 = 0; k0 < m; k0 += kGjNB) {
     const int b = std::min(kGjNB, m - k0);
     hipLaunchKernelGGL((k_gj_pivot<T>), dim3(1), dim3(256), 0, s, A, m, k0, b, Pinv, tol, d.clamped.as<int>());
-    hipLaunchKernelGGL((k_gj_panel<T>), dim3(grid_for(m, 256)), dim3(256), 0, s, A, m, k0, b, Pinv, Vt, Wt);
+    hipLaunchKernelGGL((k_gj_panel<T>), dim3((m + 63) / 64), dim3(256), 0, s, A, m, k0, b, Pinv, Vt, Wt);
     hipLaunchKernelGGL((k_gj_update<T>), g2, dim3(256), 0, s, A, m, k0, b, Vt, Wt, Pinv);
   }
   count_launch(3 * ((m + kGjNB - 1) / kGjNB));
@@ -393,27 +490,12 @@ int dense_reserve(DenseNE& d, int m, bool f32) {
   return HN_OK;
 }
 
-// Dense inverse of the assembled N in the chosen precision: afterwards
-// d.A holds -N^{-1}.
-int dense_factor(const NormalEq& ne, DenseNE& d, bool f32, hipStream_t s) {
-  const int m = (int)ne.N->nrows;
-  HN_TRY(dense_reserve(d, m, f32));
-  d.f32 = f32;
-  const long long tot = (long long)m * m;
+// In-place inverse of the dense N written by ne_numeric: afterwards d.A
+// holds -N^{-1}.
+int dense_factor(DenseNE& d, hipStream_t s) {
   const double tol = 1e-10;  // singular_tol of ToleranceConfig::industrial_defaults
   HN_HIP(hipMemsetAsync(d.clamped.p, 0, sizeof(int), s));
-  if (f32) {
-    hipLaunchKernelGGL((k_dense_fill<float>), dim3(grid_for(tot, 256, 16384)), dim3(256), 0, s, d.A.as<float>(), tot);
-    hipLaunchKernelGGL((k_dense_scatter<float>), dim3(grid_for(m, 256)), dim3(256), 0, s, ne.N->row_ptr.as<int>(),
-                       ne.N->col.as<int>(), ne.N->val64.as<double>(), m, d.A.as<float>());
-    count_launch(2);
-    return dense_invert_typed<float>(d, m, tol, s);
-  }
-  hipLaunchKernelGGL((k_dense_fill<double>), dim3(grid_for(tot, 256, 16384)), dim3(256), 0, s, d.A.as<double>(), tot);
-  hipLaunchKernelGGL((k_dense_scatter<double>), dim3(grid_for(m, 256)), dim3(256), 0, s, ne.N->row_ptr.as<int>(),
-                     ne.N->col.as<int>(), ne.N->val64.as<double>(), m, d.A.as<double>());
-  count_launch(2);
-  return dense_invert_typed<double>(d, m, tol, s);
+  return d.f32 ? dense_invert_typed<float>(d, d.m, tol, s) : dense_invert_typed<double>(d, d.m, tol, s);
 }
 
 // y = -A r  (A holds -N^{-1}); one wave per row, FP64 accumulation.

@@ -1,0 +1,213 @@
+// C++ tests of hn::backend::HipBackend, written after the reference's own
+// backend tests (tests/unit/test_gpu_backend.cpp:26-292): the same random
+// matrices (std::mt19937 recipe of make_problem, :26-49), the same
+// tolerance rule |got - ref| <= tol * max(1, |ref|) (:67-74), the same cases.
+// Usage: test_hip_backend [--no-gpu]   (--no-gpu runs only the fallback cases)
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "hip_backend.hpp"
+
+using namespace hn::backend;
+
+namespace {
+
+int g_failures = 0;
+#define EXPECT(cond)                                                                   \
+  do {                                                                                 \
+    if (!(cond)) {                                                                     \
+      std::printf("  FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond);                  \
+      ++g_failures;                                                                    \
+    }                                                                                  \
+  } while (0)
+
+struct Problem {
+  std::vector<double> values;
+  std::vector<std::size_t> col_indices, row_ptr;
+  std::vector<double> x;
+  std::size_t nrows = 0, ncols = 0;
+};
+
+Problem make_problem(std::size_t nrows, std::size_t ncols, double density, std::uint32_t seed) {
+  std::mt19937 rng(seed);
+  std::uniform_real_distribution<double> val(-4.0, 4.0);
+  Problem p;
+  p.nrows = nrows;
+  p.ncols = ncols;
+  p.row_ptr.push_back(0);
+  for (std::size_t i = 0; i < nrows; ++i) {
+    for (std::size_t j = 0; j < ncols; ++j) {
+      if (std::uniform_real_distribution<double>(0.0, 1.0)(rng) < density) {
+        p.values.push_back(val(rng));
+        p.col_indices.push_back(j);
+      }
+    }
+    p.row_ptr.push_back(p.values.size());
+  }
+  p.x.resize(ncols);
+  for (auto& v : p.x) v = val(rng);
+  return p;
+}
+
+std::vector<double> cpu_spmv(const Problem& p) {
+  std::vector<double> y(p.nrows, 0.0);
+  for (std::size_t i = 0; i < p.nrows; ++i) {
+    double s = 0.0;
+    for (std::size_t k = p.row_ptr[i]; k < p.row_ptr[i + 1]; ++k) s += p.values[k] * p.x[p.col_indices[k]];
+    y[i] = s;
+  }
+  return y;
+}
+
+void expect_close(const std::vector<double>& got, const std::vector<double>& ref, double tol) {
+  EXPECT(got.size() == ref.size());
+  if (got.size() != ref.size()) return;
+  for (std::size_t i = 0; i < ref.size(); ++i)
+    if (std::abs(got[i] - ref[i]) > tol * std::max(1.0, std::abs(ref[i]))) {
+      std::printf("  row %zu: %.17g vs %.17g\n", i, got[i], ref[i]);
+      ++g_failures;
+      return;
+    }
+}
+
+void run(const char* name, const std::function<void()>& fn) {
+  const int before = g_failures;
+  std::printf("[ RUN  ] %s\n", name);
+  std::fflush(stdout);
+  try {
+    fn();
+  } catch (const std::exception& e) {
+    std::printf("  exception: %s\n", e.what());
+    ++g_failures;
+  }
+  std::printf("[ %s ] %s\n", g_failures == before ? " OK " : "FAIL", name);
+  std::fflush(stdout);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  const bool gpu = !(argc > 1 && std::strcmp(argv[1], "--no-gpu") == 0);
+
+  run("HipBackendTest.FactoryReturnsNullWithoutLibrary", [] {
+    // gpu_backend.cpp:911-935: the factory never throws; no device -> no backend
+    EXPECT(create_hip_backend("/nonexistent/libhn_kernels.so") == nullptr);
+    EXPECT(!hip_backend_probe_error("/nonexistent/libhn_kernels.so").empty());
+    HipBackend b("/nonexistent/libhn_kernels.so");
+    b.initialize();
+    EXPECT(!b.is_available());
+    bool threw = false;
+    try {
+      std::vector<double> y;
+      b.spmv({1.0}, {0}, {0, 1}, {1.0}, y);
+    } catch (const std::runtime_error&) {
+      threw = true;
+    }
+    EXPECT(threw);
+  });
+  if (!gpu) return g_failures ? 1 : 0;
+
+  auto be = create_hip_backend();
+  if (!be) {
+    std::printf("no usable HIP device: %s\n", hip_backend_probe_error().c_str());
+    return 2;
+  }
+  std::printf("backend: %s\n", be->name().c_str());
+
+  run("HipBackendTest.SpmvShortRowsMatchesCpu", [&] {  // test_gpu_backend.cpp:211-222
+    const Problem p = make_problem(1500, 1200, 0.003, 77);
+    std::vector<double> y;
+    be->spmv(p.values, p.col_indices, p.row_ptr, p.x, y);
+    expect_close(y, cpu_spmv(p), 1e-10);
+  });
+  run("HipBackendTest.SpmvLongRowsMatchesCpu", [&] {  // :224-234
+    const Problem p = make_problem(300, 800, 0.08, 4242);
+    std::vector<double> y;
+    be->spmv(p.values, p.col_indices, p.row_ptr, p.x, y);
+    expect_close(y, cpu_spmv(p), 1e-10);
+  });
+  run("HipBackendTest.SpmvEmptyRows", [&] {  // :236-258
+    std::vector<double> vals;
+    std::vector<std::size_t> cols, rp{0};
+    for (std::size_t i = 0; i < 256; ++i) {
+      if (i % 4 != 0) {
+        vals.push_back(2.0);
+        cols.push_back(i);
+      }
+      rp.push_back(vals.size());
+    }
+    std::vector<double> x(256, 1.5), y;
+    be->spmv(vals, cols, rp, x, y);
+    EXPECT(y.size() == 256);
+    for (std::size_t i = 0; i < y.size(); ++i) EXPECT(std::abs(y[i] - ((i % 4 == 0) ? 0.0 : 3.0)) <= 1e-12);
+  });
+  run("HipBackendTest.EmptyMatrixIsSafe", [&] {  // :126-131
+    std::vector<double> y;
+    be->spmv({}, {}, {0}, {}, y);
+    EXPECT(y.empty());
+  });
+  run("HipBackendTest.SpmmMatchesCpu", [&] {  // :260-276
+    const Problem p = make_problem(400, 300, 0.05, 2024);
+    const std::size_t k = 5;
+    std::vector<double> B(p.ncols * k);
+    std::mt19937 rng(8);
+    std::uniform_real_distribution<double> val(-1.0, 1.0);
+    for (auto& v : B) v = val(rng);
+    std::vector<double> C, ref(p.nrows * k, 0.0);
+    be->spmm(p.values, p.col_indices, p.row_ptr, B, C, k);
+    for (std::size_t i = 0; i < p.nrows; ++i)
+      for (std::size_t e = p.row_ptr[i]; e < p.row_ptr[i + 1]; ++e)
+        for (std::size_t c = 0; c < k; ++c) ref[i * k + c] += p.values[e] * B[p.col_indices[e] * k + c];
+    expect_close(C, ref, 1e-10);
+  });
+  run("HipBackendTest.MatrixResidencyIsReused", [&] {  // :278-292
+    const Problem p = make_problem(2000, 1500, 0.03, 555);
+    std::vector<double> y1, y2;
+    be->spmv(p.values, p.col_indices, p.row_ptr, p.x, y1);
+    const std::size_t free1 = be->free_memory();
+    be->spmv(p.values, p.col_indices, p.row_ptr, p.x, y2);
+    EXPECT(be->free_memory() <= free1 + (1u << 20));
+    expect_close(y2, y1, 1e-12);
+  });
+  run("HipBackendTest.NewContentsSameBuffersNotStale", [&] {
+    Problem p = make_problem(500, 400, 0.02, 9);
+    std::vector<double> y1, y2;
+    be->spmv(p.values, p.col_indices, p.row_ptr, p.x, y1);
+    for (auto& v : p.values) v *= 2.0;  // same containers, new matrix
+    be->spmv(p.values, p.col_indices, p.row_ptr, p.x, y2);
+    expect_close(y2, cpu_spmv(p), 1e-10);
+  });
+  run("HipBackendTest.VectorOps", [&] {  // CPUBackend semantics, gpu_backend.cpp:124-136
+    std::vector<double> a(20000), b(20000);
+    for (std::size_t i = 0; i < a.size(); ++i) {
+      a[i] = std::sin(0.1 * i);
+      b[i] = std::cos(0.3 * i);
+    }
+    double ref = 0.0;
+    for (std::size_t i = 0; i < a.size(); ++i) ref += a[i] * b[i];
+    EXPECT(std::abs(be->dot(a, b) - ref) <= 1e-10 * std::max(1.0, std::abs(ref)));
+    std::vector<double> y = b, yr = b;
+    be->axpy(0.5, a, y);
+    for (std::size_t i = 0; i < yr.size(); ++i) yr[i] += 0.5 * a[i];
+    expect_close(y, yr, 0.0);
+    a[7] = -3.5;
+    EXPECT(be->norm_inf(a) == 3.5);
+  });
+  run("HipBackendTest.ExecutionStats", [&] {  // gpu_backend.hpp:48-50
+    be->reset_execution_stats();
+    EXPECT(!be->was_gpu_kernel_executed());
+    const Problem p = make_problem(64, 48, 0.3, 5);
+    std::vector<double> y;
+    be->spmv(p.values, p.col_indices, p.row_ptr, p.x, y);
+    EXPECT(be->was_gpu_kernel_executed() && be->gpu_kernel_executions() > 0);
+    EXPECT(be->type() == ComputeBackendType::HIP && be->is_available());
+  });
+  be->finalize();
+  std::printf("%s: %d failure(s)\n", g_failures ? "FAILED" : "PASSED", g_failures);
+  return g_failures ? 1 : 0;
+}
