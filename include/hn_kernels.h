@@ -233,6 +233,15 @@ int hn_ipm_default_opts(hn_ipm_opts* o);
  * rule, sigma, termination and status mapping as the reference; the normal
  * equations are solved by the method in opts->solver. */
 int hn_ipm_solve(const hn_lp* lp, const hn_ipm_opts* opts, hn_ipm_result* res);
+/* Solves `count` independent LPs in one persistent kernel, one workgroup per
+ * LP (largest first from a device work queue), each with the same algorithm
+ * as hn_ipm_solve and a dense LDL' of its normal equations (standard-form
+ * rows <= 20000 per LP).  opts->precision selects FP64 or FP32-factor +
+ * FP64-refinement; solver, time limit, interrupt and warm start are not
+ * used.  results[k] receives LP k (solve_ms is the wall time of the call).
+ * No reference counterpart: the reference solves one LP per call
+ * (InteriorPointSolver::solve, interior_point.cpp:402). */
+int hn_ipm_solve_batch(const hn_lp* lps, uint32_t count, const hn_ipm_opts* opts, hn_ipm_result* results);
 
 /* ---- harness helpers (device buffers, copies, events) ------------------- */
 int hn_dev_alloc(size_t bytes, void** ptr);

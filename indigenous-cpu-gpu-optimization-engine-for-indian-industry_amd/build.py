@@ -41,6 +41,7 @@ KERNEL_SOURCES = [
     "hn_stdform.cpp",
     "hn_normal.hip",
     "hn_ipm.hip",
+    "hn_batch.hip",
 ]
 
 

@@ -1,0 +1,889 @@
+// Batched interior-point solves: thousands of small LPs in one persistent
+// kernel, one workgroup per LP.
+//
+// Each workgroup takes the next LP from a device work queue (largest first,
+// so the long solves start early), runs the whole Mehrotra loop of
+// InteriorPointSolver::solve (core/lp/interior_point.cpp:443-753) on it and
+// takes another, until the queue is empty; every workgroup reaches the
+// queue's end, so the grid always drains.  Inside a workgroup:
+//   * SpMV on M and on M' (pre-built transpose) one row per thread, summed
+//     left to right - the order of the reference's CPU loops;
+//   * N = M D M' + r I assembled densely one row per thread, then factored
+//     in place by a right-looking blocked LDL' (64-wide blocks: the diagonal
+//     block in registers with one barrier per column, the panel one row per
+//     thread, the trailing update by LDS-staged 64x64 tiles), with the
+//     reference's pivot clamp (factorization.cpp:1309-1313);
+//   * solves by blocked forward/backward substitution and FP64 iterative
+//     refinement against the matrix-free N (refinement.cpp:7-36);
+//   * in mixed precision the factor is FP32 and the refinement FP64; an LP
+//     whose FP32 factor cannot be refined to 1e-10 in 5 sweeps, or whose mu
+//     falls below mu_switch, continues with an FP64 factor.
+// Everything else (start point, sigma, ratio test, guards, relaxed post-loop
+// test, status) follows the reference exactly as the single-LP driver does.
+#include "hn_device.hpp"
+#include "hn_internal.hpp"
+#include "hn_stdform.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <thread>
+#include <vector>
+
+#pragma clang fp contract(off)
+
+namespace hn {
+
+namespace {
+
+constexpr int NT = 256;
+constexpr int BK = 64;  // LDL' block width
+
+struct Inst {
+  int m, n;
+  long long mp, mz;   // M: row_ptr offset (m+1 ints), col/val offset
+  long long tp, tz;   // M': row_ptr offset (n+1 ints), col/val offset
+  long long vm, vn;   // m- and n-vector blocks
+  long long dn, w;    // dense N (m*m elements), panel workspace (m*64 elements)
+};
+
+enum VM { VM_B = 0, VM_LAM, VM_RP, VM_TM, VM_MDRD, VM_RHS, VM_RHS2, VM_DLAM, VM_MZ, VM_R, VM_ND, VM_COUNT };
+enum VN { VN_C = 0, VN_X, VN_Z, VN_RD, VN_D, VN_T, VN_DX, VN_DZ, VN_ZS, VN_COUNT };
+
+struct Out {
+  int status, iterations, fp64_from, ne_solves, refine;
+  double rp, rd, mu;
+};
+
+struct Params {
+  int max_iterations;
+  double conv_tol, comp_tol, reg, mu_switch;
+  int mixed;
+};
+
+// ---- workgroup helpers -----------------------------------------------------
+__device__ __forceinline__ void wg_spmv(const int* __restrict__ rp, const int* __restrict__ ci,
+                                        const double* __restrict__ v, const double* __restrict__ x,
+                                        double* __restrict__ y, int rows) {
+  for (int i = threadIdx.x; i < rows; i += NT) {
+    double s = 0.0;
+    int k = rp[i];
+    const int e = rp[i + 1];
+    for (; k + 4 <= e; k += 4) {  // four independent loads in flight, summed in order
+      const int c0 = ci[k], c1 = ci[k + 1], c2 = ci[k + 2], c3 = ci[k + 3];
+      const double v0 = v[k], v1 = v[k + 1], v2 = v[k + 2], v3 = v[k + 3];
+      const double x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+      s += v0 * x0;
+      s += v1 * x1;
+      s += v2 * x2;
+      s += v3 * x3;
+    }
+    for (; k < e; ++k) s += v[k] * x[ci[k]];
+    y[i] = s;
+  }
+}
+
+__device__ __forceinline__ double wg_sum(double v, double* red) { return block_sum<double, NT>(v, red); }
+__device__ __forceinline__ double wg_max(double v, double* red) { return block_max<double, NT>(v, red); }
+__device__ __forceinline__ double wg_min(double v, double* red) { return block_min<double, NT>(v, red); }
+
+// min over d < 0 of -v/d, then the reference's min(1, 0.99 alpha)
+__device__ double wg_step(const double* __restrict__ v, const double* __restrict__ d, int n, double* red) {
+  double a = 1.0;
+  for (int i = threadIdx.x; i < n; i += NT)
+    if (d[i] < 0.0) {
+      const double q = -v[i] / d[i];
+      a = (q < a) ? q : a;
+    }
+  a = wg_min(a, red);
+  const double s = 0.99 * a;
+  return (s < 1.0) ? s : 1.0;
+}
+
+// ---- dense LDL' in place (row-major, lower triangle) ------------------------
+// LDS of the factorisation (FP64: 67 KB; gfx950 has 160 KB per workgroup).
+// L11 (factor and inverse phases) and the update tiles share storage.
+template <typename T>
+struct SharedLDL {
+  struct {
+    T rk[2][BK];
+    T D[BK];
+    T L11[BK][BK + 1];
+  } f;
+  union {
+    T X[BK][BK];
+    struct {
+      T Ws[BK / 2][64];
+      T Ls[BK / 2][64];
+    } u;
+  };
+};
+
+// Returns the number of clamped pivots.
+template <typename T>
+__device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W, SharedLDL<T>& sh, double tol) {
+  const int tid = threadIdx.x;
+  const int ti = tid >> 4, tj = tid & 15;
+  int nclamp = 0;
+  for (int k0 = 0; k0 < m; k0 += BK) {
+    const int b = min(BK, m - k0);
+    // (1) diagonal block, symmetric copy in registers
+    T P[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int i = 4 * ti + a, j = 4 * tj + c;
+        const int hi = max(i, j), lo = min(i, j);
+        P[a][c] = (i < b && j < b) ? A[(long long)(k0 + hi) * m + k0 + lo] : T(i == j ? 1 : 0);
+      }
+    for (int k = 0; k < b; ++k) {
+      const int par = k & 1, kk = k & 3;
+      if (tj == (k >> 2)) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) sh.f.rk[par][4 * ti + a] = P[a][kk];
+      }
+      __syncthreads();
+      T piv = sh.f.rk[par][k];
+      if (piv <= T(tol)) {
+        piv = T(tol);
+        ++nclamp;
+      }
+      const T rinv = T(1) / piv;
+      T ci[4], cj[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) ci[a] = sh.f.rk[par][4 * ti + a];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) cj[c] = sh.f.rk[par][4 * tj + c];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int i = 4 * ti + a;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int j = 4 * tj + c;
+          if (i == k && j == k)
+            P[a][c] = piv;
+          else if (j == k && i > k)
+            P[a][c] = ci[a] * rinv;
+          else if (i == k && j > k)
+            P[a][c] = cj[c] * rinv;
+          else if (i > k && j > k)
+            P[a][c] = P[a][c] - ci[a] * (cj[c] * rinv);
+        }
+      }
+    }
+    // write back L (unit, below the diagonal) and D; keep L11, D in LDS
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int i = 4 * ti + a, j = 4 * tj + c;
+        if (i < b && j < b) {
+          if (j <= i) A[(long long)(k0 + i) * m + k0 + j] = P[a][c];
+          sh.f.L11[i][j] = (j < i) ? P[a][c] : T(0);
+          if (i == j) sh.f.D[i] = P[a][c];
+        }
+      }
+    __syncthreads();
+    // (2) panel.  X = L11^{-1} (unit lower), one column
+    //     per thread; then W = A21 L11^{-T} = A21 X' and L21 = W / D, 64 rows
+    //     at a time: thread (r = tid % 64, g = tid / 64) forms 16 columns.
+    T(*X)[BK] = sh.X;
+    if (tid < 64) {
+      const int j = tid;
+      for (int i = 0; i < BK; ++i) {
+        T acc = 0;
+        for (int k = 0; k < i; ++k)
+          if (k >= j) acc += sh.f.L11[i][k] * X[k][j];
+        X[i][j] = (i > j) ? -acc : T(i == j ? 1 : 0);
+      }
+    }
+    __syncthreads();
+    for (int i0 = k0 + b; i0 < m; i0 += 64) {
+      const int r = tid & 63, g = tid >> 6;
+      const int i = i0 + r;
+      T acc[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc[q] = 0;
+      if (i < m) {
+        const T* row = A + (long long)i * m + k0;
+        T av[BK];
+#pragma unroll
+        for (int k = 0; k < BK; ++k) av[k] = (k < b) ? row[k] : T(0);
+#pragma unroll
+        for (int k = 0; k < BK; ++k)
+#pragma unroll
+          for (int q = 0; q < 16; ++q) acc[q] += av[k] * X[16 * g + q][k];
+      }
+      __syncthreads();  // every read of the A21 rows precedes the writes below
+      if (i < m) {
+        T* row = A + (long long)i * m + k0;
+        T* wr = W + (long long)i * BK;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          const int c = 16 * g + q;
+          if (c < b) {
+            wr[c] = acc[q];
+            row[c] = acc[q] / sh.f.D[c];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // (3) trailing update of the lower triangle: A_ij -= sum_c W_ic L_jc
+    const int r0 = k0 + b;
+    const int nt = (m - r0 + 63) / 64;
+    for (int tI = 0; tI < nt; ++tI) {
+      for (int tJ = 0; tJ <= tI; ++tJ) {
+        const int i0 = r0 + 64 * tI, j0 = r0 + 64 * tJ;
+        T acc[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) acc[a][c] = 0;
+        for (int cb = 0; cb < b; cb += BK / 2) {
+          const int cw = min(BK / 2, b - cb);
+          for (int e = tid; e < 64 * (BK / 2); e += NT) {
+            const int r = e / (BK / 2), c = e % (BK / 2);
+            const int i = i0 + r, j = j0 + r;
+            sh.u.Ws[c][r] = (i < m && c < cw) ? W[(long long)i * BK + cb + c] : T(0);
+            sh.u.Ls[c][r] = (j < m && c < cw) ? A[(long long)j * m + k0 + cb + c] : T(0);
+          }
+          __syncthreads();
+          for (int c = 0; c < cw; ++c) {
+            T wv[4], lv[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) wv[a] = sh.u.Ws[c][ti + 16 * a];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) lv[a] = sh.u.Ls[c][tj + 16 * a];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+              for (int q = 0; q < 4; ++q) acc[a][q] += wv[a] * lv[q];
+          }
+          __syncthreads();
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const int i = i0 + ti + 16 * a;
+          if (i >= m) continue;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int j = j0 + tj + 16 * q;
+            if (j < m && j <= i) A[(long long)i * m + j] -= acc[a][q];
+          }
+        }
+      }
+    }
+    __syncthreads();  // the next block reads what the update wrote
+  }
+  return nclamp;
+}
+
+// x = N^{-1} r with the LDL' factor in A (in-place on x, which holds r).
+// Every row segment is loaded in full before it is used, so each thread keeps
+// up to 64 loads in flight; the current block of x is shared through LDS.
+template <typename T>
+__device__ __noinline__ void wg_ldlt_solve(const T* __restrict__ A, int m, double* __restrict__ x) {
+  __shared__ double xb[BK];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  // forward: L y = r
+  for (int k0 = 0; k0 < m; k0 += BK) {
+    const int b = min(BK, m - k0);
+    if (tid < 64) {
+      T lr[BK];
+#pragma unroll
+      for (int c = 0; c < BK; ++c) lr[c] = (lane < b && c < lane) ? A[(long long)(k0 + lane) * m + k0 + c] : T(0);
+      double y = (lane < b) ? x[k0 + lane] : 0.0;
+#pragma unroll
+      for (int c = 0; c < BK; ++c) {
+        const double yc = __shfl(y, c, 64);
+        if (lane > c) y -= (double)lr[c] * yc;
+      }
+      if (lane < b) {
+        x[k0 + lane] = y;
+        xb[lane] = y;
+      }
+    }
+    __syncthreads();
+    for (int i = k0 + b + tid; i < m; i += NT) {
+      const T* row = A + (long long)i * m + k0;
+      T lv[BK];
+#pragma unroll
+      for (int c = 0; c < BK; ++c) lv[c] = (c < b) ? row[c] : T(0);
+      double s = x[i];
+#pragma unroll
+      for (int c = 0; c < BK; ++c)
+        if (c < b) s -= (double)lv[c] * xb[c];
+      x[i] = s;
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < m; i += NT) {
+    const double d = (double)A[(long long)i * m + i];
+    if (d != 0.0) x[i] = x[i] / d;
+  }
+  __syncthreads();
+  // backward: L' x = y
+  const int nb = (m + BK - 1) / BK;
+  for (int bi = nb - 1; bi >= 0; --bi) {
+    const int k0 = bi * BK;
+    const int b = min(BK, m - k0);
+    if (tid < 64) {
+      T lc[BK];
+#pragma unroll
+      for (int c = 0; c < BK; ++c) lc[c] = (c < b && lane < c) ? A[(long long)(k0 + c) * m + k0 + lane] : T(0);
+      double y = (lane < b) ? x[k0 + lane] : 0.0;
+#pragma unroll
+      for (int c = BK - 1; c >= 0; --c) {
+        const double xc = __shfl(y, c, 64);
+        if (lane < c) y -= (double)lc[c] * xc;
+      }
+      if (lane < b) {
+        x[k0 + lane] = y;
+        xb[lane] = y;
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < k0; i += NT) {
+      T lv[BK];
+#pragma unroll
+      for (int c = 0; c < BK; ++c) lv[c] = (c < b) ? A[(long long)(k0 + c) * m + i] : T(0);
+      double s = x[i];
+#pragma unroll
+      for (int c = 0; c < BK; ++c)
+        if (c < b) s -= (double)lv[c] * xb[c];
+      x[i] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// ---- the per-LP Mehrotra loop ------------------------------------------------
+struct Ctx {
+  const int *mrp, *mci, *trp, *tci;
+  const double *mv, *tv;
+  double *vm[VM_COUNT], *vn[VN_COUNT];
+  int m, n;
+};
+
+// nd = N v = M (D (M' v)) + reg v  (matrix-free, FP64)
+__device__ __noinline__ void wg_apply_N(const Ctx& c, const double* __restrict__ v, double* __restrict__ nd, double reg) {
+  double* t = c.vn[VN_T];
+  wg_spmv(c.trp, c.tci, c.tv, v, t, c.n);
+  __syncthreads();
+  for (int j = threadIdx.x; j < c.n; j += NT) t[j] = c.vn[VN_D][j] * t[j];
+  __syncthreads();
+  wg_spmv(c.mrp, c.mci, c.mv, t, nd, c.m);
+  __syncthreads();
+  for (int i = threadIdx.x; i < c.m; i += NT) nd[i] = nd[i] + reg * v[i];
+  __syncthreads();
+}
+
+// dlam = N^{-1} rhs with refinement; returns true when refined to 1e-10.
+template <typename T>
+__device__ bool wg_solve(const Ctx& c, const T* A, const double* __restrict__ rhs, double* __restrict__ dlam,
+                         double reg, double* red, int* refine) {
+  for (int i = threadIdx.x; i < c.m; i += NT) dlam[i] = rhs[i];
+  __syncthreads();
+  wg_ldlt_solve<T>(A, c.m, dlam);
+  double* r = c.vm[VM_R];
+  double* nd = c.vm[VM_ND];
+  double nb = 0.0;
+  for (int i = threadIdx.x; i < c.m; i += NT) nb = nan_ignoring_max(nb, fabs(rhs[i]));
+  nb = wg_max(nb, red);
+  if (nb == 0.0 || nb != nb) return true;
+  for (int it = 0; it <= 5; ++it) {
+    wg_apply_N(c, dlam, nd, reg);
+    double a = 0.0;
+    for (int i = threadIdx.x; i < c.m; i += NT) {
+      const double ri = rhs[i] - nd[i];
+      r[i] = ri;
+      a = nan_ignoring_max(a, fabs(ri));
+    }
+    a = wg_max(a, red);
+    if (a / nb <= 1e-10) return true;
+    if (it == 5) break;
+    wg_ldlt_solve<T>(A, c.m, r);
+    for (int i = threadIdx.x; i < c.m; i += NT) dlam[i] = dlam[i] + r[i];
+    __syncthreads();
+    if (threadIdx.x == 0) ++*refine;
+  }
+  return false;
+}
+
+// Dense N (row-major, full rows written) + max-diag regularisation; returns r.
+template <typename T>
+__device__ __noinline__ double wg_assemble(const Ctx& c, T* __restrict__ A, double reg_min, double* red) {
+  const int m = c.m;
+  const long long tot = (long long)m * m;
+  for (long long e = threadIdx.x; e < tot; e += NT) A[e] = T(0);
+  __syncthreads();
+  const double* D = c.vn[VN_D];
+  double mxd = 0.0;
+  for (int i = threadIdx.x; i < m; i += NT) {
+    double diag = 0.0;
+    T* row = A + (long long)i * m;
+    for (int k = c.mrp[i]; k < c.mrp[i + 1]; ++k) {
+      const int j = c.mci[k];
+      const double d = D[j];
+      if (d <= 0.0) continue;
+      const double a = c.mv[k];
+      for (int q = c.trp[j]; q < c.trp[j + 1]; ++q) {
+        const int rr = c.tci[q];
+        const double val = (a * c.tv[q]) * d;
+        if (rr == i) diag += val;
+        row[rr] = row[rr] + (T)val;
+      }
+    }
+    mxd = nan_ignoring_max(mxd, diag);
+  }
+  mxd = wg_max(mxd, red);
+  const double r = fmax(reg_min, 1e-12 * mxd);
+  for (int i = threadIdx.x; i < m; i += NT) A[(long long)i * m + i] = A[(long long)i * m + i] + (T)r;
+  __syncthreads();
+  return r;
+}
+
+// Assembles and factors N; *reg_out receives its regularisation.
+template <typename T>
+__device__ void wg_factor(const Ctx& c, T* A, T* W, SharedLDL<T>& sh, double* red, double reg_min, double* reg_out) {
+  *reg_out = wg_assemble<T>(c, A, reg_min, red);
+  wg_ldlt<T>(A, c.m, W, sh, 1e-10);
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst, const int* __restrict__ order,
+                                                  int count, int* __restrict__ queue, const int* __restrict__ mrp,
+                                                  const int* __restrict__ mci, const double* __restrict__ mv,
+                                                  const int* __restrict__ trp, const int* __restrict__ tci,
+                                                  const double* __restrict__ tv, double* __restrict__ vecm,
+                                                  double* __restrict__ vecn, double* __restrict__ dense,
+                                                  double* __restrict__ work, Params prm, Out* __restrict__ out) {
+  __shared__ double red[NT / kWave];
+  __shared__ int s_next;
+  __shared__ SharedLDL<double> sh64;
+  SharedLDL<float>& sh32 = *reinterpret_cast<SharedLDL<float>*>(&sh64);
+  const int tid = threadIdx.x;
+  for (;;) {
+    if (tid == 0) s_next = atomicAdd(queue, 1);
+    __syncthreads();
+    const int q = s_next;
+    __syncthreads();
+    if (q >= count) return;
+    const int id = order[q];
+    const Inst I = inst[id];
+    Ctx c;
+    c.m = I.m;
+    c.n = I.n;
+    c.mrp = mrp + I.mp;
+    c.mci = mci + I.mz;
+    c.mv = mv + I.mz;
+    c.trp = trp + I.tp;
+    c.tci = tci + I.tz;
+    c.tv = tv + I.tz;
+    for (int k = 0; k < VM_COUNT; ++k) c.vm[k] = vecm + I.vm + (long long)k * I.m;
+    for (int k = 0; k < VN_COUNT; ++k) c.vn[k] = vecn + I.vn + (long long)k * I.n;
+    const int m = c.m, n = c.n;
+    double* A64 = dense + I.dn;
+    float* A32 = reinterpret_cast<float*>(A64);
+    double* W64 = work + I.w;
+    float* W32 = reinterpret_cast<float*>(W64);
+    double* b = c.vm[VM_B];
+    double* lam = c.vm[VM_LAM];
+    double* rp = c.vm[VM_RP];
+    double* tm = c.vm[VM_TM];
+    double* cc = c.vn[VN_C];
+    double* x = c.vn[VN_X];
+    double* z = c.vn[VN_Z];
+    double* rd = c.vn[VN_RD];
+    double* D = c.vn[VN_D];
+    double* t = c.vn[VN_T];
+    double* dx = c.vn[VN_DX];
+    double* dz = c.vn[VN_DZ];
+    double* zs = c.vn[VN_ZS];
+    double* dlam = c.vm[VM_DLAM];
+    double* rhs = c.vm[VM_RHS];
+    double* rhs2 = c.vm[VM_RHS2];
+    double* mdrd = c.vm[VM_MDRD];
+    double* mz = c.vm[VM_MZ];
+    int fp64_from = -1, ne_solves = 0, refine = 0;
+    bool fp64 = !prm.mixed;
+    // start point (interior_point.cpp:443-453)
+    double bn = 1.0, cn = 1.0;
+    for (int i = tid; i < m; i += NT) bn = nan_ignoring_max(bn, fabs(b[i]));
+    for (int j = tid; j < n; j += NT) cn = nan_ignoring_max(cn, fabs(cc[j]));
+    bn = wg_max(bn, red);
+    cn = wg_max(cn, red);
+    for (int j = tid; j < n; j += NT) {
+      x[j] = bn + 1.0;
+      z[j] = fmax(cn + 1.0, fabs(cc[j]));
+    }
+    for (int i = tid; i < m; i += NT) lam[i] = 0.0;
+    __syncthreads();
+    bool done = false, numerr = false;
+    int iterations = 0;
+    double mu = 0.0, rpn = 0.0, rdn = 0.0;
+    auto residuals = [&]() {
+      wg_spmv(c.mrp, c.mci, c.mv, x, tm, m);
+      wg_spmv(c.trp, c.tci, c.tv, lam, t, n);
+      __syncthreads();
+      double a = 0.0, bb = 0.0, s = 0.0;
+      for (int i = tid; i < m; i += NT) {
+        const double v = tm[i] - b[i];
+        rp[i] = v;
+        a = nan_ignoring_max(a, fabs(v));
+      }
+      for (int j = tid; j < n; j += NT) {
+        const double v = t[j] + z[j] - cc[j];
+        rd[j] = v;
+        bb = nan_ignoring_max(bb, fabs(v));
+        s += x[j] * z[j];
+      }
+      rpn = wg_max(a, red);
+      rdn = wg_max(bb, red);
+      mu = wg_sum(s, red) / (double)n;
+    };
+    for (int iter = 0; iter < prm.max_iterations; ++iter) {
+      residuals();
+      if (rpn < prm.conv_tol && rdn < prm.conv_tol && mu < prm.comp_tol) {
+        done = true;
+        break;
+      }
+      if (!fp64 && mu < prm.mu_switch) {
+        fp64 = true;
+        if (fp64_from < 0) fp64_from = iter;
+      }
+      for (int j = tid; j < n; j += NT) {
+        double d = x[j] / z[j];
+        if (!(d >= 0.0) || !isfinite(d) || d > 1e10) d = 1e10;
+        D[j] = d;
+        t[j] = d * rd[j];
+      }
+      __syncthreads();
+      wg_spmv(c.mrp, c.mci, c.mv, t, mdrd, m);
+      __syncthreads();
+      for (int i = tid; i < m; i += NT) rhs[i] = -rp[i] - mdrd[i] + tm[i];
+      double reg = 0.0;
+      bool ok;
+      if (fp64) {
+        wg_factor<double>(c, A64, W64, sh64, red, prm.reg, &reg);
+        ok = wg_solve<double>(c, A64, rhs, dlam, reg, red, &refine);
+      } else {
+        wg_factor<float>(c, A32, W32, sh32, red, prm.reg, &reg);
+        ok = wg_solve<float>(c, A32, rhs, dlam, reg, red, &refine);
+        if (!ok) {  // FP32 factor not refinable: FP64 from here on
+          fp64 = true;
+          if (fp64_from < 0) fp64_from = iter;
+          wg_factor<double>(c, A64, W64, sh64, red, prm.reg, &reg);
+          ok = wg_solve<double>(c, A64, rhs, dlam, reg, red, &refine);
+        }
+      }
+      ++ne_solves;
+      // predictor direction (:607-627)
+      wg_spmv(c.trp, c.tci, c.tv, dlam, t, n);
+      __syncthreads();
+      int bad = 0;
+      for (int j = tid; j < n; j += NT) {
+        const double s = t[j] + rd[j];
+        const double a = D[j] * s - x[j];
+        dx[j] = a;
+        dz[j] = -s;
+        if (!isfinite(a) || !isfinite(s)) bad = 1;
+      }
+      for (int i = tid; i < m; i += NT)
+        if (!isfinite(dlam[i])) bad = 1;
+      if (wg_max((double)bad, red) > 0.0) {
+        numerr = true;
+        break;
+      }
+      const double ap_a = wg_step(x, dx, n, red);
+      const double ad_a = wg_step(z, dz, n, red);
+      double s = 0.0;
+      for (int j = tid; j < n; j += NT) s += (x[j] + ap_a * dx[j]) * (z[j] + ad_a * dz[j]);
+      const double mu_aff = wg_sum(s, red) / (double)n;
+      double sigma;
+      if (!isfinite(mu_aff) || mu_aff < 0.0 || !(mu > 0.0)) {
+        sigma = 0.5;
+      } else {
+        sigma = pow(mu_aff / fmax(mu, 1e-300), 3.0);
+        sigma = sigma < 0.1 ? 0.1 : (sigma > 0.9 ? 0.9 : sigma);
+      }
+      const double smu = sigma * mu;
+      for (int j = tid; j < n; j += NT) {
+        const double zinv = 1.0 / z[j];
+        zs[j] = (smu - dx[j] * dz[j]) * zinv;
+      }
+      __syncthreads();
+      wg_spmv(c.mrp, c.mci, c.mv, zs, mz, m);
+      __syncthreads();
+      for (int i = tid; i < m; i += NT) rhs2[i] = rhs[i] - mz[i];
+      __syncthreads();
+      if (fp64)
+        ok = wg_solve<double>(c, A64, rhs2, dlam, reg, red, &refine);
+      else
+        ok = wg_solve<float>(c, A32, rhs2, dlam, reg, red, &refine);
+      ++ne_solves;
+      // corrector direction (:666-685)
+      wg_spmv(c.trp, c.tci, c.tv, dlam, t, n);
+      __syncthreads();
+      bad = 0;
+      for (int j = tid; j < n; j += NT) {
+        const double s2 = t[j] + rd[j];
+        const double a = D[j] * s2 - x[j] + zs[j];
+        dx[j] = a;
+        dz[j] = -s2;
+        if (!isfinite(a) || !isfinite(s2)) bad = 1;
+      }
+      for (int i = tid; i < m; i += NT)
+        if (!isfinite(dlam[i])) bad = 1;
+      if (wg_max((double)bad, red) > 0.0) {
+        numerr = true;
+        break;
+      }
+      const double ap = wg_step(x, dx, n, red);
+      const double ad = wg_step(z, dz, n, red);
+      bad = 0;
+      for (int j = tid; j < n; j += NT) {
+        const double xi = x[j] + ap * dx[j];
+        const double zi = z[j] + ad * dz[j];
+        x[j] = xi;
+        z[j] = zi;
+        if (xi < 0.0 || zi < 0.0 || !isfinite(xi) || !isfinite(zi)) bad = 1;
+      }
+      for (int i = tid; i < m; i += NT) lam[i] = lam[i] + ad * dlam[i];
+      if (wg_max((double)bad, red) > 0.0) {
+        numerr = true;
+        break;
+      }
+      iterations = iter + 1;
+    }
+    if (!done) {  // :728-753
+      residuals();
+      if (rpn < 1e-4 && rdn < 1e-4 && mu < 1e-4) done = true;
+    }
+    if (tid == 0) {
+      Out o;
+      o.status = numerr ? HN_STATUS_NUMERICAL_ERROR
+                 : done ? HN_STATUS_OPTIMAL
+                 : (rpn <= 1e-9) ? HN_STATUS_SUBOPTIMAL
+                                 : HN_STATUS_ITER_LIMIT;
+      o.iterations = iterations;
+      o.fp64_from = fp64_from;
+      o.ne_solves = ne_solves;
+      o.refine = refine;
+      o.rp = rpn;
+      o.rd = rdn;
+      o.mu = mu;
+      out[id] = o;
+    }
+    __syncthreads();
+  }
+}
+
+// Counting-sort transpose on the host (entries of each column in row order,
+// as build_normal_equations scatters them, interior_point.cpp:153-173).
+void host_transpose(const StandardForm& f, std::vector<int>& trp, std::vector<int>& tci, std::vector<double>& tv) {
+  trp.assign(f.n + 1, 0);
+  for (int64_t k = 0; k < (int64_t)f.ci.size(); ++k) trp[f.ci[k] + 1]++;
+  for (int64_t j = 0; j < f.n; ++j) trp[j + 1] += trp[j];
+  std::vector<int> cur(trp.begin(), trp.end() - 1);
+  tci.resize(f.ci.size());
+  tv.resize(f.ci.size());
+  for (int64_t i = 0; i < f.m; ++i)
+    for (int64_t k = f.rp[i]; k < f.rp[i + 1]; ++k) {
+      const int p = cur[f.ci[k]]++;
+      tci[p] = (int)i;
+      tv[p] = f.v[k];
+    }
+}
+
+struct BatchWorkspace {
+  std::mutex mtx;
+  DevBuf inst, order, queue, mrp, mci, mv, trp, tci, tv, vecm, vecn, dense, work, out;
+};
+BatchWorkspace& bws() {
+  static BatchWorkspace* w = new BatchWorkspace();
+  return *w;
+}
+
+template <typename V>
+int upload(DevBuf& d, const std::vector<V>& h, hipStream_t s) {
+  HN_TRY(d.reserve(sizeof(V) * std::max<size_t>(h.size(), 1)));
+  if (!h.empty()) HN_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(V) * h.size(), hipMemcpyHostToDevice, s));
+  return HN_OK;
+}
+
+}  // namespace
+
+int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o, hn_ipm_result* res) {
+  const auto t0 = std::chrono::steady_clock::now();
+  HN_TRY(enter());
+  hipStream_t s = ctx().stream;
+  if (count == 0) return HN_OK;
+  // host preparation, in parallel over LPs
+  std::vector<StandardForm> f(count);
+  std::vector<int> st(count, HN_OK);
+  std::vector<std::string> err(count);
+  {
+    const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned w = 0; w < nth; ++w)
+      th.emplace_back([&, w]() {
+        for (uint32_t k = w; k < count; k += nth) {
+          st[k] = build_standard_form(lps[k], f[k]);
+          if (st[k] != HN_OK) err[k] = get_error();
+        }
+      });
+    for (auto& t : th) t.join();
+  }
+  for (uint32_t k = 0; k < count; ++k) {
+    if (st[k] != HN_OK) return fail(st[k], "batch LP " + std::to_string(k) + ": " + err[k]);
+    if (f[k].m == 0 || f[k].n == 0 || lps[k].n == 0)
+      return fail(HN_ERR_UNSUPPORTED, "batch LP " + std::to_string(k) + " has an empty standard form");
+    if (f[k].m > 20000) return fail(HN_ERR_UNSUPPORTED, "batch LPs are limited to 20000 standard-form rows");
+  }
+  // pack
+  std::vector<Inst> inst(count);
+  std::vector<int> mrp, mci, trp, tci;
+  std::vector<double> mv, tv, vecm, vecn;
+  long long dn = 0, wk = 0;
+  {
+    std::vector<int> t_rp, t_ci;
+    std::vector<double> t_v;
+    for (uint32_t k = 0; k < count; ++k) {
+      const StandardForm& F = f[k];
+      Inst& I = inst[k];
+      I.m = (int)F.m;
+      I.n = (int)F.n;
+      I.mp = (long long)mrp.size();
+      I.mz = (long long)mci.size();
+      for (int64_t v : F.rp) mrp.push_back((int)v);
+      for (int64_t v : F.ci) mci.push_back((int)v);
+      mv.insert(mv.end(), F.v.begin(), F.v.end());
+      host_transpose(F, t_rp, t_ci, t_v);
+      I.tp = (long long)trp.size();
+      I.tz = (long long)tci.size();
+      trp.insert(trp.end(), t_rp.begin(), t_rp.end());
+      tci.insert(tci.end(), t_ci.begin(), t_ci.end());
+      tv.insert(tv.end(), t_v.begin(), t_v.end());
+      I.vm = (long long)vecm.size();
+      vecm.resize(vecm.size() + (size_t)VM_COUNT * F.m, 0.0);
+      std::memcpy(vecm.data() + I.vm + (long long)VM_B * F.m, F.rhs.data(), sizeof(double) * F.m);
+      I.vn = (long long)vecn.size();
+      vecn.resize(vecn.size() + (size_t)VN_COUNT * F.n, 0.0);
+      std::memcpy(vecn.data() + I.vn + (long long)VN_C * F.n, F.c.data(), sizeof(double) * F.n);
+      I.dn = dn;
+      dn += (long long)F.m * F.m;
+      I.w = wk;
+      wk += (long long)F.m * BK;
+    }
+  }
+  std::vector<int> order(count);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return inst[a].m > inst[b].m; });
+  BatchWorkspace& w = bws();
+  std::lock_guard<std::mutex> lk(w.mtx);
+  HN_TRY(upload(w.inst, inst, s));
+  HN_TRY(upload(w.order, order, s));
+  HN_TRY(upload(w.mrp, mrp, s));
+  HN_TRY(upload(w.mci, mci, s));
+  HN_TRY(upload(w.mv, mv, s));
+  HN_TRY(upload(w.trp, trp, s));
+  HN_TRY(upload(w.tci, tci, s));
+  HN_TRY(upload(w.tv, tv, s));
+  HN_TRY(upload(w.vecm, vecm, s));
+  HN_TRY(upload(w.vecn, vecn, s));
+  HN_TRY(w.dense.reserve(sizeof(double) * std::max<long long>(dn, 1)));
+  HN_TRY(w.work.reserve(sizeof(double) * std::max<long long>(wk, 1)));
+  HN_TRY(w.out.reserve(sizeof(Out) * count));
+  HN_TRY(w.queue.reserve(sizeof(int)));
+  HN_HIP(hipMemsetAsync(w.queue.p, 0, sizeof(int), s));
+  Params prm;
+  prm.max_iterations = o.max_iterations;
+  prm.conv_tol = o.convergence_tol;
+  prm.comp_tol = o.complementarity_tol;
+  prm.reg = o.regularization;
+  prm.mu_switch = o.mu_switch;
+  prm.mixed = o.precision == HN_PREC_MIXED;
+  const int grid = (int)std::min<uint32_t>(count, 2u * (uint32_t)std::max(1, ctx().info.compute_units));
+  hipLaunchKernelGGL(k_batch_ipm, dim3(grid), dim3(NT), 0, s, w.inst.as<Inst>(), w.order.as<int>(), (int)count,
+                     w.queue.as<int>(), w.mrp.as<int>(), w.mci.as<int>(), w.mv.as<double>(), w.trp.as<int>(),
+                     w.tci.as<int>(), w.tv.as<double>(), w.vecm.as<double>(), w.vecn.as<double>(),
+                     w.dense.as<double>(), w.work.as<double>(), prm, w.out.as<Out>());
+  HN_TRY(check_launch("k_batch_ipm"));
+  count_launch();
+  std::vector<Out> out(count);
+  HN_HIP(hipMemcpyAsync(out.data(), w.out.p, sizeof(Out) * count, hipMemcpyDeviceToHost, s));
+  HN_HIP(hipMemcpyAsync(vecm.data(), w.vecm.p, sizeof(double) * vecm.size(), hipMemcpyDeviceToHost, s));
+  HN_HIP(hipMemcpyAsync(vecn.data(), w.vecn.p, sizeof(double) * vecn.size(), hipMemcpyDeviceToHost, s));
+  HN_HIP(hipStreamSynchronize(s));
+  const double ms = 1e3 * std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  // results (objective as interior_point.cpp:757-780)
+  for (uint32_t k = 0; k < count; ++k) {
+    const StandardForm& F = f[k];
+    const Inst& I = inst[k];
+    hn_ipm_result& r = res[k];
+    const double* x = vecn.data() + I.vn + (long long)VN_X * I.n;
+    const double* z = vecn.data() + I.vn + (long long)VN_Z * I.n;
+    const double* lam = vecm.data() + I.vm + (long long)VM_LAM * I.m;
+    std::vector<double> primal(lps[k].n);
+    to_original_primal(F, x, primal.data());
+    double obj = 0.0;
+    bool ok = std::isfinite(lps[k].obj_offset);
+    for (int64_t j = 0; j < lps[k].n && ok; ++j) {
+      const double term = lps[k].obj[j] * primal[j];
+      if (!std::isfinite(term)) ok = false;
+      obj += term;
+    }
+    if (ok) obj += lps[k].obj_offset;
+    r.status = ok ? out[k].status : HN_STATUS_NUMERICAL_ERROR;
+    r.iterations = out[k].iterations;
+    r.objective = ok ? obj : 0.0;
+    r.solve_ms = ms;
+    r.sf_m = F.m;
+    r.sf_n = F.n;
+    r.rp_norm = out[k].rp;
+    r.rd_norm = out[k].rd;
+    r.mu = out[k].mu;
+    double bn = 0.0, cn = 0.0, ctx_ = 0.0, bty = 0.0;
+    for (int64_t i = 0; i < F.m; ++i) {
+      bn = std::max(bn, std::abs(F.rhs[i]));
+      bty += F.rhs[i] * lam[i];
+    }
+    for (int64_t j = 0; j < F.n; ++j) {
+      cn = std::max(cn, std::abs(F.c[j]));
+      ctx_ += F.c[j] * x[j];
+    }
+    r.rel_primal = out[k].rp / (1.0 + bn);
+    r.rel_dual = out[k].rd / (1.0 + cn);
+    r.rel_gap = std::abs(ctx_ - bty) / (1.0 + std::abs(ctx_));
+    r.ne_solves = out[k].ne_solves;
+    r.refine_sweeps = out[k].refine;
+    r.cg_iterations = 0;
+    r.fp64_from_iter = out[k].fp64_from;
+    r.warm_started = 0;
+    if (r.primal) std::memcpy(r.primal, primal.data(), sizeof(double) * lps[k].n);
+    if (r.x && r.cap_n >= F.n) std::memcpy(r.x, x, sizeof(double) * F.n);
+    if (r.s && r.cap_n >= F.n) std::memcpy(r.s, z, sizeof(double) * F.n);
+    if (r.y && r.cap_m >= F.m) std::memcpy(r.y, lam, sizeof(double) * F.m);
+  }
+  return HN_OK;
+}
+
+}  // namespace hn
+
+extern "C" int hn_ipm_solve_batch(const hn_lp* lps, uint32_t count, const hn_ipm_opts* opts, hn_ipm_result* results) {
+  HN_CHECK(count == 0 || (lps && results), HN_ERR_INVALID, "hn_ipm_solve_batch: null argument");
+  hn_ipm_opts o;
+  if (opts)
+    o = *opts;
+  else
+    hn_ipm_default_opts(&o);
+  HN_CHECK(o.precision == HN_PREC_F64 || o.precision == HN_PREC_MIXED, HN_ERR_INVALID,
+           "hn_ipm_solve_batch: bad precision");
+  return hn::ipm_solve_batch_impl(lps, count, o, results);
+}

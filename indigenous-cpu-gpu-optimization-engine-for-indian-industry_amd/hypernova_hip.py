@@ -575,3 +575,44 @@ class InteriorPointSolver:
 
 def ipm_solve(lp, **kw) -> InteriorPointResult:
     return InteriorPointSolver(InteriorPointOptions(**kw)).solve(lp)
+
+
+LIB.hn_ipm_solve_batch.argtypes = [C.POINTER(_HnLp), C.c_uint32, C.POINTER(_HnOpts), C.POINTER(_HnResult)]
+LIB.hn_ipm_solve_batch.restype = C.c_int
+
+
+def ipm_solve_batch(lps, precision: int = HN_PREC_F64, max_iterations: int = 100, mu_switch: float = 1e-4,
+                    want_points: bool = False):
+    """Solves many LPs in one persistent launch (hn_ipm_solve_batch).
+    Returns one InteriorPointResult per LP (solve_time_ms = whole call)."""
+    count = len(lps)
+    arr = (_HnLp * count)()
+    keep = []
+    for k, lp in enumerate(lps):
+        d = dict(rp=np.ascontiguousarray(lp.row_ptr, np.int64), ci=np.ascontiguousarray(lp.col, np.int64),
+                 v=np.ascontiguousarray(lp.val, np.float64), sense=np.ascontiguousarray(lp.sense, np.int8),
+                 rhs=np.ascontiguousarray(lp.rhs, np.float64), lb=np.ascontiguousarray(lp.lb, np.float64),
+                 ub=np.ascontiguousarray(lp.ub, np.float64), obj=np.ascontiguousarray(lp.obj, np.float64))
+        keep.append(d)
+        arr[k] = _HnLp(lp.m, lp.n, d["rp"].ctypes.data_as(_i64p), d["ci"].ctypes.data_as(_i64p), _dptr(d["v"]),
+                       d["sense"].ctypes.data_as(_i8p), _dptr(d["rhs"]), _dptr(d["lb"]), _dptr(d["ub"]),
+                       _dptr(d["obj"]), int(lp.maximize), float(getattr(lp, "obj_offset", 0.0)))
+    opts = _HnOpts()
+    check(LIB.hn_ipm_default_opts(C.byref(opts)))
+    opts.precision = precision
+    opts.max_iterations = max_iterations
+    opts.mu_switch = mu_switch
+    res = (_HnResult * count)()
+    bufs = []
+    for k, lp in enumerate(lps):
+        primal = np.zeros(max(lp.n, 1))
+        cap_m, cap_n = (lp.m + lp.n + 8, 2 * lp.n + lp.m + 8) if want_points else (0, 0)
+        x, y, s = np.zeros(max(cap_n, 1)), np.zeros(max(cap_m, 1)), np.zeros(max(cap_n, 1))
+        bufs.append((primal, x, y, s))
+        res[k].primal = _dptr(primal)
+        if want_points:
+            res[k].x, res[k].y, res[k].s = _dptr(x), _dptr(y), _dptr(s)
+            res[k].cap_m, res[k].cap_n = cap_m, cap_n
+    check(LIB.hn_ipm_solve_batch(arr, count, C.byref(opts), res))
+    return [InteriorPointResult(res[k], bufs[k][0][:lps[k].n].copy(), bufs[k][1], bufs[k][2], bufs[k][3])
+            for k in range(count)]
