@@ -10,6 +10,17 @@ and measured with HIP events on the stream the kernels run on.
 
 Printed: one JSON line with the metric, the roofline of the dominant kernel
 (the FP64 A x product) and the reference CPU baseline timed on this host.
+
+The LP legs ("lp" object, skipped by --no-lp) report LP solves/s:
+  * batch: 4,096 LPs (instance seeds 1000..5095, m log-uniform in [50, 2000],
+    n = 2.5 m, 5 nonzeros per column) solved by one persistent launch, against
+    the plain FP64 single-LP GPU solver run sequentially on a strided sample of
+    the same instances (instance sizes are i.i.d., so the sample is unbiased);
+  * single: the 2,000 x 5,000 random LP (seed 1), plain FP64 vs mixed;
+  * cpu_baseline: the reference InteriorPointSolver (crossover off, 1 thread)
+    on a prefix of the same strided sample, under a per-instance time limit.
+LP timings are host wall-clock around the whole call (standard form on the
+host, upload, solve, download), so they are end-to-end, not HBM-resident.
 """
 from __future__ import annotations
 
@@ -30,6 +41,23 @@ HBM_PEAK_GBS = 8000.0
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` as measured by the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs summarised in profiles/ (scaled by the copy
+    kernel calibration written there); None when no summary covers it."""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
+        return None
+    for fn in sorted(os.listdir(d)):
+        if not fn.endswith("_pmc.md"):
+            continue
+        for line in open(os.path.join(d, fn)):
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                return int(parts[2])
+    return None
 
 
 def spmv_bytes(nrows: int, ncols: int, nnz: int, sv: int) -> int:
@@ -180,7 +208,8 @@ def run_spmv(args) -> dict:
         "config": {"workload": "spmv_powerlaw_2Mx5M_40Mnnz_zipf1.2", "nrows": case.nrows, "ncols": case.ncols,
                    "nnz": case.nnz, "index_bytes": 4, "tiling_A": A.tiling(), "tiling_At": At.tiling()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(f"k_spmv_merge<double,{A.tiling()}>"),
                      "kernel": "k_spmv_merge<double> + k_spmv_fixup<double> (one A x product)",
                      "us_per_launch": round(ms_ax * 1e3, 2), "bytes_per_launch": ab64},
         "detail": {
@@ -199,6 +228,99 @@ def run_spmv(args) -> dict:
     return out
 
 
+def _kkt_max(results) -> dict:
+    return {"max_rel_primal": max(r.rel_primal for r in results),
+            "max_rel_dual": max(r.rel_dual for r in results),
+            "max_rel_gap": max(r.rel_gap for r in results)}
+
+
+def run_lp(args) -> dict:
+    import hypernova_hip as H
+    import lpgen
+
+    H.init(-1)
+    out = {}
+    # batched instances: one persistent launch
+    count = args.batch
+    t0 = time.time()
+    lps = [lpgen.batch_lp(i) for i in range(count)]
+    log(f"generated {count} batch LPs (sum m={sum(l.m for l in lps)}) in {time.time() - t0:.1f}s")
+    H.ipm_solve_batch([l for l in lps if l.m < 200][:64])  # warm-up: code objects
+    t0 = time.perf_counter()
+    res = H.ipm_solve_batch(lps, precision=H.HN_PREC_F64)
+    wall = time.perf_counter() - t0
+    status = {}
+    for r in res:
+        status[r.status] = status.get(r.status, 0) + 1
+    its = [r.iterations for r in res]
+    batch = {"workload": f"batch_{count}_lps_m50-2000_n2.5m_5nnz_per_col", "lps": count,
+             "wall_s": round(wall, 3), "lps_per_s": round(count / wall, 2), "status": status,
+             "iterations_mean": round(float(np.mean(its)), 2), "iterations_max": int(max(its)),
+             "precision": "f64",
+             "kkt_1e-8": sum(max(r.rel_primal, r.rel_dual, r.rel_gap) <= 1e-8 for r in res),
+             "at_iteration_limit": sum(r.iterations >= 100 for r in res)}
+    batch.update(_kkt_max(res))
+    log(f"batch: {count} LPs in {wall:.2f}s -> {count / wall:.1f} LPs/s {status}")
+    # plain FP64 single-LP GPU solver, sequential, on a strided sample
+    stride = max(1, count // args.seq_sample)
+    idx = list(range(0, count, stride))[:args.seq_sample]
+    H.ipm_solve(lps[idx[0]])
+    t0 = time.perf_counter()
+    seq = [H.ipm_solve(lps[i]) for i in idx]
+    seq_wall = time.perf_counter() - t0
+    seq_rate = len(idx) / seq_wall
+    seq_status = {}
+    for r in seq:
+        seq_status[r.status] = seq_status.get(r.status, 0) + 1
+    batch["sequential_plain"] = {"lps": len(idx), "stride": stride, "wall_s": round(seq_wall, 3),
+                                 "lps_per_s": round(seq_rate, 2), "status": seq_status,
+                                 "same_status_as_batch": sum(r.status == res[i].status for r, i in zip(seq, idx))}
+    batch["speedup_vs_sequential_plain"] = round(count / wall / seq_rate, 2)
+    log(f"sequential plain: {len(idx)} LPs in {seq_wall:.2f}s -> {seq_rate:.2f} LPs/s")
+    out["batch"] = batch
+
+    # one 2,000 x 5,000 LP, plain FP64 vs mixed precision
+    lp = lpgen.random_lp(1, 2000, 5000)
+    H.ipm_solve(lp)
+    single = {"workload": "random_lp_2000x5000_5nnz_per_col_seed1"}
+    for name, prec in (("plain_f64", H.HN_PREC_F64), ("mixed", H.HN_PREC_MIXED)):
+        t0 = time.perf_counter()
+        r = H.ipm_solve(lp, precision=prec)
+        dt = time.perf_counter() - t0
+        single[name] = {"ms": round(1e3 * dt, 2), "status": r.status, "iterations": r.iterations,
+                        "objective": r.objective_value, "rel_primal": r.rel_primal, "rel_dual": r.rel_dual,
+                        "rel_gap": r.rel_gap, "fp64_from_iter": r.fp64_from_iter}
+        log(f"single {name}: {r.status} it={r.iterations} {1e3 * dt:.1f} ms")
+    out["single"] = single
+
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O  # baseline only
+        if O.have_reference():
+            solve, kind = O.ref_ipm_solve, "reference"
+        else:
+            solve, kind = O.ipm_solve, "port"
+        done, spent, limited = 0, 0.0, 0
+        for i in idx:
+            if spent >= args.cpu_seconds:
+                break
+            t0 = time.perf_counter()
+            r = solve(lps[i], time_limit=args.cpu_lp_limit)
+            dt = time.perf_counter() - t0
+            spent += dt
+            if r.status == "OPTIMAL":
+                done += 1
+            else:
+                limited += 1
+        seen = done + limited
+        out["cpu_baseline"] = {
+            "value": round(done / spent, 3), "unit": "LPs/s", "cores": 1, "kind": kind,
+            "sample": (f"first {seen} instances of the strided batch sample, {args.cpu_lp_limit:.0f} s limit per "
+                       f"instance; {limited} stopped at the limit and count as unsolved with their time charged")}
+        log(f"cpu lp baseline {out['cpu_baseline']}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,12 +328,18 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-lp", action="store_true")
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--seq-sample", type=int, default=64)
+    ap.add_argument("--cpu-lp-limit", type=float, default=8.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is described in "
                                    "DESIGN.md and not launched", "n_gpus": args.gpus}))
         return 2
     out = run_spmv(args)
+    if not args.no_lp:
+        out["lp"] = run_lp(args)
     print(json.dumps(out), flush=True)
     return 0
 
