@@ -167,6 +167,13 @@ typedef struct hn_lp {
   const double* obj;       /* n */
   int maximize;
   double obj_offset;
+  /* Optional warm start of the scaled standard form (x0, s0: sf_n entries,
+   * y0: sf_m; all three or none), used by hn_ipm_solve_batch.  It replaces
+   * the standard start point only if it passes the centrality check
+   * described at hn_ipm_solve_batch. */
+  const double* x0;
+  const double* y0;
+  const double* s0;
 } hn_lp;
 
 /* Normal-equation solver of each Newton step. */
@@ -176,6 +183,22 @@ typedef struct hn_lp {
 /* Arithmetic of the normal-equation solve. */
 #define HN_PREC_F64 1   /* plain FP64 everywhere                              */
 #define HN_PREC_MIXED 2 /* FP32 bulk, FP64 residuals/refinement, FP64 below mu_switch */
+
+/* Pivot rule of the normal-equation factorisation.
+ * HN_PIVOT_CLAMP is the reference: N gets r = max(regularization,
+ * 1e-12 * max diag N) on its diagonal (interior_point.cpp:188-197) and LDL'
+ * pivots <= 1e-10 are replaced by 1e-10 (factorization.cpp:1309-1313).
+ * HN_PIVOT_SKIP is not in the reference: r = regularization only, and a
+ * pivot below 1e-13 (FP32 factors: 1e-6) times its row's diagonal of N is
+ * skipped - its component of the solution is set to zero, the rule of
+ * S. Wright, "Modified Cholesky factorizations in interior-point algorithms
+ * for linear programming", SIAM J. Optim. 9 (1999).  On degenerate LPs
+ * (transportation problems, many of the random batch instances) the
+ * reference rule stalls with ||rp|| far above 1e-8 until the iteration limit;
+ * the skip rule reaches the reference's 1e-8 stopping test in fewer
+ * iterations. */
+#define HN_PIVOT_CLAMP 0
+#define HN_PIVOT_SKIP 1
 
 typedef int (*hn_interrupt_fn)(void* user);
 
@@ -196,9 +219,14 @@ typedef struct hn_ipm_opts {
   int64_t dense_max_m;         /* HN_NE_AUTO threshold (40000) */
   int pcg_max_iter;            /* per solve (2000) */
   const double* x0;            /* optional warm start of the scaled standard form: */
-  const double* y0;            /* x0, s0 of length sf_n, y0 of length sf_m        */
-  const double* s0;            /* (all three or none)                              */
+  const double* y0;            /* x0, s0 of length sf_n, y0 of length sf_m (all    */
+  const double* s0;            /* three or none; else the LP's own x0/y0/s0); used */
+                               /* only if it passes the test at hn_ipm_solve_batch */
   int verbose;                 /* nonzero: per-iteration trace on stderr          */
+  int pivot_rule;              /* HN_PIVOT_CLAMP (reference, default) or HN_PIVOT_SKIP */
+  double mu_target;            /* > 0: stop at the first iterate with mu <= mu_target and
+                                  return it (status HN_STATUS_ITER_LIMIT unless the 1e-8
+                                  test was met first); 0 = off */
 } hn_ipm_opts;
 
 typedef struct hn_ipm_result {
@@ -228,6 +256,12 @@ typedef struct hn_ipm_result {
 } hn_ipm_result;
 
 int hn_ipm_default_opts(hn_ipm_opts* o);
+/* The scaled standard form the IPM iterates on (build_standard_form,
+ * interior_point.cpp:202-373, after remove_duplicate_rows :82-132): M as CSR
+ * (sf_m x sf_n), rhs, c.  Call once with null buffers to get the sizes, then
+ * with buffers of those sizes (row_ptr: sf_m + 1, col/val: nnz). */
+int hn_lp_standard_form(const hn_lp* lp, int64_t* sf_m, int64_t* sf_n, int64_t* nnz, int64_t* row_ptr,
+                        int64_t* col, double* val, double* rhs, double* c);
 /* InteriorPointSolver::solve (core/lp/interior_point.cpp:402-886) with the
  * Newton steps computed on the GPU.  Same standard form, start point, step
  * rule, sigma, termination and status mapping as the reference; the normal
@@ -237,11 +271,53 @@ int hn_ipm_solve(const hn_lp* lp, const hn_ipm_opts* opts, hn_ipm_result* res);
  * LP (largest first from a device work queue), each with the same algorithm
  * as hn_ipm_solve and a dense LDL' of its normal equations (standard-form
  * rows <= 20000 per LP).  opts->precision selects FP64 or FP32-factor +
- * FP64-refinement; solver, time limit, interrupt and warm start are not
- * used.  results[k] receives LP k (solve_ms is the wall time of the call).
+ * FP64-refinement, opts->pivot_rule the factorisation's pivot rule and
+ * opts->mu_target an early stop; solver, time limit, interrupt and the
+ * opts->x0/y0/s0 warm start are not used.  An LP's own lps[k].x0/y0/s0 warm
+ * start is used when it is finite, x0 > 0, s0 > 0, its mu0 = x0's0/n is below
+ * the mu of the standard start, min_j x0_j s0_j >= 1e-3 mu0 (centrality) and
+ * ||M x0 - b||_inf is no larger than at the standard start; otherwise the
+ * standard start point is used (results[k].warm_started tells which).  results[k] receives LP k (solve_ms is the wall time of the call).
  * No reference counterpart: the reference solves one LP per call
  * (InteriorPointSolver::solve, interior_point.cpp:402). */
 int hn_ipm_solve_batch(const hn_lp* lps, uint32_t count, const hn_ipm_opts* opts, hn_ipm_result* results);
+
+/* ---- learned warm start ------------------------------------------------- */
+/* A bipartite constraint-variable graph network (4 message-passing layers,
+ * hidden width 64, BF16 MFMA dense layers with FP32 accumulation) that maps
+ * an LP's scaled standard form (M, rhs, c) to a predicted interior point
+ * (log x, log s, y), trained on points the IPM itself produced.  No
+ * reference counterpart: the reference has no learned component; its
+ * warm-start hook is the (x0, y0, s0) start point these predictions fill. */
+typedef struct hn_gnn hn_gnn;             /* parameters + Adam state (FP32)  */
+typedef struct hn_gnn_graph hn_gnn_graph; /* a sparsity pattern + workspace  */
+#define HN_GNN_HIDDEN 64
+#define HN_GNN_LAYERS 4
+/* Glorot-uniform weights from a splitmix64 stream seeded by `seed`. */
+int hn_gnn_create(uint64_t seed, hn_gnn** out);
+int hn_gnn_free(hn_gnn* g);
+int hn_gnn_num_params(const hn_gnn* g, uint64_t* n);
+/* Flat FP32 parameter vector (host); set_params also clears the Adam state. */
+int hn_gnn_get_params(const hn_gnn* g, float* params);
+int hn_gnn_set_params(hn_gnn* g, const float* params);
+/* The pattern of M (CSR, host, 64-bit indices); instances of the same
+ * pattern reuse one graph. */
+int hn_gnn_graph_create(int64_t m, int64_t n, const int64_t* row_ptr, const int64_t* col, hn_gnn_graph** out);
+int hn_gnn_graph_free(hn_gnn_graph* gr);
+/* Forward on device inputs (val: nnz FP64 values of M, rhs: m, c: n) into
+ * device outputs log_x, log_s (n) and y (m) (each may be NULL). */
+int hn_gnn_forward_dev(hn_gnn* g, hn_gnn_graph* gr, const double* val, const double* rhs, const double* c,
+                       float* log_x, float* log_s, float* y, void* stream);
+/* Loss and its gradient with respect to every parameter for device targets
+ * tx = log x*, ts = log s* (n) and ty = y* (m); grad_host receives
+ * hn_gnn_num_params floats, loss_host one float (synchronous). */
+int hn_gnn_gradients_dev(hn_gnn* g, hn_gnn_graph* gr, const double* val, const double* rhs, const double* c,
+                         const float* tx, const float* ts, const float* ty, float* grad_host, float* loss_host);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on one instance; the loss
+ * before the step goes to *loss_dev (device, may be NULL). */
+int hn_gnn_train_step_dev(hn_gnn* g, hn_gnn_graph* gr, const double* val, const double* rhs, const double* c,
+                          const float* tx, const float* ts, const float* ty, float lr, float* loss_dev,
+                          void* stream);
 
 /* ---- harness helpers (device buffers, copies, events) ------------------- */
 int hn_dev_alloc(size_t bytes, void** ptr);

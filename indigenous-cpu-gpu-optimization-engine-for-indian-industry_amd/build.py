@@ -42,6 +42,7 @@ KERNEL_SOURCES = [
     "hn_normal.hip",
     "hn_ipm.hip",
     "hn_batch.hip",
+    "hn_gnn.hip",
 ]
 
 

@@ -12,7 +12,8 @@
 //     in place by a right-looking blocked LDL' (64-wide blocks: the diagonal
 //     block in registers with one barrier per column, the panel one row per
 //     thread, the trailing update by LDS-staged 64x64 tiles), with the
-//     reference's pivot clamp (factorization.cpp:1309-1313);
+//     reference's pivot clamp (factorization.cpp:1309-1313) or, under
+//     HN_PIVOT_SKIP, Wright's skipping of tiny pivots;
 //   * solves by blocked forward/backward substitution and FP64 iterative
 //     refinement against the matrix-free N (refinement.cpp:7-36);
 //   * in mixed precision the factor is FP32 and the refinement FP64; an LP
@@ -47,9 +48,10 @@ struct Inst {
   long long tp, tz;   // M': row_ptr offset (n+1 ints), col/val offset
   long long vm, vn;   // m- and n-vector blocks
   long long dn, w;    // dense N (m*m elements), panel workspace (m*64 elements)
+  int warm;           // 1: x, z, lam hold an accepted warm start
 };
 
-enum VM { VM_B = 0, VM_LAM, VM_RP, VM_TM, VM_MDRD, VM_RHS, VM_RHS2, VM_DLAM, VM_MZ, VM_R, VM_ND, VM_COUNT };
+enum VM { VM_B = 0, VM_LAM, VM_RP, VM_TM, VM_MDRD, VM_RHS, VM_RHS2, VM_DLAM, VM_MZ, VM_R, VM_ND, VM_DG, VM_COUNT };
 enum VN { VN_C = 0, VN_X, VN_Z, VN_RD, VN_D, VN_T, VN_DX, VN_DZ, VN_ZS, VN_COUNT };
 
 struct Out {
@@ -61,7 +63,18 @@ struct Params {
   int max_iterations;
   double conv_tol, comp_tol, reg, mu_switch;
   int mixed;
+  int skip;  // HN_PIVOT_SKIP: r = reg only, tiny pivots skipped
+  double mu_target;  // > 0: stop at the first iterate with mu <= mu_target
 };
+
+// A skipped pivot: its entry of D becomes "infinite", so its column of L and
+// its component of every solve vanish.
+template <typename T>
+__device__ __forceinline__ T skipped_pivot();
+template <>
+__device__ __forceinline__ double skipped_pivot<double>() { return 1e128; }
+template <>
+__device__ __forceinline__ float skipped_pivot<float>() { return 1e30f; }
 
 // ---- workgroup helpers -----------------------------------------------------
 __device__ __forceinline__ void wg_spmv(const int* __restrict__ rp, const int* __restrict__ ci,
@@ -121,9 +134,12 @@ struct SharedLDL {
   };
 };
 
-// Returns the number of clamped pivots.
+// Returns the number of clamped (or skipped) pivots.  skip_rel == 0: the
+// reference's clamp of pivots <= tol; skip_rel > 0: pivots <= skip_rel times
+// dg (the diagonal of N before elimination) are skipped.
 template <typename T>
-__device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W, SharedLDL<T>& sh, double tol) {
+__device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W, SharedLDL<T>& sh, double tol,
+                                    const double* __restrict__ dg, double skip_rel) {
   const int tid = threadIdx.x;
   const int ti = tid >> 4, tj = tid & 15;
   int nclamp = 0;
@@ -147,11 +163,22 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
       }
       __syncthreads();
       T piv = sh.f.rk[par][k];
-      if (piv <= T(tol)) {
-        piv = T(tol);
-        ++nclamp;
+      T rinv;
+      if (skip_rel > 0.0) {
+        if (!((double)piv > skip_rel * dg[k0 + k])) {
+          piv = skipped_pivot<T>();
+          rinv = T(0);
+          ++nclamp;
+        } else {
+          rinv = T(1) / piv;
+        }
+      } else {
+        if (piv <= T(tol)) {
+          piv = T(tol);
+          ++nclamp;
+        }
+        rinv = T(1) / piv;
       }
-      const T rinv = T(1) / piv;
       T ci[4], cj[4];
 #pragma unroll
       for (int a = 0; a < 4; ++a) ci[a] = sh.f.rk[par][4 * ti + a];
@@ -417,7 +444,8 @@ __device__ bool wg_solve(const Ctx& c, const T* A, const double* __restrict__ rh
 
 // Dense N (row-major, full rows written) + max-diag regularisation; returns r.
 template <typename T>
-__device__ __noinline__ double wg_assemble(const Ctx& c, T* __restrict__ A, double reg_min, double* red) {
+__device__ __noinline__ double wg_assemble(const Ctx& c, T* __restrict__ A, double reg_min, double rel_reg,
+                                           double* red) {
   const int m = c.m;
   const long long tot = (long long)m * m;
   for (long long e = threadIdx.x; e < tot; e += NT) A[e] = T(0);
@@ -440,19 +468,24 @@ __device__ __noinline__ double wg_assemble(const Ctx& c, T* __restrict__ A, doub
       }
     }
     mxd = nan_ignoring_max(mxd, diag);
+    c.vm[VM_DG][i] = diag;
   }
   mxd = wg_max(mxd, red);
-  const double r = fmax(reg_min, 1e-12 * mxd);
-  for (int i = threadIdx.x; i < m; i += NT) A[(long long)i * m + i] = A[(long long)i * m + i] + (T)r;
+  const double r = fmax(reg_min, rel_reg * mxd);
+  for (int i = threadIdx.x; i < m; i += NT) {
+    A[(long long)i * m + i] = A[(long long)i * m + i] + (T)r;
+    c.vm[VM_DG][i] = c.vm[VM_DG][i] + r;
+  }
   __syncthreads();
   return r;
 }
 
 // Assembles and factors N; *reg_out receives its regularisation.
 template <typename T>
-__device__ void wg_factor(const Ctx& c, T* A, T* W, SharedLDL<T>& sh, double* red, double reg_min, double* reg_out) {
-  *reg_out = wg_assemble<T>(c, A, reg_min, red);
-  wg_ldlt<T>(A, c.m, W, sh, 1e-10);
+__device__ void wg_factor(const Ctx& c, T* A, T* W, SharedLDL<T>& sh, double* red, double reg_min, bool skip,
+                          double* reg_out) {
+  *reg_out = wg_assemble<T>(c, A, reg_min, skip ? 0.0 : 1e-12, red);
+  wg_ldlt<T>(A, c.m, W, sh, 1e-10, c.vm[VM_DG], skip ? (sizeof(T) == 4 ? 1e-6 : 1e-13) : 0.0);
   __syncthreads();
 }
 
@@ -512,19 +545,22 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
     double* mz = c.vm[VM_MZ];
     int fp64_from = -1, ne_solves = 0, refine = 0;
     bool fp64 = !prm.mixed;
-    // start point (interior_point.cpp:443-453)
-    double bn = 1.0, cn = 1.0;
-    for (int i = tid; i < m; i += NT) bn = nan_ignoring_max(bn, fabs(b[i]));
-    for (int j = tid; j < n; j += NT) cn = nan_ignoring_max(cn, fabs(cc[j]));
-    bn = wg_max(bn, red);
-    cn = wg_max(cn, red);
-    for (int j = tid; j < n; j += NT) {
-      x[j] = bn + 1.0;
-      z[j] = fmax(cn + 1.0, fabs(cc[j]));
+    // start point (interior_point.cpp:443-453), unless the host accepted a
+    // warm start and packed it into x, z, lam
+    if (!I.warm) {
+      double bn = 1.0, cn = 1.0;
+      for (int i = tid; i < m; i += NT) bn = nan_ignoring_max(bn, fabs(b[i]));
+      for (int j = tid; j < n; j += NT) cn = nan_ignoring_max(cn, fabs(cc[j]));
+      bn = wg_max(bn, red);
+      cn = wg_max(cn, red);
+      for (int j = tid; j < n; j += NT) {
+        x[j] = bn + 1.0;
+        z[j] = fmax(cn + 1.0, fabs(cc[j]));
+      }
+      for (int i = tid; i < m; i += NT) lam[i] = 0.0;
     }
-    for (int i = tid; i < m; i += NT) lam[i] = 0.0;
     __syncthreads();
-    bool done = false, numerr = false;
+    bool done = false, numerr = false, stopped = false;
     int iterations = 0;
     double mu = 0.0, rpn = 0.0, rdn = 0.0;
     auto residuals = [&]() {
@@ -553,6 +589,10 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
         done = true;
         break;
       }
+      if (prm.mu_target > 0.0 && mu <= prm.mu_target) {
+        stopped = true;
+        break;
+      }
       if (!fp64 && mu < prm.mu_switch) {
         fp64 = true;
         if (fp64_from < 0) fp64_from = iter;
@@ -570,15 +610,15 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
       double reg = 0.0;
       bool ok;
       if (fp64) {
-        wg_factor<double>(c, A64, W64, sh64, red, prm.reg, &reg);
+        wg_factor<double>(c, A64, W64, sh64, red, prm.reg, prm.skip != 0, &reg);
         ok = wg_solve<double>(c, A64, rhs, dlam, reg, red, &refine);
       } else {
-        wg_factor<float>(c, A32, W32, sh32, red, prm.reg, &reg);
+        wg_factor<float>(c, A32, W32, sh32, red, prm.reg, prm.skip != 0, &reg);
         ok = wg_solve<float>(c, A32, rhs, dlam, reg, red, &refine);
         if (!ok) {  // FP32 factor not refinable: FP64 from here on
           fp64 = true;
           if (fp64_from < 0) fp64_from = iter;
-          wg_factor<double>(c, A64, W64, sh64, red, prm.reg, &reg);
+          wg_factor<double>(c, A64, W64, sh64, red, prm.reg, prm.skip != 0, &reg);
           ok = wg_solve<double>(c, A64, rhs, dlam, reg, red, &refine);
         }
       }
@@ -661,7 +701,7 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
       }
       iterations = iter + 1;
     }
-    if (!done) {  // :728-753
+    if (!done && !stopped) {  // :728-753
       residuals();
       if (rpn < 1e-4 && rdn < 1e-4 && mu < 1e-4) done = true;
     }
@@ -669,6 +709,7 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
       Out o;
       o.status = numerr ? HN_STATUS_NUMERICAL_ERROR
                  : done ? HN_STATUS_OPTIMAL
+                 : stopped ? HN_STATUS_ITER_LIMIT
                  : (rpn <= 1e-9) ? HN_STATUS_SUBOPTIMAL
                                  : HN_STATUS_ITER_LIMIT;
       o.iterations = iterations;
@@ -727,6 +768,7 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
   // host preparation, in parallel over LPs
   std::vector<StandardForm> f(count);
   std::vector<int> st(count, HN_OK);
+  std::vector<int> warm_ok(count, 0);
   std::vector<std::string> err(count);
   {
     const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -735,7 +777,12 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
       th.emplace_back([&, w]() {
         for (uint32_t k = w; k < count; k += nth) {
           st[k] = build_standard_form(lps[k], f[k]);
-          if (st[k] != HN_OK) err[k] = get_error();
+          if (st[k] != HN_OK) {
+            err[k] = get_error();
+            continue;
+          }
+          if (lps[k].x0 && lps[k].y0 && lps[k].s0)
+            warm_ok[k] = warm_start_acceptable(f[k], lps[k].x0, lps[k].y0, lps[k].s0) ? 1 : 0;
         }
       });
     for (auto& t : th) t.join();
@@ -776,6 +823,12 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
       I.vn = (long long)vecn.size();
       vecn.resize(vecn.size() + (size_t)VN_COUNT * F.n, 0.0);
       std::memcpy(vecn.data() + I.vn + (long long)VN_C * F.n, F.c.data(), sizeof(double) * F.n);
+      I.warm = warm_ok[k];
+      if (I.warm) {
+        std::memcpy(vecn.data() + I.vn + (long long)VN_X * F.n, lps[k].x0, sizeof(double) * F.n);
+        std::memcpy(vecn.data() + I.vn + (long long)VN_Z * F.n, lps[k].s0, sizeof(double) * F.n);
+        std::memcpy(vecm.data() + I.vm + (long long)VM_LAM * F.m, lps[k].y0, sizeof(double) * F.m);
+      }
       I.dn = dn;
       dn += (long long)F.m * F.m;
       I.w = wk;
@@ -809,6 +862,8 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
   prm.reg = o.regularization;
   prm.mu_switch = o.mu_switch;
   prm.mixed = o.precision == HN_PREC_MIXED;
+  prm.skip = o.pivot_rule == HN_PIVOT_SKIP;
+  prm.mu_target = o.mu_target;
   const int grid = (int)std::min<uint32_t>(count, 2u * (uint32_t)std::max(1, ctx().info.compute_units));
   hipLaunchKernelGGL(k_batch_ipm, dim3(grid), dim3(NT), 0, s, w.inst.as<Inst>(), w.order.as<int>(), (int)count,
                      w.queue.as<int>(), w.mrp.as<int>(), w.mci.as<int>(), w.mv.as<double>(), w.trp.as<int>(),
@@ -865,7 +920,7 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
     r.refine_sweeps = out[k].refine;
     r.cg_iterations = 0;
     r.fp64_from_iter = out[k].fp64_from;
-    r.warm_started = 0;
+    r.warm_started = warm_ok[k];
     if (r.primal) std::memcpy(r.primal, primal.data(), sizeof(double) * lps[k].n);
     if (r.x && r.cap_n >= F.n) std::memcpy(r.x, x, sizeof(double) * F.n);
     if (r.s && r.cap_n >= F.n) std::memcpy(r.s, z, sizeof(double) * F.n);
@@ -885,5 +940,7 @@ extern "C" int hn_ipm_solve_batch(const hn_lp* lps, uint32_t count, const hn_ipm
     hn_ipm_default_opts(&o);
   HN_CHECK(o.precision == HN_PREC_F64 || o.precision == HN_PREC_MIXED, HN_ERR_INVALID,
            "hn_ipm_solve_batch: bad precision");
+  HN_CHECK(o.pivot_rule == HN_PIVOT_CLAMP || o.pivot_rule == HN_PIVOT_SKIP, HN_ERR_INVALID,
+           "hn_ipm_solve_batch: bad pivot_rule");
   return hn::ipm_solve_batch_impl(lps, count, o, results);
 }

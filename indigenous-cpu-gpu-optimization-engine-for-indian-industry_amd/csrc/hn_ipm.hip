@@ -416,12 +416,12 @@ __global__ __launch_bounds__(kNT) void k_pcg_init(const double* __restrict__ rhs
 
 // r = max(reg, 1e-12 max diag) for the matrix-free operator
 __global__ __launch_bounds__(kNT) void k_pcg_reg(const double* __restrict__ diag, int m, double reg,
-                                                 double* __restrict__ S) {
+                                                 double rel_reg, double* __restrict__ S) {
   __shared__ double red[kNT / kWave];
   double a = 0.0;
   for (int i = threadIdx.x; i < m; i += kNT) a = nan_ignoring_max(a, diag[i]);
   a = block_max<double, kNT>(a, red);
-  if (threadIdx.x == 0) S[S_REG] = fmax(reg, 1e-12 * a);
+  if (threadIdx.x == 0) S[S_REG] = fmax(reg, rel_reg * a);
 }
 
 struct Timer {
@@ -504,16 +504,19 @@ struct Solver {
 
   int spmv(const hn_csr* A, const double* xin, double* yout) { return spmv_launch(A, HN_F64, xin, yout, s); }
 
+  double rel_reg() const { return o.pivot_rule == HN_PIVOT_SKIP ? 0.0 : kRelReg; }
+
   // Factor (dense) or prepare (PCG) the normal equations for the current D.
   int prepare_ne() {
     if (use_dense) {
-      HN_TRY(ne_numeric(M, Mt, D.as<double>(), o.regularization, ne, S.as<double>() + S_REG, &dense, !fp64_solve, s));
-      HN_TRY(dense_factor(dense, s));
+      HN_TRY(ne_numeric(M, Mt, D.as<double>(), o.regularization, rel_reg(), ne, S.as<double>() + S_REG, &dense,
+                        !fp64_solve, s));
+      HN_TRY(dense_factor(dense, o.pivot_rule, s));
       dense_ready = true;
     } else {
       HN_TRY(ne_jacobi_diag(M, D.as<double>(), pcg_diag.as<double>(), s));
       hipLaunchKernelGGL(k_pcg_reg, dim3(1), dim3(kNT), 0, s, pcg_diag.as<double>(), m, o.regularization,
-                         S.as<double>());
+                         rel_reg(), S.as<double>());
       HN_TRY(check_launch("k_pcg_reg"));
     }
     return HN_OK;
@@ -597,8 +600,9 @@ struct Solver {
     if (!ok && !fp64_solve) {  // FP32 inverse could not be refined: switch to FP64 for good
       fp64_solve = true;
       *escalated = true;
-      HN_TRY(ne_numeric(M, Mt, D.as<double>(), o.regularization, ne, S.as<double>() + S_REG, &dense, false, s));
-      HN_TRY(dense_factor(dense, s));
+      HN_TRY(ne_numeric(M, Mt, D.as<double>(), o.regularization, rel_reg(), ne, S.as<double>() + S_REG, &dense,
+                        false, s));
+      HN_TRY(dense_factor(dense, o.pivot_rule, s));
       HN_TRY(solve_dense(r_in, out, &ok));
     }
     return HN_OK;
@@ -671,10 +675,15 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
   // start point (:443-453) or the caller's warm start
   {
     std::vector<double> x0(n), z0(n), l0(m, 0.0);
-    if (o.x0 && o.y0 && o.s0) {
-      std::memcpy(x0.data(), o.x0, sizeof(double) * n);
-      std::memcpy(z0.data(), o.s0, sizeof(double) * n);
-      std::memcpy(l0.data(), o.y0, sizeof(double) * m);
+    // the caller's warm start (options first, then the LP's own), if it
+    // passes the acceptance test; otherwise the standard start point
+    const double* wx = o.x0 ? o.x0 : lp.x0;
+    const double* wy = o.x0 ? o.y0 : lp.y0;
+    const double* ws = o.x0 ? o.s0 : lp.s0;
+    if (wx && wy && ws && warm_start_acceptable(f, wx, wy, ws)) {
+      std::memcpy(x0.data(), wx, sizeof(double) * n);
+      std::memcpy(z0.data(), ws, sizeof(double) * n);
+      std::memcpy(l0.data(), wy, sizeof(double) * m);
       res.warm_started = 1;
     } else {
       double b_norm = 1.0, c_norm = 1.0;
@@ -693,7 +702,7 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
 
   const dim3 B(256);
   const int np_n = parts_for(std::max(n, m));
-  bool done = false, time_up = false, interrupted = false, numerical_error = false;
+  bool done = false, time_up = false, interrupted = false, numerical_error = false, stopped = false;
   auto deadline = [&]() { return o.time_limit_seconds > 0.0 && clock.s() >= o.time_limit_seconds; };
   // residuals, mu and their norms on the device; host copies in hS
   auto residuals = [&]() -> int {
@@ -724,6 +733,10 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
     mu = hS[S_XZ] / (double)n;
     if (hS[S_RPN] < o.convergence_tol && hS[S_RDN] < o.convergence_tol && mu < o.complementarity_tol) {
       done = true;  // :525-530
+      break;
+    }
+    if (o.mu_target > 0.0 && mu <= o.mu_target) {  // early stop (not in the reference)
+      stopped = true;
       break;
     }
     if (!sv.fp64_solve && mu < o.mu_switch) {
@@ -803,11 +816,11 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
     res.iterations = iter + 1;
   }
 
-  if (!done && !interrupted) {  // :728-753
+  if (!done && !interrupted && !stopped) {  // :728-753
     HN_TRY(residuals());
     mu = hS[S_XZ] / (double)n;
     if (hS[S_RPN] < 1e-4 && hS[S_RDN] < 1e-4 && mu < 1e-4) done = true;
-  } else if (interrupted) {
+  } else if (interrupted || stopped) {
     HN_TRY(residuals());
     mu = hS[S_XZ] / (double)n;
   }
@@ -836,6 +849,7 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
   res.status = numerical_error ? HN_STATUS_NUMERICAL_ERROR
                : interrupted   ? HN_STATUS_INTERRUPTED
                : done          ? HN_STATUS_OPTIMAL
+               : stopped       ? HN_STATUS_ITER_LIMIT
                : feasible_incumbent ? HN_STATUS_SUBOPTIMAL
                : time_up       ? HN_STATUS_TIME_LIMIT
                                : HN_STATUS_ITER_LIMIT;
@@ -871,6 +885,7 @@ extern "C" int hn_ipm_default_opts(hn_ipm_opts* o) {
   o->mu_switch = 1e-4;
   o->dense_max_m = 40000;
   o->pcg_max_iter = 2000;
+  o->pivot_rule = HN_PIVOT_CLAMP;
   return HN_OK;
 }
 
@@ -884,5 +899,7 @@ extern "C" int hn_ipm_solve(const hn_lp* lp, const hn_ipm_opts* opts, hn_ipm_res
   HN_CHECK(o.precision == HN_PREC_F64 || o.precision == HN_PREC_MIXED, HN_ERR_INVALID, "hn_ipm_solve: bad precision");
   HN_CHECK(o.solver >= HN_NE_AUTO && o.solver <= HN_NE_PCG, HN_ERR_INVALID, "hn_ipm_solve: bad solver");
   HN_CHECK(o.max_iterations >= 0, HN_ERR_INVALID, "hn_ipm_solve: negative max_iterations");
+  HN_CHECK(o.pivot_rule == HN_PIVOT_CLAMP || o.pivot_rule == HN_PIVOT_SKIP, HN_ERR_INVALID,
+           "hn_ipm_solve: bad pivot_rule");
   return hn::ipm_solve_impl(*lp, o, *res);
 }

@@ -17,17 +17,30 @@ struct NormalEq {
 struct DenseNE {
   int m = 0;
   bool f32 = false;
-  DevBuf A, pinv, vt, wt, clamped;
+  DevBuf A;              // N, then L (lower), then -N^{-1} (full)
+  DevBuf G;              // L^{-1} (lower triangular; 64 x 64 diagonal blocks stored in full)
+  DevBuf dinv;           // 1 / pivot (0 for a skipped pivot)
+  DevBuf wt, lt;         // panel W = L D and L, k-major (64 x m)
+  DevBuf clamped;        // number of clamped or skipped pivots
+  DevBuf diag0;  // diagonal of N (regularisation included) before elimination
 };
+
+// Skip-rule thresholds relative to the diagonal of N (HN_PIVOT_SKIP).
+constexpr double kSkipRel64 = 1e-13;
+constexpr double kSkipRel32 = 1e-6;
+// Relative regularisation of the reference rule: r = max(reg, 1e-12 max diag N).
+constexpr double kRelReg = 1e-12;
 
 int ne_symbolic(const hn_csr* M, const hn_csr* Mt, NormalEq& ne, hipStream_t s);
 // Numeric N for the current D; also writes it densely (zeros elsewhere) into
 // `dense` in FP32 or FP64 when dense is not null.
-int ne_numeric(const hn_csr* M, const hn_csr* Mt, const double* D, double reg, NormalEq& ne, double* reg_out,
-               DenseNE* dense, bool f32, hipStream_t s);
+// rel_reg: kRelReg under the reference's pivot rule, 0 under the skip rule.
+int ne_numeric(const hn_csr* M, const hn_csr* Mt, const double* D, double reg, double rel_reg, NormalEq& ne,
+               double* reg_out, DenseNE* dense, bool f32, hipStream_t s);
 int dense_reserve(DenseNE& d, int m, bool f32);
-// Inverts the dense N in place (d.A becomes -N^{-1}).
-int dense_factor(DenseNE& d, hipStream_t s);
+// Inverts the dense N in place (d.A becomes -N^{-1}) under the pivot rule
+// HN_PIVOT_CLAMP or HN_PIVOT_SKIP.
+int dense_factor(DenseNE& d, int pivot_rule, hipStream_t s);
 int dense_apply(const DenseNE& d, const double* r, double* y, bool accumulate, hipStream_t s);
 int ne_jacobi_diag(const hn_csr* M, const double* D, double* diag, hipStream_t s);
 

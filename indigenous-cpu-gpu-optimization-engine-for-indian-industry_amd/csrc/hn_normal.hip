@@ -15,12 +15,11 @@
 //   * numeric   - one thread per row accumulates its row in column order
 //     (deterministic, no atomics), then r is added on the diagonal;
 //   * dense     - for moderate m the matrix is scattered into a dense
-//     row-major buffer and inverted in place by the blocked symmetric sweep
-//     operator (Gauss-Jordan on SPD blocks): each step inverts an NB x NB
-//     pivot block in LDS (its scalar pivots are the LDL' pivots in natural
-//     order, clamped like the reference) and applies a rank-NB update to the
-//     whole matrix with LDS-staged 64x64 tiles.  Solves are then one dense
-//     GEMV; FP64 refinement against the sparse N corrects them;
+//     row-major buffer, factored N = L D L' by 64-wide blocks (pivots in
+//     natural order, clamped like the reference or skipped), and -N^{-1} =
+//     -L^{-T} D^{-1} L^{-1} is formed over it from the triangular factor with
+//     LDS-staged 64x64 tiles.  Solves are then one dense GEMV; FP64
+//     refinement against the sparse N corrects them;
 //   * PCG       - matrix-free products M (D (M' p)) + r p and the Jacobi
 //     diagonal sum_j m_ij^2 D_j + r, for LPs too large to densify.
 #include "hn_device.hpp"
@@ -203,7 +202,8 @@ __global__ __launch_bounds__(256) void k_ne_numeric(
   }
 }
 
-// Per-block maximum of the diagonal, then r = max(reg, 1e-12 * max) added.
+// Per-block maximum of the diagonal, then r = max(reg, rel_reg * max) added
+// (rel_reg = 1e-12 under the reference's rule, 0 under the skip rule).
 __global__ __launch_bounds__(256) void k_ne_diag_max(const double* __restrict__ nval,
                                                      const int* __restrict__ diag_pos, int m,
                                                      double* __restrict__ part) {
@@ -216,18 +216,20 @@ __global__ __launch_bounds__(256) void k_ne_diag_max(const double* __restrict__ 
 }
 
 __global__ __launch_bounds__(256) void k_ne_add_reg(const double* __restrict__ part, int np, double reg,
-                                                    double* __restrict__ nval, const int* __restrict__ diag_pos,
-                                                    int m, double* __restrict__ reg_out, double* __restrict__ d64,
-                                                    float* __restrict__ d32) {
+                                                    double rel_reg, double* __restrict__ nval,
+                                                    const int* __restrict__ diag_pos, int m,
+                                                    double* __restrict__ reg_out, double* __restrict__ d64,
+                                                    float* __restrict__ d32, double* __restrict__ diag0) {
   // every block recomputes the (tiny) final maximum so no extra launch is needed
   __shared__ double red[4];
   double mx = 0.0;
   for (int i = threadIdx.x; i < np; i += 256) mx = nan_ignoring_max(mx, part[i]);
   mx = block_max<double, 256>(mx, red);
-  const double r = fmax(reg, 1e-12 * mx);
+  const double r = fmax(reg, rel_reg * mx);
   for (int i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256) {
     const double dv = nval[diag_pos[i]] + r;
     nval[diag_pos[i]] = dv;
+    if (diag0) diag0[i] = dv;
     if (d64) d64[(long long)i * m + i] = dv;
     if (d32) d32[(long long)i * m + i] = (float)dv;
   }
@@ -241,8 +243,8 @@ __global__ void k_dense_fill(T* __restrict__ A, long long total) {
     A[i] = T(0);
 }
 
-int ne_numeric(const hn_csr* M, const hn_csr* Mt, const double* D, double reg, NormalEq& ne, double* reg_out,
-               DenseNE* dense, bool f32, hipStream_t s) {
+int ne_numeric(const hn_csr* M, const hn_csr* Mt, const double* D, double reg, double rel_reg, NormalEq& ne,
+               double* reg_out, DenseNE* dense, bool f32, hipStream_t s) {
   const int m = (int)M->nrows;
   hn_csr* N = ne.N;
   const dim3 gw((unsigned)(((long long)m * kWave + 255) / 256));
@@ -273,83 +275,63 @@ int ne_numeric(const hn_csr* M, const hn_csr* Mt, const double* D, double reg, N
   HN_TRY(ne.part.reserve(sizeof(double) * 256));
   hipLaunchKernelGGL(k_ne_diag_max, dim3(nb), dim3(256), 0, s, N->val64.as<double>(), ne.diag_pos.as<int>(), m,
                      ne.part.as<double>());
-  hipLaunchKernelGGL(k_ne_add_reg, dim3(nb), dim3(256), 0, s, ne.part.as<double>(), nb, reg, N->val64.as<double>(),
-                     ne.diag_pos.as<int>(), m, reg_out, d64, d32);
+  hipLaunchKernelGGL(k_ne_add_reg, dim3(nb), dim3(256), 0, s, ne.part.as<double>(), nb, reg, rel_reg,
+                     N->val64.as<double>(), ne.diag_pos.as<int>(), m, reg_out, d64, d32,
+                     dense ? dense->diag0.as<double>() : nullptr);
   count_launch(dense ? 4 : 3);
   return check_launch("k_ne_add_reg");
 }
 
 // ---------------------------------------------------------------------------
-// dense inverse by blocked symmetric sweeps
+// dense N^{-1} through a blocked LDL' factorisation
 // ---------------------------------------------------------------------------
-constexpr int kGjNB = 64;
-constexpr int kGjTile = 64;
+// N = L D L' (right-looking, 64-wide blocks, natural pivot order - the pivots
+// are the reference's LDL' pivots and are clamped or skipped like them), then
+// G = L^{-1} (block rows from the inverted diagonal blocks) and finally
+// -N^{-1} = -G' D^{-1} G written over N, so every solve stays one GEMV.
+// Forming the inverse from the triangular factor keeps it as accurate as
+// triangular solves; a Gauss-Jordan sweep of N itself loses several more
+// digits once N's condition passes ~1e10, which the skip rule reaches.
+constexpr int kNB = 64;
 
-// Symmetric sweep inverse of a 64 x 64 block held in registers by a
-// 256-thread workgroup: thread (ti, tj) = (tid / 16, tid % 16) owns rows
-// 4ti..4ti+3 and columns 4tj..4tj+3.  Step k publishes row k and column k in
-// double-buffered LDS (one barrier per step), then every thread updates its
-// 16 entries:  a_kk <- -1/p,  a_kj <- a_kj/p,  a_ik <- a_ik/p,
-// a_ij <- a_ij - a_ik a_kj / p.  Pivots p <= tol (and NaN-free non-positive
-// ones) are replaced by tol, the reference's LDL' clamp; steps k >= b leave
-// the block untouched.  Returns the number of clamped pivots (thread 0).
+// acc[a][c] += sum_k As[k][ty + 16 a] * Bs[k][tx + 16 c], k < kw.
 template <typename T>
-__device__ int sweep_block64(T (&P)[4][4], int b, double tol, T (*rk)[64], T (*ck)[64]) {
-  const int tid = threadIdx.x;
-  const int ti = tid >> 4, tj = tid & 15;
-  int nclamp = 0;
-  for (int k = 0; k < b; ++k) {
-    const int par = k & 1;
-    const int kk = k & 3;
-    if (ti == (k >> 2)) {
+__device__ __forceinline__ void tile_acc(T (&acc)[4][4], const T (*As)[kNB], const T (*Bs)[kNB], int kw) {
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  for (int k = 0; k < kw; ++k) {
+    T av[4], bv[4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) rk[par][4 * tj + c] = P[kk][c];
-    }
-    if (tj == (k >> 2)) {
+    for (int a = 0; a < 4; ++a) av[a] = As[k][ty + 16 * a];
 #pragma unroll
-      for (int a = 0; a < 4; ++a) ck[par][4 * ti + a] = P[a][kk];
-    }
-    __syncthreads();
-    T piv = rk[par][k];
-    if (piv <= T(tol)) {
-      piv = T(tol);
-      ++nclamp;
-    }
-    const T rinv = T(1) / piv;
-    T rkj[4], cik[4];
+    for (int c = 0; c < 4; ++c) bv[c] = Bs[k][tx + 16 * c];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) rkj[c] = rk[par][4 * tj + c] * rinv;
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int a = 0; a < 4; ++a) cik[a] = ck[par][4 * ti + a];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const int i = 4 * ti + a;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const int j = 4 * tj + c;
-        T v;
-        if (i == k && j == k)
-          v = -rinv;
-        else if (i == k)
-          v = rkj[c];
-        else if (j == k)
-          v = cik[a] * rinv;
-        else
-          v = P[a][c] - cik[a] * rkj[c];
-        P[a][c] = v;
-      }
-    }
+      for (int c = 0; c < 4; ++c) acc[a][c] += av[a] * bv[c];
   }
-  return nclamp;
 }
 
-// Inverts the pivot block A[k0:k0+b, k0:k0+b] (identity beyond b) and writes
-// Pinv (64 x 64, row-major); clamped pivots are added to *clamped.
 template <typename T>
-__global__ __launch_bounds__(256) void k_gj_pivot(const T* __restrict__ A, int m, int k0, int b,
-                                                  T* __restrict__ Pinv, double tol, int* __restrict__ clamped) {
-  __shared__ T rk[2][64];
-  __shared__ T ck[2][64];
+__device__ __forceinline__ void tile_zero(T (&acc)[4][4]) {
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[a][c] = T(0);
+}
+
+// (1) LDL' of the diagonal block A[k0:k0+b, k0:k0+b] (lower triangle read,
+// symmetric copy in registers: thread (ti, tj) owns rows 4ti.., columns
+// 4tj..).  Writes the strictly lower part of L, dinv[k0:k0+b] (1/pivot; 0 for
+// a skipped pivot) and X = L_KK^{-1} as the full 64 x 64 diagonal block of
+// G = L^{-1} (zeros above).  Pivots p <= tol are clamped to tol
+// (skip_rel == 0) or, with skip_rel > 0, p <= skip_rel * diag0 is skipped.
+template <typename T>
+__global__ __launch_bounds__(256) void k_ldl_diag(T* __restrict__ A, int m, int k0, int b, T* __restrict__ dinv,
+                                                  T* __restrict__ G, double tol, const double* __restrict__ diag0,
+                                                  double skip_rel, int* __restrict__ clamped) {
+  __shared__ T rk[2][kNB];
+  __shared__ T Ls[kNB][kNB + 1];
+  __shared__ T Xs[kNB][kNB + 1];
   const int tid = threadIdx.x;
   const int ti = tid >> 4, tj = tid & 15;
   T P[4][4];
@@ -358,144 +340,286 @@ __global__ __launch_bounds__(256) void k_gj_pivot(const T* __restrict__ A, int m
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       const int i = 4 * ti + a, j = 4 * tj + c;
-      P[a][c] = (i < b && j < b) ? A[(long long)(k0 + i) * m + k0 + j] : T(i == j ? 1 : 0);
+      const int hi = max(i, j), lo = min(i, j);
+      P[a][c] = (i < b && j < b) ? A[(long long)(k0 + hi) * m + k0 + lo] : T(i == j ? 1 : 0);
     }
-  const int nclamp = sweep_block64<T>(P, b, tol, rk, ck);
-  if (tid == 0 && nclamp) atomicAdd(clamped, nclamp);
+  int ncl = 0;
+  for (int k = 0; k < b; ++k) {
+    const int par = k & 1, kk = k & 3;
+    if (tj == (k >> 2)) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a) rk[par][4 * ti + a] = P[a][kk];
+    }
+    __syncthreads();
+    T piv = rk[par][k];
+    T rinv;
+    if (skip_rel > 0.0) {
+      if (!((double)piv > skip_rel * diag0[k0 + k])) {
+        rinv = T(0);
+        ++ncl;
+      } else {
+        rinv = T(1) / piv;
+      }
+    } else {
+      if (piv <= T(tol)) {
+        piv = T(tol);
+        ++ncl;
+      }
+      rinv = T(1) / piv;
+    }
+    if (tid == 0) dinv[k0 + k] = rinv;
+    T ci[4], cj[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) ci[a] = rk[par][4 * ti + a];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) cj[c] = rk[par][4 * tj + c];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int i = 4 * ti + a;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int j = 4 * tj + c;
+        if (j == k && i > k)
+          P[a][c] = ci[a] * rinv;
+        else if (i == k && j > k)
+          P[a][c] = cj[c] * rinv;
+        else if (i > k && j > k)
+          P[a][c] = P[a][c] - ci[a] * (cj[c] * rinv);
+      }
+    }
+  }
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) Pinv[(4 * ti + a) * 64 + 4 * tj + c] = -P[a][c];
-}
-
-// Vt[k][i] = A[i][k0+k]; Wt[c][i] = sum_k Vt[k][i] Pinv[k][c]  (k-major, m
-// columns).  64 rows per workgroup: the rows' pivot columns and Pinv are
-// staged in LDS, each thread produces 16 outputs.
-template <typename T>
-__global__ __launch_bounds__(256) void k_gj_panel(const T* __restrict__ A, int m, int k0, int b,
-                                                  const T* __restrict__ Pinv, T* __restrict__ Vt,
-                                                  T* __restrict__ Wt) {
-  constexpr int NB = kGjNB, TL = 64;
-  __shared__ T Ps[NB][NB + 1];
-  __shared__ T Vs[TL][NB + 1];
-  const int i0 = blockIdx.x * TL;
-  const int tid = threadIdx.x;
-  for (int e = tid; e < NB * NB; e += 256) Ps[e / NB][e % NB] = Pinv[e];
-  for (int e = tid; e < TL * NB; e += 256) {
-    const int r = e / NB, k = e % NB;
-    const int i = i0 + r;
-    const T val = (i < m && k < b) ? A[(long long)i * m + k0 + k] : T(0);
-    Vs[r][k] = val;
+    for (int c = 0; c < 4; ++c) {
+      const int i = 4 * ti + a, j = 4 * tj + c;
+      const bool low = i < b && j < i;
+      if (low) A[(long long)(k0 + i) * m + k0 + j] = P[a][c];
+      Ls[i][j] = low ? P[a][c] : T(0);
+    }
+  if (tid == 0 && ncl) atomicAdd(clamped, ncl);
+  __syncthreads();
+  if (tid < kNB) {  // column j of X: x_i = delta_ij - sum_{j<=k<i} L_ik x_k
+    const int j = tid;
+    for (int i = 0; i < kNB; ++i) {
+      T sx = (i == j) ? T(1) : T(0);
+      if (i > j)
+        for (int k = j; k < i; ++k) sx -= Ls[i][k] * Xs[k][j];
+      Xs[i][j] = sx;
+    }
   }
   __syncthreads();
-  for (int e = tid; e < TL * NB; e += 256) {
-    const int k = e / TL, r = e % TL;
-    if (i0 + r < m && k < b) Vt[(long long)k * m + i0 + r] = Vs[r][k];
+  for (int e = tid; e < kNB * kNB; e += 256) {
+    const int i = e / kNB, j = e % kNB;
+    if (i < b && j < b) G[(long long)(k0 + i) * m + k0 + j] = Xs[i][j];
   }
-  // outputs: column c = tid / 4 (0..63), rows r = (tid % 4) + 4 t
-  const int c = tid >> 2;
+}
+
+// (2) panel, 64 rows r >= k0 + b per workgroup: W = A[r, K] X' (so that
+// W L_KK' = A[r, K]) and L[r, K] = W dinv; W and L are also written k-major
+// for the trailing update.  Each thread forms 16 outputs of one column.
+// Workgroups past the panel finish block row K of G = L^{-1}:
+// G[K, J] = X_K R[K, J] for J < K, where R[K, J] holds the accumulated
+// -sum L[K, K'] G[K', J] of the earlier steps (G[K, K] = X_K already).
+template <typename T>
+__global__ __launch_bounds__(256) void k_ldl_panel(T* __restrict__ A, int m, int k0, int b, int npanel,
+                                                   const T* __restrict__ dinv, T* __restrict__ G,
+                                                   T* __restrict__ Wt, T* __restrict__ Lt) {
+  __shared__ T Xs[kNB][kNB + 1];
+  __shared__ T Vs[kNB][kNB + 1];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < kNB * kNB; e += 256) {
+    const int i = e / kNB, k = e % kNB;
+    Xs[i][k] = (i < b && k < b) ? G[(long long)(k0 + i) * m + k0 + k] : T(0);
+  }
+  if ((int)blockIdx.x >= npanel) {
+    const int j0 = ((int)blockIdx.x - npanel) * kNB;  // a full block left of K
+    for (int e = tid; e < kNB * kNB; e += 256) {
+      const int k = e / kNB, c = e % kNB;
+      Vs[k][c] = k < b ? G[(long long)(k0 + k) * m + j0 + c] : T(0);
+    }
+    __syncthreads();
+    const int c = tid & 63, rg = tid >> 6;
+    for (int r = rg; r < b; r += 4) {
+      T acc = 0;
+      for (int k = 0; k <= r; ++k) acc += Xs[r][k] * Vs[k][c];
+      G[(long long)(k0 + r) * m + j0 + c] = acc;  // every read of the tile preceded the barrier
+    }
+    return;
+  }
+  const int i0 = k0 + b + blockIdx.x * kNB;
+  for (int e = tid; e < kNB * kNB; e += 256) {
+    const int i = e / kNB, k = e % kNB;
+    Vs[i][k] = (i0 + i < m && k < b) ? A[(long long)(i0 + i) * m + k0 + k] : T(0);
+  }
+  __syncthreads();
+  const int c = tid >> 2;  // output column (pivot index within the block)
+  if (c >= b) return;
+  const T dc = dinv[k0 + c];
 #pragma unroll 4
-  for (int t = 0; t < TL / 4; ++t) {
+  for (int t = 0; t < kNB / 4; ++t) {
     const int r = (tid & 3) + 4 * t;
-    T acc = 0;
-#pragma unroll
-    for (int k = 0; k < NB; ++k) acc += Vs[r][k] * Ps[k][c];
-    if (i0 + r < m && c < b) Wt[(long long)c * m + i0 + r] = acc;
+    const int i = i0 + r;
+    if (i >= m) break;
+    T w = 0;
+#pragma unroll 16
+    for (int k = 0; k <= c; ++k) w += Vs[r][k] * Xs[c][k];
+    const T l = w * dc;
+    A[(long long)i * m + k0 + c] = l;
+    Wt[(long long)c * m + i] = w;
+    Lt[(long long)c * m + i] = l;
   }
 }
 
-// A -= W V' outside the pivot rows/columns; pivot rows/columns get W, the
-// pivot block gets -Pinv.  64 x 64 output tile per workgroup, 4 x 4 per thread.
+// (3) trailing update.  Grid x: K + 1 G-columns then nt A-columns; grid y:
+// the nt block rows below K.  A tiles (J >= I excluded):
+// A[I, J] -= W[I, K] L[J, K]'.  G tiles (J <= K): R[I, J] = -L[I, K] X_K for
+// J = K (first touch), R[I, J] -= L[I, K] G[K, J] for J < K.
 template <typename T>
-__global__ __launch_bounds__(256) void k_gj_update(T* __restrict__ A, int m, int k0, int b,
-                                                   const T* __restrict__ Vt, const T* __restrict__ Wt,
-                                                   const T* __restrict__ Pinv) {
-  constexpr int NB = kGjNB, TL = kGjTile;
-  __shared__ T Ws[NB][TL];
-  __shared__ T Vs[NB][TL];
-  const int i0 = blockIdx.y * TL, j0 = blockIdx.x * TL;
+__global__ __launch_bounds__(256) void k_ldl_update(T* __restrict__ A, T* __restrict__ G, int m, int k0, int b,
+                                                    int K, const T* __restrict__ Wt, const T* __restrict__ Lt) {
+  __shared__ T Ws[kNB][kNB];
+  __shared__ T Ls[kNB][kNB];
+  const int r0 = k0 + b;
+  const int tI = blockIdx.y;
+  const int i0 = r0 + kNB * tI;
   const int tid = threadIdx.x;
-  for (int e = tid; e < NB * TL; e += 256) {
-    const int k = e / TL, r = e % TL;
-    const bool okk = k < b;
-    Ws[k][r] = (okk && i0 + r < m) ? Wt[(long long)k * m + i0 + r] : T(0);
-    Vs[k][r] = (okk && j0 + r < m) ? Vt[(long long)k * m + j0 + r] : T(0);
-  }
-  __syncthreads();
   const int tx = tid & 15, ty = tid >> 4;
   T acc[4][4];
+  tile_zero(acc);
+  if ((int)blockIdx.x <= K) {  // G tile (I, J)
+    const int J = blockIdx.x;
+    const int j0 = J * kNB;
+    for (int e = tid; e < kNB * kNB; e += 256) {
+      const int k = e / kNB, q = e % kNB;
+      Ws[k][q] = (k < b && i0 + q < m) ? Lt[(long long)k * m + i0 + q] : T(0);
+      Ls[k][q] = k < b ? G[(long long)(k0 + k) * m + j0 + q] : T(0);
+    }
+    __syncthreads();
+    tile_acc(acc, Ws, Ls, b);
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 4; ++a) {
+      const int i = i0 + ty + 16 * a;
+      if (i >= m) continue;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) acc[a][c] = 0;
-  for (int k = 0; k < b; ++k) {
-    T wv[4], vv[4];
+      for (int c = 0; c < 4; ++c) {
+        T* dst = G + (long long)i * m + j0 + tx + 16 * c;
+        *dst = (J == K) ? -acc[a][c] : *dst - acc[a][c];
+      }
+    }
+    return;
+  }
+  const int tJ = (int)blockIdx.x - (K + 1);
+  if (tJ > tI) return;
+  const int j0 = r0 + kNB * tJ;
+  for (int e = tid; e < kNB * kNB; e += 256) {
+    const int k = e / kNB, q = e % kNB;
+    Ws[k][q] = (k < b && i0 + q < m) ? Wt[(long long)k * m + i0 + q] : T(0);
+    Ls[k][q] = (k < b && j0 + q < m) ? Lt[(long long)k * m + j0 + q] : T(0);
+  }
+  __syncthreads();
+  tile_acc(acc, Ws, Ls, b);
 #pragma unroll
-    for (int a = 0; a < 4; ++a) wv[a] = Ws[k][ty + 16 * a];
+  for (int a = 0; a < 4; ++a) {
+    const int i = i0 + ty + 16 * a;
+    if (i >= m) continue;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) vv[c] = Vs[k][tx + 16 * c];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) acc[a][c] += wv[a] * vv[c];
+    for (int c = 0; c < 4; ++c) {
+      const int j = j0 + tx + 16 * c;
+      if (j <= i && j < m) A[(long long)i * m + j] -= acc[a][c];
+    }
+  }
+}
+
+// (4) A[I, J] = A[J, I] = -sum_{K >= I} G[K, I]' diag(dinv_K) G[K, J] for the
+// lower tiles I >= J: the full symmetric -N^{-1}.
+template <typename T>
+__global__ __launch_bounds__(256) void k_inv_syrk(const T* __restrict__ G, const T* __restrict__ dinv, int m,
+                                                  int nb, T* __restrict__ A) {
+  const int I = blockIdx.y, J = blockIdx.x;
+  if (J > I) return;
+  __shared__ T As[kNB][kNB];
+  __shared__ T Bs[kNB][kNB];
+  const int i0 = I * kNB, j0 = J * kNB;
+  const int tid = threadIdx.x;
+  const int tx = tid & 15, ty = tid >> 4;
+  T acc[4][4];
+  tile_zero(acc);
+  for (int K = I; K < nb; ++K) {
+    const int kb = K * kNB;
+    const int kw = min(kNB, m - kb);
+    __syncthreads();
+    for (int e = tid; e < kNB * kNB; e += 256) {
+      const int k = e / kNB, q = e % kNB;
+      const bool ok = k < kw;
+      As[k][q] = (ok && i0 + q < m) ? G[(long long)(kb + k) * m + i0 + q] : T(0);
+      Bs[k][q] = (ok && j0 + q < m) ? dinv[kb + k] * G[(long long)(kb + k) * m + j0 + q] : T(0);
+    }
+    __syncthreads();
+    tile_acc(acc, As, Bs, kw);
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     const int i = i0 + ty + 16 * a;
     if (i >= m) continue;
-    const bool ik = (i >= k0 && i < k0 + b);
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       const int j = j0 + tx + 16 * c;
-      if (j >= m) continue;
-      const bool jk = (j >= k0 && j < k0 + b);
-      T* dst = A + (long long)i * m + j;
-      if (!ik && !jk)
-        *dst -= acc[a][c];
-      else if (!ik)
-        *dst = Wt[(long long)(j - k0) * m + i];
-      else if (!jk)
-        *dst = Wt[(long long)(i - k0) * m + j];
-      else
-        *dst = -Pinv[(i - k0) * NB + (j - k0)];
+      // on a diagonal tile (i, j) and (j, i) are formed by two threads with
+      // different roundings: only the lower one is written, and mirrored
+      if (j >= m || j > i) continue;
+      A[(long long)i * m + j] = -acc[a][c];
+      A[(long long)j * m + i] = -acc[a][c];
     }
   }
 }
 
 template <typename T>
-static int dense_invert_typed(DenseNE& d, int m, double tol, hipStream_t s) {
+static int dense_invert_typed(DenseNE& d, int m, double tol, double skip_rel, hipStream_t s) {
   T* A = d.A.as<T>();
-  T* Pinv = d.pinv.as<T>();
-  T* Vt = d.vt.as<T>();
+  T* G = d.G.as<T>();
+  T* dinv = d.dinv.as<T>();
   T* Wt = d.wt.as<T>();
-  const dim3 g2((m + kGjTile - 1) / kGjTile, (m + kGjTile - 1) / kGjTile);
-  for (int k0 = 0; k0 < m; k0 += kGjNB) {
-    const int b = std::min(kGjNB, m - k0);
-    hipLaunchKernelGGL((k_gj_pivot<T>), dim3(1), dim3(256), 0, s, A, m, k0, b, Pinv, tol, d.clamped.as<int>());
-    hipLaunchKernelGGL((k_gj_panel<T>), dim3((m + 63) / 64), dim3(256), 0, s, A, m, k0, b, Pinv, Vt, Wt);
-    hipLaunchKernelGGL((k_gj_update<T>), g2, dim3(256), 0, s, A, m, k0, b, Vt, Wt, Pinv);
+  T* Lt = d.lt.as<T>();
+  const int nb = (m + kNB - 1) / kNB;
+  for (int K = 0; K < nb; ++K) {
+    const int k0 = K * kNB;
+    const int b = std::min(kNB, m - k0);
+    hipLaunchKernelGGL((k_ldl_diag<T>), dim3(1), dim3(256), 0, s, A, m, k0, b, dinv, G, tol, d.diag0.as<double>(),
+                       skip_rel, d.clamped.as<int>());
+    const int rest = m - k0 - b;
+    const int nt = (rest + kNB - 1) / kNB;
+    if (nt + K > 0)
+      hipLaunchKernelGGL((k_ldl_panel<T>), dim3(nt + K), dim3(256), 0, s, A, m, k0, b, nt, dinv, G, Wt, Lt);
+    if (nt > 0)
+      hipLaunchKernelGGL((k_ldl_update<T>), dim3(K + 1 + nt, nt), dim3(256), 0, s, A, G, m, k0, b, K, Wt, Lt);
   }
-  count_launch(3 * ((m + kGjNB - 1) / kGjNB));
+  hipLaunchKernelGGL((k_inv_syrk<T>), dim3(nb, nb), dim3(256), 0, s, G, dinv, m, nb, A);
+  count_launch(3 * nb + 1);
   return check_launch("dense invert");
 }
 
 int dense_reserve(DenseNE& d, int m, bool f32) {
   const size_t es = f32 ? sizeof(float) : sizeof(double);
   HN_TRY(d.A.reserve(es * (size_t)m * m));
-  HN_TRY(d.pinv.reserve(es * kGjNB * kGjNB));
-  HN_TRY(d.vt.reserve(es * (size_t)kGjNB * m));
-  HN_TRY(d.wt.reserve(es * (size_t)kGjNB * m));
+  HN_TRY(d.G.reserve(es * (size_t)m * m));
+  HN_TRY(d.dinv.reserve(es * (size_t)m));
+  HN_TRY(d.wt.reserve(es * (size_t)kNB * m));
+  HN_TRY(d.lt.reserve(es * (size_t)kNB * m));
   HN_TRY(d.clamped.reserve(sizeof(int)));
+  HN_TRY(d.diag0.reserve(sizeof(double) * (size_t)m));
   d.m = m;
   return HN_OK;
 }
 
 // In-place inverse of the dense N written by ne_numeric: afterwards d.A
 // holds -N^{-1}.
-int dense_factor(DenseNE& d, hipStream_t s) {
+int dense_factor(DenseNE& d, int pivot_rule, hipStream_t s) {
   const double tol = 1e-10;  // singular_tol of ToleranceConfig::industrial_defaults
   HN_HIP(hipMemsetAsync(d.clamped.p, 0, sizeof(int), s));
-  return d.f32 ? dense_invert_typed<float>(d, d.m, tol, s) : dense_invert_typed<double>(d, d.m, tol, s);
+  const bool skip = pivot_rule == HN_PIVOT_SKIP;
+  return d.f32 ? dense_invert_typed<float>(d, d.m, tol, skip ? kSkipRel32 : 0.0, s)
+               : dense_invert_typed<double>(d, d.m, tol, skip ? kSkipRel64 : 0.0, s);
 }
 
 // y = -A r  (A holds -N^{-1}); one wave per row, FP64 accumulation.

@@ -33,9 +33,10 @@ struct Trip {
 // sparse_matrix.cpp:8-53,230-262).
 void csr_from_triplets(int64_t rows, std::vector<Trip>& t, std::vector<int64_t>& rp,
                        std::vector<int64_t>& ci, std::vector<double>& v) {
-  std::stable_sort(t.begin(), t.end(), [](const Trip& a, const Trip& b) {
-    return a.r != b.r ? a.r < b.r : a.c < b.c;
-  });
+  const auto less = [](const Trip& a, const Trip& b) { return a.r != b.r ? a.r < b.r : a.c < b.c; };
+  // triplets usually arrive in order (rows in turn, columns increasing);
+  // the stable sort of an ordered sequence is the sequence itself
+  if (!std::is_sorted(t.begin(), t.end(), less)) std::stable_sort(t.begin(), t.end(), less);
   rp.assign(rows + 1, 0);
   ci.clear();
   v.clear();
@@ -247,4 +248,58 @@ void to_original_primal(const StandardForm& f, const double* x, double* primal) 
   }
 }
 
+// The warm-start acceptance test (no counterpart in the reference, whose
+// solver always starts at interior_point.cpp:443-453): finite, strictly
+// positive x0 and s0; mu0 below the standard start's mu; every product
+// x0_j s0_j at least 1e-3 mu0; ||M x0 - b||_inf no larger than at the
+// standard start.
+bool warm_start_acceptable(const StandardForm& f, const double* x0, const double* y0, const double* s0) {
+  if (!x0 || !y0 || !s0) return false;
+  const int64_t m = f.m, n = f.n;
+  if (n == 0) return false;
+  double bn = 1.0, cn = 1.0;
+  for (double v : f.rhs) bn = std::max(bn, std::abs(v));
+  for (double v : f.c) cn = std::max(cn, std::abs(v));
+  double mu_cold = 0.0, mu0 = 0.0, pmin = HUGE_VAL;
+  for (int64_t j = 0; j < n; ++j) {
+    if (!(x0[j] > 0.0) || !(s0[j] > 0.0) || !std::isfinite(x0[j]) || !std::isfinite(s0[j])) return false;
+    mu_cold += (bn + 1.0) * std::max(cn + 1.0, std::abs(f.c[j]));
+    const double p = x0[j] * s0[j];
+    mu0 += p;
+    pmin = std::min(pmin, p);
+  }
+  for (int64_t i = 0; i < m; ++i)
+    if (!std::isfinite(y0[i])) return false;
+  mu_cold /= (double)n;
+  mu0 /= (double)n;
+  if (!(mu0 < mu_cold) || !(pmin >= 1e-3 * mu0)) return false;
+  double rw = 0.0, rc = 0.0;
+  for (int64_t i = 0; i < m; ++i) {
+    double sw = 0.0, sc = 0.0;
+    for (int64_t k = f.rp[i]; k < f.rp[i + 1]; ++k) {
+      sw += f.v[k] * x0[f.ci[k]];
+      sc += f.v[k] * (bn + 1.0);
+    }
+    rw = std::max(rw, std::abs(sw - f.rhs[i]));
+    rc = std::max(rc, std::abs(sc - f.rhs[i]));
+  }
+  return rw <= rc;
+}
+
 }  // namespace hn
+
+extern "C" int hn_lp_standard_form(const hn_lp* lp, int64_t* sf_m, int64_t* sf_n, int64_t* nnz, int64_t* row_ptr,
+                                   int64_t* col, double* val, double* rhs, double* c) {
+  HN_CHECK(lp && sf_m && sf_n && nnz, HN_ERR_INVALID, "hn_lp_standard_form: null argument");
+  hn::StandardForm f;
+  HN_TRY(hn::build_standard_form(*lp, f));
+  *sf_m = f.m;
+  *sf_n = f.n;
+  *nnz = (int64_t)f.ci.size();
+  if (row_ptr) std::memcpy(row_ptr, f.rp.data(), sizeof(int64_t) * f.rp.size());
+  if (col) std::memcpy(col, f.ci.data(), sizeof(int64_t) * f.ci.size());
+  if (val) std::memcpy(val, f.v.data(), sizeof(double) * f.v.size());
+  if (rhs) std::memcpy(rhs, f.rhs.data(), sizeof(double) * f.rhs.size());
+  if (c) std::memcpy(c, f.c.data(), sizeof(double) * f.c.size());
+  return HN_OK;
+}

@@ -39,4 +39,9 @@ int build_standard_form(const hn_lp& lp, StandardForm& f);
 // interior_point.cpp:375-395
 void to_original_primal(const StandardForm& f, const double* x, double* primal);
 
+// True when (x0, y0, s0) may replace the standard start point: finite,
+// x0 > 0, s0 > 0, mu0 below the standard start's mu, min_j x0_j s0_j >=
+// 1e-3 mu0, and ||M x0 - b||_inf no larger than at the standard start.
+bool warm_start_acceptable(const StandardForm& f, const double* x0, const double* y0, const double* s0);
+
 }  // namespace hn

@@ -432,6 +432,7 @@ def gpu_visible() -> bool:
 # ---------------------------------------------------------------------------
 HN_NE_AUTO, HN_NE_DENSE, HN_NE_PCG = 0, 1, 2
 HN_PREC_F64, HN_PREC_MIXED = 1, 2
+HN_PIVOT_CLAMP, HN_PIVOT_SKIP = 0, 1
 STATUS = ["UNKNOWN", "OPTIMAL", "INFEASIBLE", "UNBOUNDED", "SUBOPTIMAL", "TIME_LIMIT",
           "ITER_LIMIT", "NUMERICAL_ERROR", "INTERRUPTED"]
 
@@ -443,7 +444,7 @@ INTERRUPT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)
 class _HnLp(C.Structure):
     _fields_ = [("m", C.c_int64), ("n", C.c_int64), ("row_ptr", _i64p), ("col", _i64p), ("val", _dp),
                 ("sense", _i8p), ("rhs", _dp), ("lb", _dp), ("ub", _dp), ("obj", _dp), ("maximize", C.c_int),
-                ("obj_offset", C.c_double)]
+                ("obj_offset", C.c_double), ("x0", _dp), ("y0", _dp), ("s0", _dp)]
 
 
 class _HnOpts(C.Structure):
@@ -452,7 +453,8 @@ class _HnOpts(C.Structure):
                 ("regularization", C.c_double), ("interrupt_callback", INTERRUPT_FN),
                 ("interrupt_user", C.c_void_p), ("solver", C.c_int), ("precision", C.c_int),
                 ("mu_switch", C.c_double), ("dense_max_m", C.c_int64), ("pcg_max_iter", C.c_int),
-                ("x0", _dp), ("y0", _dp), ("s0", _dp), ("verbose", C.c_int)]
+                ("x0", _dp), ("y0", _dp), ("s0", _dp), ("verbose", C.c_int), ("pivot_rule", C.c_int),
+                ("mu_target", C.c_double)]
 
 
 class _HnResult(C.Structure):
@@ -490,6 +492,8 @@ class InteriorPointOptions:
         self.dense_max_m = 40000
         self.pcg_max_iter = 2000
         self.warm_start = None          # (x0, y0, s0) of the scaled standard form
+        self.pivot_rule = HN_PIVOT_CLAMP  # HN_PIVOT_SKIP: Wright's skipped pivots, r = regularization
+        self.mu_target = 0.0            # > 0: return the first iterate with mu <= mu_target
         self.verbose = False
         for k, v in kw.items():
             if not hasattr(self, k):
@@ -558,6 +562,8 @@ class InteriorPointSolver:
         opts.dense_max_m = o.dense_max_m
         opts.pcg_max_iter = o.pcg_max_iter
         opts.verbose = int(o.verbose)
+        opts.pivot_rule = o.pivot_rule
+        opts.mu_target = o.mu_target
         if o.warm_start is not None:
             ws = [np.ascontiguousarray(a, np.float64) for a in o.warm_start]
             keep["ws"] = ws
@@ -582,9 +588,12 @@ LIB.hn_ipm_solve_batch.restype = C.c_int
 
 
 def ipm_solve_batch(lps, precision: int = HN_PREC_F64, max_iterations: int = 100, mu_switch: float = 1e-4,
-                    want_points: bool = False):
+                    want_points: bool = False, pivot_rule: int = HN_PIVOT_CLAMP, mu_target: float = 0.0,
+                    warm=None):
     """Solves many LPs in one persistent launch (hn_ipm_solve_batch).
-    Returns one InteriorPointResult per LP (solve_time_ms = whole call)."""
+    warm: optional list (one entry per LP, or None) of (x0, y0, s0) warm
+    starts of the scaled standard forms.  Returns one InteriorPointResult per
+    LP (solve_time_ms = whole call)."""
     count = len(lps)
     arr = (_HnLp * count)()
     keep = []
@@ -597,11 +606,16 @@ def ipm_solve_batch(lps, precision: int = HN_PREC_F64, max_iterations: int = 100
         arr[k] = _HnLp(lp.m, lp.n, d["rp"].ctypes.data_as(_i64p), d["ci"].ctypes.data_as(_i64p), _dptr(d["v"]),
                        d["sense"].ctypes.data_as(_i8p), _dptr(d["rhs"]), _dptr(d["lb"]), _dptr(d["ub"]),
                        _dptr(d["obj"]), int(lp.maximize), float(getattr(lp, "obj_offset", 0.0)))
+        if warm is not None and warm[k] is not None:
+            d["warm"] = [np.ascontiguousarray(a, np.float64) for a in warm[k]]
+            arr[k].x0, arr[k].y0, arr[k].s0 = (_dptr(a) for a in d["warm"])
     opts = _HnOpts()
     check(LIB.hn_ipm_default_opts(C.byref(opts)))
     opts.precision = precision
     opts.max_iterations = max_iterations
     opts.mu_switch = mu_switch
+    opts.pivot_rule = pivot_rule
+    opts.mu_target = mu_target
     res = (_HnResult * count)()
     bufs = []
     for k, lp in enumerate(lps):
@@ -616,3 +630,36 @@ def ipm_solve_batch(lps, precision: int = HN_PREC_F64, max_iterations: int = 100
     check(LIB.hn_ipm_solve_batch(arr, count, C.byref(opts), res))
     return [InteriorPointResult(res[k], bufs[k][0][:lps[k].n].copy(), bufs[k][1], bufs[k][2], bufs[k][3])
             for k in range(count)]
+
+
+LIB.hn_lp_standard_form.argtypes = [C.POINTER(_HnLp), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_int64), _i64p, _i64p, _dp, _dp, _dp]
+LIB.hn_lp_standard_form.restype = C.c_int
+
+
+def lp_standard_form(lp) -> dict:
+    """The scaled standard form (M as CSR, rhs, c) the IPM iterates on
+    (hn_lp_standard_form); host-side, no device needed."""
+    d = dict(rp=np.ascontiguousarray(lp.row_ptr, np.int64), ci=np.ascontiguousarray(lp.col, np.int64),
+             v=np.ascontiguousarray(lp.val, np.float64), sense=np.ascontiguousarray(lp.sense, np.int8),
+             rhs=np.ascontiguousarray(lp.rhs, np.float64), lb=np.ascontiguousarray(lp.lb, np.float64),
+             ub=np.ascontiguousarray(lp.ub, np.float64), obj=np.ascontiguousarray(lp.obj, np.float64))
+    P = _HnLp(lp.m, lp.n, d["rp"].ctypes.data_as(_i64p), d["ci"].ctypes.data_as(_i64p), _dptr(d["v"]),
+              d["sense"].ctypes.data_as(_i8p), _dptr(d["rhs"]), _dptr(d["lb"]), _dptr(d["ub"]), _dptr(d["obj"]),
+              int(lp.maximize), float(getattr(lp, "obj_offset", 0.0)))
+    # one call into buffers of guaranteed capacity: every kept row gives one
+    # row and every boxed column one more; a column gives at most two slots
+    # plus a slack of its bound row, a row at most one slack
+    m0, n0, z0 = int(lp.m), int(lp.n), int(d["rp"][-1]) if lp.m else 0
+    cap_m, cap_n, cap_z = m0 + n0, 3 * n0 + m0, 2 * z0 + m0 + 2 * n0
+    rp = np.empty(cap_m + 1, np.int64)
+    ci = np.empty(max(cap_z, 1), np.int64)
+    v = np.empty(max(cap_z, 1))
+    rhs = np.empty(max(cap_m, 1))
+    c = np.empty(max(cap_n, 1))
+    sm, sn, nz = C.c_int64(), C.c_int64(), C.c_int64()
+    check(LIB.hn_lp_standard_form(C.byref(P), C.byref(sm), C.byref(sn), C.byref(nz), rp.ctypes.data_as(_i64p),
+                                  ci.ctypes.data_as(_i64p), _dptr(v), _dptr(rhs), _dptr(c)))
+    m, n, z = sm.value, sn.value, nz.value
+    return dict(m=m, n=n, rp=rp[:m + 1].copy(), ci=ci[:z].copy(), v=v[:z].copy(), rhs=rhs[:m].copy(),
+                c=c[:n].copy())

@@ -211,8 +211,17 @@ int64_t or_adat_pattern(int64_t m, int64_t n, const int64_t* rp, const int64_t* 
   return total;
 }
 
+static double adat_dense_rel(int64_t m, int64_t n, const int64_t* rp, const int64_t* ci, const double* v,
+                             const double* D, double reg, double rel_reg, double* N);
+
 double or_adat_dense(int64_t m, int64_t n, const int64_t* rp, const int64_t* ci,
                      const double* v, const double* D, double reg, double* N) {
+  return adat_dense_rel(m, n, rp, ci, v, D, reg, 1e-12, N);
+}
+
+/* rel_reg = 1e-12 is the reference's rule; 0 leaves r = reg (the skip rule). */
+static double adat_dense_rel(int64_t m, int64_t n, const int64_t* rp, const int64_t* ci, const double* v,
+                             const double* D, double reg, double rel_reg, double* N) {
   /* interior_point.cpp:144-200 */
   int64_t* crp = (int64_t*)malloc(sizeof(int64_t) * (n + 1));
   int64_t* cri = (int64_t*)malloc(sizeof(int64_t) * (rp[m] > 0 ? rp[m] : 1));
@@ -227,7 +236,7 @@ double or_adat_dense(int64_t m, int64_t n, const int64_t* rp, const int64_t* ci,
   }
   double max_diag = 0.0;
   for (int64_t i = 0; i < m; ++i) max_diag = (max_diag < N[i * m + i]) ? N[i * m + i] : max_diag;
-  double dyn = 1e-12 * max_diag;
+  double dyn = rel_reg * max_diag;
   if (dyn < reg) dyn = reg;
   for (int64_t i = 0; i < m; ++i) N[i * m + i] += dyn;
   free(crp);
@@ -241,7 +250,11 @@ double or_adat_dense(int64_t m, int64_t n, const int64_t* rp, const int64_t* ci,
  * L unit lower, D diagonal, pivots <= singular_tol (1e-10, tolerance.hpp:70)
  * replaced by singular_tol.  L is stored row-major in the lower triangle of F
  * with D on the diagonal. */
-static void ldlt_factor(int64_t m, const double* N, double* F, double singular_tol) {
+/* skip_rel > 0 replaces the clamp by the skip rule of S. Wright (SIAM J.
+ * Optim. 9, 1999): a pivot <= skip_rel * N(i,i) becomes 1e128, so its column
+ * of L and its component of every solve vanish.  This rule is not in the
+ * reference; it backs the HN_PIVOT_SKIP option of the GPU solvers. */
+static void ldlt_factor(int64_t m, const double* N, double* F, double singular_tol, double skip_rel) {
   double* w = (double*)malloc(sizeof(double) * (m > 0 ? m : 1));
   for (int64_t i = 0; i < m; ++i) {
     double* Li = F + i * m;
@@ -254,7 +267,11 @@ static void ldlt_factor(int64_t m, const double* N, double* F, double singular_t
     }
     double sum = N[i * m + i];
     for (int64_t k = 0; k < i; ++k) sum -= Li[k] * w[k];
-    if (sum <= singular_tol) sum = singular_tol;
+    if (skip_rel > 0.0) {
+      if (!(sum > skip_rel * N[i * m + i])) sum = 1e128;
+    } else if (sum <= singular_tol) {
+      sum = singular_tol;
+    }
     Li[i] = sum;
   }
   free(w);
@@ -614,6 +631,8 @@ void or_default_options(or_options* o) {
   o->complementarity_tol = 1e-8;
   o->regularization = 1e-8;
   o->interrupt = 0;
+  o->pivot_rule = 0;
+  o->x0 = o->y0 = o->s0 = NULL;
 }
 
 int or_standard_form(const or_problem* p, int64_t* sm, int64_t* sn, int64_t* snnz, int64_t* rp,
@@ -721,6 +740,11 @@ int or_ipm_solve(const or_problem* p_in, const or_options* o, or_result* r) {
       const double a = c_norm + 1.0, b = fabs(f.c[j]);
       z[j] = (a < b) ? b : a;
     }
+    if (o->x0 && o->y0 && o->s0) { /* caller's warm start of the scaled standard form */
+      memcpy(x, o->x0, sizeof(double) * n);
+      memcpy(z, o->s0, sizeof(double) * n);
+      memcpy(lam, o->y0, sizeof(double) * m);
+    }
     double* rp = (double*)calloc((size_t)m, sizeof(double));
     double* rd = (double*)calloc((size_t)n, sizeof(double));
     double* D = (double*)malloc(sizeof(double) * n);
@@ -758,9 +782,10 @@ int or_ipm_solve(const or_problem* p_in, const or_options* o, or_result* r) {
       for (int64_t j = 0; j < n; ++j) D[j] = x[j] / z[j];           /* :532-539 */
       for (int64_t j = 0; j < n; ++j)
         if (!(D[j] >= 0.0) || !isfinite(D[j]) || D[j] > 1e10) D[j] = 1e10;
-      or_adat_dense(m, n, f.M.rp, f.M.ci, f.M.v, D, o->regularization, N); /* :541 */
+      adat_dense_rel(m, n, f.M.rp, f.M.ci, f.M.v, D, o->regularization,       /* :541 */
+                     o->pivot_rule ? 0.0 : 1e-12, N);
       if (DEADLINE()) { time_up = 1; break; }
-      ldlt_factor(m, N, F, SINGULAR_TOL);                           /* :558-576 */
+      ldlt_factor(m, N, F, SINGULAR_TOL, o->pivot_rule ? 1e-13 : 0.0); /* :558-576 */
       if (DEADLINE()) { time_up = 1; break; }
       for (int64_t j = 0; j < n; ++j) tmp_n[j] = D[j] * rd[j];      /* :584 */
       or_spmv(m, f.M.rp, f.M.ci, f.M.v, tmp_n, mdrd);               /* :585-590 */

@@ -83,6 +83,12 @@ typedef struct or_options {   /* InteriorPointOptions, interior_point.hpp:16-27 
   double complementarity_tol; /* 1e-8 */
   double regularization;      /* 1e-8 */
   int interrupt;              /* nonzero: the interrupt callback returns true */
+  int pivot_rule;             /* 0: the reference (r = max(reg, 1e-12 max diag),
+                                 pivots <= 1e-10 clamped); 1: not in the reference -
+                                 r = reg, pivots <= 1e-13 N(i,i) skipped (Wright) */
+  const double* x0;           /* optional start point of the scaled standard form */
+  const double* y0;           /* (x0, s0: sf_n entries, y0: sf_m; all or none)   */
+  const double* s0;
 } or_options;
 
 typedef struct or_result {

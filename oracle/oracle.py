@@ -45,7 +45,8 @@ class _Problem(C.Structure):
 class _Options(C.Structure):
     _fields_ = [("max_iterations", C.c_int), ("time_limit_seconds", C.c_double),
                 ("convergence_tol", C.c_double), ("complementarity_tol", C.c_double),
-                ("regularization", C.c_double), ("interrupt", C.c_int)]
+                ("regularization", C.c_double), ("interrupt", C.c_int), ("pivot_rule", C.c_int),
+                ("x0", _dp), ("y0", _dp), ("s0", _dp)]
 
 
 class _Result(C.Structure):
@@ -231,14 +232,20 @@ def standard_form(lp):
 
 
 def ipm_solve(lp, max_iterations: int = 100, time_limit: float = 0.0, interrupt: bool = False,
-              trace_cap: int = 200) -> IpmResult:
-    """The restated reference IPM (crossover off)."""
+              trace_cap: int = 200, pivot_rule: int = 0, warm=None) -> IpmResult:
+    """The restated reference IPM (crossover off).  pivot_rule=1 selects the
+    skip rule that is not in the reference (r = reg, tiny pivots skipped);
+    warm = (x0, y0, s0) of the scaled standard form replaces the start point."""
     p, keep = _problem(lp)
     o = _Options()
     OR.or_default_options(C.byref(o))
     o.max_iterations = max_iterations
     o.time_limit_seconds = time_limit
     o.interrupt = int(interrupt)
+    o.pivot_rule = int(pivot_rule)
+    if warm is not None:
+        keep["warm"] = [_f64(a) for a in warm]
+        o.x0, o.y0, o.s0 = (_p(a, _dp) for a in keep["warm"])
     cap_m = lp.m + lp.n + 8
     cap_n = 2 * lp.n + lp.m + 8
     primal = np.zeros(max(lp.n, 1))

@@ -62,3 +62,19 @@ def test_batch_repeated_calls_and_sizes(hip):
     c = hip.ipm_solve_batch([lpgen.batch_lp(i) for i in range(3)])
     assert [r.objective_value for r in a] == [r.objective_value for r in c]
     assert b[0].status == "OPTIMAL"
+
+
+@pytest.mark.parametrize("precision", [1, 2])
+def test_batch_skip_rule_matches_oracle(hip, precision):
+    # HN_PIVOT_SKIP (not in the reference): every LP meets the 1e-8 test
+    lps = [lpgen.batch_lp(i) for i in range(60)]
+    lps = [lp for lp in lps if lp.m <= 400][:12] + [lpgen.batch_lp(3), lpgen.transport_lp(1, 40, 60)]
+    print(f"[case] batch skip rule of {len(lps)} precision={precision}", flush=True)
+    out = hip.ipm_solve_batch(lps, precision=precision, pivot_rule=hip.HN_PIVOT_SKIP)
+    for k, (lp, r) in enumerate(zip(lps, out)):
+        ref = O.ipm_solve(lp, pivot_rule=1)
+        assert r.status == ref.status == "OPTIMAL", (k, r.status, ref.status)
+        assert r.iterations < 100, (k, r.iterations)
+        assert abs(r.iterations - ref.iterations) <= max(2, ref.iterations // 10), (k, r.iterations, ref.iterations)
+        assert rel(r.objective_value, ref.objective) <= 1e-8, (k, r.objective_value, ref.objective)
+        assert r.rel_primal <= 1e-8 and r.rel_dual <= 1e-8

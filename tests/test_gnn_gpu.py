@@ -1,0 +1,77 @@
+"""Warm-start network kernels (hn_gnn_*) against the CPU oracle
+(oracle/gnn_oracle.py, which emulates the same BF16 rounding points).
+
+Tolerances: the oracle sums in a different order than the device, so a
+value feeding a BF16 rounding can land on the other side of a rounding
+boundary; results agree to about 1e-2 relative in norm, not bit for bit."""
+import numpy as np
+import pytest
+
+import gnn_oracle as G
+import hypernova_hip as HH
+import lpgen
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ws(hip):
+    import warmstart
+    return warmstart
+
+
+def _case(ws, seed=0, S=12, T=18):
+    lp = lpgen.transport_lp(seed, S, T)
+    sf = HH.lp_standard_form(lp)
+    g = ws.Graph(sf["m"], sf["n"], sf["rp"], sf["ci"])
+    og = G.Graph(sf["m"], sf["n"], sf["rp"], sf["ci"])
+    dev = {k: HH.DeviceArray.from_numpy(sf[k]) for k in ("v", "rhs", "c")}
+    return lp, sf, g, og, dev
+
+
+def relnorm(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_gnn_init_matches_oracle(ws):
+    net = ws.Network(seed=11)
+    assert net.nparams == G.NPARAMS
+    assert np.array_equal(net.get_params(), G.init_params(11))
+
+
+def test_gnn_forward_matches_oracle(ws):
+    net = ws.Network(seed=3)
+    lp, sf, g, og, dev = _case(ws)
+    print(f"[case] gnn forward transport 12x18 m={sf['m']} n={sf['n']}", flush=True)
+    lx, ls, ly = net.forward(g, dev["v"], dev["rhs"], dev["c"])
+    act = G.forward(net.get_params(), og, sf["v"], sf["rhs"], sf["c"])
+    assert relnorm(lx, act["lx"]) < 1e-2 and relnorm(ls, act["ls"]) < 1e-2 and relnorm(ly, act["ly"]) < 1e-2
+
+
+def test_gnn_gradients_match_oracle(ws):
+    net = ws.Network(seed=5)
+    lp, sf, g, og, dev = _case(ws, seed=1)
+    rng = np.random.default_rng(0)
+    tx = rng.normal(size=sf["n"]).astype(np.float32)
+    ts = rng.normal(size=sf["n"]).astype(np.float32)
+    ty = rng.normal(size=sf["m"]).astype(np.float32)
+    print(f"[case] gnn gradients transport 12x18", flush=True)
+    loss, grad = net.gradients(g, dev["v"], dev["rhs"], dev["c"], HH.DeviceArray.from_numpy(tx),
+                               HH.DeviceArray.from_numpy(ts), HH.DeviceArray.from_numpy(ty))
+    ref_loss, ref = G.loss_and_grad(net.get_params(), og, sf["v"], sf["rhs"], sf["c"], tx, ts, ty)
+    assert abs(loss - ref_loss) <= 1e-3 * abs(ref_loss)
+    for name, (o, nn) in G.LAYOUT.items():
+        if name == "total":
+            continue
+        assert relnorm(grad[o:o + nn], ref[o:o + nn]) < 3e-2, name
+
+
+def test_gnn_training_lowers_loss(ws):
+    # eight instances of a small family, fitted to their own IPM targets
+    def make(s):
+        return lpgen.transport_lp(s, 12, 18)
+    ds = ws.make_dataset(make, range(8), chunk=8, log=lambda *a: None)
+    net = ws.Network(seed=1)
+    hist = ws.train(net, ds, epochs=30, lr=3e-3, lr_final=1e-3, log=lambda *a: None)
+    print(f"  losses {hist[0]:.4f} -> {hist[-1]:.4f}", flush=True)
+    assert np.all(np.isfinite(hist)) and hist[-1] < 0.5 * hist[0]

@@ -208,3 +208,53 @@ def test_standard_form_invariants():
     M = dense(sf["rp"], sf["ci"], sf["v"], sf["n"])
     assert np.allclose(np.max(np.abs(M), axis=0)[np.any(M != 0, axis=0)], 1.0)   # column max 1 after scaling
     assert abs(sf["offset"] - (1.0 * 1.0 + (-1.0) * 0.0 + 2.0 * 3.0 + (-1.0) * 0.0)) < 1e-15
+
+
+def _highs(lp):
+    """Optimum of an LP (LE/GE/EQ rows, bounds) by scipy's HiGHS."""
+    scipy_opt = pytest.importorskip("scipy.optimize")
+    sparse = pytest.importorskip("scipy.sparse")
+    A = sparse.csr_matrix((lp.val, lp.col, lp.row_ptr), shape=(lp.m, lp.n))
+    sense = np.asarray(lp.sense)
+    le, ge, eq = sense == 0, sense == 1, sense == 2
+    A_ub = sparse.vstack([A[le], -A[ge]])
+    b_ub = np.concatenate([lp.rhs[le], -lp.rhs[ge]])
+    bounds = list(zip(np.where(np.isfinite(lp.lb), lp.lb, None), np.where(np.isfinite(lp.ub), lp.ub, None)))
+    res = scipy_opt.linprog(-lp.obj if lp.maximize else lp.obj, A_ub=A_ub if A_ub.shape[0] else None,
+                            b_ub=b_ub if A_ub.shape[0] else None, A_eq=A[eq] if eq.any() else None,
+                            b_eq=lp.rhs[eq] if eq.any() else None, bounds=bounds, method="highs",
+                            options={"primal_feasibility_tolerance": 1e-10, "dual_feasibility_tolerance": 1e-10})
+    assert res.status == 0
+    return (-res.fun if lp.maximize else res.fun) + lp.obj_offset
+
+
+@pytest.mark.parametrize("name", ["transport_40x60_s0", "transport_40x60_s2", "batch_1", "batch_4", "random_80x200_s2"])
+def test_skip_rule_objective_against_highs(name):
+    # pivot_rule=1 is not in the reference: r = reg and tiny pivots skipped
+    # (S. Wright, SIAM J. Optim. 9, 1999); it must reach the same optimum
+    kind, rest = name.split("_", 1)
+    if kind == "transport":
+        lp = lpgen.transport_lp(int(rest.rsplit("s", 1)[1]), 40, 60)
+    elif kind == "batch":
+        lp = lpgen.batch_lp(int(rest))
+    else:
+        lp = lpgen.random_lp(2, 80, 200)
+    r = O.ipm_solve(lp, pivot_rule=1)
+    assert r.status == "OPTIMAL" and r.iterations < 100
+    assert r.rp_norm < 1e-8 and r.rd_norm < 1e-8 and r.mu < 1e-8
+    # the 1e-8 stop is absolute on mu, so the objective is exact to ~n*mu
+    opt = _highs(lp)
+    assert abs(r.objective - opt) <= max(1e-7 * abs(opt), 2.0 * r.mu * r.x.size)
+
+
+def test_skip_rule_clears_the_reference_stall():
+    # 200 x 300 transportation LP: under the reference's rule the primal
+    # residual stalls near 5e-7 and the loop runs to its 100-iteration limit
+    # (relaxed exit); the skip rule meets the absolute 1e-8 test
+    lp = lpgen.transport_lp(0)
+    ref_rule = O.ipm_solve(lp)
+    skip = O.ipm_solve(lp, pivot_rule=1)
+    assert ref_rule.iterations == 100 and ref_rule.rp_norm > 1e-8
+    assert skip.status == "OPTIMAL" and skip.iterations < 40 and skip.rp_norm < 1e-8
+    opt = _highs(lp)
+    assert abs(skip.objective - opt) <= 2.0 * skip.mu * skip.x.size
