@@ -13,10 +13,12 @@ Printed: one JSON line with the metric, the roofline of the dominant kernel
 
 The LP legs ("lp" object, skipped by --no-lp) report LP solves/s:
   * batch: 4,096 LPs (instance seeds 1000..5095, m log-uniform in [50, 2000],
-    n = 2.5 m, 5 nonzeros per column) solved by one persistent launch, against
-    the plain FP64 single-LP GPU solver run sequentially on a strided sample of
-    the same instances (instance sizes are i.i.d., so the sample is unbiased);
-  * single: the 2,000 x 5,000 random LP (seed 1), plain FP64 vs mixed;
+    n = 2.5 m, 5 nonzeros per column) solved by one persistent launch, under
+    the reference's pivot rule and under the skip rule, against the plain
+    FP64 single-LP GPU solver (reference rule) run sequentially on a strided
+    sample of the same instances (sizes are i.i.d., so the sample is unbiased);
+  * single: the 2,000 x 5,000 random LP (seed 1), plain FP64 vs mixed, under
+    both pivot rules;
   * cpu_baseline: the reference InteriorPointSolver (crossover off, 1 thread)
     on a prefix of the same strided sample, under a per-instance time limit.
 LP timings are host wall-clock around the whole call (standard form on the
@@ -246,21 +248,26 @@ def run_lp(args) -> dict:
     lps = [lpgen.batch_lp(i) for i in range(count)]
     log(f"generated {count} batch LPs (sum m={sum(l.m for l in lps)}) in {time.time() - t0:.1f}s")
     H.ipm_solve_batch([l for l in lps if l.m < 200][:64])  # warm-up: code objects
-    t0 = time.perf_counter()
-    res = H.ipm_solve_batch(lps, precision=H.HN_PREC_F64)
-    wall = time.perf_counter() - t0
-    status = {}
-    for r in res:
-        status[r.status] = status.get(r.status, 0) + 1
-    its = [r.iterations for r in res]
-    batch = {"workload": f"batch_{count}_lps_m50-2000_n2.5m_5nnz_per_col", "lps": count,
-             "wall_s": round(wall, 3), "lps_per_s": round(count / wall, 2), "status": status,
+    batch = {"workload": f"batch_{count}_lps_m50-2000_n2.5m_5nnz_per_col", "lps": count}
+    runs = {}
+    for rule, rname in ((H.HN_PIVOT_SKIP, "skip_rule"), (H.HN_PIVOT_CLAMP, "reference_rule")):
+        t0 = time.perf_counter()
+        res = H.ipm_solve_batch(lps, precision=H.HN_PREC_F64, pivot_rule=rule)
+        wall = time.perf_counter() - t0
+        status = {}
+        for r in res:
+            status[r.status] = status.get(r.status, 0) + 1
+        its = [r.iterations for r in res]
+        b = {"wall_s": round(wall, 3), "lps_per_s": round(count / wall, 2), "status": status,
              "iterations_mean": round(float(np.mean(its)), 2), "iterations_max": int(max(its)),
              "precision": "f64",
-             "kkt_1e-8": sum(max(r.rel_primal, r.rel_dual, r.rel_gap) <= 1e-8 for r in res),
+             "stop_test_1e-8": sum(r.rp_norm < 1e-8 and r.rd_norm < 1e-8 and r.mu < 1e-8 for r in res),
              "at_iteration_limit": sum(r.iterations >= 100 for r in res)}
-    batch.update(_kkt_max(res))
-    log(f"batch: {count} LPs in {wall:.2f}s -> {count / wall:.1f} LPs/s {status}")
+        b.update(_kkt_max(res))
+        batch[rname] = b
+        runs[rname] = res
+        log(f"batch {rname}: {count} LPs in {wall:.2f}s -> {count / wall:.1f} LPs/s {status}")
+    res = runs["reference_rule"]
     # plain FP64 single-LP GPU solver, sequential, on a strided sample
     stride = max(1, count // args.seq_sample)
     idx = list(range(0, count, stride))[:args.seq_sample]
@@ -275,7 +282,8 @@ def run_lp(args) -> dict:
     batch["sequential_plain"] = {"lps": len(idx), "stride": stride, "wall_s": round(seq_wall, 3),
                                  "lps_per_s": round(seq_rate, 2), "status": seq_status,
                                  "same_status_as_batch": sum(r.status == res[i].status for r, i in zip(seq, idx))}
-    batch["speedup_vs_sequential_plain"] = round(count / wall / seq_rate, 2)
+    batch["speedup_vs_sequential_plain"] = {
+        k: round(batch[k]["lps_per_s"] / seq_rate, 2) for k in ("skip_rule", "reference_rule")}
     log(f"sequential plain: {len(idx)} LPs in {seq_wall:.2f}s -> {seq_rate:.2f} LPs/s")
     out["batch"] = batch
 
@@ -283,9 +291,11 @@ def run_lp(args) -> dict:
     lp = lpgen.random_lp(1, 2000, 5000)
     H.ipm_solve(lp)
     single = {"workload": "random_lp_2000x5000_5nnz_per_col_seed1"}
-    for name, prec in (("plain_f64", H.HN_PREC_F64), ("mixed", H.HN_PREC_MIXED)):
+    for name, prec, rule in (("plain_f64", H.HN_PREC_F64, H.HN_PIVOT_CLAMP), ("mixed", H.HN_PREC_MIXED, H.HN_PIVOT_CLAMP),
+                             ("skip_rule_f64", H.HN_PREC_F64, H.HN_PIVOT_SKIP),
+                             ("skip_rule_mixed", H.HN_PREC_MIXED, H.HN_PIVOT_SKIP)):
         t0 = time.perf_counter()
-        r = H.ipm_solve(lp, precision=prec)
+        r = H.ipm_solve(lp, precision=prec, pivot_rule=rule)
         dt = time.perf_counter() - t0
         single[name] = {"ms": round(1e3 * dt, 2), "status": r.status, "iterations": r.iterations,
                         "objective": r.objective_value, "rel_primal": r.rel_primal, "rel_dual": r.rel_dual,

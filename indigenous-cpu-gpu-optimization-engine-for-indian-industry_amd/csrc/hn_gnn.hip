@@ -97,16 +97,31 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
 }
 
 // ---- features -----------------------------------------------------------------
-// S[0] = mean |c| (1 if 0), S[1] = mean |b| (1 if 0)
-__global__ __launch_bounds__(256) void k_gnn_scales(const double* __restrict__ c, int n, const double* __restrict__ b,
-                                                    int m, float* __restrict__ S) {
+// S[0] = mean |c| (1 if 0), S[1] = mean |b| (1 if 0): per-block partial
+// sums, then one block adds them in a fixed order.
+constexpr int kScaleBlocks = 64;
+__global__ __launch_bounds__(256) void k_gnn_scale_part(const double* __restrict__ c, int n,
+                                                        const double* __restrict__ b, int m,
+                                                        double* __restrict__ part) {
   __shared__ double red[4];
   double sc = 0.0, sb = 0.0;
-  for (int j = threadIdx.x; j < n; j += 256) sc += fabs(c[j]);
-  for (int i = threadIdx.x; i < m; i += 256) sb += fabs(b[i]);
+  for (int j = blockIdx.x * 256 + threadIdx.x; j < n; j += kScaleBlocks * 256) sc += fabs(c[j]);
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < m; i += kScaleBlocks * 256) sb += fabs(b[i]);
   sc = block_sum<double, 256>(sc, red);
   sb = block_sum<double, 256>(sb, red);
   if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sc;
+    part[2 * blockIdx.x + 1] = sb;
+  }
+}
+
+__global__ void k_gnn_scale_fin(const double* __restrict__ part, int n, int m, float* __restrict__ S) {
+  if (threadIdx.x == 0) {
+    double sc = 0.0, sb = 0.0;
+    for (int q = 0; q < kScaleBlocks; ++q) {
+      sc += part[2 * q];
+      sb += part[2 * q + 1];
+    }
     const double mc = n ? sc / n : 0.0, mb = m ? sb / m : 0.0;
     S[0] = (float)(mc > 0.0 ? mc : 1.0);
     S[1] = (float)(mb > 0.0 ? mb : 1.0);
@@ -182,17 +197,15 @@ __global__ void k_gnn_feat_v(const int* __restrict__ trp, const int* __restrict_
   f[7] = 0.f;
 }
 
-// Constraint features, one thread per row.
-__global__ void k_gnn_feat_c(const int* __restrict__ rp, const int* __restrict__ ci, const int* __restrict__ trp,
-                             const double* __restrict__ b, const float* __restrict__ S, int m, float* __restrict__ F) {
+// Constraint features, one thread per row (slack: the row holds a column
+// of degree 1, a property of the pattern computed once per graph).
+__global__ void k_gnn_feat_c(const int* __restrict__ slackrow, const double* __restrict__ b,
+                             const float* __restrict__ S, int m, float* __restrict__ F) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
-  float slack = 0.f;
-  for (int k = rp[i]; k < rp[i + 1]; ++k)
-    if (trp[ci[k] + 1] - trp[ci[k]] == 1) slack = 1.f;
   float* f = F + (long long)i * 4;
   f[0] = (float)(b[i] / S[1]);
-  f[1] = slack;
+  f[1] = slackrow[i] ? 1.f : 0.f;
   f[2] = 1.f;
   f[3] = 0.f;
 }
@@ -232,6 +245,43 @@ __global__ __launch_bounds__(256) void k_gnn_spmm(const int* __restrict__ rp, co
   }
   float* o = out + (long long)r * H + f;
   *o = accumulate ? *o + acc : acc;
+}
+
+// The same product for long rows: one workgroup per row, wave w takes the
+// entries w, w + 4, ... and the four partial rows are added in a fixed order.
+__global__ __launch_bounds__(256) void k_gnn_spmm_long(const int* __restrict__ rp, const int* __restrict__ col,
+                                                       const int* __restrict__ src, const double* __restrict__ val,
+                                                       const int* __restrict__ orp, int self_deg,
+                                                       const float* __restrict__ in, float* __restrict__ out,
+                                                       int accumulate) {
+  __shared__ float part[4][H];
+  const int r = blockIdx.x;
+  const int w = threadIdx.x / kWave, f = threadIdx.x & (kWave - 1);
+  const int k0 = rp[r], k1 = rp[r + 1];
+  const float inv_self = k1 > k0 ? 1.f / (float)(k1 - k0) : 0.f;
+  float a0 = 0.f, a1 = 0.f;
+  int k = k0 + w;
+  for (; k + 4 < k1; k += 8) {
+    const int c0 = col[k], c1 = col[k + 4];
+    const float v0 = (float)val[src ? src[k] : k], v1 = (float)val[src ? src[k + 4] : k + 4];
+    const float w0 = self_deg ? v0 * inv_self : v0 / (float)(orp[c0 + 1] - orp[c0]);
+    const float w1 = self_deg ? v1 * inv_self : v1 / (float)(orp[c1 + 1] - orp[c1]);
+    a0 += w0 * in[(long long)c0 * H + f];
+    a1 += w1 * in[(long long)c1 * H + f];
+  }
+  for (; k < k1; k += 4) {
+    const int c0 = col[k];
+    const float v0 = (float)val[src ? src[k] : k];
+    const float w0 = self_deg ? v0 * inv_self : v0 / (float)(orp[c0 + 1] - orp[c0]);
+    a0 += w0 * in[(long long)c0 * H + f];
+  }
+  part[w][f] = a0 + a1;
+  __syncthreads();
+  if (w == 0) {
+    const float acc = ((part[0][f] + part[1][f]) + part[2][f]) + part[3][f];
+    float* o = out + (long long)r * H + f;
+    *o = accumulate ? *o + acc : acc;
+  }
 }
 
 // ---- dense layers on MFMA -------------------------------------------------------
@@ -329,53 +379,79 @@ __global__ __launch_bounds__(256) void k_gnn_gemm_dx(const float* __restrict__ d
 }
 
 // Partial weight gradient of kChunk rows: part[blk] = ([X1, X2]' dZ (128 x 64),
-// column sums of dZ (64)), dZ = dY * [Y > 0].
+// column sums of dZ (64)), dZ = dY * [Y > 0].  Sub-chunks of 64 rows are
+// staged in LDS (coalesced loads, BF16, transposed so that each MFMA operand
+// is one 8-byte read); the bias sums use the FP32 dZ.
 __global__ __launch_bounds__(256) void k_gnn_gemm_dw(const float* __restrict__ X1, const float* __restrict__ X2,
                                                      const float* __restrict__ dY, const float* __restrict__ Y,
                                                      int rows, float* __restrict__ part) {
-  const int wave = threadIdx.x / kWave, l = threadIdx.x & (kWave - 1);
+  __shared__ short XT[2 * H][64 + 4];  // XT[c][r] = bf16(X[r][c])
+  __shared__ short ZT[H][64 + 4];      // ZT[j][r] = bf16(dZ[r][j])
+  __shared__ float Zf[64][H + 1];      // FP32 dZ for the bias
+  const int tid = threadIdx.x;
+  const int wave = tid / kWave, l = tid & (kWave - 1);
   const int li = l & 15, kq = l >> 4;
-  const int r0 = blockIdx.x * kChunk;
   f32x4 acc[2][4];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int jb = 0; jb < 4; ++jb) acc[a][jb] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int ks = 0; ks < kChunk / 16; ++ks) {
-    bf16x4 av[2], bv[4];
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      const int cc = (2 * wave + a) * 16 + li;  // input feature 0..127
-      const float* X = cc < H ? X1 : X2;
-      const int cx = cc & (H - 1);
-      short t4[4];
+  float bsum = 0.f;
+  for (int sub = 0; sub < kChunk / 64; ++sub) {
+    const int r0 = blockIdx.x * kChunk + sub * 64;
+    if (r0 >= rows) break;
+    __syncthreads();
+    // X: 64 rows x 128 features, float4 per thread-step
+    for (int e = tid; e < 64 * (2 * H / 4); e += 256) {
+      const int r = e / (2 * H / 4), c4 = (e % (2 * H / 4)) * 4;
+      const int row = r0 + r;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (row < rows) v = *reinterpret_cast<const float4*>((c4 < H ? X1 : X2) + (long long)row * H + (c4 & (H - 1)));
+      XT[c4][r] = f2bf(v.x);
+      XT[c4 + 1][r] = f2bf(v.y);
+      XT[c4 + 2][r] = f2bf(v.z);
+      XT[c4 + 3][r] = f2bf(v.w);
+    }
+    for (int e = tid; e < 64 * (H / 4); e += 256) {
+      const int r = e / (H / 4), j4 = (e % (H / 4)) * 4;
+      const int row = r0 + r;
+      float g[4] = {0.f, 0.f, 0.f, 0.f};
+      if (row < rows) {
+        const float4 gv = *reinterpret_cast<const float4*>(dY + (long long)row * H + j4);
+        const float4 yv = *reinterpret_cast<const float4*>(Y + (long long)row * H + j4);
+        g[0] = yv.x > 0.f ? gv.x : 0.f;
+        g[1] = yv.y > 0.f ? gv.y : 0.f;
+        g[2] = yv.z > 0.f ? gv.z : 0.f;
+        g[3] = yv.w > 0.f ? gv.w : 0.f;
+      }
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        const int row = r0 + ks * 16 + 4 * kq + t;
-        t4[t] = row < rows ? f2bf(X[(long long)row * H + cx]) : (short)0;
+        ZT[j4 + t][r] = f2bf(g[t]);
+        Zf[r][j4 + t] = g[t];
       }
-      av[a] = bf16x4{t4[0], t4[1], t4[2], t4[3]};
     }
+    __syncthreads();
 #pragma unroll
-    for (int jb = 0; jb < 4; ++jb) {
-      const int j = jb * 16 + li;
-      short t4[4];
+    for (int ks = 0; ks < 4; ++ks) {
+      const int rr = ks * 16 + 4 * kq;
+      bf16x4 av[2], bv[4];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int row = r0 + ks * 16 + 4 * kq + t;
-        float g = 0.f;
-        if (row < rows) {
-          const long long o = (long long)row * H + j;
-          g = Y[o] > 0.f ? dY[o] : 0.f;
-        }
-        t4[t] = f2bf(g);
+      for (int a = 0; a < 2; ++a) {
+        const short* p = &XT[(2 * wave + a) * 16 + li][rr];
+        av[a] = bf16x4{p[0], p[1], p[2], p[3]};
       }
-      bv[jb] = bf16x4{t4[0], t4[1], t4[2], t4[3]};
+#pragma unroll
+      for (int jb = 0; jb < 4; ++jb) {
+        const short* p = &ZT[jb * 16 + li][rr];
+        bv[jb] = bf16x4{p[0], p[1], p[2], p[3]};
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb) acc[a][jb] = mfma16(av[a], bv[jb], acc[a][jb]);
     }
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int jb = 0; jb < 4; ++jb) acc[a][jb] = mfma16(av[a], bv[jb], acc[a][jb]);
+    if (tid < H)
+      for (int r = 0; r < 64; ++r) bsum += Zf[r][tid];
   }
   float* p = part + (long long)blockIdx.x * (2 * H * H + H);
 #pragma unroll
@@ -388,18 +464,7 @@ __global__ __launch_bounds__(256) void k_gnn_gemm_dw(const float* __restrict__ X
         const int j = jb * 16 + li;                       // D column = output feature
         p[cc * H + j] = acc[a][jb][r];
       }
-  // bias gradient: column sums of dZ over the chunk (FP32)
-  if (threadIdx.x < H) {
-    const int j = threadIdx.x;
-    float s = 0.f;
-    for (int t = 0; t < kChunk; ++t) {
-      const int row = r0 + t;
-      if (row >= rows) break;
-      const long long o = (long long)row * H + j;
-      s += Y[o] > 0.f ? dY[o] : 0.f;
-    }
-    p[2 * H * H + j] = s;
-  }
+  if (tid < H) p[2 * H * H + tid] = bsum;
 }
 
 // dst[e] = sum_b part[b][e] (fixed order), e < width.
@@ -596,7 +661,7 @@ struct hn_gnn_graph {
   int m = 0, n = 0;
   long long nnz = 0;
   hn::DevBuf rp, ci, trp, tci, tsrc;
-  hn::DevBuf S, rank, gap, Fv, Fc;
+  hn::DevBuf S, spart, slackrow, rank, gap, Fv, Fc;
   hn::DevBuf hv[hn::NL + 1], hc[hn::NL + 1], av[hn::NL], ac[hn::NL];
   hn::DevBuf lx, ls, ly, dlx, dls, dly;
   hn::DevBuf dhv[2], dhc[2], dav, dac;
@@ -637,6 +702,18 @@ std::vector<float> init_params(uint64_t seed) {
   return p;
 }
 
+// Row-wise aggregation: a workgroup per row when the rows are long (mean
+// length >= 32), else a wave per row.
+void spmm(hn_gnn_graph* gr, const int* rp, const int* col, const int* src, const double* val, const int* orp,
+          int self_deg, int rows, const float* in, float* out, int accumulate, hipStream_t s) {
+  if (gr->nnz >= 32LL * rows)
+    hipLaunchKernelGGL(k_gnn_spmm_long, dim3(rows), dim3(256), 0, s, rp, col, src, val, orp, self_deg, in, out,
+                       accumulate);
+  else
+    hipLaunchKernelGGL(k_gnn_spmm, dim3(wgrid(rows)), dim3(256), 0, s, rp, col, src, val, orp, self_deg, rows, in,
+                       out, accumulate);
+}
+
 int forward(hn_gnn* g, hn_gnn_graph* gr, const double* val, const double* b, const double* c, hipStream_t s) {
   const Layout& Lo = L();
   const float* P = g->p.as<float>();
@@ -647,23 +724,23 @@ int forward(hn_gnn* g, hn_gnn_graph* gr, const double* val, const double* b, con
   const int* tci = gr->tci.as<int>();
   const int* tsrc = gr->tsrc.as<int>();
   float* S = gr->S.as<float>();
-  hipLaunchKernelGGL(k_gnn_scales, dim3(1), dim3(256), 0, s, c, n, b, m, S);
+  hipLaunchKernelGGL(k_gnn_scale_part, dim3(kScaleBlocks), dim3(256), 0, s, c, n, b, m, gr->spart.as<double>());
+  hipLaunchKernelGGL(k_gnn_scale_fin, dim3(1), dim3(64), 0, s, gr->spart.as<double>(), n, m, S);
   hipLaunchKernelGGL(k_gnn_row_rank, dim3(m), dim3(256), 0, s, rp, ci, c, S, gr->rank.as<float>(),
                      gr->gap.as<float>());
   hipLaunchKernelGGL(k_gnn_feat_v, dim3(tgrid(n)), dim3(256), 0, s, trp, tsrc, gr->rank.as<float>(),
                      gr->gap.as<float>(), c, S, n, gr->Fv.as<float>());
-  hipLaunchKernelGGL(k_gnn_feat_c, dim3(tgrid(m)), dim3(256), 0, s, rp, ci, trp, b, S, m, gr->Fc.as<float>());
+  hipLaunchKernelGGL(k_gnn_feat_c, dim3(tgrid(m)), dim3(256), 0, s, gr->slackrow.as<int>(), b, S, m,
+                     gr->Fc.as<float>());
   hipLaunchKernelGGL(k_gnn_embed, dim3(wgrid(n)), dim3(256), 0, s, gr->Fv.as<float>(), 8, FV, P + Lo.wev,
                      P + Lo.bev, n, gr->hv[0].as<float>());
   hipLaunchKernelGGL(k_gnn_embed, dim3(wgrid(m)), dim3(256), 0, s, gr->Fc.as<float>(), 4, FC, P + Lo.wec,
                      P + Lo.bec, m, gr->hc[0].as<float>());
   for (int l = 0; l < NL; ++l) {
-    hipLaunchKernelGGL(k_gnn_spmm, dim3(wgrid(m)), dim3(256), 0, s, rp, ci, (const int*)nullptr, val, trp, 1, m,
-                       gr->hv[l].as<float>(), gr->ac[l].as<float>(), 0);
+    spmm(gr, rp, ci, nullptr, val, trp, 1, m, gr->hv[l].as<float>(), gr->ac[l].as<float>(), 0, s);
     hipLaunchKernelGGL(k_gnn_gemm_fwd, dim3((m + 63) / 64), dim3(256), 0, s, gr->hc[l].as<float>(),
                        gr->ac[l].as<float>(), P + Lo.wc[l], P + Lo.bc[l], m, gr->hc[l + 1].as<float>());
-    hipLaunchKernelGGL(k_gnn_spmm, dim3(wgrid(n)), dim3(256), 0, s, trp, tci, tsrc, val, trp, 1, n,
-                       gr->hc[l + 1].as<float>(), gr->av[l].as<float>(), 0);
+    spmm(gr, trp, tci, tsrc, val, trp, 1, n, gr->hc[l + 1].as<float>(), gr->av[l].as<float>(), 0, s);
     hipLaunchKernelGGL(k_gnn_gemm_fwd, dim3((n + 63) / 64), dim3(256), 0, s, gr->hv[l].as<float>(),
                        gr->av[l].as<float>(), P + Lo.wv[l], P + Lo.bv[l], n, gr->hv[l + 1].as<float>());
   }
@@ -725,8 +802,7 @@ int backward(hn_gnn* g, hn_gnn_graph* gr, const double* val, const float* tx, co
     hipLaunchKernelGGL(k_gnn_gemm_dx, dim3((n + 63) / 64), dim3(256), 0, s, gr->dhv[cur].as<float>(),
                        gr->hv[l + 1].as<float>(), P + Lo.wv[l], n, gr->dhv[nxt].as<float>(), gr->dav.as<float>());
     // av[l] = Avc hc[l+1]  ->  dhc[l+1] += Avc' dav  (rows of M, weight M_ij / deg_j)
-    hipLaunchKernelGGL(k_gnn_spmm, dim3(wgrid(m)), dim3(256), 0, s, rp, ci, (const int*)nullptr, val, trp, 0, m,
-                       gr->dav.as<float>(), gr->dhc[cur].as<float>(), 1);
+    spmm(gr, rp, ci, nullptr, val, trp, 0, m, gr->dav.as<float>(), gr->dhc[cur].as<float>(), 1, s);
     // constraint update: hc[l+1] = relu([hc[l], ac[l]] Wc + bc)
     hipLaunchKernelGGL(k_gnn_gemm_dw, dim3(cgrid(m)), dim3(256), 0, s, gr->hc[l].as<float>(), gr->ac[l].as<float>(),
                        gr->dhc[cur].as<float>(), gr->hc[l + 1].as<float>(), m, gr->part.as<float>());
@@ -734,8 +810,7 @@ int backward(hn_gnn* g, hn_gnn_graph* gr, const double* val, const float* tx, co
     hipLaunchKernelGGL(k_gnn_gemm_dx, dim3((m + 63) / 64), dim3(256), 0, s, gr->dhc[cur].as<float>(),
                        gr->hc[l + 1].as<float>(), P + Lo.wc[l], m, gr->dhc[nxt].as<float>(), gr->dac.as<float>());
     // ac[l] = Acv hv[l]  ->  dhv[l] += Acv' dac  (rows of M', weight M_ij / deg_i)
-    hipLaunchKernelGGL(k_gnn_spmm, dim3(wgrid(n)), dim3(256), 0, s, trp, tci, tsrc, val, rp, 0, n,
-                       gr->dac.as<float>(), gr->dhv[nxt].as<float>(), 1);
+    spmm(gr, trp, tci, tsrc, val, rp, 0, n, gr->dac.as<float>(), gr->dhv[nxt].as<float>(), 1, s);
     cur = nxt;
   }
   // embeddings
@@ -861,6 +936,14 @@ extern "C" int hn_gnn_graph_create(int64_t m, int64_t n, const int64_t* row_ptr,
   if (st == HN_OK) st = up(gr->tci, tci);
   if (st == HN_OK) st = up(gr->tsrc, tsrc);
   if (st == HN_OK) st = gr->S.reserve(sizeof(float) * 4);
+  if (st == HN_OK) st = gr->spart.reserve(sizeof(double) * 2 * kScaleBlocks);
+  {
+    std::vector<int> slack(m, 0);
+    for (int64_t i = 0; i < m; ++i)
+      for (int k = rp[i]; k < rp[i + 1]; ++k)
+        if (trp[ci[k] + 1] - trp[ci[k]] == 1) slack[i] = 1;
+    if (st == HN_OK) st = up(gr->slackrow, slack);
+  }
   if (st == HN_OK) st = gr->rank.reserve(sizeof(float) * std::max<long long>(nnz, 1));
   if (st == HN_OK) st = gr->gap.reserve(sizeof(float) * std::max<long long>(nnz, 1));
   if (st == HN_OK) st = gr->Fv.reserve(sizeof(float) * (size_t)n * 8);

@@ -319,39 +319,29 @@ __device__ __forceinline__ void tile_zero(T (&acc)[4][4]) {
     for (int c = 0; c < 4; ++c) acc[a][c] = T(0);
 }
 
-// (1) LDL' of the diagonal block A[k0:k0+b, k0:k0+b] (lower triangle read,
-// symmetric copy in registers: thread (ti, tj) owns rows 4ti.., columns
-// 4tj..).  Writes the strictly lower part of L, dinv[k0:k0+b] (1/pivot; 0 for
-// a skipped pivot) and X = L_KK^{-1} as the full 64 x 64 diagonal block of
-// G = L^{-1} (zeros above).  Pivots p <= tol are clamped to tol
+// (1) LDL' of the diagonal block A[k0:k0+b, k0:k0+b] by one wave, the block
+// resident in LDS (lane j owns column j): step k scales column k below the
+// pivot (lane i scales row i) and lane j > k applies a_ij -= l_i a_kj down
+// its column.  Writes the strictly lower part of L, dinv[k0:k0+b] (1/pivot;
+// 0 for a skipped pivot) and X = L_KK^{-1} as the full 64 x 64 diagonal
+// block of G = L^{-1} (zeros above).  Pivots p <= tol are clamped to tol
 // (skip_rel == 0) or, with skip_rel > 0, p <= skip_rel * diag0 is skipped.
 template <typename T>
-__global__ __launch_bounds__(256) void k_ldl_diag(T* __restrict__ A, int m, int k0, int b, T* __restrict__ dinv,
-                                                  T* __restrict__ G, double tol, const double* __restrict__ diag0,
-                                                  double skip_rel, int* __restrict__ clamped) {
-  __shared__ T rk[2][kNB];
-  __shared__ T Ls[kNB][kNB + 1];
+__global__ __launch_bounds__(kWave) void k_ldl_diag(T* __restrict__ A, int m, int k0, int b, T* __restrict__ dinv,
+                                                    T* __restrict__ G, double tol, const double* __restrict__ diag0,
+                                                    double skip_rel, int* __restrict__ clamped) {
+  __shared__ T S[kNB][kNB + 1];
   __shared__ T Xs[kNB][kNB + 1];
-  const int tid = threadIdx.x;
-  const int ti = tid >> 4, tj = tid & 15;
-  T P[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int i = 4 * ti + a, j = 4 * tj + c;
-      const int hi = max(i, j), lo = min(i, j);
-      P[a][c] = (i < b && j < b) ? A[(long long)(k0 + hi) * m + k0 + lo] : T(i == j ? 1 : 0);
-    }
+  __shared__ T Ts[3 * 256];
+  const int j = threadIdx.x;
+  for (int i = 0; i < kNB; ++i) {
+    const int hi = i > j ? i : j, lo = i > j ? j : i;
+    S[i][j] = (i < b && j < b) ? A[(long long)(k0 + hi) * m + k0 + lo] : T(i == j ? 1 : 0);
+  }
+  __syncthreads();
   int ncl = 0;
   for (int k = 0; k < b; ++k) {
-    const int par = k & 1, kk = k & 3;
-    if (tj == (k >> 2)) {
-#pragma unroll
-      for (int a = 0; a < 4; ++a) rk[par][4 * ti + a] = P[a][kk];
-    }
-    __syncthreads();
-    T piv = rk[par][k];
+    T piv = S[k][k];
     T rinv;
     if (skip_rel > 0.0) {
       if (!((double)piv > skip_rel * diag0[k0 + k])) {
@@ -367,107 +357,115 @@ __global__ __launch_bounds__(256) void k_ldl_diag(T* __restrict__ A, int m, int 
       }
       rinv = T(1) / piv;
     }
-    if (tid == 0) dinv[k0 + k] = rinv;
-    T ci[4], cj[4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) ci[a] = rk[par][4 * ti + a];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) cj[c] = rk[par][4 * tj + c];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const int i = 4 * ti + a;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const int j = 4 * tj + c;
-        if (j == k && i > k)
-          P[a][c] = ci[a] * rinv;
-        else if (i == k && j > k)
-          P[a][c] = cj[c] * rinv;
-        else if (i > k && j > k)
-          P[a][c] = P[a][c] - ci[a] * (cj[c] * rinv);
-      }
+    if (j == 0) dinv[k0 + k] = rinv;
+    if (j > k) S[j][k] = S[j][k] * rinv;
+    __syncthreads();
+    if (j > k) {
+      const T akj = S[k][j];
+#pragma unroll 8
+      for (int i = k + 1; i < kNB; ++i) S[i][j] -= S[i][k] * akj;
     }
+    __syncthreads();
   }
+  for (int i = j + 1; i < b; ++i) A[(long long)(k0 + i) * m + k0 + j] = S[i][j];
+  if (j == 0 && ncl) atomicAdd(clamped, ncl);
+  // X = L_KK^{-1} by 16 x 16 blocks: the four diagonal blocks by forward
+  // substitution (a column per lane, in registers), then block rows
+  // I = 1..3: X_IJ = -X_II sum_{J<=K<I} L_IK X_KJ.  Only the strictly lower
+  // part of S (= L) is read; entries of X above the diagonal are zero.
+  {
+    const int o = 16 * (j >> 4), cc = j & 15;
+    T x[16];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+    for (int i = 0; i < 16; ++i) {
+      T sx = (i == cc) ? T(1) : T(0);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int i = 4 * ti + a, j = 4 * tj + c;
-      const bool low = i < b && j < i;
-      if (low) A[(long long)(k0 + i) * m + k0 + j] = P[a][c];
-      Ls[i][j] = low ? P[a][c] : T(0);
+      for (int k = 0; k < i; ++k) sx -= S[o + i][o + k] * x[k];
+      x[i] = sx;
     }
-  if (tid == 0 && ncl) atomicAdd(clamped, ncl);
-  __syncthreads();
-  if (tid < kNB) {  // column j of X: x_i = delta_ij - sum_{j<=k<i} L_ik x_k
-    const int j = tid;
-    for (int i = 0; i < kNB; ++i) {
-      T sx = (i == j) ? T(1) : T(0);
-      if (i > j)
-        for (int k = j; k < i; ++k) sx -= Ls[i][k] * Xs[k][j];
-      Xs[i][j] = sx;
-    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) Xs[o + i][o + cc] = x[i];
   }
   __syncthreads();
-  for (int e = tid; e < kNB * kNB; e += 256) {
-    const int i = e / kNB, j = e % kNB;
-    if (i < b && j < b) G[(long long)(k0 + i) * m + k0 + j] = Xs[i][j];
+  for (int I = 1; I < 4; ++I) {
+    for (int e = j; e < I * 256; e += kWave) {
+      const int J = e >> 8, r = (e >> 4) & 15, cc = e & 15;
+      T sx = T(0);
+      for (int K = J; K < I; ++K)
+#pragma unroll
+        for (int k = 0; k < 16; ++k) sx += S[16 * I + r][16 * K + k] * Xs[16 * K + k][16 * J + cc];
+      Ts[e] = sx;
+    }
+    __syncthreads();
+    for (int e = j; e < I * 256; e += kWave) {
+      const int J = e >> 8, r = (e >> 4) & 15, cc = e & 15;
+      T sx = T(0);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) sx += Xs[16 * I + r][16 * I + k] * Ts[(J << 8) + (k << 4) + cc];
+      Xs[16 * I + r][16 * J + cc] = -sx;
+    }
+    __syncthreads();
   }
+  for (int i = 0; i < b; ++i)
+    if (j < b) G[(long long)(k0 + i) * m + k0 + j] = (j <= i) ? Xs[i][j] : T(0);
 }
 
 // (2) panel, 64 rows r >= k0 + b per workgroup: W = A[r, K] X' (so that
 // W L_KK' = A[r, K]) and L[r, K] = W dinv; W and L are also written k-major
-// for the trailing update.  Each thread forms 16 outputs of one column.
-// Workgroups past the panel finish block row K of G = L^{-1}:
-// G[K, J] = X_K R[K, J] for J < K, where R[K, J] holds the accumulated
-// -sum L[K, K'] G[K', J] of the earlier steps (G[K, K] = X_K already).
+// for the trailing update.  Workgroups past the panel finish block row K of
+// G = L^{-1}: G[K, J] = X_K R[K, J] for J < K, where R[K, J] holds the
+// accumulated -sum L[K, K'] G[K', J] of the earlier steps (G[K, K] = X_K
+// already).  Both are 64 x 64 x 64 products, LDS-staged, 4 x 4 per thread.
 template <typename T>
 __global__ __launch_bounds__(256) void k_ldl_panel(T* __restrict__ A, int m, int k0, int b, int npanel,
                                                    const T* __restrict__ dinv, T* __restrict__ G,
                                                    T* __restrict__ Wt, T* __restrict__ Lt) {
-  __shared__ T Xs[kNB][kNB + 1];
-  __shared__ T Vs[kNB][kNB + 1];
+  __shared__ T As[kNB][kNB];
+  __shared__ T Bs[kNB][kNB];
   const int tid = threadIdx.x;
-  for (int e = tid; e < kNB * kNB; e += 256) {
-    const int i = e / kNB, k = e % kNB;
-    Xs[i][k] = (i < b && k < b) ? G[(long long)(k0 + i) * m + k0 + k] : T(0);
-  }
+  const int tx = tid & 15, ty = tid >> 4;
+  T acc[4][4];
+  tile_zero(acc);
   if ((int)blockIdx.x >= npanel) {
     const int j0 = ((int)blockIdx.x - npanel) * kNB;  // a full block left of K
     for (int e = tid; e < kNB * kNB; e += 256) {
-      const int k = e / kNB, c = e % kNB;
-      Vs[k][c] = k < b ? G[(long long)(k0 + k) * m + j0 + c] : T(0);
+      const int r = e / kNB, k = e % kNB;
+      As[k][r] = (r < b && k < b) ? G[(long long)(k0 + r) * m + k0 + k] : T(0);  // X[r][k]
+      Bs[r][k] = r < b ? G[(long long)(k0 + r) * m + j0 + k] : T(0);              // R[r][c], c = k
     }
     __syncthreads();
-    const int c = tid & 63, rg = tid >> 6;
-    for (int r = rg; r < b; r += 4) {
-      T acc = 0;
-      for (int k = 0; k <= r; ++k) acc += Xs[r][k] * Vs[k][c];
-      G[(long long)(k0 + r) * m + j0 + c] = acc;  // every read of the tile preceded the barrier
+    tile_acc(acc, As, Bs, b);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int r = ty + 16 * a;
+      if (r >= b) continue;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) G[(long long)(k0 + r) * m + j0 + tx + 16 * q] = acc[a][q];
     }
     return;
   }
   const int i0 = k0 + b + blockIdx.x * kNB;
   for (int e = tid; e < kNB * kNB; e += 256) {
-    const int i = e / kNB, k = e % kNB;
-    Vs[i][k] = (i0 + i < m && k < b) ? A[(long long)(i0 + i) * m + k0 + k] : T(0);
+    const int r = e / kNB, k = e % kNB;
+    As[k][r] = (i0 + r < m && k < b) ? A[(long long)(i0 + r) * m + k0 + k] : T(0);  // V[r][k]
+    Bs[k][r] = (r < b && k < b) ? G[(long long)(k0 + r) * m + k0 + k] : T(0);       // X[c][k], c = r
   }
   __syncthreads();
-  const int c = tid >> 2;  // output column (pivot index within the block)
-  if (c >= b) return;
-  const T dc = dinv[k0 + c];
-#pragma unroll 4
-  for (int t = 0; t < kNB / 4; ++t) {
-    const int r = (tid & 3) + 4 * t;
-    const int i = i0 + r;
-    if (i >= m) break;
-    T w = 0;
-#pragma unroll 16
-    for (int k = 0; k <= c; ++k) w += Vs[r][k] * Xs[c][k];
-    const T l = w * dc;
-    A[(long long)i * m + k0 + c] = l;
-    Wt[(long long)c * m + i] = w;
-    Lt[(long long)c * m + i] = l;
+  tile_acc(acc, As, Bs, b);
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int i = i0 + ty + 16 * a;
+    if (i >= m) continue;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = tx + 16 * q;
+      if (c >= b) continue;
+      const T w = acc[a][q];
+      const T l = w * dinv[k0 + c];
+      A[(long long)i * m + k0 + c] = l;
+      Wt[(long long)c * m + i] = w;
+      Lt[(long long)c * m + i] = l;
+    }
   }
 }
 
@@ -585,7 +583,7 @@ static int dense_invert_typed(DenseNE& d, int m, double tol, double skip_rel, hi
   for (int K = 0; K < nb; ++K) {
     const int k0 = K * kNB;
     const int b = std::min(kNB, m - k0);
-    hipLaunchKernelGGL((k_ldl_diag<T>), dim3(1), dim3(256), 0, s, A, m, k0, b, dinv, G, tol, d.diag0.as<double>(),
+    hipLaunchKernelGGL((k_ldl_diag<T>), dim3(1), dim3(kWave), 0, s, A, m, k0, b, dinv, G, tol, d.diag0.as<double>(),
                        skip_rel, d.clamped.as<int>());
     const int rest = m - k0 - b;
     const int nt = (rest + kNB - 1) / kNB;
