@@ -252,7 +252,11 @@ typedef struct hn_ipm_result {
   int64_t refine_sweeps;       /* FP64 refinement sweeps */
   int64_t cg_iterations;       /* PCG iterations */
   int64_t fp64_from_iter;      /* first iteration solved in FP64 (-1: never) */
-  int warm_started;            /* 1 when x0/y0/s0 were used */
+  int warm_started;            /* 1 when x0/y0/s0 were used; 2 when they were used
+                                  and then abandoned: mu fell below its tolerance
+                                  while a residual stayed above its own for three
+                                  iterations, and the solve restarted from the
+                                  standard point */
 } hn_ipm_result;
 
 int hn_ipm_default_opts(hn_ipm_opts* o);
@@ -308,6 +312,12 @@ int hn_gnn_graph_free(hn_gnn_graph* gr);
  * device outputs log_x, log_s (n) and y (m) (each may be NULL). */
 int hn_gnn_forward_dev(hn_gnn* g, hn_gnn_graph* gr, const double* val, const double* rhs, const double* c,
                        float* log_x, float* log_s, float* y, void* stream);
+/* The warm start (x0, y0, s0) of the scaled standard form from one forward
+ * pass: x0 = exp(log x), y0 = y, s0 = max(c - M'y, exp(log s)) - dual
+ * feasible wherever the predicted reduced cost covers the predicted s.
+ * Device FP64 outputs (n, m, n), ready for hn_lp.x0/y0/s0. */
+int hn_gnn_warm_start_dev(hn_gnn* g, hn_gnn_graph* gr, const double* val, const double* rhs, const double* c,
+                          double* x0, double* y0, double* s0, void* stream);
 /* Loss and its gradient with respect to every parameter for device targets
  * tx = log x*, ts = log s* (n) and ty = y* (m); grad_host receives
  * hn_gnn_num_params floats, loss_host one float (synchronous). */

@@ -54,8 +54,10 @@ struct Inst {
 enum VM { VM_B = 0, VM_LAM, VM_RP, VM_TM, VM_MDRD, VM_RHS, VM_RHS2, VM_DLAM, VM_MZ, VM_R, VM_ND, VM_DG, VM_COUNT };
 enum VN { VN_C = 0, VN_X, VN_Z, VN_RD, VN_D, VN_T, VN_DX, VN_DZ, VN_ZS, VN_COUNT };
 
+constexpr int kStallIters = 3;
+
 struct Out {
-  int status, iterations, fp64_from, ne_solves, refine;
+  int status, iterations, fp64_from, ne_solves, refine, restarted;
   double rp, rd, mu;
 };
 
@@ -547,7 +549,7 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
     bool fp64 = !prm.mixed;
     // start point (interior_point.cpp:443-453), unless the host accepted a
     // warm start and packed it into x, z, lam
-    if (!I.warm) {
+    auto cold_start = [&]() {
       double bn = 1.0, cn = 1.0;
       for (int i = tid; i < m; i += NT) bn = nan_ignoring_max(bn, fabs(b[i]));
       for (int j = tid; j < n; j += NT) cn = nan_ignoring_max(cn, fabs(cc[j]));
@@ -558,9 +560,13 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
         z[j] = fmax(cn + 1.0, fabs(cc[j]));
       }
       for (int i = tid; i < m; i += NT) lam[i] = 0.0;
-    }
+      __syncthreads();
+    };
+    if (!I.warm) cold_start();
     __syncthreads();
     bool done = false, numerr = false, stopped = false;
+    bool warm_live = I.warm != 0, restarted = false;
+    int stuck = 0;
     int iterations = 0;
     double mu = 0.0, rpn = 0.0, rdn = 0.0;
     auto residuals = [&]() {
@@ -592,6 +598,18 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
       if (prm.mu_target > 0.0 && mu <= prm.mu_target) {
         stopped = true;
         break;
+      }
+      // a warm start that lets mu fall below its tolerance while the
+      // residuals stay above theirs for kStallIters iterations has stalled:
+      // restart once from the standard point
+      if (warm_live) {
+        stuck = (mu < prm.comp_tol && (rpn >= prm.conv_tol || rdn >= prm.conv_tol)) ? stuck + 1 : 0;
+        if (stuck >= kStallIters) {
+          warm_live = false;
+          restarted = true;
+          cold_start();
+          residuals();
+        }
       }
       if (!fp64 && mu < prm.mu_switch) {
         fp64 = true;
@@ -716,6 +734,7 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
       o.fp64_from = fp64_from;
       o.ne_solves = ne_solves;
       o.refine = refine;
+      o.restarted = restarted ? 1 : 0;
       o.rp = rpn;
       o.rd = rdn;
       o.mu = mu;
@@ -920,7 +939,7 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
     r.refine_sweeps = out[k].refine;
     r.cg_iterations = 0;
     r.fp64_from_iter = out[k].fp64_from;
-    r.warm_started = warm_ok[k];
+    r.warm_started = warm_ok[k] ? (out[k].restarted ? 2 : 1) : 0;
     if (r.primal) std::memcpy(r.primal, primal.data(), sizeof(double) * lps[k].n);
     if (r.x && r.cap_n >= F.n) std::memcpy(r.x, x, sizeof(double) * F.n);
     if (r.s && r.cap_n >= F.n) std::memcpy(r.s, z, sizeof(double) * F.n);

@@ -644,6 +644,23 @@ __global__ void k_gnn_adam(float* __restrict__ p, const float* __restrict__ g, f
   p[i] -= lr * (a / c1) / (sqrtf(b / c2) + 1e-8f);
 }
 
+// Warm point from the network's outputs, one thread per column j:
+// x0 = exp(log x), s0 = max(c - M'y, exp(log s)) (dual feasible wherever the
+// predicted reduced cost is at least the predicted s), y0 = y (FP64).
+__global__ void k_gnn_warm_point(const int* __restrict__ trp, const int* __restrict__ tci,
+                                 const int* __restrict__ tsrc, const double* __restrict__ val,
+                                 const double* __restrict__ c, const float* __restrict__ lx,
+                                 const float* __restrict__ ls, const float* __restrict__ ly, int n, int m,
+                                 double* __restrict__ x0, double* __restrict__ y0, double* __restrict__ s0) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < m) y0[j] = (double)ly[j];
+  if (j >= n) return;
+  double mty = 0.0;
+  for (int q = trp[j]; q < trp[j + 1]; ++q) mty += val[tsrc[q]] * (double)ly[tci[q]];
+  x0[j] = exp((double)lx[j]);
+  s0[j] = fmax(c[j] - mty, exp((double)ls[j]));
+}
+
 unsigned wgrid(long long rows) { return (unsigned)((rows * kWave + 255) / 256); }
 unsigned tgrid(long long n) { return (unsigned)((n + 255) / 256); }
 unsigned cgrid(long long rows) { return (unsigned)((rows + kChunk - 1) / kChunk); }
@@ -1027,4 +1044,19 @@ extern "C" int hn_gnn_train_step_dev(hn_gnn* g, hn_gnn_graph* gr, const double* 
   if (loss_dev) HN_HIP(hipMemcpyAsync(loss_dev, gr->loss.p, sizeof(float), hipMemcpyDeviceToDevice, s));
   count_launch();
   return check_launch("gnn train step");
+}
+
+extern "C" int hn_gnn_warm_start_dev(hn_gnn* g, hn_gnn_graph* gr, const double* val, const double* rhs,
+                                     const double* c, double* x0, double* y0, double* s0, void* stream) {
+  HN_TRY(check_graph_inputs(g, gr));
+  HN_CHECK(val && rhs && c && x0 && y0 && s0, HN_ERR_INVALID, "hn_gnn_warm_start_dev: null argument");
+  HN_TRY(enter());
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx().stream;
+  HN_TRY(forward(g, gr, val, rhs, c, s));
+  const int nm = gr->n > gr->m ? gr->n : gr->m;
+  hipLaunchKernelGGL(k_gnn_warm_point, dim3(tgrid(nm)), dim3(256), 0, s, gr->trp.as<int>(), gr->tci.as<int>(),
+                     gr->tsrc.as<int>(), val, c, gr->lx.as<float>(), gr->ls.as<float>(), gr->ly.as<float>(), gr->n,
+                     gr->m, x0, y0, s0);
+  count_launch();
+  return check_launch("gnn warm start");
 }

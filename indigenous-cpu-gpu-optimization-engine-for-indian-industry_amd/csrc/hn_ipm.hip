@@ -317,42 +317,59 @@ __global__ __launch_bounds__(kNT) void k_fin_final(const double* __restrict__ pa
 }
 
 // ---- PCG pieces (Jacobi, matrix-free N = M D M' + r I) -----------------------
-__global__ void k_mul(const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ o, int n) {
+// Vectors of type T (FP64, or FP32 for the bulk solves of mixed precision);
+// scalars and reductions in FP64.
+template <typename T>
+__global__ void k_mul(const T* __restrict__ a, const T* __restrict__ b, T* __restrict__ o, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) o[i] = a[i] * b[i];
 }
 
+template <typename Ti, typename To>
+__global__ void k_cvt(const Ti* __restrict__ a, To* __restrict__ o, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) o[i] = (To)a[i];
+}
+
+// y += (double) e
+__global__ void k_add_f32(const float* __restrict__ e, double* __restrict__ y, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = y[i] + (double)e[i];
+}
+
 // q += r p; partial p'q
-__global__ __launch_bounds__(kNT) void k_pcg_pq(double* __restrict__ q, const double* __restrict__ p,
+template <typename T>
+__global__ __launch_bounds__(kNT) void k_pcg_pq(T* __restrict__ q, const T* __restrict__ p,
                                                 const double* __restrict__ S, int m, double* __restrict__ part) {
   __shared__ double red[kNT / kWave];
-  const double reg = S[S_REG];
+  const T reg = (T)S[S_REG];
   double s = 0.0;
   for (int i = blockIdx.x * kNT + threadIdx.x; i < m; i += gridDim.x * kNT) {
-    const double qi = q[i] + reg * p[i];
+    const T qi = q[i] + reg * p[i];
     q[i] = qi;
-    s += p[i] * qi;
+    s += (double)(p[i] * qi);
   }
   s = block_sum<double, kNT>(s, red);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
 // x += a p; r -= a q; z = r / diag; partials r'z and max |r|
-__global__ __launch_bounds__(kNT) void k_pcg_step(double* __restrict__ xk, double* __restrict__ r,
-                                                  double* __restrict__ zk, const double* __restrict__ p,
-                                                  const double* __restrict__ q, const double* __restrict__ dinv,
-                                                  const double* __restrict__ S, int m, double* __restrict__ part) {
+template <typename T>
+__global__ __launch_bounds__(kNT) void k_pcg_step(T* __restrict__ xk, T* __restrict__ r, T* __restrict__ zk,
+                                                  const T* __restrict__ p, const T* __restrict__ q,
+                                                  const T* __restrict__ dinv, const double* __restrict__ S, int m,
+                                                  double* __restrict__ part) {
   __shared__ double red[kNT / kWave];
-  const double alpha = S[S_PCG_RZ] / S[S_PCG_PQ];
+  const T alpha = (T)(S[S_PCG_RZ] / S[S_PCG_PQ]);
   double s = 0.0, a = 0.0;
   for (int i = blockIdx.x * kNT + threadIdx.x; i < m; i += gridDim.x * kNT) {
     xk[i] = xk[i] + alpha * p[i];
-    const double ri = r[i] - alpha * q[i];
+    const T ri = r[i] - alpha * q[i];
     r[i] = ri;
-    const double zi = ri * dinv[i];
+    const T zi = ri * dinv[i];
     zk[i] = zi;
-    s += ri * zi;
-    a = nan_ignoring_max(a, fabs(ri));
+    s += (double)(ri * zi);
+    a = nan_ignoring_max(a, fabs((double)ri));
   }
   s = block_sum<double, kNT>(s, red);
   a = block_max<double, kNT>(a, red);
@@ -378,33 +395,34 @@ __global__ __launch_bounds__(kNT) void k_pcg_fin(const double* __restrict__ part
   }
 }
 
-__global__ void k_pcg_dir(double* __restrict__ p, const double* __restrict__ zk, const double* __restrict__ S, int m) {
+template <typename T>
+__global__ void k_pcg_dir(T* __restrict__ p, const T* __restrict__ zk, const double* __restrict__ S, int m) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
-  const double beta = S[S_PCG_RZ] / S[S_PCG_RZ_OLD];
+  const T beta = (T)(S[S_PCG_RZ] / S[S_PCG_RZ_OLD]);
   p[i] = zk[i] + beta * p[i];
 }
 
 // r = rhs (x0 = 0), z = r/diag, p = z, dinv = 1/(diag + reg); partials r'z, max|r|
-__global__ __launch_bounds__(kNT) void k_pcg_init(const double* __restrict__ rhs, const double* __restrict__ diag,
-                                                  const double* __restrict__ S, double* __restrict__ xk,
-                                                  double* __restrict__ r, double* __restrict__ zk,
-                                                  double* __restrict__ p, double* __restrict__ dinv, int m,
-                                                  double* __restrict__ part) {
+template <typename T>
+__global__ __launch_bounds__(kNT) void k_pcg_init(const T* __restrict__ rhs, const double* __restrict__ diag,
+                                                  const double* __restrict__ S, T* __restrict__ xk,
+                                                  T* __restrict__ r, T* __restrict__ zk, T* __restrict__ p,
+                                                  T* __restrict__ dinv, int m, double* __restrict__ part) {
   __shared__ double red[kNT / kWave];
   const double reg = S[S_REG];
   double s = 0.0, a = 0.0;
   for (int i = blockIdx.x * kNT + threadIdx.x; i < m; i += gridDim.x * kNT) {
-    const double di = 1.0 / (diag[i] + reg);
+    const T di = (T)(1.0 / (diag[i] + reg));
     dinv[i] = di;
-    xk[i] = 0.0;
-    const double ri = rhs[i];
+    xk[i] = T(0);
+    const T ri = rhs[i];
     r[i] = ri;
-    const double zi = ri * di;
+    const T zi = ri * di;
     zk[i] = zi;
     p[i] = zi;
-    s += ri * zi;
-    a = nan_ignoring_max(a, fabs(ri));
+    s += (double)(ri * zi);
+    a = nan_ignoring_max(a, fabs((double)ri));
   }
   s = block_sum<double, kNT>(s, red);
   a = block_max<double, kNT>(a, red);
@@ -443,6 +461,8 @@ struct Solver {
   // device vectors
   DevBuf b, c, x, z, lam, rp, rd, D, Drd, tn, tm, mdrd, rhs, rhs2, dlam, dx, dz, zs, mz, ir_r, ir_nd, ir_e;
   DevBuf pcg_r, pcg_z, pcg_p, pcg_q, pcg_dinv, pcg_diag, pcg_tn;
+  // FP32 copies for the bulk PCG of mixed precision
+  DevBuf f_r, f_z, f_p, f_q, f_x, f_dinv, f_tn, f_D, f_rhs;
   DevBuf S, flags, part;
   double* hS = nullptr;  // pinned mirror of S
   int* hF = nullptr;
@@ -496,6 +516,11 @@ struct Solver {
     } else {
       for (DevBuf* d : {&pcg_r, &pcg_z, &pcg_p, &pcg_q, &pcg_dinv, &pcg_diag}) HN_TRY(d->reserve(bm));
       HN_TRY(pcg_tn.reserve(bn));
+      if (o.precision == HN_PREC_MIXED) {
+        const size_t fm = sizeof(float) * std::max(m, 1), fn = sizeof(float) * std::max(n, 1);
+        for (DevBuf* d : {&f_r, &f_z, &f_p, &f_q, &f_x, &f_dinv, &f_rhs}) HN_TRY(d->reserve(fm));
+        for (DevBuf* d : {&f_tn, &f_D}) HN_TRY(d->reserve(fn));
+      }
     }
     fp64_solve = o.precision != HN_PREC_MIXED;
     HN_HIP(hipStreamSynchronize(s));
@@ -517,6 +542,7 @@ struct Solver {
       HN_TRY(ne_jacobi_diag(M, D.as<double>(), pcg_diag.as<double>(), s));
       hipLaunchKernelGGL(k_pcg_reg, dim3(1), dim3(kNT), 0, s, pcg_diag.as<double>(), m, o.regularization,
                          rel_reg(), S.as<double>());
+      if (!fp64_solve) hipLaunchKernelGGL((k_cvt<double, float>), g(n), dim3(256), 0, s, D.as<double>(), f_D.as<float>(), n);
       HN_TRY(check_launch("k_pcg_reg"));
     }
     return HN_OK;
@@ -526,9 +552,18 @@ struct Solver {
   int apply_N(const double* v, double* out) {
     if (use_dense) return spmv(ne.N, v, out);
     HN_TRY(spmv(Mt, v, pcg_tn.as<double>()));
-    hipLaunchKernelGGL(k_mul, g(n), dim3(256), 0, s, pcg_tn.as<double>(), D.as<double>(), pcg_tn.as<double>(), n);
+    hipLaunchKernelGGL((k_mul<double>), g(n), dim3(256), 0, s, pcg_tn.as<double>(), D.as<double>(),
+                       pcg_tn.as<double>(), n);
     HN_TRY(spmv(M, pcg_tn.as<double>(), out));
     return HN_OK;
+  }
+
+  // q = M (D (M' p)) in FP32 (the regularisation is added by k_pcg_pq)
+  int apply_N32(const float* p, float* q) {
+    HN_TRY(spmv_launch(Mt, HN_F32, p, f_tn.as<float>(), s));
+    hipLaunchKernelGGL((k_mul<float>), g(n), dim3(256), 0, s, f_tn.as<float>(), f_D.as<float>(), f_tn.as<float>(),
+                       n);
+    return spmv_launch(M, HN_F32, f_tn.as<float>(), q, s);
   }
 
   // Dense: dlam = N^{-1} rhs then refinement against N (refinement.cpp:7-36):
@@ -564,8 +599,8 @@ struct Solver {
   // tolerance follows mu (tighter as the iterates approach the optimum).
   int solve_pcg(const double* r_in, double* out, double mu) {
     const int np = parts_for(m);
-    hipLaunchKernelGGL(k_pcg_init, dim3(np), dim3(kNT), 0, s, r_in, pcg_diag.as<double>(), S.as<double>(), out,
-                       pcg_r.as<double>(), pcg_z.as<double>(), pcg_p.as<double>(), pcg_dinv.as<double>(), m,
+    hipLaunchKernelGGL((k_pcg_init<double>), dim3(np), dim3(kNT), 0, s, r_in, pcg_diag.as<double>(), S.as<double>(),
+                       out, pcg_r.as<double>(), pcg_z.as<double>(), pcg_p.as<double>(), pcg_dinv.as<double>(), m,
                        part.as<double>());
     hipLaunchKernelGGL(k_pcg_fin, dim3(1), dim3(kNT), 0, s, part.as<double>(), np, S.as<double>());
     HN_TRY(pull());
@@ -575,14 +610,15 @@ struct Solver {
     ++res.ne_solves;
     for (int it = 0; it < o.pcg_max_iter; ++it) {
       HN_TRY(apply_N(pcg_p.as<double>(), pcg_q.as<double>()));
-      hipLaunchKernelGGL(k_pcg_pq, dim3(np), dim3(kNT), 0, s, pcg_q.as<double>(), pcg_p.as<double>(), S.as<double>(),
-                         m, part.as<double>());
+      hipLaunchKernelGGL((k_pcg_pq<double>), dim3(np), dim3(kNT), 0, s, pcg_q.as<double>(), pcg_p.as<double>(),
+                         S.as<double>(), m, part.as<double>());
       hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(kNT), 0, s, part.as<double>(), np, S.as<double>(), (int)S_PCG_PQ);
-      hipLaunchKernelGGL(k_pcg_step, dim3(np), dim3(kNT), 0, s, out, pcg_r.as<double>(), pcg_z.as<double>(),
+      hipLaunchKernelGGL((k_pcg_step<double>), dim3(np), dim3(kNT), 0, s, out, pcg_r.as<double>(), pcg_z.as<double>(),
                          pcg_p.as<double>(), pcg_q.as<double>(), pcg_dinv.as<double>(), S.as<double>(), m,
                          part.as<double>());
       hipLaunchKernelGGL(k_pcg_fin, dim3(1), dim3(kNT), 0, s, part.as<double>(), np, S.as<double>());
-      hipLaunchKernelGGL(k_pcg_dir, g(m), dim3(256), 0, s, pcg_p.as<double>(), pcg_z.as<double>(), S.as<double>(), m);
+      hipLaunchKernelGGL((k_pcg_dir<double>), g(m), dim3(256), 0, s, pcg_p.as<double>(), pcg_z.as<double>(),
+                         S.as<double>(), m);
       ++res.cg_iterations;
       if ((it & 7) == 7) {
         HN_TRY(pull());
@@ -592,9 +628,65 @@ struct Solver {
     return check_launch("pcg");
   }
 
+  // Mixed precision: FP32 Jacobi-PCG on the FP32 operator for the bulk of
+  // the work, FP64 refinement around it - the residual r_in - N out is
+  // recomputed in FP64 and solved for again, until it meets the tolerance
+  // of the FP64 PCG (or 8 sweeps).
+  int solve_pcg_mixed(const double* r_in, double* out, double mu) {
+    const int np = parts_for(m);
+    const double rel = std::max(1e-11, std::min(1e-3, 0.1 * mu));
+    hipLaunchKernelGGL(k_fill, g(m), dim3(256), 0, s, out, 0.0, m);
+    const double* res_src = r_in;
+    double r0 = -1.0;
+    ++res.ne_solves;
+    for (int sweep = 0; sweep < 8; ++sweep) {
+      if (sweep > 0) {
+        HN_TRY(apply_N(out, ir_nd.as<double>()));
+        hipLaunchKernelGGL(k_red_resid, dim3(np), dim3(kNT), 0, s, r_in, ir_nd.as<double>(), ir_r.as<double>(), m,
+                           part.as<double>());
+        hipLaunchKernelGGL(k_fin_max2, dim3(1), dim3(kNT), 0, s, part.as<double>(), np, S.as<double>(), S_IRR, S_IRB);
+        HN_TRY(pull());
+        r0 = hS[S_IRB];
+        if (!(hS[S_IRR] > rel * r0)) break;
+        res_src = ir_r.as<double>();
+        ++res.refine_sweeps;
+      }
+      hipLaunchKernelGGL((k_cvt<double, float>), g(m), dim3(256), 0, s, res_src, f_rhs.as<float>(), m);
+      hipLaunchKernelGGL((k_pcg_init<float>), dim3(np), dim3(kNT), 0, s, f_rhs.as<float>(), pcg_diag.as<double>(),
+                         S.as<double>(), f_x.as<float>(), f_r.as<float>(), f_z.as<float>(), f_p.as<float>(),
+                         f_dinv.as<float>(), m, part.as<double>());
+      hipLaunchKernelGGL(k_pcg_fin, dim3(1), dim3(kNT), 0, s, part.as<double>(), np, S.as<double>());
+      HN_TRY(pull());
+      const double f0 = hS[S_PCG_RN];
+      if (f0 == 0.0) break;
+      // FP32 can resolve about 1e-6 of the residual it starts from
+      const double frel = std::max(1e-6, sweep == 0 ? rel : 1e-6);
+      for (int it = 0; it < o.pcg_max_iter; ++it) {
+        HN_TRY(apply_N32(f_p.as<float>(), f_q.as<float>()));
+        hipLaunchKernelGGL((k_pcg_pq<float>), dim3(np), dim3(kNT), 0, s, f_q.as<float>(), f_p.as<float>(),
+                           S.as<double>(), m, part.as<double>());
+        hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(kNT), 0, s, part.as<double>(), np, S.as<double>(),
+                           (int)S_PCG_PQ);
+        hipLaunchKernelGGL((k_pcg_step<float>), dim3(np), dim3(kNT), 0, s, f_x.as<float>(), f_r.as<float>(),
+                           f_z.as<float>(), f_p.as<float>(), f_q.as<float>(), f_dinv.as<float>(), S.as<double>(), m,
+                           part.as<double>());
+        hipLaunchKernelGGL(k_pcg_fin, dim3(1), dim3(kNT), 0, s, part.as<double>(), np, S.as<double>());
+        hipLaunchKernelGGL((k_pcg_dir<float>), g(m), dim3(256), 0, s, f_p.as<float>(), f_z.as<float>(),
+                           S.as<double>(), m);
+        ++res.cg_iterations;
+        if ((it & 7) == 7) {
+          HN_TRY(pull());
+          if (!(hS[S_PCG_RN] > frel * f0)) break;
+        }
+      }
+      hipLaunchKernelGGL(k_add_f32, g(m), dim3(256), 0, s, f_x.as<float>(), out, m);
+    }
+    return check_launch("pcg mixed");
+  }
+
   int solve(const double* r_in, double* out, double mu, bool* escalated) {
     *escalated = false;
-    if (!use_dense) return solve_pcg(r_in, out, mu);
+    if (!use_dense) return fp64_solve ? solve_pcg(r_in, out, mu) : solve_pcg_mixed(r_in, out, mu);
     bool ok = false;
     HN_TRY(solve_dense(r_in, out, &ok));
     if (!ok && !fp64_solve) {  // FP32 inverse could not be refined: switch to FP64 for good
@@ -673,6 +765,21 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
   double* hS = sv.hS;
 
   // start point (:443-453) or the caller's warm start
+  auto cold_start = [&]() -> int {
+    std::vector<double> x0(n), z0(n), l0(m, 0.0);
+    double b_norm = 1.0, c_norm = 1.0;
+    for (double v : f.rhs) b_norm = std::max(b_norm, std::abs(v));
+    for (double v : f.c) c_norm = std::max(c_norm, std::abs(v));
+    for (int j = 0; j < n; ++j) {
+      x0[j] = b_norm + 1.0;
+      z0[j] = std::max(c_norm + 1.0, std::abs(f.c[j]));
+    }
+    HN_HIP(hipMemcpyAsync(sv.x.p, x0.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    HN_HIP(hipMemcpyAsync(sv.z.p, z0.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    HN_HIP(hipMemcpyAsync(sv.lam.p, l0.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+    HN_HIP(hipStreamSynchronize(s));
+    return HN_OK;
+  };
   {
     std::vector<double> x0(n), z0(n), l0(m, 0.0);
     // the caller's warm start (options first, then the LP's own), if it
@@ -703,6 +810,8 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
   const dim3 B(256);
   const int np_n = parts_for(std::max(n, m));
   bool done = false, time_up = false, interrupted = false, numerical_error = false, stopped = false;
+  bool warm_live = res.warm_started != 0;
+  int stuck = 0;
   auto deadline = [&]() { return o.time_limit_seconds > 0.0 && clock.s() >= o.time_limit_seconds; };
   // residuals, mu and their norms on the device; host copies in hS
   auto residuals = [&]() -> int {
@@ -738,6 +847,20 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
     if (o.mu_target > 0.0 && mu <= o.mu_target) {  // early stop (not in the reference)
       stopped = true;
       break;
+    }
+    // a warm start that lets mu fall below its tolerance while the residuals
+    // stay above theirs for 3 iterations has stalled: restart once from the
+    // standard point
+    if (warm_live) {
+      stuck = (mu < o.complementarity_tol &&
+               (hS[S_RPN] >= o.convergence_tol || hS[S_RDN] >= o.convergence_tol)) ? stuck + 1 : 0;
+      if (stuck >= 3) {
+        warm_live = false;
+        res.warm_started = 2;
+        HN_TRY(cold_start());
+        HN_TRY(residuals());
+        mu = hS[S_XZ] / (double)n;
+      }
     }
     if (!sv.fp64_solve && mu < o.mu_switch) {
       sv.fp64_solve = true;

@@ -42,9 +42,10 @@ _L.hn_gnn_graph_free.argtypes = [_vp]
 _L.hn_gnn_forward_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
 _L.hn_gnn_gradients_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _fp, _fp]
 _L.hn_gnn_train_step_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_float, _vp, _vp]
+_L.hn_gnn_warm_start_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
 for _f in ("hn_gnn_create", "hn_gnn_free", "hn_gnn_num_params", "hn_gnn_get_params", "hn_gnn_set_params",
            "hn_gnn_graph_create", "hn_gnn_graph_free", "hn_gnn_forward_dev", "hn_gnn_gradients_dev",
-           "hn_gnn_train_step_dev"):
+           "hn_gnn_train_step_dev", "hn_gnn_warm_start_dev"):
     getattr(_L, _f).restype = C.c_int
 
 
@@ -103,6 +104,15 @@ class Network:
         HH.check(_L.hn_gnn_forward_dev(self.h, g.h, val.ptr, rhs.ptr, c.ptr, lx.ptr, ls.ptr, ly.ptr, None))
         HH.stream_sync()
         return lx.to_numpy(), ls.to_numpy(), ly.to_numpy()
+
+    def warm_start(self, g: Graph, val, rhs, c):
+        """(x0, y0, s0) on the host, built on the device (hn_gnn_warm_start_dev)."""
+        x0 = HH.DeviceArray(g.n)
+        y0 = HH.DeviceArray(g.m)
+        s0 = HH.DeviceArray(g.n)
+        HH.check(_L.hn_gnn_warm_start_dev(self.h, g.h, val.ptr, rhs.ptr, c.ptr, x0.ptr, y0.ptr, s0.ptr, None))
+        HH.stream_sync()
+        return x0.to_numpy(), y0.to_numpy(), s0.to_numpy()
 
     def gradients(self, g: Graph, val, rhs, c, tx, ts, ty):
         grad = np.empty(self.nparams, np.float32)
@@ -211,17 +221,29 @@ def train(net: Network, ds: Dataset, epochs: int, lr: float = 1e-3, lr_final: fl
 
 
 def predict(net: Network, lps, graph: Optional[Graph] = None):
-    """Warm starts (x0, y0, s0) of the scaled standard forms of `lps`."""
+    """Warm starts (x0, y0, s0) of the scaled standard forms of `lps`:
+    x0 = exp(log x), y0 = y, s0 = max(c - M'y, exp(log s)), formed on the
+    device.  The dual-feasible s0 matters: a start whose mu is small next to
+    its dual residual lets mu collapse first and the dual steps then stall."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=16) as pool:   # the ctypes calls release the GIL
+        sfs = list(pool.map(HH.lp_standard_form, lps))
     out = []
-    for lp in lps:
-        sf = HH.lp_standard_form(lp)
-        g = graph if graph is not None and graph.nnz == sf["ci"].size and graph.m == sf["m"] else \
-            Graph(sf["m"], sf["n"], sf["rp"], sf["ci"])
-        val = HH.DeviceArray.from_numpy(sf["v"])
-        rhs = HH.DeviceArray.from_numpy(sf["rhs"])
-        c = HH.DeviceArray.from_numpy(sf["c"])
-        lx, ls, y = net.forward(g, val, rhs, c)
-        out.append((np.exp(lx.astype(np.float64)), y.astype(np.float64), np.exp(ls.astype(np.float64))))
+    bufs = None
+    for sf in sfs:
+        g = graph if graph is not None and graph.nnz == sf["ci"].size and graph.m == sf["m"] and \
+            np.array_equal(graph.col, sf["ci"]) else Graph(sf["m"], sf["n"], sf["rp"], sf["ci"])
+        shape = (sf["m"], sf["n"], sf["ci"].size)
+        if bufs is None or bufs[0] != shape:  # device buffers reused across instances
+            bufs = (shape, HH.DeviceArray(shape[2]), HH.DeviceArray(shape[0]), HH.DeviceArray(shape[1]),
+                    HH.DeviceArray(shape[1]), HH.DeviceArray(shape[0]), HH.DeviceArray(shape[1]))
+        _, val, rhs, c, x0, y0, s0 = bufs
+        val.copy_from(sf["v"])
+        rhs.copy_from(sf["rhs"])
+        c.copy_from(sf["c"])
+        HH.check(_L.hn_gnn_warm_start_dev(net.h, g.h, val.ptr, rhs.ptr, c.ptr, x0.ptr, y0.ptr, s0.ptr, None))
+        HH.stream_sync()
+        out.append((x0.to_numpy(), y0.to_numpy(), s0.to_numpy()))
     return out
 
 
@@ -250,6 +272,8 @@ def evaluate(net: Network, make_lp, seeds: Sequence[int], graph: Optional[Graph]
         "instances": len(lps),
         "cold_iterations_mean": float(ci.mean()), "warm_iterations_mean": float(wi.mean()),
         "cold_iterations_max": int(ci.max()), "warm_iterations_max": int(wi.max()),
+        "cold_iterations_median": float(np.median(ci)), "warm_iterations_median": float(np.median(wi)),
+        "cold_stalled": int((ci >= 100).sum()), "warm_stalled": int((wi >= 100).sum()),
         "accepted": int(acc.sum()),
         "cold_optimal": int(sum(r.status == "OPTIMAL" for r in cold)),
         "warm_optimal": int(sum(r.status == "OPTIMAL" for r in warm)),
@@ -301,9 +325,16 @@ if __name__ == "__main__":
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--out-params", default=None)
     ap.add_argument("--report", default=None)
+    ap.add_argument("--params", default=None, help="evaluate these trained parameters (.npy) without training")
     a = ap.parse_args()
-    r = run_experiment(a.train, a.test, a.epochs, lr=a.lr, out_params=a.out_params,
-                       log=lambda s: print(s, flush=True))
+    if a.params:
+        HH.init(-1)
+        net = Network(0)
+        net.set_params(np.load(a.params))
+        r = evaluate(net, transport_family(), range(1_000_000, 1_000_000 + a.test), log=lambda s: print(s, flush=True))
+    else:
+        r = run_experiment(a.train, a.test, a.epochs, lr=a.lr, out_params=a.out_params,
+                           log=lambda s: print(s, flush=True))
     print(json.dumps(r), flush=True)
     if a.report:
         with open(a.report, "w") as f:

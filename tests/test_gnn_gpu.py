@@ -75,3 +75,20 @@ def test_gnn_training_lowers_loss(ws):
     hist = ws.train(net, ds, epochs=30, lr=3e-3, lr_final=1e-3, log=lambda *a: None)
     print(f"  losses {hist[0]:.4f} -> {hist[-1]:.4f}", flush=True)
     assert np.all(np.isfinite(hist)) and hist[-1] < 0.5 * hist[0]
+
+
+def test_trained_warm_start_saves_iterations(ws):
+    # the network trained on 20,000 plain-IPM targets of the 200 x 300
+    # transportation family (data/warmstart_transport_200x300.npy), applied
+    # to held-out seeds: same optimum, fewer iterations than the standard start
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(ws.__file__)), "data", "warmstart_transport_200x300.npy")
+    net = ws.Network(0)
+    net.set_params(np.load(path))
+    print("[case] trained warm start, 64 held-out 200x300 transportation LPs", flush=True)
+    r = ws.evaluate(net, ws.transport_family(), range(1_000_000, 1_000_064), log=lambda *a: None)
+    print(f"  {r}", flush=True)
+    assert r["cold_optimal"] == r["warm_optimal"] == 64
+    assert r["accepted"] >= 60
+    assert r["max_rel_objective_diff"] <= 1e-6
+    assert r["warm_iterations_median"] <= r["cold_iterations_median"] - 3

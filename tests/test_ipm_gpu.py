@@ -187,3 +187,16 @@ def test_ipm_skip_rule_full_size_transport(hip):
     assert abs(r.iterations - ref.iterations) <= max(2, ref.iterations // 10)
     assert r.rp_norm < 1e-8 and r.rd_norm < 1e-8 and r.mu < 1e-8
     assert rel(r.objective_value, ref.objective) <= 1e-8
+
+
+@pytest.mark.parametrize("name", ["random_60x150_s2", "powerlaw_120x300_s3"])
+def test_ipm_pcg_mixed_precision(hip, name):
+    # FP32 Jacobi-PCG bulk with FP64 refinement of every Newton step; FP64
+    # PCG once mu < mu_switch
+    lp = _lp(name)
+    ref = O.ipm_solve(lp)
+    r = hip.ipm_solve(lp, solver=hip.HN_NE_PCG, precision=hip.HN_PREC_MIXED, pcg_max_iter=20000)
+    print(f"  mixed pcg {r} cg={r.cg_iterations} sweeps={r.refine_sweeps} fp64_from={r.fp64_from_iter}", flush=True)
+    assert r.status == ref.status == "OPTIMAL"
+    assert r.cg_iterations > 0 and r.fp64_from_iter > 0
+    assert rel(r.objective_value, ref.objective) <= 1e-6
