@@ -247,12 +247,13 @@ def run_lp(args) -> dict:
     t0 = time.time()
     lps = [lpgen.batch_lp(i) for i in range(count)]
     log(f"generated {count} batch LPs (sum m={sum(l.m for l in lps)}) in {time.time() - t0:.1f}s")
-    H.ipm_solve_batch([l for l in lps if l.m < 200][:64])  # warm-up: code objects
+    # warm-up: code objects, device arena and page-locked staging sized for this batch
+    H.ipm_solve_batch(lps, pivot_rule=H.HN_PIVOT_SKIP)
     batch = {"workload": f"batch_{count}_lps_m50-2000_n2.5m_5nnz_per_col", "lps": count}
     runs = {}
     for rule, rname in ((H.HN_PIVOT_SKIP, "skip_rule"), (H.HN_PIVOT_CLAMP, "reference_rule")):
         t0 = time.perf_counter()
-        res = H.ipm_solve_batch(lps, precision=H.HN_PREC_F64, pivot_rule=rule)
+        res = H.ipm_solve_batch(lps, precision=H.HN_PREC_F64, pivot_rule=rule, verbose=True)
         wall = time.perf_counter() - t0
         status = {}
         for r in res:

@@ -46,13 +46,18 @@ struct Inst {
   int m, n;
   long long mp, mz;   // M: row_ptr offset (m+1 ints), col/val offset
   long long tp, tz;   // M': row_ptr offset (n+1 ints), col/val offset
-  long long vm, vn;   // m- and n-vector blocks
+  long long vm, vn;   // device-only m- and n-vector scratch blocks
+  long long om, on;   // b and lambda (m each), c (n) and [x, z] (2n) in the host-visible arrays
   long long dn, w;    // dense N (m*m elements), panel workspace (m*64 elements)
   int warm;           // 1: x, z, lam hold an accepted warm start
 };
 
+// Per-LP vectors.  B, LAM, C, X and Z cross the host link and live in their
+// own arrays (b and c uploaded, lam, x and z uploaded for warm starts and
+// downloaded); the rest is device-only scratch.
 enum VM { VM_B = 0, VM_LAM, VM_RP, VM_TM, VM_MDRD, VM_RHS, VM_RHS2, VM_DLAM, VM_MZ, VM_R, VM_ND, VM_DG, VM_COUNT };
 enum VN { VN_C = 0, VN_X, VN_Z, VN_RD, VN_D, VN_T, VN_DX, VN_DZ, VN_ZS, VN_COUNT };
+constexpr int kVmIo = 2, kVnIo = 3;  // leading slots held in the host-visible arrays
 
 constexpr int kStallIters = 3;
 
@@ -496,7 +501,9 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
                                                   const int* __restrict__ mci, const double* __restrict__ mv,
                                                   const int* __restrict__ trp, const int* __restrict__ tci,
                                                   const double* __restrict__ tv, double* __restrict__ vecm,
-                                                  double* __restrict__ vecn, double* __restrict__ dense,
+                                                  double* __restrict__ vecn, double* __restrict__ bio,
+                                                  double* __restrict__ lio, double* __restrict__ cio,
+                                                  double* __restrict__ xzio, double* __restrict__ dense,
                                                   double* __restrict__ work, Params prm, Out* __restrict__ out) {
   __shared__ double red[NT / kWave];
   __shared__ int s_next;
@@ -520,8 +527,13 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
     c.trp = trp + I.tp;
     c.tci = tci + I.tz;
     c.tv = tv + I.tz;
-    for (int k = 0; k < VM_COUNT; ++k) c.vm[k] = vecm + I.vm + (long long)k * I.m;
-    for (int k = 0; k < VN_COUNT; ++k) c.vn[k] = vecn + I.vn + (long long)k * I.n;
+    c.vm[VM_B] = bio + I.om;
+    c.vm[VM_LAM] = lio + I.om;
+    for (int k = kVmIo; k < VM_COUNT; ++k) c.vm[k] = vecm + I.vm + (long long)(k - kVmIo) * I.m;
+    c.vn[VN_C] = cio + I.on;
+    c.vn[VN_X] = xzio + 2 * I.on;
+    c.vn[VN_Z] = xzio + 2 * I.on + I.n;
+    for (int k = kVnIo; k < VN_COUNT; ++k) c.vn[k] = vecn + I.vn + (long long)(k - kVnIo) * I.n;
     const int m = c.m, n = c.n;
     double* A64 = dense + I.dn;
     float* A32 = reinterpret_cast<float*>(A64);
@@ -746,13 +758,13 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
 
 // Counting-sort transpose on the host (entries of each column in row order,
 // as build_normal_equations scatters them, interior_point.cpp:153-173).
-void host_transpose(const StandardForm& f, std::vector<int>& trp, std::vector<int>& tci, std::vector<double>& tv) {
-  trp.assign(f.n + 1, 0);
+// CSR of M' by a counting scatter in row order (stable), written into
+// caller-provided slices: trp (n+1), tci and tv (nnz).
+void host_transpose(const StandardForm& f, int* trp, int* tci, double* tv) {
+  std::fill(trp, trp + f.n + 1, 0);
   for (int64_t k = 0; k < (int64_t)f.ci.size(); ++k) trp[f.ci[k] + 1]++;
   for (int64_t j = 0; j < f.n; ++j) trp[j + 1] += trp[j];
-  std::vector<int> cur(trp.begin(), trp.end() - 1);
-  tci.resize(f.ci.size());
-  tv.resize(f.ci.size());
+  std::vector<int> cur(trp, trp + f.n);
   for (int64_t i = 0; i < f.m; ++i)
     for (int64_t k = f.rp[i]; k < f.rp[i + 1]; ++k) {
       const int p = cur[f.ci[k]]++;
@@ -761,9 +773,47 @@ void host_transpose(const StandardForm& f, std::vector<int>& trp, std::vector<in
     }
 }
 
+// Page-locked host staging that only grows (hipHostMalloc; copies from it
+// run at the host link's DMA rate).
+struct PinnedBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int reserve(size_t want) {
+    if (want <= bytes) return HN_OK;
+    if (p) hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    HN_HIP(hipHostMalloc(&p, want, hipHostMallocDefault));
+    bytes = want;
+    return HN_OK;
+  }
+  ~PinnedBuf() {
+    if (p) hipHostFree(p);
+  }
+  template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Runs f(k) for k in [0, count) on up to 16 host threads.
+template <typename F>
+void parallel_for(uint32_t count, F&& f) {
+  const unsigned nth = std::max(1u, std::min({16u, std::thread::hardware_concurrency(), count}));
+  if (nth <= 1) {
+    for (uint32_t k = 0; k < count; ++k) f(k);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (unsigned w = 0; w < nth; ++w)
+    th.emplace_back([&, w]() {
+      for (uint32_t k = w; k < count; k += nth) f(k);
+    });
+  for (auto& t : th) t.join();
+}
+
 struct BatchWorkspace {
   std::mutex mtx;
-  DevBuf inst, order, queue, mrp, mci, mv, trp, tci, tv, vecm, vecn, dense, work, out;
+  DevBuf inst, order, queue, mrp, mci, mv, trp, tci, tv, vecm, vecn, bio, lio, cio, xzio, dense, work, out;
+  PinnedBuf h_mrp, h_mci, h_mv, h_trp, h_tci, h_tv, h_bio, h_lio, h_cio, h_xzio;
+  hipEvent_t ev[2] = {nullptr, nullptr};  // brackets the kernel for the verbose timing line
 };
 BatchWorkspace& bws() {
   static BatchWorkspace* w = new BatchWorkspace();
@@ -774,6 +824,14 @@ template <typename V>
 int upload(DevBuf& d, const std::vector<V>& h, hipStream_t s) {
   HN_TRY(d.reserve(sizeof(V) * std::max<size_t>(h.size(), 1)));
   if (!h.empty()) HN_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(V) * h.size(), hipMemcpyHostToDevice, s));
+  return HN_OK;
+}
+
+// Uploads n elements of pinned staging into d (grown as needed).
+template <typename V>
+int upload_pinned(DevBuf& d, const PinnedBuf& h, long long n, hipStream_t s) {
+  HN_TRY(d.reserve(sizeof(V) * std::max<long long>(n, 1)));
+  if (n > 0) HN_HIP(hipMemcpyAsync(d.p, h.p, sizeof(V) * n, hipMemcpyHostToDevice, s));
   return HN_OK;
 }
 
@@ -789,91 +847,124 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
   std::vector<int> st(count, HN_OK);
   std::vector<int> warm_ok(count, 0);
   std::vector<std::string> err(count);
-  {
-    const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    std::vector<std::thread> th;
-    for (unsigned w = 0; w < nth; ++w)
-      th.emplace_back([&, w]() {
-        for (uint32_t k = w; k < count; k += nth) {
-          st[k] = build_standard_form(lps[k], f[k]);
-          if (st[k] != HN_OK) {
-            err[k] = get_error();
-            continue;
-          }
-          if (lps[k].x0 && lps[k].y0 && lps[k].s0)
-            warm_ok[k] = warm_start_acceptable(f[k], lps[k].x0, lps[k].y0, lps[k].s0) ? 1 : 0;
-        }
-      });
-    for (auto& t : th) t.join();
-  }
+  parallel_for(count, [&](uint32_t k) {
+    st[k] = build_standard_form(lps[k], f[k]);
+    if (st[k] != HN_OK) {
+      err[k] = get_error();
+      return;
+    }
+    if (lps[k].x0 && lps[k].y0 && lps[k].s0)
+      warm_ok[k] = warm_start_acceptable(f[k], lps[k].x0, lps[k].y0, lps[k].s0) ? 1 : 0;
+  });
+  const auto t_prep = std::chrono::steady_clock::now();
   for (uint32_t k = 0; k < count; ++k) {
     if (st[k] != HN_OK) return fail(st[k], "batch LP " + std::to_string(k) + ": " + err[k]);
     if (f[k].m == 0 || f[k].n == 0 || lps[k].n == 0)
       return fail(HN_ERR_UNSUPPORTED, "batch LP " + std::to_string(k) + " has an empty standard form");
     if (f[k].m > 20000) return fail(HN_ERR_UNSUPPORTED, "batch LPs are limited to 20000 standard-form rows");
   }
-  // pack
+  // pack: offsets first, then every LP's slices filled in parallel straight
+  // into page-locked staging
   std::vector<Inst> inst(count);
-  std::vector<int> mrp, mci, trp, tci;
-  std::vector<double> mv, tv, vecm, vecn;
-  long long dn = 0, wk = 0;
-  {
-    std::vector<int> t_rp, t_ci;
-    std::vector<double> t_v;
-    for (uint32_t k = 0; k < count; ++k) {
-      const StandardForm& F = f[k];
-      Inst& I = inst[k];
-      I.m = (int)F.m;
-      I.n = (int)F.n;
-      I.mp = (long long)mrp.size();
-      I.mz = (long long)mci.size();
-      for (int64_t v : F.rp) mrp.push_back((int)v);
-      for (int64_t v : F.ci) mci.push_back((int)v);
-      mv.insert(mv.end(), F.v.begin(), F.v.end());
-      host_transpose(F, t_rp, t_ci, t_v);
-      I.tp = (long long)trp.size();
-      I.tz = (long long)tci.size();
-      trp.insert(trp.end(), t_rp.begin(), t_rp.end());
-      tci.insert(tci.end(), t_ci.begin(), t_ci.end());
-      tv.insert(tv.end(), t_v.begin(), t_v.end());
-      I.vm = (long long)vecm.size();
-      vecm.resize(vecm.size() + (size_t)VM_COUNT * F.m, 0.0);
-      std::memcpy(vecm.data() + I.vm + (long long)VM_B * F.m, F.rhs.data(), sizeof(double) * F.m);
-      I.vn = (long long)vecn.size();
-      vecn.resize(vecn.size() + (size_t)VN_COUNT * F.n, 0.0);
-      std::memcpy(vecn.data() + I.vn + (long long)VN_C * F.n, F.c.data(), sizeof(double) * F.n);
-      I.warm = warm_ok[k];
-      if (I.warm) {
-        std::memcpy(vecn.data() + I.vn + (long long)VN_X * F.n, lps[k].x0, sizeof(double) * F.n);
-        std::memcpy(vecn.data() + I.vn + (long long)VN_Z * F.n, lps[k].s0, sizeof(double) * F.n);
-        std::memcpy(vecm.data() + I.vm + (long long)VM_LAM * F.m, lps[k].y0, sizeof(double) * F.m);
-      }
-      I.dn = dn;
-      dn += (long long)F.m * F.m;
-      I.w = wk;
-      wk += (long long)F.m * BK;
-    }
+  long long n_mp = 0, n_mz = 0, n_tp = 0, n_vm = 0, n_vn = 0, n_om = 0, n_on = 0, dn = 0, wk = 0;
+  bool any_warm = false;
+  for (uint32_t k = 0; k < count; ++k) {
+    const StandardForm& F = f[k];
+    Inst& I = inst[k];
+    I.m = (int)F.m;
+    I.n = (int)F.n;
+    I.mp = n_mp;
+    n_mp += F.m + 1;
+    I.mz = n_mz;
+    I.tz = n_mz;
+    n_mz += (long long)F.ci.size();
+    I.tp = n_tp;
+    n_tp += F.n + 1;
+    I.vm = n_vm;
+    n_vm += (long long)(VM_COUNT - kVmIo) * F.m;
+    I.vn = n_vn;
+    n_vn += (long long)(VN_COUNT - kVnIo) * F.n;
+    I.om = n_om;
+    n_om += F.m;
+    I.on = n_on;
+    n_on += F.n;
+    I.dn = dn;
+    dn += (long long)F.m * F.m;
+    I.w = wk;
+    wk += (long long)F.m * BK;
+    I.warm = warm_ok[k];
+    any_warm = any_warm || I.warm;
   }
+  BatchWorkspace& w = bws();
+  std::lock_guard<std::mutex> lk(w.mtx);
+  HN_TRY(w.h_mrp.reserve(sizeof(int) * n_mp));
+  HN_TRY(w.h_mci.reserve(sizeof(int) * std::max(n_mz, 1LL)));
+  HN_TRY(w.h_mv.reserve(sizeof(double) * std::max(n_mz, 1LL)));
+  HN_TRY(w.h_trp.reserve(sizeof(int) * n_tp));
+  HN_TRY(w.h_tci.reserve(sizeof(int) * std::max(n_mz, 1LL)));
+  HN_TRY(w.h_tv.reserve(sizeof(double) * std::max(n_mz, 1LL)));
+  HN_TRY(w.h_bio.reserve(sizeof(double) * n_om));
+  HN_TRY(w.h_lio.reserve(sizeof(double) * n_om));
+  HN_TRY(w.h_cio.reserve(sizeof(double) * n_on));
+  HN_TRY(w.h_xzio.reserve(sizeof(double) * 2 * n_on));
+  {
+    int* mrp = w.h_mrp.as<int>();
+    int* mci = w.h_mci.as<int>();
+    double* mv = w.h_mv.as<double>();
+    int* trp = w.h_trp.as<int>();
+    int* tci = w.h_tci.as<int>();
+    double* tv = w.h_tv.as<double>();
+    double* bio = w.h_bio.as<double>();
+    double* lio = w.h_lio.as<double>();
+    double* cio = w.h_cio.as<double>();
+    double* xzio = w.h_xzio.as<double>();
+    parallel_for(count, [&](uint32_t k) {
+      const StandardForm& F = f[k];
+      const Inst& I = inst[k];
+      for (int64_t i = 0; i <= F.m; ++i) mrp[I.mp + i] = (int)F.rp[i];
+      for (size_t q = 0; q < F.ci.size(); ++q) mci[I.mz + q] = (int)F.ci[q];
+      std::memcpy(mv + I.mz, F.v.data(), sizeof(double) * F.v.size());
+      host_transpose(F, trp + I.tp, tci + I.tz, tv + I.tz);
+      std::memcpy(bio + I.om, F.rhs.data(), sizeof(double) * F.m);
+      std::memcpy(cio + I.on, F.c.data(), sizeof(double) * F.n);
+      if (I.warm) {
+        std::memcpy(xzio + 2 * I.on, lps[k].x0, sizeof(double) * F.n);
+        std::memcpy(xzio + 2 * I.on + F.n, lps[k].s0, sizeof(double) * F.n);
+        std::memcpy(lio + I.om, lps[k].y0, sizeof(double) * F.m);
+      }
+    });
+  }
+  const auto t_pack = std::chrono::steady_clock::now();
   std::vector<int> order(count);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return inst[a].m > inst[b].m; });
-  BatchWorkspace& w = bws();
-  std::lock_guard<std::mutex> lk(w.mtx);
   HN_TRY(upload(w.inst, inst, s));
   HN_TRY(upload(w.order, order, s));
-  HN_TRY(upload(w.mrp, mrp, s));
-  HN_TRY(upload(w.mci, mci, s));
-  HN_TRY(upload(w.mv, mv, s));
-  HN_TRY(upload(w.trp, trp, s));
-  HN_TRY(upload(w.tci, tci, s));
-  HN_TRY(upload(w.tv, tv, s));
-  HN_TRY(upload(w.vecm, vecm, s));
-  HN_TRY(upload(w.vecn, vecn, s));
+  HN_TRY(upload_pinned<int>(w.mrp, w.h_mrp, n_mp, s));
+  HN_TRY(upload_pinned<int>(w.mci, w.h_mci, n_mz, s));
+  HN_TRY(upload_pinned<double>(w.mv, w.h_mv, n_mz, s));
+  HN_TRY(upload_pinned<int>(w.trp, w.h_trp, n_tp, s));
+  HN_TRY(upload_pinned<int>(w.tci, w.h_tci, n_mz, s));
+  HN_TRY(upload_pinned<double>(w.tv, w.h_tv, n_mz, s));
+  HN_TRY(upload_pinned<double>(w.bio, w.h_bio, n_om, s));
+  HN_TRY(upload_pinned<double>(w.cio, w.h_cio, n_on, s));
+  HN_TRY(w.lio.reserve(sizeof(double) * n_om));
+  HN_TRY(w.xzio.reserve(sizeof(double) * 2 * n_on));
+  if (any_warm) {  // cold LPs' entries are overwritten by the kernel's start point
+    HN_HIP(hipMemcpyAsync(w.lio.p, w.h_lio.p, sizeof(double) * n_om, hipMemcpyHostToDevice, s));
+    HN_HIP(hipMemcpyAsync(w.xzio.p, w.h_xzio.p, sizeof(double) * 2 * n_on, hipMemcpyHostToDevice, s));
+  }
+  HN_TRY(w.vecm.reserve(sizeof(double) * std::max(n_vm, 1LL)));
+  HN_TRY(w.vecn.reserve(sizeof(double) * std::max(n_vn, 1LL)));
+  HN_HIP(hipMemsetAsync(w.vecm.p, 0, sizeof(double) * n_vm, s));
+  HN_HIP(hipMemsetAsync(w.vecn.p, 0, sizeof(double) * n_vn, s));
   HN_TRY(w.dense.reserve(sizeof(double) * std::max<long long>(dn, 1)));
   HN_TRY(w.work.reserve(sizeof(double) * std::max<long long>(wk, 1)));
   HN_TRY(w.out.reserve(sizeof(Out) * count));
   HN_TRY(w.queue.reserve(sizeof(int)));
   HN_HIP(hipMemsetAsync(w.queue.p, 0, sizeof(int), s));
+  for (hipEvent_t& e : w.ev)
+    if (!e) HN_HIP(hipEventCreate(&e));
   Params prm;
   prm.max_iterations = o.max_iterations;
   prm.conv_tol = o.convergence_tol;
@@ -884,26 +975,32 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
   prm.skip = o.pivot_rule == HN_PIVOT_SKIP;
   prm.mu_target = o.mu_target;
   const int grid = (int)std::min<uint32_t>(count, 2u * (uint32_t)std::max(1, ctx().info.compute_units));
+  HN_HIP(hipEventRecord(w.ev[0], s));
   hipLaunchKernelGGL(k_batch_ipm, dim3(grid), dim3(NT), 0, s, w.inst.as<Inst>(), w.order.as<int>(), (int)count,
                      w.queue.as<int>(), w.mrp.as<int>(), w.mci.as<int>(), w.mv.as<double>(), w.trp.as<int>(),
                      w.tci.as<int>(), w.tv.as<double>(), w.vecm.as<double>(), w.vecn.as<double>(),
+                     w.bio.as<double>(), w.lio.as<double>(), w.cio.as<double>(), w.xzio.as<double>(),
                      w.dense.as<double>(), w.work.as<double>(), prm, w.out.as<Out>());
   HN_TRY(check_launch("k_batch_ipm"));
   count_launch();
+  HN_HIP(hipEventRecord(w.ev[1], s));
   std::vector<Out> out(count);
   HN_HIP(hipMemcpyAsync(out.data(), w.out.p, sizeof(Out) * count, hipMemcpyDeviceToHost, s));
-  HN_HIP(hipMemcpyAsync(vecm.data(), w.vecm.p, sizeof(double) * vecm.size(), hipMemcpyDeviceToHost, s));
-  HN_HIP(hipMemcpyAsync(vecn.data(), w.vecn.p, sizeof(double) * vecn.size(), hipMemcpyDeviceToHost, s));
+  HN_HIP(hipMemcpyAsync(w.h_lio.p, w.lio.p, sizeof(double) * n_om, hipMemcpyDeviceToHost, s));
+  HN_HIP(hipMemcpyAsync(w.h_xzio.p, w.xzio.p, sizeof(double) * 2 * n_on, hipMemcpyDeviceToHost, s));
   HN_HIP(hipStreamSynchronize(s));
-  const double ms = 1e3 * std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  const auto t_dev = std::chrono::steady_clock::now();
+  const double ms = 1e3 * std::chrono::duration<double>(t_dev - t0).count();
   // results (objective as interior_point.cpp:757-780)
-  for (uint32_t k = 0; k < count; ++k) {
+  const double* lio = w.h_lio.as<double>();
+  const double* xzio = w.h_xzio.as<double>();
+  parallel_for(count, [&](uint32_t k) {
     const StandardForm& F = f[k];
     const Inst& I = inst[k];
     hn_ipm_result& r = res[k];
-    const double* x = vecn.data() + I.vn + (long long)VN_X * I.n;
-    const double* z = vecn.data() + I.vn + (long long)VN_Z * I.n;
-    const double* lam = vecm.data() + I.vm + (long long)VM_LAM * I.m;
+    const double* x = xzio + 2 * I.on;
+    const double* z = x + I.n;
+    const double* lam = lio + I.om;
     std::vector<double> primal(lps[k].n);
     to_original_primal(F, x, primal.data());
     double obj = 0.0;
@@ -944,6 +1041,18 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
     if (r.x && r.cap_n >= F.n) std::memcpy(r.x, x, sizeof(double) * F.n);
     if (r.s && r.cap_n >= F.n) std::memcpy(r.s, z, sizeof(double) * F.n);
     if (r.y && r.cap_m >= F.m) std::memcpy(r.y, lam, sizeof(double) * F.m);
+  });
+  if (o.verbose) {
+    float kms = 0.f;
+    HN_HIP(hipEventElapsedTime(&kms, w.ev[0], w.ev[1]));
+    auto span = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+      return 1e3 * std::chrono::duration<double>(b - a).count();
+    };
+    std::fprintf(stderr,
+                 "[hn batch] %u LPs: standard forms %.1f ms, packing %.1f ms, upload+kernel+download %.1f ms "
+                 "(kernel %.1f ms), results %.1f ms\n",
+                 count, span(t0, t_prep), span(t_prep, t_pack), span(t_pack, t_dev), (double)kms,
+                 span(t_dev, std::chrono::steady_clock::now()));
   }
   return HN_OK;
 }

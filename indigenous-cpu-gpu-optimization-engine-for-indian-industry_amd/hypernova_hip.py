@@ -589,11 +589,12 @@ LIB.hn_ipm_solve_batch.restype = C.c_int
 
 def ipm_solve_batch(lps, precision: int = HN_PREC_F64, max_iterations: int = 100, mu_switch: float = 1e-4,
                     want_points: bool = False, pivot_rule: int = HN_PIVOT_CLAMP, mu_target: float = 0.0,
-                    warm=None):
+                    warm=None, verbose: bool = False):
     """Solves many LPs in one persistent launch (hn_ipm_solve_batch).
     warm: optional list (one entry per LP, or None) of (x0, y0, s0) warm
-    starts of the scaled standard forms.  Returns one InteriorPointResult per
-    LP (solve_time_ms = whole call)."""
+    starts of the scaled standard forms.  verbose: the library prints the
+    time of each phase of the call on stderr.  Returns one
+    InteriorPointResult per LP (solve_time_ms = whole call)."""
     count = len(lps)
     arr = (_HnLp * count)()
     keep = []
@@ -616,6 +617,7 @@ def ipm_solve_batch(lps, precision: int = HN_PREC_F64, max_iterations: int = 100
     opts.mu_switch = mu_switch
     opts.pivot_rule = pivot_rule
     opts.mu_target = mu_target
+    opts.verbose = int(verbose)
     res = (_HnResult * count)()
     bufs = []
     for k, lp in enumerate(lps):

@@ -231,8 +231,10 @@ def predict(net: Network, lps, graph: Optional[Graph] = None):
     out = []
     bufs = None
     for sf in sfs:
-        g = graph if graph is not None and graph.nnz == sf["ci"].size and graph.m == sf["m"] and \
-            np.array_equal(graph.col, sf["ci"]) else Graph(sf["m"], sf["n"], sf["rp"], sf["ci"])
+        if graph is None or graph.nnz != sf["ci"].size or graph.m != sf["m"] or graph.n != sf["n"] or \
+                not np.array_equal(graph.row_ptr, sf["rp"]) or not np.array_equal(graph.col, sf["ci"]):
+            graph = Graph(sf["m"], sf["n"], sf["rp"], sf["ci"])  # kept for the next instance of this structure
+        g = graph
         shape = (sf["m"], sf["n"], sf["ci"].size)
         if bufs is None or bufs[0] != shape:  # device buffers reused across instances
             bufs = (shape, HH.DeviceArray(shape[2]), HH.DeviceArray(shape[0]), HH.DeviceArray(shape[1]),
