@@ -108,7 +108,8 @@ def run_spmv(args) -> dict:
     log("uploaded A and built A' on the device")
     tA = A.autotune(10)
     tT = At.autotune(10)
-    log(f"autotune: A tiling {A.tiling()} ({tA:.1f} us), A' tiling {At.tiling()} ({tT:.1f} us)")
+    log(f"autotune: A tiling {A.tiling()} slab {A.slab()} ({tA:.1f} us), "
+        f"A' tiling {At.tiling()} slab {At.slab()} ({tT:.1f} us)")
 
     x = H.DeviceArray.from_numpy(case.x)
     w = H.DeviceArray.from_numpy(case.w)
@@ -194,6 +195,12 @@ def run_spmv(args) -> dict:
     tb32 = spmv_bytes(case.ncols, case.nrows, case.nnz, 4)
     value = (ab64 + tb64) / (ms_step * 1e-3) / 1e9
     achieved = ab64 / (ms_ax * 1e-3) / 1e9
+    if A.slab():
+        kname = "k_spmv_slab<double,4>"
+        kdesc = f"k_spmv_slab<double,4> (one A x product, slab layout 2^{A.slab()} columns)"
+    else:
+        kname = f"k_spmv_merge<double,{A.tiling()}>"
+        kdesc = "k_spmv_merge<double> + k_spmv_fixup<double> (one A x product)"
     out = {
         "metric": "SpMV GB/s (y=A x and z=A' w, FP64 CSR, algorithmic bytes)",
         "value": round(value, 2),
@@ -208,11 +215,12 @@ def run_spmv(args) -> dict:
         "dtype": "f64",
         "data": "synthetic (seeded generator, seed 2)",
         "config": {"workload": "spmv_powerlaw_2Mx5M_40Mnnz_zipf1.2", "nrows": case.nrows, "ncols": case.ncols,
-                   "nnz": case.nnz, "index_bytes": 4, "tiling_A": A.tiling(), "tiling_At": At.tiling()},
+                   "nnz": case.nnz, "index_bytes": 4, "tiling_A": A.tiling(), "tiling_At": At.tiling(),
+                   "slab_log2_A": A.slab(), "slab_log2_At": At.slab()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(f"k_spmv_merge<double,{A.tiling()}>"),
-                     "kernel": "k_spmv_merge<double> + k_spmv_fixup<double> (one A x product)",
+                     "traffic": measured_traffic(kname),
+                     "kernel": kdesc,
                      "us_per_launch": round(ms_ax * 1e3, 2), "bytes_per_launch": ab64},
         "detail": {
             "ax_f64_us": round(ms_ax * 1e3, 2), "ax_f64_gbs": round(ab64 / (ms_ax * 1e-3) / 1e9, 1),

@@ -103,9 +103,21 @@ int hn_csr_download(const hn_csr* a, double* values, uint64_t* col_indices,
  * 256*items_per_thread.  Valid items_per_thread: 4, 8, 16. */
 int hn_csr_set_tiling(hn_csr* a, int items_per_thread);
 int hn_csr_get_tiling(const hn_csr* a, int* items_per_thread);
-/* Time every valid tiling on this matrix (iters launches each, FP64 and FP32
- * separately when both copies exist) and keep the fastest.  best_us (may be
- * NULL) receives the best FP64 time per product in microseconds. */
+/* Slab layout of SpMV, for matrices whose x is larger than L2 (no reference
+ * counterpart: a storage choice of this library behind hn_spmv).  Rows are
+ * grouped into blocks of at most 2048 rows with balanced nonzeros; inside a
+ * block the nonzeros are stored by column slab (2^slab_log2 columns), then
+ * row, then column.  One workgroup per block walks the slabs in order with
+ * its rows' sums in LDS, so the workgroups of an XCD gather from one
+ * L2-resident slab of x at a time.  slab_log2 in [4, 30]; 0 returns to the
+ * merge path (as does hn_csr_set_tiling).  The layout holds its own copy of
+ * the values. */
+int hn_csr_set_slab(hn_csr* a, int slab_log2);
+int hn_csr_get_slab(const hn_csr* a, int* slab_log2);
+/* Time every valid layout on this matrix (merge path with each valid
+ * items_per_thread; slab layout with 2^16..2^19 columns per slab when x spans
+ * more than one slab; iters launches each) and keep the fastest.  best_us
+ * (may be NULL) receives the best time per product in microseconds. */
 int hn_csr_autotune(hn_csr* a, int iters, double* best_us);
 
 /* ---- drop-in host-buffer operations (IComputeBackend semantics) -------- */

@@ -36,6 +36,7 @@ HIP_FLAGS = [
 KERNEL_SOURCES = [
     "hn_runtime.hip",
     "hn_spmv.hip",
+    "hn_spmv_slab.hip",
     "hn_transpose.hip",
     "hn_vec.hip",
     "hn_stdform.cpp",

@@ -88,7 +88,18 @@ struct Tiling {
   DevBuf carry_row;       // int[ntiles]
   DevBuf carry_val64;     // double[ntiles]
   DevBuf carry_val32;     // float[ntiles]
+  // Slab layout (used instead of the merge path when slab_log2 > 0): row
+  // blocks x column slabs of 2^slab_log2 columns, values copied at build.
+  int slab_log2 = 0, slab_nb = 0, slab_K = 0;
+  DevBuf slab_rb;         // int[nb + 1]: first row of each block
+  DevBuf slab_seg;        // int[nb * K + 1]: start of (block, slab) runs
+  DevBuf slab_col;        // int[nnz]
+  DevBuf slab_row;        // uint16[nnz]: row within the block
+  DevBuf slab_val64;      // double[nnz] (if HN_F64)
+  DevBuf slab_val32;      // float[nnz]  (if HN_F32)
 };
+
+constexpr int kSlabRows = 2048;  // rows per slab-layout block (LDS accumulators)
 
 }  // namespace hn
 
@@ -109,6 +120,12 @@ namespace hn {
 int build_tiling(hn_csr* a, int ipt, hipStream_t s);
 // Allocates an empty CSR of the given shape (values per flags).
 int csr_alloc(hn_csr* a, int64_t nrows, int64_t ncols, int64_t nnz, int flags);
+
+// Builds the slab layout with 2^slab_log2 columns per slab (0: back to the
+// merge path).
+int build_slab(hn_csr* a, int slab_log2, hipStream_t s);
+template <typename V>
+int spmv_slab_launch(const hn_csr* a, const V* x, V* y, hipStream_t s);
 
 // y = A x on device buffers (FP64 or FP32) with the handle's current tiling.
 int spmv_launch(const hn_csr* a, int prec, const void* x, void* y, hipStream_t s);

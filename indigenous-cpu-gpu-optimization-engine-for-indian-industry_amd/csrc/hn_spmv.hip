@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstring>
+#include <vector>
 
 namespace hn {
 
@@ -228,6 +229,7 @@ static int spmv_typed(const hn_csr* a, const V* val, const V* x, V* y, hipStream
                        (long long)a->nrows);
     return check_launch("k_fill");
   }
+  if (a->tiling.slab_log2 > 0) return spmv_slab_launch<V>(a, x, y, s);
   switch (a->tiling.ipt) {
     case 4: return spmv_run<V, 4>(a, val, x, y, s);
     case 8: return spmv_run<V, 8>(a, val, x, y, s);
@@ -420,7 +422,20 @@ extern "C" int hn_csr_set_tiling(hn_csr* a, int items_per_thread) {
   HN_CHECK(a, HN_ERR_INVALID, "hn_csr_set_tiling: null matrix");
   HN_TRY(enter());
   HN_TRY(build_tiling(a, items_per_thread, ctx().stream));
+  a->tiling.slab_log2 = 0;
   HN_HIP(hipStreamSynchronize(ctx().stream));
+  return HN_OK;
+}
+
+extern "C" int hn_csr_set_slab(hn_csr* a, int slab_log2) {
+  HN_CHECK(a, HN_ERR_INVALID, "hn_csr_set_slab: null matrix");
+  HN_TRY(enter());
+  return build_slab(a, slab_log2, ctx().stream);
+}
+
+extern "C" int hn_csr_get_slab(const hn_csr* a, int* slab_log2) {
+  HN_CHECK(a && slab_log2, HN_ERR_INVALID, "hn_csr_get_slab: null argument");
+  *slab_log2 = a->tiling.slab_log2;
   return HN_OK;
 }
 
@@ -444,12 +459,20 @@ extern "C" int hn_csr_autotune(hn_csr* a, int iters, double* best_us) {
   hipEvent_t e0, e1;
   HN_HIP(hipEventCreate(&e0));
   HN_HIP(hipEventCreate(&e1));
-  const int cand[3] = {4, 8, 16};
-  int best = 8;
+  // candidates: merge path with 4, 8, 16 items per thread, then the slab
+  // layout with 2^16 .. 2^19 columns per slab (only when x spans several slabs)
+  struct Cand {
+    int ipt, slab;
+  };
+  std::vector<Cand> cand = {{4, 0}, {8, 0}, {16, 0}};
+  for (int lg = 16; lg <= 19; ++lg)
+    if (a->nnz > 0 && a->nrows > 0 && (a->ncols >> lg) >= 1) cand.push_back({8, lg});
+  Cand best{8, 0};
   float best_ms = 1e30f;
   int st = HN_OK;
-  for (int c = 0; c < 3 && st == HN_OK; ++c) {
-    st = build_tiling(a, cand[c], s);
+  for (size_t c = 0; c < cand.size() && st == HN_OK; ++c) {
+    st = build_tiling(a, cand[c].ipt, s);
+    if (st == HN_OK) st = build_slab(a, cand[c].slab, s);
     if (st == HN_OK) st = spmv_launch(a, prec, x.p, y.p, s);  // warm-up
     if (st != HN_OK) break;
     (void)hipEventRecord(e0, s);
@@ -466,7 +489,8 @@ extern "C" int hn_csr_autotune(hn_csr* a, int iters, double* best_us) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   HN_TRY(st);
-  HN_TRY(build_tiling(a, best, s));
+  HN_TRY(build_tiling(a, best.ipt, s));
+  HN_TRY(build_slab(a, best.slab, s));
   HN_HIP(hipStreamSynchronize(s));
   if (best_us) *best_us = 1000.0 * best_ms / iters;
   return HN_OK;

@@ -65,6 +65,8 @@ def _load_kernels() -> C.CDLL:
         "hn_csr_download": (C.c_int, [_vp, _dp, _u64p, _u64p]),
         "hn_csr_set_tiling": (C.c_int, [_vp, C.c_int]),
         "hn_csr_get_tiling": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+        "hn_csr_set_slab": (C.c_int, [_vp, C.c_int]),
+        "hn_csr_get_slab": (C.c_int, [_vp, C.POINTER(C.c_int)]),
         "hn_csr_autotune": (C.c_int, [_vp, C.c_int, _dp]),
         "hn_spmv": (C.c_int, [_vp, _dp, _dp]),
         "hn_spmm": (C.c_int, [_vp, _dp, _dp, _u64]),
@@ -226,6 +228,15 @@ class CsrMatrix:
     def tiling(self) -> int:
         v = C.c_int()
         check(LIB.hn_csr_get_tiling(self.h, C.byref(v)))
+        return v.value
+
+    def set_slab(self, slab_log2: int) -> None:
+        """Slab layout with 2**slab_log2 columns per slab (0: merge path)."""
+        check(LIB.hn_csr_set_slab(self.h, slab_log2))
+
+    def slab(self) -> int:
+        v = C.c_int()
+        check(LIB.hn_csr_get_slab(self.h, C.byref(v)))
         return v.value
 
     def autotune(self, iters: int = 20) -> float:

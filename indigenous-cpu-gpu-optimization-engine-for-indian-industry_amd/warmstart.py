@@ -220,32 +220,45 @@ def train(net: Network, ds: Dataset, epochs: int, lr: float = 1e-3, lr_final: fl
     return hist
 
 
-def predict(net: Network, lps, graph: Optional[Graph] = None):
+def _same_pattern(sf, m: int, n: int, row_ptr, col) -> bool:
+    return sf["m"] == m and sf["n"] == n and sf["ci"].size == col.size and \
+        np.array_equal(sf["rp"], row_ptr) and np.array_equal(sf["ci"], col)
+
+
+def predict(net: Network, lps, graph: Optional[Graph] = None, chunk: int = 256):
     """Warm starts (x0, y0, s0) of the scaled standard forms of `lps`:
     x0 = exp(log x), y0 = y, s0 = max(c - M'y, exp(log s)), formed on the
     device.  The dual-feasible s0 matters: a start whose mu is small next to
-    its dual residual lets mu collapse first and the dual steps then stall."""
+    its dual residual lets mu collapse first and the dual steps then stall.
+    Runs of up to `chunk` instances with one sparsity pattern share one
+    upload, back-to-back network launches on the library stream and one
+    download."""
     from concurrent.futures import ThreadPoolExecutor
     with ThreadPoolExecutor(max_workers=16) as pool:   # the ctypes calls release the GIL
         sfs = list(pool.map(HH.lp_standard_form, lps))
-    out = []
-    bufs = None
-    for sf in sfs:
-        if graph is None or graph.nnz != sf["ci"].size or graph.m != sf["m"] or graph.n != sf["n"] or \
-                not np.array_equal(graph.row_ptr, sf["rp"]) or not np.array_equal(graph.col, sf["ci"]):
-            graph = Graph(sf["m"], sf["n"], sf["rp"], sf["ci"])  # kept for the next instance of this structure
-        g = graph
-        shape = (sf["m"], sf["n"], sf["ci"].size)
-        if bufs is None or bufs[0] != shape:  # device buffers reused across instances
-            bufs = (shape, HH.DeviceArray(shape[2]), HH.DeviceArray(shape[0]), HH.DeviceArray(shape[1]),
-                    HH.DeviceArray(shape[1]), HH.DeviceArray(shape[0]), HH.DeviceArray(shape[1]))
-        _, val, rhs, c, x0, y0, s0 = bufs
-        val.copy_from(sf["v"])
-        rhs.copy_from(sf["rhs"])
-        c.copy_from(sf["c"])
-        HH.check(_L.hn_gnn_warm_start_dev(net.h, g.h, val.ptr, rhs.ptr, c.ptr, x0.ptr, y0.ptr, s0.ptr, None))
+    out = [None] * len(sfs)
+    k = 0
+    while k < len(sfs):
+        sf = sfs[k]
+        if graph is None or not _same_pattern(sf, graph.m, graph.n, graph.row_ptr, graph.col):
+            graph = Graph(sf["m"], sf["n"], sf["rp"], sf["ci"])  # kept for the next instances of this pattern
+        j = k + 1
+        while j < len(sfs) and j - k < chunk and _same_pattern(sfs[j], graph.m, graph.n, graph.row_ptr, graph.col):
+            j += 1
+        cnt, m, n, nnz = j - k, graph.m, graph.n, graph.nnz
+        val = HH.DeviceArray.from_numpy(np.concatenate([q["v"] for q in sfs[k:j]]))
+        rhs = HH.DeviceArray.from_numpy(np.concatenate([q["rhs"] for q in sfs[k:j]]))
+        c = HH.DeviceArray.from_numpy(np.concatenate([q["c"] for q in sfs[k:j]]))
+        x0, y0, s0 = HH.DeviceArray(cnt * n), HH.DeviceArray(cnt * m), HH.DeviceArray(cnt * n)
+        for q in range(cnt):
+            HH.check(_L.hn_gnn_warm_start_dev(net.h, graph.h, _ptr_at(val, q * nnz), _ptr_at(rhs, q * m),
+                                              _ptr_at(c, q * n), _ptr_at(x0, q * n), _ptr_at(y0, q * m),
+                                              _ptr_at(s0, q * n), None))
         HH.stream_sync()
-        out.append((x0.to_numpy(), y0.to_numpy(), s0.to_numpy()))
+        X, Y, S = x0.to_numpy().reshape(cnt, n), y0.to_numpy().reshape(cnt, m), s0.to_numpy().reshape(cnt, n)
+        for q in range(cnt):
+            out[k + q] = (X[q], Y[q], S[q])
+        k = j
     return out
 
 

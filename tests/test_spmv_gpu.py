@@ -95,6 +95,57 @@ def test_spmv_skewed_rows_span_tiles(hip, ipt):
     close(A.spmv(x), O.spmv(rp, ci, v, x), 1e-12)
 
 
+@pytest.mark.parametrize("slab_log2", [4, 6, 10])
+def test_spmv_slab_layout(hip, golden_backend, slab_log2):
+    # the slab layout on the reference's own cases, FP64 and FP32
+    for name in CASES:
+        nrows, ncols, rp, ci, v, x, y_ref = _case(golden_backend, name)
+        print(f"[case] slab spmv {name} {nrows}x{ncols} nnz={v.size} slab_log2={slab_log2}", flush=True)
+        A = hip.CsrMatrix.upload(v, ci, rp, ncols, hip.HN_F64 | hip.HN_F32)
+        A.set_slab(slab_log2)
+        assert A.slab() == (slab_log2 if v.size and nrows and ncols else 0)
+        close(A.spmv(x), y_ref, 1e-10)
+        dx = hip.DeviceArray.from_numpy(x.astype(np.float32))
+        dy = hip.DeviceArray(nrows, np.float32)
+        A.spmv_dev(dx, dy, hip.HN_F32)
+        ref32 = O.spmv(rp, ci, v.astype(np.float32).astype(np.float64), x.astype(np.float32).astype(np.float64))
+        l1 = O.spmv(rp, ci, np.abs(v), np.abs(x))
+        assert np.all(np.abs(dy.to_numpy().astype(np.float64) - ref32) <= 1e-5 * np.maximum(1.0, l1))
+
+
+@pytest.mark.parametrize("slab_log2", [4, 9, 14])
+def test_spmv_slab_skewed_and_blocks(hip, slab_log2):
+    # rows longer than a chunk of one slab, runs of empty rows, more rows than
+    # one block holds (2048), ncols not a multiple of the slab width
+    rp, ci, v, x = _skewed(11, 9000, 20011, [(3, 9000), (4, 1), (2500, 12000), (2047, 3000), (2048, 5000),
+                                            (8999, 7000)])
+    print(f"[case] slab skewed nnz={v.size} slab_log2={slab_log2}", flush=True)
+    A = hip.CsrMatrix.upload(v, ci, rp, 20011, hip.HN_F64)
+    A.set_slab(slab_log2)
+    ref = O.spmv(rp, ci, v, x)
+    y1 = A.spmv(x)
+    close(y1, ref, 1e-12)
+    assert np.array_equal(A.spmv(x), y1)  # deterministic
+    A.set_tiling(8)  # back to the merge path
+    assert A.slab() == 0
+    close(A.spmv(x), ref, 1e-12)
+
+
+def test_spmv_slab_empty_rows_exact(hip):
+    # test_gpu_backend.cpp:236-258 under the slab layout: exact values
+    n = 5000
+    rp, ci, v = [0], [], []
+    for i in range(n):
+        if i % 4:
+            ci.append(i)
+            v.append(2.0)
+        rp.append(len(ci))
+    A = hip.CsrMatrix.upload(np.array(v), np.array(ci), np.array(rp), n)
+    A.set_slab(5)
+    y = A.spmv(np.full(n, 1.5))
+    assert np.array_equal(y, np.where(np.arange(n) % 4 == 0, 0.0, 3.0))
+
+
 def test_transpose_bit_exact(hip, golden_backend):
     for name in CASES:
         nrows, ncols, rp, ci, v, x, _ = _case(golden_backend, name)
@@ -187,5 +238,5 @@ def test_autotune_keeps_results(hip):
     rp, ci, v, x = _skewed(9, 20000, 30000, [(100, 6000)])
     A = hip.CsrMatrix.upload(v, ci, rp, 30000, hip.HN_F64 | hip.HN_F32)
     us = A.autotune(5)
-    assert us > 0 and A.tiling() in (4, 8, 16)
+    assert us > 0 and A.tiling() in (4, 8, 16) and A.slab() in (0, 16, 17, 18, 19)
     close(A.spmv(x), O.spmv(rp, ci, v, x), 1e-12)
