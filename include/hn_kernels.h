@@ -109,13 +109,13 @@ int hn_csr_get_tiling(const hn_csr* a, int* items_per_thread);
  * block the nonzeros are stored by column slab (2^slab_log2 columns), then
  * row, then column.  One workgroup per block walks the slabs in order with
  * its rows' sums in LDS, so the workgroups of an XCD gather from one
- * L2-resident slab of x at a time.  slab_log2 in [4, 30]; 0 returns to the
+ * L2-resident slab of x at a time.  slab_log2 in [4, 21]; 0 returns to the
  * merge path (as does hn_csr_set_tiling).  The layout holds its own copy of
  * the values. */
 int hn_csr_set_slab(hn_csr* a, int slab_log2);
 int hn_csr_get_slab(const hn_csr* a, int* slab_log2);
 /* Time every valid layout on this matrix (merge path with each valid
- * items_per_thread; slab layout with 2^16..2^19 columns per slab when x spans
+ * items_per_thread; slab layout with 2^14..2^19 columns per slab when x spans
  * more than one slab; iters launches each) and keep the fastest.  best_us
  * (may be NULL) receives the best time per product in microseconds. */
 int hn_csr_autotune(hn_csr* a, int iters, double* best_us);

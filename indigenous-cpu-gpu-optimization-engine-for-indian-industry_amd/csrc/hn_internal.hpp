@@ -93,8 +93,7 @@ struct Tiling {
   int slab_log2 = 0, slab_nb = 0, slab_K = 0;
   DevBuf slab_rb;         // int[nb + 1]: first row of each block
   DevBuf slab_seg;        // int[nb * K + 1]: start of (block, slab) runs
-  DevBuf slab_col;        // int[nnz]
-  DevBuf slab_row;        // uint16[nnz]: row within the block
+  DevBuf slab_pk;         // uint32[nnz]: row within the block << 21 | column within the slab
   DevBuf slab_val64;      // double[nnz] (if HN_F64)
   DevBuf slab_val32;      // float[nnz]  (if HN_F32)
 };

@@ -460,12 +460,12 @@ extern "C" int hn_csr_autotune(hn_csr* a, int iters, double* best_us) {
   HN_HIP(hipEventCreate(&e0));
   HN_HIP(hipEventCreate(&e1));
   // candidates: merge path with 4, 8, 16 items per thread, then the slab
-  // layout with 2^16 .. 2^19 columns per slab (only when x spans several slabs)
+  // layout with 2^14 .. 2^19 columns per slab (only when x spans several slabs)
   struct Cand {
     int ipt, slab;
   };
   std::vector<Cand> cand = {{4, 0}, {8, 0}, {16, 0}};
-  for (int lg = 16; lg <= 19; ++lg)
+  for (int lg = 14; lg <= 19; ++lg)
     if (a->nnz > 0 && a->nrows > 0 && (a->ncols >> lg) >= 1) cand.push_back({8, lg});
   Cand best{8, 0};
   float best_ms = 1e30f;

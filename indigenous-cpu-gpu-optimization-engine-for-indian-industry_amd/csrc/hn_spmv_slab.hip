@@ -16,23 +16,26 @@
 //     balanced number of nonzeros (one workgroup per block);
 //   * columns are cut into K slabs of 2^slab_log2 columns;
 //   * inside a block the nonzeros are stored slab by slab, and inside a
-//     slab in CSR order (row, then column), each with its column (int32),
-//     value and row within the block (uint16);
+//     slab in CSR order (row, then column), each as its value and one
+//     32-bit word (row within the block << 21 | column within the slab);
 //   * seg[b*K + s] is where block b's slab s starts (seg[nb*K] = nnz).
 // Product: the workgroup of block b keeps the block's partial sums in LDS
 // and walks the slabs in order.  Per chunk of up to 256*IPT nonzeros of a
-// slab it stages the products and row indices in LDS; the thread that holds
-// the first entry of each row's run sums the run in order and adds it to
-// the row's LDS accumulator.  Each row has one run per chunk, so every
-// accumulator is updated by one thread at a time, in a fixed order: the
-// result is deterministic.  All blocks walk the slabs in the same order and
-// blocks are dealt round-robin to the XCDs, so the resident workgroups of an
-// XCD share the slab they are on.
+// slab it stages the products and row indices in LDS (entry k of the chunk
+// at k = u*256 + thread, so neighbouring lanes touch neighbouring banks);
+// the thread that holds the first entry of each row's run sums the run in
+// order and adds it to the row's LDS accumulator.  Each row has one run per
+// chunk, so every accumulator is updated by one thread at a time, in a
+// fixed order: the result is deterministic.  Blocks are sized so that all
+// of them are resident at once; all walk the slabs in the same order, so
+// the workgroups of an XCD gather from the same L2-resident slab.  What
+// remains is the gather itself: one L2 access per nonzero.
 #include "hn_device.hpp"
 #include "hn_internal.hpp"
 
 #include <algorithm>
 #include <climits>
+#include <cmath>
 #include <vector>
 
 namespace hn {
@@ -40,9 +43,11 @@ namespace hn {
 namespace {
 
 constexpr int kNT = 256;
+constexpr int kSlabIpt = 4;  // nonzeros per thread and chunk
 constexpr int kRowsPerThread = kSlabRows / kNT;
 static_assert(kSlabRows % kNT == 0, "block rows must split evenly over the threads");
-static_assert(kSlabRows <= 65536, "row within a block is stored as uint16");
+constexpr int kSlabColBits = 21;  // column within a slab; the row within the block takes the upper 11 bits
+static_assert(kSlabRows <= (1 << (32 - kSlabColBits)), "row within a block must fit the packed word");
 
 // Exclusive prefix of v over the workgroup (thread order); *total receives
 // the sum.  sh holds kNT/64 ints.
@@ -75,9 +80,8 @@ __device__ int block_exclusive_scan(int v, int* sh, int* total) {
 __global__ __launch_bounds__(kNT) void k_slab_build(const int* __restrict__ rowptr, const int* __restrict__ col,
                                                     const double* __restrict__ v64, const float* __restrict__ v32,
                                                     const int* __restrict__ rb, int K, int slab_log2,
-                                                    int* __restrict__ seg, int* __restrict__ scol,
-                                                    double* __restrict__ sv64, float* __restrict__ sv32,
-                                                    unsigned short* __restrict__ slrow) {
+                                                    int* __restrict__ seg, unsigned* __restrict__ pk,
+                                                    double* __restrict__ sv64, float* __restrict__ sv32) {
   __shared__ int sh[kNT / kWave];
   const int b = blockIdx.x, tid = threadIdx.x;
   const int r0 = rb[b], r1 = rb[b + 1];
@@ -90,7 +94,8 @@ __global__ __launch_bounds__(kNT) void k_slab_build(const int* __restrict__ rowp
   }
   int off = rowptr[r0];
   for (int s = 0; s < K; ++s) {
-    const long long lim = (long long)(s + 1) << slab_log2;
+    const long long base = (long long)s << slab_log2;
+    const long long lim = base + (1LL << slab_log2);
     int cnt[kRowsPerThread];
     int mine = 0;
 #pragma unroll
@@ -105,12 +110,12 @@ __global__ __launch_bounds__(kNT) void k_slab_build(const int* __restrict__ rowp
     if (tid == 0) seg[(long long)b * K + s] = off;
 #pragma unroll
     for (int q = 0; q < kRowsPerThread; ++q) {
+      const unsigned row_bits = (unsigned)(tid * kRowsPerThread + q) << kSlabColBits;
       for (int e = 0; e < cnt[q]; ++e, ++dst) {
         const int src = cur[q] + e;
-        scol[dst] = col[src];
+        pk[dst] = row_bits | (unsigned)(col[src] - base);
         if (sv64) sv64[dst] = v64[src];
         if (sv32) sv32[dst] = v32[src];
-        slrow[dst] = (unsigned short)(tid * kRowsPerThread + q);
       }
       cur[q] += cnt[q];
     }
@@ -120,10 +125,10 @@ __global__ __launch_bounds__(kNT) void k_slab_build(const int* __restrict__ rowp
 
 template <typename V, int IPT>
 __global__ __launch_bounds__(kNT) void k_spmv_slab(const int* __restrict__ rb, const int* __restrict__ seg,
-                                                   const int* __restrict__ scol, const V* __restrict__ sval,
-                                                   const unsigned short* __restrict__ slrow,
-                                                   const V* __restrict__ x, V* __restrict__ y, int K) {
+                                                   const unsigned* __restrict__ pk, const V* __restrict__ sval,
+                                                   const V* __restrict__ x, V* __restrict__ y, int K, int slab_log2) {
   constexpr int CH = kNT * IPT;
+  constexpr unsigned kColMask = (1u << kSlabColBits) - 1u;
   __shared__ V ys[kSlabRows];
   __shared__ V prod[CH];
   __shared__ unsigned short lr[CH];
@@ -134,23 +139,23 @@ __global__ __launch_bounds__(kNT) void k_spmv_slab(const int* __restrict__ rb, c
   const int* sg = seg + (long long)b * K;
   for (int s = 0; s < K; ++s) {
     const int beg = sg[s], end = sg[s + 1];
+    const V* xs = x + ((long long)s << slab_log2);
     for (int c0 = beg; c0 < end; c0 += CH) {
       const int n = min(CH, end - c0);
 #pragma unroll
       for (int u = 0; u < IPT; ++u) {
         const int k = u * kNT + tid;
         if (k < n) {
-          const int e = c0 + k;
-          const int c = __builtin_nontemporal_load(scol + e);
-          const V v = __builtin_nontemporal_load(sval + e);
-          lr[k] = __builtin_nontemporal_load(slrow + e);
-          prod[k] = v * x[c];
+          const unsigned w = __builtin_nontemporal_load(pk + c0 + k);
+          const V v = __builtin_nontemporal_load(sval + c0 + k);
+          lr[k] = (unsigned short)(w >> kSlabColBits);
+          prod[k] = v * xs[w & kColMask];
         }
       }
       __syncthreads();  // also orders the zeroing of ys before the first adds
 #pragma unroll
       for (int u = 0; u < IPT; ++u) {
-        const int q = tid * IPT + u;
+        const int q = u * kNT + tid;
         if (q < n && (q == 0 || lr[q] != lr[q - 1])) {
           const int key = lr[q];
           V acc = prod[q];
@@ -173,7 +178,7 @@ int build_slab(hn_csr* a, int slab_log2, hipStream_t s) {
     tl.slab_log2 = 0;
     return HN_OK;
   }
-  HN_CHECK(slab_log2 >= 4 && slab_log2 <= 30, HN_ERR_INVALID, "slab_log2 must be 0 or in [4, 30]");
+  HN_CHECK(slab_log2 >= 4 && slab_log2 <= kSlabColBits, HN_ERR_INVALID, "slab_log2 must be 0 or in [4, 21]");
   if (a->nrows == 0 || a->nnz == 0 || a->ncols == 0) {  // nothing to gather: the merge path handles these
     tl.slab_log2 = 0;
     return HN_OK;
@@ -181,10 +186,17 @@ int build_slab(hn_csr* a, int slab_log2, hipStream_t s) {
   std::vector<int> rp(a->nrows + 1);
   HN_HIP(hipMemcpyAsync(rp.data(), a->row_ptr.p, sizeof(int) * (a->nrows + 1), hipMemcpyDeviceToHost, s));
   HN_HIP(hipStreamSynchronize(s));
-  // Blocks of at most kSlabRows rows and about nnz / (6 per CU) nonzeros, so
-  // one round of resident workgroups covers the matrix.
-  const long long slots = 6LL * std::max(1, ctx().info.compute_units);
-  const long long target = std::max<long long>(1024, (a->nnz + slots - 1) / slots);
+  // Blocks of at most kSlabRows rows and about nnz / (0.97 * resident
+  // workgroups) nonzeros, so that every block is resident at once and all
+  // walk the slabs together (a second round would start at slab 0 while the
+  // first is at the end, and the two would evict each other's slabs).
+  int per_cu = 0;
+  if (a->flags & HN_F64)
+    HN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_spmv_slab<double, kSlabIpt>, kNT, 0));
+  else
+    HN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_spmv_slab<float, kSlabIpt>, kNT, 0));
+  const double slots = 0.97 * std::max(1, per_cu) * std::max(1, ctx().info.compute_units);
+  const long long target = std::max<long long>(1024, (long long)std::ceil((double)a->nnz / slots));
   std::vector<int> rb{0};
   for (int64_t r = 0; r < a->nrows;) {
     const int64_t start = r;
@@ -202,8 +214,7 @@ int build_slab(hn_csr* a, int slab_log2, hipStream_t s) {
   HN_CHECK(nb * K < INT_MAX && nb < INT_MAX, HN_ERR_UNSUPPORTED, "slab layout: too many blocks x slabs");
   HN_TRY(tl.slab_rb.reserve(sizeof(int) * (nb + 1)));
   HN_TRY(tl.slab_seg.reserve(sizeof(int) * (nb * K + 1)));
-  HN_TRY(tl.slab_col.reserve(sizeof(int) * a->nnz));
-  HN_TRY(tl.slab_row.reserve(sizeof(unsigned short) * a->nnz));
+  HN_TRY(tl.slab_pk.reserve(sizeof(unsigned) * a->nnz));
   if (a->flags & HN_F64) HN_TRY(tl.slab_val64.reserve(sizeof(double) * a->nnz));
   if (a->flags & HN_F32) HN_TRY(tl.slab_val32.reserve(sizeof(float) * a->nnz));
   HN_HIP(hipMemcpyAsync(tl.slab_rb.p, rb.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, s));
@@ -212,9 +223,9 @@ int build_slab(hn_csr* a, int slab_log2, hipStream_t s) {
   hipLaunchKernelGGL(k_slab_build, dim3((unsigned)nb), dim3(kNT), 0, s, a->row_ptr.as<int>(), a->col.as<int>(),
                      (a->flags & HN_F64) ? a->val64.as<double>() : nullptr,
                      (a->flags & HN_F32) ? a->val32.as<float>() : nullptr, tl.slab_rb.as<int>(), (int)K, slab_log2,
-                     tl.slab_seg.as<int>(), tl.slab_col.as<int>(),
+                     tl.slab_seg.as<int>(), tl.slab_pk.as<unsigned>(),
                      (a->flags & HN_F64) ? tl.slab_val64.as<double>() : nullptr,
-                     (a->flags & HN_F32) ? tl.slab_val32.as<float>() : nullptr, tl.slab_row.as<unsigned short>());
+                     (a->flags & HN_F32) ? tl.slab_val32.as<float>() : nullptr);
   HN_TRY(check_launch("k_slab_build"));
   HN_HIP(hipStreamSynchronize(s));  // rb and last live on this stack frame
   tl.slab_log2 = slab_log2;
@@ -227,9 +238,8 @@ template <typename V>
 int spmv_slab_launch(const hn_csr* a, const V* x, V* y, hipStream_t s) {
   const Tiling& tl = a->tiling;
   const V* sval = std::is_same<V, double>::value ? (const V*)tl.slab_val64.p : (const V*)tl.slab_val32.p;
-  hipLaunchKernelGGL((k_spmv_slab<V, 4>), dim3(tl.slab_nb), dim3(kNT), 0, s, tl.slab_rb.as<int>(),
-                     tl.slab_seg.as<int>(), tl.slab_col.as<int>(), sval, tl.slab_row.as<unsigned short>(), x, y,
-                     tl.slab_K);
+  hipLaunchKernelGGL((k_spmv_slab<V, kSlabIpt>), dim3(tl.slab_nb), dim3(kNT), 0, s, tl.slab_rb.as<int>(),
+                     tl.slab_seg.as<int>(), tl.slab_pk.as<unsigned>(), sval, x, y, tl.slab_K, tl.slab_log2);
   HN_TRY(check_launch("k_spmv_slab"));
   count_launch();
   return HN_OK;

@@ -238,5 +238,5 @@ def test_autotune_keeps_results(hip):
     rp, ci, v, x = _skewed(9, 20000, 30000, [(100, 6000)])
     A = hip.CsrMatrix.upload(v, ci, rp, 30000, hip.HN_F64 | hip.HN_F32)
     us = A.autotune(5)
-    assert us > 0 and A.tiling() in (4, 8, 16) and A.slab() in (0, 16, 17, 18, 19)
+    assert us > 0 and A.tiling() in (4, 8, 16) and A.slab() in (0, 14, 15, 16, 17, 18, 19)
     close(A.spmv(x), O.spmv(rp, ci, v, x), 1e-12)
