@@ -108,8 +108,8 @@ def run_spmv(args) -> dict:
     log("uploaded A and built A' on the device")
     tA = A.autotune(10)
     tT = At.autotune(10)
-    log(f"autotune: A tiling {A.tiling()} slab {A.slab()} ({tA:.1f} us), "
-        f"A' tiling {At.tiling()} slab {At.slab()} ({tT:.1f} us)")
+    log(f"autotune: A tiling {A.tiling()} slab {A.slab()} chunks {A.chunks()} ({tA:.1f} us), "
+        f"A' tiling {At.tiling()} slab {At.slab()} chunks {At.chunks()} ({tT:.1f} us)")
 
     x = H.DeviceArray.from_numpy(case.x)
     w = H.DeviceArray.from_numpy(case.w)
@@ -195,7 +195,11 @@ def run_spmv(args) -> dict:
     tb32 = spmv_bytes(case.ncols, case.nrows, case.nnz, 4)
     value = (ab64 + tb64) / (ms_step * 1e-3) / 1e9
     achieved = ab64 / (ms_ax * 1e-3) / 1e9
-    if A.slab():
+    if A.chunks()[0]:
+        ci, sp, nt = A.chunks()
+        kname = f"k_spmv_chunk<double,{ci},{nt}>"
+        kdesc = f"{kname} (one A x product, chunked stream, chunks spanning <= 2^{sp} columns)"
+    elif A.slab():
         kname = "k_spmv_slab<double,4>"
         kdesc = f"k_spmv_slab<double,4> (one A x product, slab layout 2^{A.slab()} columns)"
     else:
@@ -216,7 +220,8 @@ def run_spmv(args) -> dict:
         "data": "synthetic (seeded generator, seed 2)",
         "config": {"workload": "spmv_powerlaw_2Mx5M_40Mnnz_zipf1.2", "nrows": case.nrows, "ncols": case.ncols,
                    "nnz": case.nnz, "index_bytes": 4, "tiling_A": A.tiling(), "tiling_At": At.tiling(),
-                   "slab_log2_A": A.slab(), "slab_log2_At": At.slab()},
+                   "slab_log2_A": A.slab(), "slab_log2_At": At.slab(),
+                   "chunks_A": list(A.chunks()), "chunks_At": list(At.chunks())},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(kname),

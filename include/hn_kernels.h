@@ -114,10 +114,30 @@ int hn_csr_get_tiling(const hn_csr* a, int* items_per_thread);
  * the values. */
 int hn_csr_set_slab(hn_csr* a, int slab_log2);
 int hn_csr_get_slab(const hn_csr* a, int* slab_log2);
+/* Chunked stream of SpMV (no reference counterpart: a storage choice of this
+ * library behind hn_spmv; it takes precedence over the two layouts above
+ * until hn_csr_set_tiling, hn_csr_set_slab or items_per_thread 0 drops it).
+ * Rows are grouped into blocks of at most 8*threads rows with balanced
+ * nonzeros, sized so that every block is resident at once; each block's
+ * nonzeros, in column order (by slabs of 2^12 columns), are cut into chunks
+ * of at most threads*items_per_thread nonzeros spanning at most
+ * 2^span_log2 columns, and each chunk is stored sorted by row.  One
+ * workgroup of `threads` threads per block takes its chunks in order with
+ * the rows' sums in LDS: each thread gathers items_per_thread nonzeros at
+ * once while the next chunk streams in, and a chunk needs one barrier.
+ * items_per_thread: 0, 4 or 8; span_log2 in [4, 19]; threads 256, 512 or
+ * 1024.  Sums are in a fixed order (deterministic). */
+int hn_csr_set_chunks(hn_csr* a, int items_per_thread, int span_log2, int threads);
+/* items_per_thread, span_log2 and threads of the chunked stream (all 0 when
+ * it is not in use). */
+int hn_csr_get_chunks(const hn_csr* a, int* items_per_thread, int* span_log2, int* threads);
 /* Time every valid layout on this matrix (merge path with each valid
- * items_per_thread; slab layout with 2^14..2^19 columns per slab when x spans
- * more than one slab; iters launches each) and keep the fastest.  best_us
- * (may be NULL) receives the best time per product in microseconds. */
+ * items_per_thread; slab layout with 2^14..2^19 columns per slab and chunked
+ * stream with 256 or 1024 threads, 4 or 8 items per thread and spans
+ * 2^16..2^19, when x spans more than one slab; iters launches each) and keep
+ * the fastest.
+ * best_us (may be NULL) receives the best time per product in
+ * microseconds. */
 int hn_csr_autotune(hn_csr* a, int iters, double* best_us);
 
 /* ---- drop-in host-buffer operations (IComputeBackend semantics) -------- */

@@ -96,6 +96,18 @@ struct Tiling {
   DevBuf slab_pk;         // uint32[nnz]: row within the block << 21 | column within the slab
   DevBuf slab_val64;      // double[nnz] (if HN_F64)
   DevBuf slab_val32;      // float[nnz]  (if HN_F32)
+  // Chunked stream (used when chunk_ipt > 0, ahead of both layouts above):
+  // row blocks of at most 8*chunk_threads rows, each block's nonzeros in
+  // column-slab order cut into chunks of at most chunk_threads*chunk_ipt
+  // nonzeros spanning at most 2^chunk_span columns, each chunk stored sorted
+  // by row.
+  int chunk_ipt = 0, chunk_threads = 256, chunk_span = 0, chunk_nb = 0, chunk_n = 0;
+  DevBuf chunk_rb;        // int[nb + 1]: first row of each block
+  DevBuf chunk_bc;        // int[nb + 1]: first chunk of each block
+  DevBuf chunk_rec;       // int4[n]: (offset, count, first column, 0); offsets are multiples of chunk_ipt
+  DevBuf chunk_pk;        // uint32[padded]: row within the block << 19 | column - first column
+  DevBuf chunk_val64;     // double[padded] (if HN_F64)
+  DevBuf chunk_val32;     // float[padded]  (if HN_F32)
 };
 
 constexpr int kSlabRows = 2048;  // rows per slab-layout block (LDS accumulators)
@@ -125,6 +137,12 @@ int csr_alloc(hn_csr* a, int64_t nrows, int64_t ncols, int64_t nnz, int flags);
 int build_slab(hn_csr* a, int slab_log2, hipStream_t s);
 template <typename V>
 int spmv_slab_launch(const hn_csr* a, const V* x, V* y, hipStream_t s);
+// Builds the chunked stream for workgroups of `threads` (256, 512, 1024)
+// threads, threads*ipt nonzeros per chunk (ipt 4 or 8) and chunks spanning
+// at most 2^span_log2 columns (ipt 0: drops it).
+int build_chunks(hn_csr* a, int ipt, int span_log2, int threads, hipStream_t s);
+template <typename V>
+int spmv_chunk_launch(const hn_csr* a, const V* x, V* y, hipStream_t s);
 
 // y = A x on device buffers (FP64 or FP32) with the handle's current tiling.
 int spmv_launch(const hn_csr* a, int prec, const void* x, void* y, hipStream_t s);

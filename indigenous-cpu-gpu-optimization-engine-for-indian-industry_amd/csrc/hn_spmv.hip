@@ -229,6 +229,7 @@ static int spmv_typed(const hn_csr* a, const V* val, const V* x, V* y, hipStream
                        (long long)a->nrows);
     return check_launch("k_fill");
   }
+  if (a->tiling.chunk_ipt > 0) return spmv_chunk_launch<V>(a, x, y, s);
   if (a->tiling.slab_log2 > 0) return spmv_slab_launch<V>(a, x, y, s);
   switch (a->tiling.ipt) {
     case 4: return spmv_run<V, 4>(a, val, x, y, s);
@@ -423,6 +424,7 @@ extern "C" int hn_csr_set_tiling(hn_csr* a, int items_per_thread) {
   HN_TRY(enter());
   HN_TRY(build_tiling(a, items_per_thread, ctx().stream));
   a->tiling.slab_log2 = 0;
+  a->tiling.chunk_ipt = 0;
   HN_HIP(hipStreamSynchronize(ctx().stream));
   return HN_OK;
 }
@@ -430,7 +432,24 @@ extern "C" int hn_csr_set_tiling(hn_csr* a, int items_per_thread) {
 extern "C" int hn_csr_set_slab(hn_csr* a, int slab_log2) {
   HN_CHECK(a, HN_ERR_INVALID, "hn_csr_set_slab: null matrix");
   HN_TRY(enter());
-  return build_slab(a, slab_log2, ctx().stream);
+  HN_TRY(build_slab(a, slab_log2, ctx().stream));
+  a->tiling.chunk_ipt = 0;
+  return HN_OK;
+}
+
+extern "C" int hn_csr_set_chunks(hn_csr* a, int items_per_thread, int span_log2, int threads) {
+  HN_CHECK(a, HN_ERR_INVALID, "hn_csr_set_chunks: null matrix");
+  HN_TRY(enter());
+  return build_chunks(a, items_per_thread, span_log2, threads, ctx().stream);
+}
+
+extern "C" int hn_csr_get_chunks(const hn_csr* a, int* items_per_thread, int* span_log2, int* threads) {
+  HN_CHECK(a && items_per_thread && span_log2 && threads, HN_ERR_INVALID, "hn_csr_get_chunks: null argument");
+  const bool on = a->tiling.chunk_ipt > 0;
+  *items_per_thread = a->tiling.chunk_ipt;
+  *span_log2 = on ? a->tiling.chunk_span : 0;
+  *threads = on ? a->tiling.chunk_threads : 0;
+  return HN_OK;
 }
 
 extern "C" int hn_csr_get_slab(const hn_csr* a, int* slab_log2) {
@@ -459,20 +478,28 @@ extern "C" int hn_csr_autotune(hn_csr* a, int iters, double* best_us) {
   hipEvent_t e0, e1;
   HN_HIP(hipEventCreate(&e0));
   HN_HIP(hipEventCreate(&e1));
-  // candidates: merge path with 4, 8, 16 items per thread, then the slab
-  // layout with 2^14 .. 2^19 columns per slab (only when x spans several slabs)
+  // candidates: merge path with 4, 8, 16 items per thread; the slab layout
+  // with 2^14 .. 2^19 columns per slab and the chunked stream (256- or
+  // 1024-thread workgroups, 4 or 8 items per thread, chunks spanning 2^16 ..
+  // 2^19 columns), both only when x spans several slabs
   struct Cand {
-    int ipt, slab;
+    int ipt, slab, chunk, span, threads;
   };
-  std::vector<Cand> cand = {{4, 0}, {8, 0}, {16, 0}};
+  std::vector<Cand> cand = {{4, 0, 0, 0, 0}, {8, 0, 0, 0, 0}, {16, 0, 0, 0, 0}};
+  const bool layouts = a->nnz > 0 && a->nrows > 0;
   for (int lg = 14; lg <= 19; ++lg)
-    if (a->nnz > 0 && a->nrows > 0 && (a->ncols >> lg) >= 1) cand.push_back({8, lg});
-  Cand best{8, 0};
+    if (layouts && (a->ncols >> lg) >= 1) cand.push_back({8, lg, 0, 0, 0});
+  for (int nt : {256, 1024})
+    for (int ci : {4, 8})
+      for (int sp = 16; sp <= 19; ++sp)
+        if (layouts && (a->ncols >> 14) >= 1) cand.push_back({8, 0, ci, sp, nt});
+  Cand best{8, 0, 0, 0, 0};
   float best_ms = 1e30f;
   int st = HN_OK;
   for (size_t c = 0; c < cand.size() && st == HN_OK; ++c) {
     st = build_tiling(a, cand[c].ipt, s);
     if (st == HN_OK) st = build_slab(a, cand[c].slab, s);
+    if (st == HN_OK) st = build_chunks(a, cand[c].chunk, cand[c].span, cand[c].threads, s);
     if (st == HN_OK) st = spmv_launch(a, prec, x.p, y.p, s);  // warm-up
     if (st != HN_OK) break;
     (void)hipEventRecord(e0, s);
@@ -491,6 +518,7 @@ extern "C" int hn_csr_autotune(hn_csr* a, int iters, double* best_us) {
   HN_TRY(st);
   HN_TRY(build_tiling(a, best.ipt, s));
   HN_TRY(build_slab(a, best.slab, s));
+  HN_TRY(build_chunks(a, best.chunk, best.span, best.threads, s));
   HN_HIP(hipStreamSynchronize(s));
   if (best_us) *best_us = 1000.0 * best_ms / iters;
   return HN_OK;

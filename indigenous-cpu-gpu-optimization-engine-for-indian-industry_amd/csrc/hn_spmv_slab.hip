@@ -48,6 +48,11 @@ constexpr int kRowsPerThread = kSlabRows / kNT;
 static_assert(kSlabRows % kNT == 0, "block rows must split evenly over the threads");
 constexpr int kSlabColBits = 21;  // column within a slab; the row within the block takes the upper 11 bits
 static_assert(kSlabRows <= (1 << (32 - kSlabColBits)), "row within a block must fit the packed word");
+// chunked stream: up to 8192 rows per block (13 bits), columns relative to
+// the chunk's first column in the low 19 bits
+constexpr int kChunkColBits = 19;
+constexpr int kChunkMaxRows = 1 << (32 - kChunkColBits);
+constexpr int kChunkMaxItems = 8192;  // nonzeros per chunk (13-bit position in the pack sort key)
 
 // Exclusive prefix of v over the workgroup (thread order); *total receives
 // the sum.  sh holds kNT/64 ints.
@@ -73,22 +78,24 @@ __device__ int block_exclusive_scan(int v, int* sh, int* total) {
   return base + inc - v;
 }
 
-// One workgroup per block.  Thread t owns rows [t*8, t*8+8) of the block
-// and keeps a cursor into each; per slab it counts each row's run (columns
-// are sorted within a row, so a run is the next stretch of the row), scans
-// the counts in row order and copies the runs to their places.
+// One workgroup per block.  Thread t owns rows [t*RPT, t*RPT+RPT) of the
+// block and keeps a cursor into each; per slab it counts each row's run
+// (columns are sorted within a row, so a run is the next stretch of the
+// row), scans the counts in row order and copies the runs to their places as
+// (row within the block << shift | column within the slab).
+template <int RPT>
 __global__ __launch_bounds__(kNT) void k_slab_build(const int* __restrict__ rowptr, const int* __restrict__ col,
                                                     const double* __restrict__ v64, const float* __restrict__ v32,
-                                                    const int* __restrict__ rb, int K, int slab_log2,
+                                                    const int* __restrict__ rb, int K, int slab_log2, int shift,
                                                     int* __restrict__ seg, unsigned* __restrict__ pk,
                                                     double* __restrict__ sv64, float* __restrict__ sv32) {
   __shared__ int sh[kNT / kWave];
   const int b = blockIdx.x, tid = threadIdx.x;
   const int r0 = rb[b], r1 = rb[b + 1];
-  int cur[kRowsPerThread], end[kRowsPerThread];
+  int cur[RPT], end[RPT];
 #pragma unroll
-  for (int q = 0; q < kRowsPerThread; ++q) {
-    const int i = r0 + tid * kRowsPerThread + q;
+  for (int q = 0; q < RPT; ++q) {
+    const int i = r0 + tid * RPT + q;
     cur[q] = (i < r1) ? rowptr[i] : 0;
     end[q] = (i < r1) ? rowptr[i + 1] : 0;
   }
@@ -96,10 +103,10 @@ __global__ __launch_bounds__(kNT) void k_slab_build(const int* __restrict__ rowp
   for (int s = 0; s < K; ++s) {
     const long long base = (long long)s << slab_log2;
     const long long lim = base + (1LL << slab_log2);
-    int cnt[kRowsPerThread];
+    int cnt[RPT];
     int mine = 0;
 #pragma unroll
-    for (int q = 0; q < kRowsPerThread; ++q) {
+    for (int q = 0; q < RPT; ++q) {
       int c = 0;
       while (cur[q] + c < end[q] && (long long)col[cur[q] + c] < lim) ++c;
       cnt[q] = c;
@@ -109,8 +116,8 @@ __global__ __launch_bounds__(kNT) void k_slab_build(const int* __restrict__ rowp
     int dst = off + block_exclusive_scan(mine, sh, &total);
     if (tid == 0) seg[(long long)b * K + s] = off;
 #pragma unroll
-    for (int q = 0; q < kRowsPerThread; ++q) {
-      const unsigned row_bits = (unsigned)(tid * kRowsPerThread + q) << kSlabColBits;
+    for (int q = 0; q < RPT; ++q) {
+      const unsigned row_bits = (unsigned)(tid * RPT + q) << shift;
       for (int e = 0; e < cnt[q]; ++e, ++dst) {
         const int src = cur[q] + e;
         pk[dst] = row_bits | (unsigned)(col[src] - base);
@@ -170,6 +177,353 @@ __global__ __launch_bounds__(kNT) void k_spmv_slab(const int* __restrict__ rb, c
   for (int i = tid; i < R; i += kNT) y[r0 + i] = ys[i];
 }
 
+
+// ---- chunked stream ---------------------------------------------------------
+// The per-slab walk above pays one dependent round trip (segment bounds, the
+// matrix stream, the gather, two barriers) per slab for a few hundred
+// nonzeros.  The chunked stream keeps the column-slab order of each block
+// but cuts the block's stream into chunks of up to 256*IPT nonzeros that run
+// across slab boundaries: each thread owns IPT consecutive nonzeros (vector
+// loads), gathers all of them at once, and the chunk needs one barrier.
+// Inside a chunk the nonzeros are sorted by row, so a row forms one run per
+// chunk; runs are summed in registers (within a thread), by a wave64
+// segmented scan (across threads) and a 4-entry cross-wave pass, always in
+// the same order: deterministic.  The packed word holds the row within the
+// block and the column relative to the chunk's first column, so a chunk may
+// span at most 2^21 (or the chosen 2^span) columns.
+
+// One thread per block: cuts [seg[b*K], seg[b*K+K]) into chunks of at most
+// CH nonzeros whose slabs lie in [s, s + span_slabs), s the slab of the
+// chunk's first nonzero.  Block b writes its chunks to rec[cs[b]...] as
+// (stream offset, count, first column of slab s, block) and their number to
+// cnt[b]; cs[] reserves enough slots (see build_chunks).
+__global__ void k_chunk_cut(const int* __restrict__ seg, int nb, int K, int slab_log2, int span_slabs, int CH,
+                            const int* __restrict__ cs, int4* __restrict__ rec, int* __restrict__ cnt) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int* sg = seg + (long long)b * K;
+  int o = sg[0];
+  const int e = sg[K];
+  int s = 0, n = 0;
+  int4* out = rec + cs[b];
+  const int cap = cs[b + 1] - cs[b];
+  while (o < e) {
+    if (n == cap) {  // cannot happen with the host's slot count; reported, never written past
+      cnt[b] = -1;
+      return;
+    }
+    while (sg[s + 1] <= o) ++s;  // seg[s] <= o < seg[s+1]; ends at s <= K-1 since o < seg[K]
+    const int lim = sg[min(K, s + span_slabs)];
+    const int end = min(o + CH, lim);
+    out[n++] = make_int4(o, end - o, s << slab_log2, b);
+    o = end;
+  }
+  cnt[b] = n;
+}
+
+// One workgroup per chunk: sorts the chunk's nonzeros by (row, stream
+// position) in LDS (bitonic) and writes them at the chunk's padded offset
+// as (row << 19 | column - first column) and value.  src = (stream offset,
+// count, first column, block); dst_off = padded output offset.  The stream
+// words are (row << 19 | column within the slab).
+__global__ __launch_bounds__(1024) void k_chunk_pack(const int4* __restrict__ src, const int* __restrict__ dst_off,
+                                                     const int* __restrict__ seg, int K, int slab_log2,
+                                                     const unsigned* __restrict__ spk,
+                                                     const double* __restrict__ sv64, const float* __restrict__ sv32,
+                                                     unsigned* __restrict__ pk, double* __restrict__ v64,
+                                                     float* __restrict__ v32) {
+  __shared__ unsigned key[kChunkMaxItems];
+  const int c = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+  const int4 r = src[c];
+  const int off = r.x, n = r.y, base = r.z, b = r.w;
+  const int out = dst_off[c];
+  int P = 1;
+  while (P < n) P <<= 1;
+  for (int i = tid; i < P; i += nt)
+    key[i] = (i < n) ? (((spk[off + i] >> kChunkColBits) << 13) | (unsigned)i) : 0xFFFFFFFFu;
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = tid; i < P; i += nt) {
+        const int l = i ^ j;
+        if (l > i) {
+          const unsigned ki = key[i], kl = key[l];
+          if ((ki > kl) == ((i & k) == 0)) {
+            key[i] = kl;
+            key[l] = ki;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const int* sg = seg + (long long)b * K;
+  const unsigned in_slab = (1u << slab_log2) - 1u;
+  for (int i = tid; i < n; i += nt) {
+    const unsigned kk = key[i];
+    const int p = off + (int)(kk & 8191u);
+    // slab of stream position p: the largest s with seg[s] <= p
+    int lo = 0, hi = K - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (sg[mid] <= p)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const int colg = (lo << slab_log2) | (int)(spk[p] & in_slab);
+    pk[out + i] = ((kk >> 13) << kChunkColBits) | (unsigned)(colg - base);
+    if (v64) v64[out + i] = sv64[p];
+    if (v32) v32[out + i] = sv32[p];
+  }
+}
+
+typedef unsigned hn_u4 __attribute__((ext_vector_type(4)));
+typedef double hn_d2 __attribute__((ext_vector_type(2)));
+typedef float hn_f4 __attribute__((ext_vector_type(4)));
+
+template <int IPT>
+__device__ __forceinline__ void load_words(const unsigned* p, unsigned (&w)[IPT]) {
+#pragma unroll
+  for (int q = 0; q < IPT / 4; ++q) {
+    const hn_u4 t = __builtin_nontemporal_load(reinterpret_cast<const hn_u4*>(p) + q);
+    w[4 * q + 0] = t.x;
+    w[4 * q + 1] = t.y;
+    w[4 * q + 2] = t.z;
+    w[4 * q + 3] = t.w;
+  }
+}
+
+template <int IPT>
+__device__ __forceinline__ void load_vals(const double* p, double (&v)[IPT]) {
+#pragma unroll
+  for (int q = 0; q < IPT / 2; ++q) {
+    const hn_d2 t = __builtin_nontemporal_load(reinterpret_cast<const hn_d2*>(p) + q);
+    v[2 * q + 0] = t.x;
+    v[2 * q + 1] = t.y;
+  }
+}
+
+template <int IPT>
+__device__ __forceinline__ void load_vals(const float* p, float (&v)[IPT]) {
+#pragma unroll
+  for (int q = 0; q < IPT / 4; ++q) {
+    const hn_f4 t = __builtin_nontemporal_load(reinterpret_cast<const hn_f4*>(p) + q);
+    v[4 * q + 0] = t.x;
+    v[4 * q + 1] = t.y;
+    v[4 * q + 2] = t.z;
+    v[4 * q + 3] = t.w;
+  }
+}
+
+// pk and val are not __restrict__: the compiler would then treat their loads
+// as invariant and move them past the fence that keeps the next chunk's
+// stream ahead of the wait for the gathers.
+template <typename V, int IPT, int NT>
+__global__ __launch_bounds__(NT) void k_spmv_chunk(const int* __restrict__ rb, const int* __restrict__ bc,
+                                                   const int4* __restrict__ rec, const unsigned* pk, const V* val,
+                                                   const V* __restrict__ x, V* __restrict__ y) {
+  constexpr int NW = NT / kWave;
+  constexpr unsigned kColMask = (1u << kChunkColBits) - 1u;
+  extern __shared__ __align__(16) unsigned char hn_chunk_lds[];
+  V* ys = reinterpret_cast<V*>(hn_chunk_lds);  // the block's row sums (8*NT rows at most)
+  // per wave and chunk parity: inclusive tail value at lane 63, whether a
+  // run starts in the wave, the first row of lane 0 and the last of lane 63
+  __shared__ V wv[2][NW];
+  __shared__ int wf[2][NW], wfirst[2][NW], wlast[2][NW];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int lane = tid & (kWave - 1), w = tid / kWave;
+  const int r0 = rb[b];
+  const int R = rb[b + 1] - r0;
+  for (int i = tid; i < R; i += NT) ys[i] = V(0);
+  const int c_beg = bc[b], c_end = bc[b + 1];
+  const int k0 = tid * IPT;
+  // Two register sets, A and B: a chunk's words and values are loaded into
+  // one while the other chunk's gathers are in flight (the loop is unrolled
+  // by two, so no register copy makes the compiler wait for the loads).
+  // Every load is issued unconditionally, so the compiler can wait for the
+  // gathers alone: a thread past the end of a chunk reads the chunk's first
+  // words instead, and words past the end inside a thread are zero padding,
+  // so every gather stays inside the chunk's columns; such values are never
+  // summed.
+  unsigned wdA[IPT], wdB[IPT];
+  V vA[IPT], vB[IPT];
+  int nA = 0, baseA = 0, nB = 0, baseB = 0;
+  if (c_beg < c_end) {
+    const int4 r = rec[c_beg];
+    nA = r.y;
+    baseA = r.z;
+    const int kk = (k0 < nA) ? k0 : 0;
+    load_words<IPT>(pk + r.x + kk, wdA);
+    load_vals<IPT>(val + r.x + kk, vA);
+  }
+  // one chunk: gathers with (wd, v), next chunk's stream into (wd2, v2)
+  auto step = [&](const unsigned (&wd)[IPT], const V (&v)[IPT], int n, int base, unsigned (&wd2)[IPT],
+                  V (&v2)[IPT], int& n2, int& base2, int c) {
+    const int par = (c - c_beg) & 1;
+    const int nv = min(IPT, max(0, n - k0));  // valid nonzeros of this thread (a prefix)
+    V xv[IPT];
+    {
+      const V* xb = x + base;
+#pragma unroll
+      for (int u = 0; u < IPT; ++u) xv[u] = xb[wd[u] & kColMask];
+    }
+    __builtin_amdgcn_sched_barrier(0);  // gathers first, then the next chunk's stream
+    {
+      const bool more = c + 1 < c_end;
+      const int4 r2 = rec[more ? c + 1 : c];
+      n2 = more ? r2.y : 0;
+      base2 = r2.z;
+      const int kk = (k0 < r2.y) ? k0 : 0;
+      load_words<IPT>(pk + r2.x + kk, wd2);
+      load_vals<IPT>(val + r2.x + kk, v2);
+    }
+    // keep the stream loads here, ahead of the wait for the gathers (an
+    // empty asm with a memory clobber stops the optimiser from sinking them;
+    // it reads no register, so it waits for nothing)
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    int row[IPT];
+    V sum[IPT];  // sum[u]: sum of the thread's nonzeros of row[u] up to u
+#pragma unroll
+    for (int u = 0; u < IPT; ++u) {
+      row[u] = (u < nv) ? (int)(wd[u] >> kChunkColBits) : -1;
+      const V p = v[u] * xv[u];
+      sum[u] = (u > 0 && row[u] == row[u - 1]) ? sum[u - 1] + p : p;
+    }
+    int first = -1, last = -1;
+    V tail = V(0);
+    if (nv > 0) first = row[0];
+#pragma unroll
+    for (int u = 0; u < IPT; ++u)
+      if (u == nv - 1) {
+        last = row[u];
+        tail = sum[u];
+      }
+    const bool single = nv > 0 && first == last;
+    // does this thread's first run continue the previous thread's last run?
+    const int prev_last = __shfl_up(last, 1, kWave);
+    bool cont = lane > 0 && nv > 0 && prev_last == first;
+    // segmented scan of the tail runs; lane 0 of a wave assumes it continues
+    // the previous wave when single (its carry is added below or not)
+    int f = (nv == 0 || !single) ? 1 : ((lane == 0) ? 0 : (cont ? 0 : 1));
+    V sv = tail;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const V ov = __shfl_up(sv, off, kWave);
+      const int of = __shfl_up(f, off, kWave);
+      if (lane >= off) {
+        if (!f) sv = ov + sv;
+        f |= of;
+      }
+    }
+    if (lane == kWave - 1) {
+      wv[par][w] = sv;
+      wf[par][w] = f;
+      wlast[par][w] = last;
+    }
+    if (lane == 0) wfirst[par][w] = first;
+    __syncthreads();
+    // tail value accumulated through the end of wave w-1
+    V carry = V(0);
+    for (int q = 0; q < w; ++q) {
+      const bool cq = q > 0 && wfirst[par][q] >= 0 && wlast[par][q - 1] == wfirst[par][q];
+      carry = wf[par][q] ? wv[par][q] : ((cq ? carry : V(0)) + wv[par][q]);
+    }
+    if (lane == 0) cont = w > 0 && nv > 0 && wlast[par][w - 1] == first;
+    const bool wave_cont = w > 0 && wfirst[par][w] >= 0 && wlast[par][w - 1] == wfirst[par][w];
+    const V S = f ? sv : ((wave_cont ? carry : V(0)) + sv);  // inclusive block-wide tail value
+    V S_prev = __shfl_up(S, 1, kWave);
+    if (lane == 0) S_prev = carry;
+    int next_first = __shfl_down(first, 1, kWave);
+    if (lane == kWave - 1) next_first = (w + 1 < NW) ? wfirst[par][w + 1] : -1;
+    if (nv > 0) {
+      bool first_run = true;
+#pragma unroll
+      for (int u = 0; u + 1 < IPT; ++u) {
+        if (u + 1 < nv && row[u + 1] != row[u]) {
+          V r = sum[u];
+          if (first_run && cont) r = S_prev + r;
+          first_run = false;
+          ys[row[u]] += r;
+        }
+      }
+      if (next_first != last) ys[last] += S;
+    }
+  };
+  for (int c = c_beg; c < c_end; c += 2) {
+    step(wdA, vA, nA, baseA, wdB, vB, nB, baseB, c);
+    if (c + 1 < c_end) step(wdB, vB, nB, baseB, wdA, vA, nA, baseA, c + 1);
+  }
+  __syncthreads();  // blocks without chunks still zeroed ys
+  for (int i = tid; i < R; i += NT) y[r0 + i] = ys[i];
+}
+}  // namespace
+
+namespace {
+
+// Row blocks of at most kSlabRows rows and about nnz / (0.97 * resident
+// workgroups) nonzeros, so that every block is resident at once and all walk
+// the columns together (a second round would start at the first columns
+// while the first is at the end, and the two would evict each other's part
+// of x from L2).
+std::vector<int> row_blocks(const std::vector<int>& rp, int64_t nrows, int64_t nnz, int per_cu, int max_rows) {
+  const double slots = 0.97 * std::max(1, per_cu) * std::max(1, ctx().info.compute_units);
+  const long long target = std::max<long long>(1024, (long long)std::ceil((double)nnz / slots));
+  std::vector<int> rb{0};
+  for (int64_t r = 0; r < nrows;) {
+    const int64_t start = r;
+    long long acc = 0;
+    while (r < nrows && r - start < max_rows) {
+      const long long len = rp[r + 1] - rp[r];
+      if (r > start && acc + len > target) break;
+      acc += len;
+      ++r;
+    }
+    rb.push_back((int)r);
+  }
+  return rb;
+}
+
+// Orders the nonzeros of every row block by column slab (then row, then
+// column) into pk/sv64/sv32 as (row within the block << shift | column
+// within the slab), with seg[b*K + s] the start of block b's slab s and
+// seg[nb*K] = nnz.  drb receives rb.  Blocks hold at most 2048 rows
+// (rows_per_thread 8) or 8192 (32).
+int slab_stream(const hn_csr* a, const std::vector<int>& rb, int slab_log2, long long K, int shift,
+                int rows_per_thread, DevBuf& drb, DevBuf& seg, DevBuf& pk, DevBuf& sv64, DevBuf& sv32,
+                hipStream_t s) {
+  const long long nb = (long long)rb.size() - 1;
+  HN_TRY(drb.reserve(sizeof(int) * (nb + 1)));
+  HN_TRY(seg.reserve(sizeof(int) * (nb * K + 1)));
+  HN_TRY(pk.reserve(sizeof(unsigned) * a->nnz));
+  if (a->flags & HN_F64) HN_TRY(sv64.reserve(sizeof(double) * a->nnz));
+  if (a->flags & HN_F32) HN_TRY(sv32.reserve(sizeof(float) * a->nnz));
+  HN_HIP(hipMemcpyAsync(drb.p, rb.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, s));
+  const int last = (int)a->nnz;
+  HN_HIP(hipMemcpyAsync(seg.as<int>() + nb * K, &last, sizeof(int), hipMemcpyHostToDevice, s));
+  const double* v64 = (a->flags & HN_F64) ? a->val64.as<double>() : nullptr;
+  const float* v32 = (a->flags & HN_F32) ? a->val32.as<float>() : nullptr;
+  double* o64 = (a->flags & HN_F64) ? sv64.as<double>() : nullptr;
+  float* o32 = (a->flags & HN_F32) ? sv32.as<float>() : nullptr;
+  if (rows_per_thread == 8)
+    hipLaunchKernelGGL(k_slab_build<8>, dim3((unsigned)nb), dim3(kNT), 0, s, a->row_ptr.as<int>(), a->col.as<int>(),
+                       v64, v32, drb.as<int>(), (int)K, slab_log2, shift, seg.as<int>(), pk.as<unsigned>(), o64, o32);
+  else
+    hipLaunchKernelGGL(k_slab_build<32>, dim3((unsigned)nb), dim3(kNT), 0, s, a->row_ptr.as<int>(), a->col.as<int>(),
+                       v64, v32, drb.as<int>(), (int)K, slab_log2, shift, seg.as<int>(), pk.as<unsigned>(), o64, o32);
+  HN_TRY(check_launch("k_slab_build"));
+  HN_HIP(hipStreamSynchronize(s));  // rb and last live in the caller's frame
+  return HN_OK;
+}
+
+int download_row_ptr(const hn_csr* a, std::vector<int>& rp, hipStream_t s) {
+  rp.resize(a->nrows + 1);
+  HN_HIP(hipMemcpyAsync(rp.data(), a->row_ptr.p, sizeof(int) * (a->nrows + 1), hipMemcpyDeviceToHost, s));
+  HN_HIP(hipStreamSynchronize(s));
+  return HN_OK;
+}
+
 }  // namespace
 
 int build_slab(hn_csr* a, int slab_log2, hipStream_t s) {
@@ -183,51 +537,19 @@ int build_slab(hn_csr* a, int slab_log2, hipStream_t s) {
     tl.slab_log2 = 0;
     return HN_OK;
   }
-  std::vector<int> rp(a->nrows + 1);
-  HN_HIP(hipMemcpyAsync(rp.data(), a->row_ptr.p, sizeof(int) * (a->nrows + 1), hipMemcpyDeviceToHost, s));
-  HN_HIP(hipStreamSynchronize(s));
-  // Blocks of at most kSlabRows rows and about nnz / (0.97 * resident
-  // workgroups) nonzeros, so that every block is resident at once and all
-  // walk the slabs together (a second round would start at slab 0 while the
-  // first is at the end, and the two would evict each other's slabs).
+  std::vector<int> rp;
+  HN_TRY(download_row_ptr(a, rp, s));
   int per_cu = 0;
   if (a->flags & HN_F64)
     HN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_spmv_slab<double, kSlabIpt>, kNT, 0));
   else
     HN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_spmv_slab<float, kSlabIpt>, kNT, 0));
-  const double slots = 0.97 * std::max(1, per_cu) * std::max(1, ctx().info.compute_units);
-  const long long target = std::max<long long>(1024, (long long)std::ceil((double)a->nnz / slots));
-  std::vector<int> rb{0};
-  for (int64_t r = 0; r < a->nrows;) {
-    const int64_t start = r;
-    long long acc = 0;
-    while (r < a->nrows && r - start < kSlabRows) {
-      const long long len = rp[r + 1] - rp[r];
-      if (r > start && acc + len > target) break;
-      acc += len;
-      ++r;
-    }
-    rb.push_back((int)r);
-  }
+  const std::vector<int> rb = row_blocks(rp, a->nrows, a->nnz, per_cu, kSlabRows);
   const long long nb = (long long)rb.size() - 1;
   const long long K = (a->ncols + (1LL << slab_log2) - 1) >> slab_log2;
   HN_CHECK(nb * K < INT_MAX && nb < INT_MAX, HN_ERR_UNSUPPORTED, "slab layout: too many blocks x slabs");
-  HN_TRY(tl.slab_rb.reserve(sizeof(int) * (nb + 1)));
-  HN_TRY(tl.slab_seg.reserve(sizeof(int) * (nb * K + 1)));
-  HN_TRY(tl.slab_pk.reserve(sizeof(unsigned) * a->nnz));
-  if (a->flags & HN_F64) HN_TRY(tl.slab_val64.reserve(sizeof(double) * a->nnz));
-  if (a->flags & HN_F32) HN_TRY(tl.slab_val32.reserve(sizeof(float) * a->nnz));
-  HN_HIP(hipMemcpyAsync(tl.slab_rb.p, rb.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, s));
-  const int last = (int)a->nnz;
-  HN_HIP(hipMemcpyAsync(tl.slab_seg.as<int>() + nb * K, &last, sizeof(int), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_slab_build, dim3((unsigned)nb), dim3(kNT), 0, s, a->row_ptr.as<int>(), a->col.as<int>(),
-                     (a->flags & HN_F64) ? a->val64.as<double>() : nullptr,
-                     (a->flags & HN_F32) ? a->val32.as<float>() : nullptr, tl.slab_rb.as<int>(), (int)K, slab_log2,
-                     tl.slab_seg.as<int>(), tl.slab_pk.as<unsigned>(),
-                     (a->flags & HN_F64) ? tl.slab_val64.as<double>() : nullptr,
-                     (a->flags & HN_F32) ? tl.slab_val32.as<float>() : nullptr);
-  HN_TRY(check_launch("k_slab_build"));
-  HN_HIP(hipStreamSynchronize(s));  // rb and last live on this stack frame
+  HN_TRY(slab_stream(a, rb, slab_log2, K, kSlabColBits, kRowsPerThread, tl.slab_rb, tl.slab_seg, tl.slab_pk,
+                     tl.slab_val64, tl.slab_val32, s));
   tl.slab_log2 = slab_log2;
   tl.slab_nb = (int)nb;
   tl.slab_K = (int)K;
@@ -247,5 +569,175 @@ int spmv_slab_launch(const hn_csr* a, const V* x, V* y, hipStream_t s) {
 
 template int spmv_slab_launch<double>(const hn_csr*, const double*, double*, hipStream_t);
 template int spmv_slab_launch<float>(const hn_csr*, const float*, float*, hipStream_t);
+
+namespace {
+
+template <typename V, int IPT, int NT>
+void* chunk_kernel() {
+  return reinterpret_cast<void*>(&k_spmv_chunk<V, IPT, NT>);
+}
+
+template <typename V>
+void* chunk_kernel_for(int ipt, int nt) {
+  if (ipt == 4) {
+    if (nt == 256) return chunk_kernel<V, 4, 256>();
+    if (nt == 512) return chunk_kernel<V, 4, 512>();
+    return chunk_kernel<V, 4, 1024>();
+  }
+  if (nt == 256) return chunk_kernel<V, 8, 256>();
+  if (nt == 512) return chunk_kernel<V, 8, 512>();
+  return chunk_kernel<V, 8, 1024>();
+}
+
+}  // namespace
+
+int build_chunks(hn_csr* a, int ipt, int span_log2, int threads, hipStream_t s) {
+  Tiling& tl = a->tiling;
+  if (ipt == 0) {
+    tl.chunk_ipt = 0;
+    return HN_OK;
+  }
+  HN_CHECK(ipt == 4 || ipt == 8, HN_ERR_INVALID, "chunk items_per_thread must be 0, 4 or 8");
+  HN_CHECK(span_log2 >= 4 && span_log2 <= kChunkColBits, HN_ERR_INVALID, "chunk span_log2 must be in [4, 19]");
+  HN_CHECK(threads == 256 || threads == 512 || threads == 1024, HN_ERR_INVALID,
+           "chunk threads must be 256, 512 or 1024");
+  if (a->nrows == 0 || a->nnz == 0 || a->ncols == 0) {
+    tl.chunk_ipt = 0;
+    return HN_OK;
+  }
+  std::vector<int> rp;
+  HN_TRY(download_row_ptr(a, rp, s));
+  // rows per block: 8 per thread, summed in LDS
+  const int max_rows = 8 * threads;
+  static_assert(8 * 1024 <= kChunkMaxRows, "rows per block must fit the packed word");
+  const size_t vb = (a->flags & HN_F64) ? sizeof(double) : sizeof(float);
+  const size_t lds = vb * max_rows;
+  int per_cu = 0;
+  HN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, (const void*)((a->flags & HN_F64) ? chunk_kernel_for<double>(ipt, threads)
+                                                 : chunk_kernel_for<float>(ipt, threads)),
+      threads, lds));
+  HN_CHECK(per_cu > 0, HN_ERR_UNSUPPORTED, "chunked layout: kernel does not fit a CU");
+  const std::vector<int> rb = row_blocks(rp, a->nrows, a->nnz, per_cu, max_rows);
+  const long long nb = (long long)rb.size() - 1;
+  // slabs only steer the cut: 2^12 columns, or the span if narrower
+  const int slab_log2 = std::min(span_log2, 12);
+  const long long K = (a->ncols + (1LL << slab_log2) - 1) >> slab_log2;
+  const int span_slabs = 1 << (span_log2 - slab_log2);
+  const int CH = threads * ipt;
+  HN_CHECK(nb * K < INT_MAX, HN_ERR_UNSUPPORTED, "chunked layout: too many blocks x slabs");
+  DevBuf drb, seg, spk, sv64, sv32;
+  HN_TRY(slab_stream(a, rb, slab_log2, K, kChunkColBits, max_rows > kSlabRows ? 32 : 8, drb, seg, spk, sv64, sv32,
+                     s));
+  // Slots per block: each chunk is full (CH), ends a span window (a new
+  // window starts at least span_slabs slabs further) or ends the block.
+  std::vector<int> cs(nb + 1, 0);
+  for (long long b = 0; b < nb; ++b) {
+    const long long nz = rp[rb[b + 1]] - rp[rb[b]];
+    cs[b + 1] = (int)(cs[b] + nz / CH + (K + span_slabs - 1) / span_slabs + 1);
+  }
+  const long long slots = cs[nb];
+  HN_CHECK(slots < INT_MAX / 2, HN_ERR_UNSUPPORTED, "chunked layout: too many chunks");
+  DevBuf dcs, drec, dcnt;
+  HN_TRY(dcs.reserve(sizeof(int) * (nb + 1)));
+  HN_TRY(drec.reserve(sizeof(int4) * slots));
+  HN_TRY(dcnt.reserve(sizeof(int) * nb));
+  HN_HIP(hipMemcpyAsync(dcs.p, cs.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_chunk_cut, dim3((unsigned)((nb + 63) / 64)), dim3(64), 0, s, seg.as<int>(), (int)nb, (int)K,
+                     slab_log2, span_slabs, CH, dcs.as<int>(), drec.as<int4>(), dcnt.as<int>());
+  HN_TRY(check_launch("k_chunk_cut"));
+  std::vector<int4> rec(slots);
+  std::vector<int> cnt(nb);
+  HN_HIP(hipMemcpyAsync(rec.data(), drec.p, sizeof(int4) * slots, hipMemcpyDeviceToHost, s));
+  HN_HIP(hipMemcpyAsync(cnt.data(), dcnt.p, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
+  HN_HIP(hipStreamSynchronize(s));
+  // compact the chunk list; every chunk starts at a multiple of ipt
+  std::vector<int4> src, out;
+  std::vector<int> bc(nb + 1, 0), dst;
+  long long padded = 0;
+  for (long long b = 0; b < nb; ++b) {
+    HN_CHECK(cnt[b] >= 0 && cnt[b] <= cs[b + 1] - cs[b], HN_ERR_UNSUPPORTED, "chunked layout: chunk slots overflow");
+    for (int i = 0; i < cnt[b]; ++i) {
+      const int4 r = rec[cs[b] + i];
+      src.push_back(r);
+      dst.push_back((int)padded);
+      out.push_back(make_int4((int)padded, r.y, r.z, 0));
+      padded += (r.y + ipt - 1) / ipt * ipt;
+    }
+    bc[b + 1] = (int)src.size();
+  }
+  HN_CHECK(padded < INT_MAX, HN_ERR_UNSUPPORTED, "chunked layout: too many nonzeros");
+  const long long nch = (long long)src.size();
+  DevBuf dsrc, ddst;
+  HN_TRY(dsrc.reserve(sizeof(int4) * std::max<long long>(nch, 1)));
+  HN_TRY(ddst.reserve(sizeof(int) * std::max<long long>(nch, 1)));
+  HN_TRY(tl.chunk_rb.reserve(sizeof(int) * (nb + 1)));
+  HN_TRY(tl.chunk_bc.reserve(sizeof(int) * (nb + 1)));
+  HN_TRY(tl.chunk_rec.reserve(sizeof(int4) * std::max<long long>(nch, 1)));
+  HN_TRY(tl.chunk_pk.reserve(sizeof(unsigned) * padded));
+  if (a->flags & HN_F64) HN_TRY(tl.chunk_val64.reserve(sizeof(double) * padded));
+  if (a->flags & HN_F32) HN_TRY(tl.chunk_val32.reserve(sizeof(float) * padded));
+  // padding words read as row 0, column 0 of the chunk (never summed)
+  HN_HIP(hipMemsetAsync(tl.chunk_pk.p, 0, sizeof(unsigned) * padded, s));
+  if (a->flags & HN_F64) HN_HIP(hipMemsetAsync(tl.chunk_val64.p, 0, sizeof(double) * padded, s));
+  if (a->flags & HN_F32) HN_HIP(hipMemsetAsync(tl.chunk_val32.p, 0, sizeof(float) * padded, s));
+  HN_HIP(hipMemcpyAsync(tl.chunk_rb.p, rb.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, s));
+  HN_HIP(hipMemcpyAsync(tl.chunk_bc.p, bc.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, s));
+  if (nch) {
+    HN_HIP(hipMemcpyAsync(tl.chunk_rec.p, out.data(), sizeof(int4) * nch, hipMemcpyHostToDevice, s));
+    HN_HIP(hipMemcpyAsync(dsrc.p, src.data(), sizeof(int4) * nch, hipMemcpyHostToDevice, s));
+    HN_HIP(hipMemcpyAsync(ddst.p, dst.data(), sizeof(int) * nch, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_chunk_pack, dim3((unsigned)nch), dim3(1024), 0, s, dsrc.as<int4>(), ddst.as<int>(),
+                       seg.as<int>(), (int)K, slab_log2, spk.as<unsigned>(),
+                       (a->flags & HN_F64) ? sv64.as<double>() : nullptr,
+                       (a->flags & HN_F32) ? sv32.as<float>() : nullptr, tl.chunk_pk.as<unsigned>(),
+                       (a->flags & HN_F64) ? tl.chunk_val64.as<double>() : nullptr,
+                       (a->flags & HN_F32) ? tl.chunk_val32.as<float>() : nullptr);
+    HN_TRY(check_launch("k_chunk_pack"));
+  }
+  HN_HIP(hipStreamSynchronize(s));  // host vectors and the stream buffers end with this frame
+  tl.chunk_ipt = ipt;
+  tl.chunk_threads = threads;
+  tl.chunk_span = span_log2;
+  tl.chunk_nb = (int)nb;
+  tl.chunk_n = (int)nch;
+  return HN_OK;
+}
+
+template <typename V>
+int spmv_chunk_launch(const hn_csr* a, const V* x, V* y, hipStream_t s) {
+  const Tiling& tl = a->tiling;
+  const V* val = std::is_same<V, double>::value ? (const V*)tl.chunk_val64.p : (const V*)tl.chunk_val32.p;
+  const int* rb = tl.chunk_rb.as<int>();
+  const int* bc = tl.chunk_bc.as<int>();
+  const int4* rec = tl.chunk_rec.as<int4>();
+  const unsigned* pk = tl.chunk_pk.as<unsigned>();
+  const size_t lds = sizeof(V) * 8 * tl.chunk_threads;
+  const dim3 g(tl.chunk_nb), t(tl.chunk_threads);
+#define HN_CHUNK_LAUNCH(IPT, NT) \
+  hipLaunchKernelGGL((k_spmv_chunk<V, IPT, NT>), g, t, lds, s, rb, bc, rec, pk, val, x, y)
+  if (tl.chunk_ipt == 4) {
+    if (tl.chunk_threads == 256)
+      HN_CHUNK_LAUNCH(4, 256);
+    else if (tl.chunk_threads == 512)
+      HN_CHUNK_LAUNCH(4, 512);
+    else
+      HN_CHUNK_LAUNCH(4, 1024);
+  } else {
+    if (tl.chunk_threads == 256)
+      HN_CHUNK_LAUNCH(8, 256);
+    else if (tl.chunk_threads == 512)
+      HN_CHUNK_LAUNCH(8, 512);
+    else
+      HN_CHUNK_LAUNCH(8, 1024);
+  }
+#undef HN_CHUNK_LAUNCH
+  HN_TRY(check_launch("k_spmv_chunk"));
+  count_launch();
+  return HN_OK;
+}
+
+template int spmv_chunk_launch<double>(const hn_csr*, const double*, double*, hipStream_t);
+template int spmv_chunk_launch<float>(const hn_csr*, const float*, float*, hipStream_t);
 
 }  // namespace hn

@@ -67,6 +67,8 @@ def _load_kernels() -> C.CDLL:
         "hn_csr_get_tiling": (C.c_int, [_vp, C.POINTER(C.c_int)]),
         "hn_csr_set_slab": (C.c_int, [_vp, C.c_int]),
         "hn_csr_get_slab": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+        "hn_csr_set_chunks": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+        "hn_csr_get_chunks": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "hn_csr_autotune": (C.c_int, [_vp, C.c_int, _dp]),
         "hn_spmv": (C.c_int, [_vp, _dp, _dp]),
         "hn_spmm": (C.c_int, [_vp, _dp, _dp, _u64]),
@@ -238,6 +240,19 @@ class CsrMatrix:
         v = C.c_int()
         check(LIB.hn_csr_get_slab(self.h, C.byref(v)))
         return v.value
+
+    def set_chunks(self, items_per_thread: int, span_log2: int = 19, threads: int = 256) -> None:
+        """Chunked stream: workgroups of `threads` threads, threads *
+        items_per_thread nonzeros per chunk spanning at most 2**span_log2
+        columns (items_per_thread 0: drop it)."""
+        check(LIB.hn_csr_set_chunks(self.h, items_per_thread, span_log2, threads))
+
+    def chunks(self) -> tuple[int, int, int]:
+        """(items_per_thread, span_log2, threads) of the chunked stream,
+        (0, 0, 0) if off."""
+        v, sp, nt = C.c_int(), C.c_int(), C.c_int()
+        check(LIB.hn_csr_get_chunks(self.h, C.byref(v), C.byref(sp), C.byref(nt)))
+        return v.value, sp.value, nt.value
 
     def autotune(self, iters: int = 20) -> float:
         us = C.c_double()

@@ -146,6 +146,75 @@ def test_spmv_slab_empty_rows_exact(hip):
     assert np.array_equal(y, np.where(np.arange(n) % 4 == 0, 0.0, 3.0))
 
 
+@pytest.mark.parametrize("ipt,span,nt", [(4, 4, 256), (8, 4, 1024), (4, 9, 512), (8, 19, 256), (8, 16, 1024)])
+def test_spmv_chunk_layout(hip, golden_backend, ipt, span, nt):
+    # the chunked stream on the reference's own cases, FP64 and FP32
+    for name in CASES:
+        nrows, ncols, rp, ci, v, x, y_ref = _case(golden_backend, name)
+        print(f"[case] chunk spmv {name} {nrows}x{ncols} nnz={v.size} ipt={ipt} span={span} nt={nt}", flush=True)
+        A = hip.CsrMatrix.upload(v, ci, rp, ncols, hip.HN_F64 | hip.HN_F32)
+        A.set_chunks(ipt, span, nt)
+        assert A.chunks() == ((ipt, span, nt) if v.size and nrows and ncols else (0, 0, 0))
+        close(A.spmv(x), y_ref, 1e-10)
+        dx = hip.DeviceArray.from_numpy(x.astype(np.float32))
+        dy = hip.DeviceArray(nrows, np.float32)
+        A.spmv_dev(dx, dy, hip.HN_F32)
+        ref32 = O.spmv(rp, ci, v.astype(np.float32).astype(np.float64), x.astype(np.float32).astype(np.float64))
+        l1 = O.spmv(rp, ci, np.abs(v), np.abs(x))
+        assert np.all(np.abs(dy.to_numpy().astype(np.float64) - ref32) <= 1e-5 * np.maximum(1.0, l1))
+
+
+@pytest.mark.parametrize("ipt,span,nt", [(4, 4, 256), (8, 10, 256), (4, 16, 1024), (8, 19, 512), (8, 17, 1024)])
+def test_spmv_chunk_skewed_and_blocks(hip, ipt, span, nt):
+    # rows longer than a chunk (9000, 12000 > 2048 nonzeros), runs of empty
+    # rows, more rows than one block holds (2048), ncols not a power of two;
+    # narrow spans force chunks cut by the span window
+    rp, ci, v, x = _skewed(11, 9000, 20011, [(3, 9000), (4, 1), (2500, 12000), (2047, 3000), (2048, 5000),
+                                            (8999, 7000)])
+    print(f"[case] chunk skewed nnz={v.size} ipt={ipt} span={span} nt={nt}", flush=True)
+    A = hip.CsrMatrix.upload(v, ci, rp, 20011, hip.HN_F64 | hip.HN_F32)
+    A.set_chunks(ipt, span, nt)
+    ref = O.spmv(rp, ci, v, x)
+    y1 = A.spmv(x)
+    close(y1, ref, 1e-12)
+    assert np.array_equal(A.spmv(x), y1)  # deterministic
+    dx = hip.DeviceArray.from_numpy(x.astype(np.float32))
+    dy = hip.DeviceArray(9000, np.float32)
+    A.spmv_dev(dx, dy, hip.HN_F32)
+    ref32 = O.spmv(rp, ci, v.astype(np.float32).astype(np.float64), x.astype(np.float32).astype(np.float64))
+    l1 = O.spmv(rp, ci, np.abs(v), np.abs(x))
+    assert np.all(np.abs(dy.to_numpy().astype(np.float64) - ref32) <= 1e-5 * np.maximum(1.0, l1))
+    A.set_tiling(8)  # back to the merge path
+    assert A.chunks() == (0, 0, 0) and A.slab() == 0
+    close(A.spmv(x), ref, 1e-12)
+
+
+def test_spmv_chunk_single_long_row_and_exact(hip):
+    # one row holding every nonzero (runs across all threads, waves and
+    # chunks), then the empty-row diagonal case of test_gpu_backend.cpp:236-258
+    n = 50000
+    rng = np.random.default_rng(5)
+    ci = np.sort(rng.choice(n, 20000, replace=False))
+    v = rng.standard_normal(20000)
+    rp = np.array([0, 20000, 20000])
+    x = rng.uniform(-1, 1, n)
+    A = hip.CsrMatrix.upload(v, ci, rp, n)
+    for ipt, span, nt in ((4, 12, 256), (8, 19, 1024)):
+        A.set_chunks(ipt, span, nt)
+        close(A.spmv(x), O.spmv(rp, ci, v, x), 1e-12)
+    n = 5000
+    rp, ci, v = [0], [], []
+    for i in range(n):
+        if i % 4:
+            ci.append(i)
+            v.append(2.0)
+        rp.append(len(ci))
+    A = hip.CsrMatrix.upload(np.array(v), np.array(ci), np.array(rp), n)
+    A.set_chunks(8, 5, 512)
+    y = A.spmv(np.full(n, 1.5))
+    assert np.array_equal(y, np.where(np.arange(n) % 4 == 0, 0.0, 3.0))
+
+
 def test_transpose_bit_exact(hip, golden_backend):
     for name in CASES:
         nrows, ncols, rp, ci, v, x, _ = _case(golden_backend, name)
@@ -239,4 +308,7 @@ def test_autotune_keeps_results(hip):
     A = hip.CsrMatrix.upload(v, ci, rp, 30000, hip.HN_F64 | hip.HN_F32)
     us = A.autotune(5)
     assert us > 0 and A.tiling() in (4, 8, 16) and A.slab() in (0, 14, 15, 16, 17, 18, 19)
+    ci_, sp_, nt_ = A.chunks()
+    assert (ci_, sp_, nt_) == (0, 0, 0) or (ci_ in (4, 8) and 16 <= sp_ <= 19 and nt_ in (256, 1024))
+    assert not (A.slab() and A.chunks()[0])
     close(A.spmv(x), O.spmv(rp, ci, v, x), 1e-12)
