@@ -106,9 +106,19 @@ def run_spmv(args) -> dict:
     A = H.CsrMatrix.upload(case.val, case.col, case.row_ptr, case.ncols, H.HN_F64 | H.HN_F32)
     At = A.transpose()
     log("uploaded A and built A' on the device")
-    tA = A.autotune(10)
-    tT = At.autotune(10)
-    log(f"autotune: A tiling {A.tiling()} slab {A.slab()} chunks {A.chunks()} ({tA:.1f} us), "
+    # layouts: autotuned, or fixed by --chunks-A / --chunks-At (so that a
+    # profiler run, whose counters distort the autotune's timings, measures
+    # the layout the plain run chose)
+    tA = tT = float("nan")
+    if args.chunks_A:
+        A.set_chunks(*(int(t) for t in args.chunks_A.split(",")))
+    else:
+        tA = A.autotune(10)
+    if args.chunks_At:
+        At.set_chunks(*(int(t) for t in args.chunks_At.split(",")))
+    else:
+        tT = At.autotune(10)
+    log(f"layouts: A tiling {A.tiling()} slab {A.slab()} chunks {A.chunks()} ({tA:.1f} us), "
         f"A' tiling {At.tiling()} slab {At.slab()} chunks {At.chunks()} ({tT:.1f} us)")
 
     x = H.DeviceArray.from_numpy(case.x)
@@ -353,6 +363,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-lp", action="store_true")
+    ap.add_argument("--chunks-A", default="", help="fixed chunked layout of A: items,span_log2,threads")
+    ap.add_argument("--chunks-At", default="", help="fixed chunked layout of A': items,span_log2,threads")
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--seq-sample", type=int, default=64)
     ap.add_argument("--cpu-lp-limit", type=float, default=8.0)

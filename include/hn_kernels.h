@@ -133,8 +133,8 @@ int hn_csr_set_chunks(hn_csr* a, int items_per_thread, int span_log2, int thread
 int hn_csr_get_chunks(const hn_csr* a, int* items_per_thread, int* span_log2, int* threads);
 /* Time every valid layout on this matrix (merge path with each valid
  * items_per_thread; slab layout with 2^14..2^19 columns per slab and chunked
- * stream with 256 or 1024 threads, 4 or 8 items per thread and spans
- * 2^16..2^19, when x spans more than one slab; iters launches each) and keep
+ * stream with 256, 512 or 1024 threads, 4 or 8 items per thread and spans
+ * 2^17..2^19, when x spans more than one slab; iters launches each) and keep
  * the fastest.
  * best_us (may be NULL) receives the best time per product in
  * microseconds. */

@@ -479,9 +479,9 @@ extern "C" int hn_csr_autotune(hn_csr* a, int iters, double* best_us) {
   HN_HIP(hipEventCreate(&e0));
   HN_HIP(hipEventCreate(&e1));
   // candidates: merge path with 4, 8, 16 items per thread; the slab layout
-  // with 2^14 .. 2^19 columns per slab and the chunked stream (256- or
-  // 1024-thread workgroups, 4 or 8 items per thread, chunks spanning 2^16 ..
-  // 2^19 columns), both only when x spans several slabs
+  // with 2^14 .. 2^19 columns per slab and the chunked stream (256-, 512-
+  // or 1024-thread workgroups, 4 or 8 items per thread, chunks spanning
+  // 2^17 .. 2^19 columns), both only when x spans several slabs
   struct Cand {
     int ipt, slab, chunk, span, threads;
   };
@@ -489,9 +489,9 @@ extern "C" int hn_csr_autotune(hn_csr* a, int iters, double* best_us) {
   const bool layouts = a->nnz > 0 && a->nrows > 0;
   for (int lg = 14; lg <= 19; ++lg)
     if (layouts && (a->ncols >> lg) >= 1) cand.push_back({8, lg, 0, 0, 0});
-  for (int nt : {256, 1024})
+  for (int nt : {256, 512, 1024})
     for (int ci : {4, 8})
-      for (int sp = 16; sp <= 19; ++sp)
+      for (int sp = 17; sp <= 19; ++sp)
         if (layouts && (a->ncols >> 14) >= 1) cand.push_back({8, 0, ci, sp, nt});
   Cand best{8, 0, 0, 0, 0};
   float best_ms = 1e30f;
