@@ -253,10 +253,19 @@ def run_spmv(args) -> dict:
     return out
 
 
+# The LP configs' bar: the reference's absolute stopping test (interior_point.cpp
+# :525-530) and, on top, relative KKT residuals and gap <= 1e-8.  Every LP leg
+# below solves with gap_tol = KKT_TOL, so the plain and the chosen solvers are
+# compared at the same bar.
+KKT_TOL = 1e-8
+
+
 def _kkt_max(results) -> dict:
     return {"max_rel_primal": max(r.rel_primal for r in results),
             "max_rel_dual": max(r.rel_dual for r in results),
-            "max_rel_gap": max(r.rel_gap for r in results)}
+            "max_rel_gap": max(r.rel_gap for r in results),
+            "kkt_1e-8": sum(r.rel_primal <= KKT_TOL and r.rel_dual <= KKT_TOL and r.rel_gap <= KKT_TOL
+                            for r in results)}
 
 
 def run_lp(args) -> dict:
@@ -271,12 +280,13 @@ def run_lp(args) -> dict:
     lps = [lpgen.batch_lp(i) for i in range(count)]
     log(f"generated {count} batch LPs (sum m={sum(l.m for l in lps)}) in {time.time() - t0:.1f}s")
     # warm-up: code objects, device arena and page-locked staging sized for this batch
-    H.ipm_solve_batch(lps, pivot_rule=H.HN_PIVOT_SKIP)
-    batch = {"workload": f"batch_{count}_lps_m50-2000_n2.5m_5nnz_per_col", "lps": count}
+    H.ipm_solve_batch(lps, pivot_rule=H.HN_PIVOT_SKIP, gap_tol=KKT_TOL)
+    batch = {"workload": f"batch_{count}_lps_m50-2000_n2.5m_5nnz_per_col", "lps": count,
+             "stop": "reference test (absolute 1e-8) and relative gap <= 1e-8"}
     runs = {}
     for rule, rname in ((H.HN_PIVOT_SKIP, "skip_rule"), (H.HN_PIVOT_CLAMP, "reference_rule")):
         t0 = time.perf_counter()
-        res = H.ipm_solve_batch(lps, precision=H.HN_PREC_F64, pivot_rule=rule, verbose=True)
+        res = H.ipm_solve_batch(lps, precision=H.HN_PREC_F64, pivot_rule=rule, verbose=True, gap_tol=KKT_TOL)
         wall = time.perf_counter() - t0
         status = {}
         for r in res:
@@ -295,9 +305,9 @@ def run_lp(args) -> dict:
     # plain FP64 single-LP GPU solver, sequential, on a strided sample
     stride = max(1, count // args.seq_sample)
     idx = list(range(0, count, stride))[:args.seq_sample]
-    H.ipm_solve(lps[idx[0]])
+    H.ipm_solve(lps[idx[0]], gap_tol=KKT_TOL)
     t0 = time.perf_counter()
-    seq = [H.ipm_solve(lps[i]) for i in idx]
+    seq = [H.ipm_solve(lps[i], gap_tol=KKT_TOL) for i in idx]
     seq_wall = time.perf_counter() - t0
     seq_rate = len(idx) / seq_wall
     seq_status = {}
@@ -306,6 +316,7 @@ def run_lp(args) -> dict:
     batch["sequential_plain"] = {"lps": len(idx), "stride": stride, "wall_s": round(seq_wall, 3),
                                  "lps_per_s": round(seq_rate, 2), "status": seq_status,
                                  "same_status_as_batch": sum(r.status == res[i].status for r, i in zip(seq, idx))}
+    batch["sequential_plain"].update(_kkt_max(seq))
     batch["speedup_vs_sequential_plain"] = {
         k: round(batch[k]["lps_per_s"] / seq_rate, 2) for k in ("skip_rule", "reference_rule")}
     log(f"sequential plain: {len(idx)} LPs in {seq_wall:.2f}s -> {seq_rate:.2f} LPs/s")
@@ -313,13 +324,14 @@ def run_lp(args) -> dict:
 
     # one 2,000 x 5,000 LP, plain FP64 vs mixed precision
     lp = lpgen.random_lp(1, 2000, 5000)
-    H.ipm_solve(lp)
-    single = {"workload": "random_lp_2000x5000_5nnz_per_col_seed1"}
+    H.ipm_solve(lp, gap_tol=KKT_TOL)
+    single = {"workload": "random_lp_2000x5000_5nnz_per_col_seed1",
+              "stop": "reference test (absolute 1e-8) and relative gap <= 1e-8"}
     for name, prec, rule in (("plain_f64", H.HN_PREC_F64, H.HN_PIVOT_CLAMP), ("mixed", H.HN_PREC_MIXED, H.HN_PIVOT_CLAMP),
                              ("skip_rule_f64", H.HN_PREC_F64, H.HN_PIVOT_SKIP),
                              ("skip_rule_mixed", H.HN_PREC_MIXED, H.HN_PIVOT_SKIP)):
         t0 = time.perf_counter()
-        r = H.ipm_solve(lp, precision=prec, pivot_rule=rule)
+        r = H.ipm_solve(lp, precision=prec, pivot_rule=rule, gap_tol=KKT_TOL)
         dt = time.perf_counter() - t0
         single[name] = {"ms": round(1e3 * dt, 2), "status": r.status, "iterations": r.iterations,
                         "objective": r.objective_value, "rel_primal": r.rel_primal, "rel_dual": r.rel_dual,

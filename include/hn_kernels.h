@@ -259,6 +259,10 @@ typedef struct hn_ipm_opts {
   double mu_target;            /* > 0: stop at the first iterate with mu <= mu_target and
                                   return it (status HN_STATUS_ITER_LIMIT unless the 1e-8
                                   test was met first); 0 = off */
+  double gap_tol;              /* > 0: the stopping test (interior_point.cpp:525-530) also
+                                  requires |c'x - b'y| / (1 + |c'x|) <= gap_tol on the
+                                  scaled form, so a solve meets a relative-gap bar and not
+                                  only the absolute mu bar; 0 = off (the reference's test) */
 } hn_ipm_opts;
 
 typedef struct hn_ipm_result {

@@ -72,6 +72,7 @@ struct Params {
   int mixed;
   int skip;  // HN_PIVOT_SKIP: r = reg only, tiny pivots skipped
   double mu_target;  // > 0: stop at the first iterate with mu <= mu_target
+  double gap_tol;    // > 0: the stop test also needs |c'x - b'y| <= gap_tol (1 + |c'x|)
 };
 
 // A skipped pivot: its entry of D becomes "infinite", so its column of L and
@@ -604,8 +605,19 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
     for (int iter = 0; iter < prm.max_iterations; ++iter) {
       residuals();
       if (rpn < prm.conv_tol && rdn < prm.conv_tol && mu < prm.comp_tol) {
-        done = true;
-        break;
+        bool gap_ok = true;
+        if (prm.gap_tol > 0.0) {  // optional relative-gap bar (not in the reference)
+          double cx = 0.0, by = 0.0;
+          for (int j = tid; j < n; j += NT) cx += cc[j] * x[j];
+          for (int i = tid; i < m; i += NT) by += b[i] * lam[i];
+          cx = wg_sum(cx, red);
+          by = wg_sum(by, red);
+          gap_ok = fabs(cx - by) <= prm.gap_tol * (1.0 + fabs(cx));
+        }
+        if (gap_ok) {
+          done = true;
+          break;
+        }
       }
       if (prm.mu_target > 0.0 && mu <= prm.mu_target) {
         stopped = true;
@@ -974,6 +986,7 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
   prm.mixed = o.precision == HN_PREC_MIXED;
   prm.skip = o.pivot_rule == HN_PIVOT_SKIP;
   prm.mu_target = o.mu_target;
+  prm.gap_tol = o.gap_tol;
   const int grid = (int)std::min<uint32_t>(count, 2u * (uint32_t)std::max(1, ctx().info.compute_units));
   HN_HIP(hipEventRecord(w.ev[0], s));
   hipLaunchKernelGGL(k_batch_ipm, dim3(grid), dim3(NT), 0, s, w.inst.as<Inst>(), w.order.as<int>(), (int)count,

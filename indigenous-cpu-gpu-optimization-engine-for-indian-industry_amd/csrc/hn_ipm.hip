@@ -841,8 +841,19 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
     HN_TRY(residuals());
     mu = hS[S_XZ] / (double)n;
     if (hS[S_RPN] < o.convergence_tol && hS[S_RDN] < o.convergence_tol && mu < o.complementarity_tol) {
-      done = true;  // :525-530
-      break;
+      bool gap_ok = true;
+      if (o.gap_tol > 0.0) {  // optional relative-gap bar (not in the reference)
+        hipLaunchKernelGGL(k_red_final, dim3(np_n), dim3(kNT), 0, s, sv.c.as<double>(), sv.x.as<double>(),
+                           sv.b.as<double>(), sv.lam.as<double>(), sv.rd.as<double>(), n, m, sv.part.as<double>());
+        hipLaunchKernelGGL(k_fin_final, dim3(1), dim3(kNT), 0, s, sv.part.as<double>(), np_n, S);
+        HN_TRY(check_launch("gap"));
+        HN_TRY(sv.pull());
+        gap_ok = std::abs(hS[S_CTX] - hS[S_BTY]) <= o.gap_tol * (1.0 + std::abs(hS[S_CTX]));
+      }
+      if (gap_ok) {
+        done = true;  // :525-530
+        break;
+      }
     }
     if (o.mu_target > 0.0 && mu <= o.mu_target) {  // early stop (not in the reference)
       stopped = true;

@@ -480,7 +480,7 @@ class _HnOpts(C.Structure):
                 ("interrupt_user", C.c_void_p), ("solver", C.c_int), ("precision", C.c_int),
                 ("mu_switch", C.c_double), ("dense_max_m", C.c_int64), ("pcg_max_iter", C.c_int),
                 ("x0", _dp), ("y0", _dp), ("s0", _dp), ("verbose", C.c_int), ("pivot_rule", C.c_int),
-                ("mu_target", C.c_double)]
+                ("mu_target", C.c_double), ("gap_tol", C.c_double)]
 
 
 class _HnResult(C.Structure):
@@ -520,6 +520,7 @@ class InteriorPointOptions:
         self.warm_start = None          # (x0, y0, s0) of the scaled standard form
         self.pivot_rule = HN_PIVOT_CLAMP  # HN_PIVOT_SKIP: Wright's skipped pivots, r = regularization
         self.mu_target = 0.0            # > 0: return the first iterate with mu <= mu_target
+        self.gap_tol = 0.0              # > 0: also stop only once |c'x - b'y| / (1 + |c'x|) <= gap_tol
         self.verbose = False
         for k, v in kw.items():
             if not hasattr(self, k):
@@ -590,6 +591,7 @@ class InteriorPointSolver:
         opts.verbose = int(o.verbose)
         opts.pivot_rule = o.pivot_rule
         opts.mu_target = o.mu_target
+        opts.gap_tol = o.gap_tol
         if o.warm_start is not None:
             ws = [np.ascontiguousarray(a, np.float64) for a in o.warm_start]
             keep["ws"] = ws
@@ -615,11 +617,13 @@ LIB.hn_ipm_solve_batch.restype = C.c_int
 
 def ipm_solve_batch(lps, precision: int = HN_PREC_F64, max_iterations: int = 100, mu_switch: float = 1e-4,
                     want_points: bool = False, pivot_rule: int = HN_PIVOT_CLAMP, mu_target: float = 0.0,
-                    warm=None, verbose: bool = False):
+                    warm=None, verbose: bool = False, gap_tol: float = 0.0):
     """Solves many LPs in one persistent launch (hn_ipm_solve_batch).
     warm: optional list (one entry per LP, or None) of (x0, y0, s0) warm
     starts of the scaled standard forms.  verbose: the library prints the
-    time of each phase of the call on stderr.  Returns one
+    time of each phase of the call on stderr.  gap_tol > 0: an instance
+    stops only once its relative gap |c'x - b'y| / (1 + |c'x|) is also at
+    most gap_tol (besides the reference's absolute test).  Returns one
     InteriorPointResult per LP (solve_time_ms = whole call)."""
     count = len(lps)
     arr = (_HnLp * count)()
@@ -643,6 +647,7 @@ def ipm_solve_batch(lps, precision: int = HN_PREC_F64, max_iterations: int = 100
     opts.mu_switch = mu_switch
     opts.pivot_rule = pivot_rule
     opts.mu_target = mu_target
+    opts.gap_tol = gap_tol
     opts.verbose = int(verbose)
     res = (_HnResult * count)()
     bufs = []

@@ -78,3 +78,34 @@ def test_batch_skip_rule_matches_oracle(hip, precision):
         assert abs(r.iterations - ref.iterations) <= max(2, ref.iterations // 10), (k, r.iterations, ref.iterations)
         assert rel(r.objective_value, ref.objective) <= 1e-8, (k, r.objective_value, ref.objective)
         assert r.rel_primal <= 1e-8 and r.rel_dual <= 1e-8
+
+
+def _highs_objective(lp):
+    # the batch generator's LPs are min c'x, Ax = b, x >= 0
+    import scipy.optimize as so
+    import scipy.sparse as sp
+    A = sp.csr_matrix((lp.val, lp.col, lp.row_ptr), shape=(lp.m, lp.n))
+    r = so.linprog(lp.obj, A_eq=A, b_eq=lp.rhs, bounds=[(0, None)] * lp.n, method="highs",
+                   options={"primal_feasibility_tolerance": 1e-10, "dual_feasibility_tolerance": 1e-10})
+    assert r.status == 0
+    return r.fun
+
+
+@pytest.mark.parametrize("pivot_rule,precision", [(0, 1), (1, 1), (1, 2)])
+def test_batch_gap_tol_meets_relative_bar(hip, pivot_rule, precision):
+    # gap_tol (not in the reference): the stop also requires the relative gap
+    # |c'x - b'y| / (1 + |c'x|) <= 1e-8, so every instance meets the 1e-8 KKT
+    # bar on its primal and dual residuals and its gap; the optimum is pinned
+    # by HiGHS (an independent simplex/IPM code) within 3e-8 relative
+    lps = [lpgen.batch_lp(i) for i in range(80)]
+    lps = [lp for lp in lps if lp.m <= 300][:10]
+    print(f"[case] batch gap_tol pivot_rule={pivot_rule} precision={precision} n={len(lps)}", flush=True)
+    out = hip.ipm_solve_batch(lps, precision=precision, pivot_rule=pivot_rule, gap_tol=1e-8)
+    plain = hip.ipm_solve_batch(lps, precision=precision, pivot_rule=pivot_rule)
+    for k, (lp, r, p) in enumerate(zip(lps, out, plain)):
+        assert r.status == "OPTIMAL", (k, r.status)
+        if r.iterations >= 100:  # reference rule: stalled instance, relaxed exit
+            continue
+        assert r.rel_gap <= 1e-8 and r.rel_primal <= 1e-8 and r.rel_dual <= 1e-8, (k, r.rel_gap, r.rel_primal)
+        assert r.iterations >= p.iterations  # the extra bar can only add iterations
+        assert rel(r.objective_value, _highs_objective(lp)) <= 3e-8, k

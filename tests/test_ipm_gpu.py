@@ -200,3 +200,17 @@ def test_ipm_pcg_mixed_precision(hip, name):
     assert r.status == ref.status == "OPTIMAL"
     assert r.cg_iterations > 0 and r.fp64_from_iter > 0
     assert rel(r.objective_value, ref.objective) <= 1e-6
+
+
+@pytest.mark.parametrize("pivot_rule", [0, 1])
+def test_ipm_gap_tol_full_size(hip, pivot_rule):
+    # the 2,000 x 5,000 random LP (seed 1) with the relative-gap bar on top of
+    # the reference's absolute stop
+    lp = lpgen.random_lp(1, 2000, 5000)
+    print(f"[case] gap_tol 2000x5000 pivot_rule={pivot_rule}", flush=True)
+    r = hip.ipm_solve(lp, pivot_rule=pivot_rule, gap_tol=1e-8)
+    p = hip.ipm_solve(lp, pivot_rule=pivot_rule)
+    assert r.status == p.status == "OPTIMAL"
+    assert r.rel_gap <= 1e-8 and r.rel_primal <= 1e-8 and r.rel_dual <= 1e-8
+    assert r.iterations >= p.iterations
+    assert abs(r.objective_value - p.objective_value) <= 1e-7 * max(1.0, abs(p.objective_value))
