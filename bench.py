@@ -339,6 +339,23 @@ def run_lp(args) -> dict:
         log(f"single {name}: {r.status} it={r.iterations} {1e3 * dt:.1f} ms")
     out["single"] = single
 
+    # learned warm start: the 200 x 300 transportation family, the committed
+    # trained network on 1,000 held-out seeds, cold vs warm batched solves
+    import warmstart as WS
+    net = WS.Network(0)
+    net.set_params(np.load(os.path.join(PKG, "data", "warmstart_transport_200x300.npy")))
+    # This degenerate family (rank m-1) stalls in its dual steps once mu is
+    # near 1e-8 and does not reach a relative gap of 1e-8 within 100
+    # iterations, so both solves stop at the reference's test; the relative
+    # gaps they reach are reported.
+    ev = WS.evaluate(net, WS.transport_family(), range(1_000_000, 1_000_000 + args.ws_test),
+                     log=lambda *a: None)
+    ev["workload"] = f"transport_200x300_heldout_{args.ws_test}"
+    ev["stop"] = "reference test (absolute 1e-8 on rp, rd, mu)"
+    out["warm_start"] = ev
+    log(f"warm start: iterations {ev['cold_iterations_mean']:.2f} cold -> {ev['warm_iterations_mean']:.2f} warm, "
+        f"wall {ev['cold_wall_s']:.2f} s -> {ev['warm_wall_s']:.2f} s (inference {ev['predict_s']:.2f} s)")
+
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O  # baseline only
@@ -364,6 +381,17 @@ def run_lp(args) -> dict:
             "sample": (f"first {seen} instances of the strided batch sample, {args.cpu_lp_limit:.0f} s limit per "
                        f"instance; {limited} stopped at the limit and count as unsolved with their time charged")}
         log(f"cpu lp baseline {out['cpu_baseline']}")
+        # the reference IPM, cold start, on the first held-out transportation LPs
+        tl = []
+        for sd in range(1_000_000, 1_000_002):
+            t0 = time.perf_counter()
+            r = solve(lpgen.transport_lp(sd, 200, 300), time_limit=args.cpu_lp_limit)
+            tl.append((time.perf_counter() - t0, r.status, r.iterations))
+        out["warm_start"]["cpu_baseline"] = {
+            "value": round(len(tl) / sum(t for t, _, _ in tl), 4), "unit": "LPs/s", "cores": 1, "kind": kind,
+            "sample": (f"{len(tl)} held-out 200x300 transportation LPs, cold start, {args.cpu_lp_limit:.0f} s "
+                       f"limit each; status/iterations " + ", ".join(f"{st}/{it}" for _, st, it in tl))}
+        log(f"cpu warm-start-family baseline {out['warm_start']['cpu_baseline']}")
     return out
 
 
@@ -380,6 +408,7 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--seq-sample", type=int, default=64)
     ap.add_argument("--cpu-lp-limit", type=float, default=8.0)
+    ap.add_argument("--ws-test", type=int, default=1000, help="held-out transportation LPs of the warm-start leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is described in "

@@ -319,10 +319,21 @@ __device__ __forceinline__ void tile_zero(T (&acc)[4][4]) {
     for (int c = 0; c < 4; ++c) acc[a][c] = T(0);
 }
 
+// Value of v in lane `lane` (a wave-uniform lane index), through readlane.
+__device__ __forceinline__ double read_lane(double v, int lane) {
+  const long long w = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)w, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(w >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float read_lane(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
 // (1) LDL' of the diagonal block A[k0:k0+b, k0:k0+b] by one wave, the block
-// resident in LDS (lane j owns column j): step k scales column k below the
-// pivot (lane i scales row i) and lane j > k applies a_ij -= l_i a_kj down
-// its column.  Writes the strictly lower part of L, dinv[k0:k0+b] (1/pivot;
+// in registers (lane j owns column j): step k forms the multipliers l_i of
+// column k (lane i its own) and lane j > k applies a_ij -= l_i a_kj down its
+// column.  Writes the strictly lower part of L, dinv[k0:k0+b] (1/pivot;
 // 0 for a skipped pivot) and X = L_KK^{-1} as the full 64 x 64 diagonal
 // block of G = L^{-1} (zeros above).  Pivots p <= tol are clamped to tol
 // (skip_rel == 0) or, with skip_rel > 0, p <= skip_rel * diag0 is skipped.
@@ -333,41 +344,81 @@ __global__ __launch_bounds__(kWave) void k_ldl_diag(T* __restrict__ A, int m, in
   __shared__ T S[kNB][kNB + 1];
   __shared__ T Xs[kNB][kNB + 1];
   __shared__ T Ts[3 * 256];
+  __shared__ T Lk[kNB];  // the scaled column k of the current step
   const int j = threadIdx.x;
-  for (int i = 0; i < kNB; ++i) {
-    const int hi = i > j ? i : j, lo = i > j ? j : i;
-    S[i][j] = (i < b && j < b) ? A[(long long)(k0 + hi) * m + k0 + lo] : T(i == j ? 1 : 0);
+  // The block's lower triangle, row by row (lane j reads column j of row i:
+  // coalesced), mirrored into the upper half; rows past b are the identity.
+  // The loads carry no branch (addresses are clamped into the block's lower
+  // triangle, unused values dropped), so 16 rows are in flight at once.
+  for (int i0 = 0; i0 < kNB; i0 += 16) {
+    T v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int ii = min(i0 + u, b - 1);
+      const int jj = min(j, ii);
+      v[u] = A[(long long)(k0 + ii) * m + k0 + jj];
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int i = i0 + u;
+      if (j <= i) {
+        const T w = (i < b) ? v[u] : T(i == j ? 1 : 0);
+        S[i][j] = w;
+        S[j][i] = w;
+      }
+    }
   }
   __syncthreads();
+  // Elimination in registers: lane j holds column j (c[i] = S[i][j]); the
+  // k loop is unrolled so every register index is static.  The trailing
+  // block stays symmetric, so lane i's c[k] = S[k][i] = S[i][k] gives lane i
+  // its own multiplier l_i = S[i][k] / d_k, which it keeps in c[k] (lane i
+  // ends holding row i of L); the multipliers reach the other lanes through
+  // one LDS vector per step and the pivot through readlane.
+  T c[kNB];
+#pragma unroll
+  for (int i = 0; i < kNB; ++i) c[i] = S[i][j];
+  const double d0j = (skip_rel > 0.0 && j < b) ? diag0[k0 + j] : 1.0;
   int ncl = 0;
-  for (int k = 0; k < b; ++k) {
-    T piv = S[k][k];
-    T rinv;
-    if (skip_rel > 0.0) {
-      if (!((double)piv > skip_rel * diag0[k0 + k])) {
-        rinv = T(0);
-        ++ncl;
+#pragma unroll
+  for (int k = 0; k < kNB; ++k) {
+    if (k < b) {
+      T piv = read_lane(c[k], k);
+      T rinv;
+      if (skip_rel > 0.0) {
+        if (!((double)piv > skip_rel * read_lane(d0j, k))) {
+          rinv = T(0);
+          ++ncl;
+        } else {
+          rinv = T(1) / piv;
+        }
       } else {
+        if (piv <= T(tol)) {
+          piv = T(tol);
+          ++ncl;
+        }
         rinv = T(1) / piv;
       }
-    } else {
-      if (piv <= T(tol)) {
-        piv = T(tol);
-        ++ncl;
+      if (j == 0) dinv[k0 + k] = rinv;
+      const T akj = c[k];  // S[k][j]
+      const T l = (j > k) ? akj * rinv : T(0);
+      Lk[j] = l;
+      __syncthreads();
+      if (j > k) {
+#pragma unroll
+        for (int i = k + 1; i < kNB; ++i) c[i] -= Lk[i] * akj;
+        c[k] = l;
       }
-      rinv = T(1) / piv;
+      __syncthreads();  // Lk is rewritten by the next step
     }
-    if (j == 0) dinv[k0 + k] = rinv;
-    if (j > k) S[j][k] = S[j][k] * rinv;
-    __syncthreads();
-    if (j > k) {
-      const T akj = S[k][j];
-#pragma unroll 8
-      for (int i = k + 1; i < kNB; ++i) S[i][j] -= S[i][k] * akj;
-    }
-    __syncthreads();
   }
-  for (int i = j + 1; i < b; ++i) A[(long long)(k0 + i) * m + k0 + j] = S[i][j];
+  // S takes L (lane j writes row j: L[j][k] for k < j) for the write-back
+  // and for X below; the entries on and above the diagonal are not read.
+#pragma unroll
+  for (int k = 0; k < kNB; ++k) S[j][k] = c[k];
+  __syncthreads();
+  for (int i = 1; i < b; ++i)  // L below the diagonal, row by row (coalesced)
+    if (j < i) A[(long long)(k0 + i) * m + k0 + j] = S[i][j];
   if (j == 0 && ncl) atomicAdd(clamped, ncl);
   // X = L_KK^{-1} by 16 x 16 blocks: the four diagonal blocks by forward
   // substitution (a column per lane, in registers), then block rows

@@ -263,20 +263,21 @@ def predict(net: Network, lps, graph: Optional[Graph] = None, chunk: int = 256):
 
 
 def evaluate(net: Network, make_lp, seeds: Sequence[int], graph: Optional[Graph] = None,
-             pivot_rule: int = HH.HN_PIVOT_SKIP, log=print) -> dict:
+             pivot_rule: int = HH.HN_PIVOT_SKIP, log=print, gap_tol: float = 0.0) -> dict:
     """Cold vs warm-started batched solves of the held-out seeds: iterations,
     wall time (the warm time includes the network's inference), acceptance
-    of the predictions, and agreement of the two optima."""
+    of the predictions, and agreement of the two optima.  gap_tol > 0: both
+    solves also stop only at a relative gap <= gap_tol (see hn_ipm_opts)."""
     lps = [make_lp(int(s)) for s in seeds]
-    HH.ipm_solve_batch(lps[:4], pivot_rule=pivot_rule)  # warm-up
+    HH.ipm_solve_batch(lps[:4], pivot_rule=pivot_rule, gap_tol=gap_tol)  # warm-up
     t0 = time.perf_counter()
-    cold = HH.ipm_solve_batch(lps, pivot_rule=pivot_rule)
+    cold = HH.ipm_solve_batch(lps, pivot_rule=pivot_rule, gap_tol=gap_tol)
     t_cold = time.perf_counter() - t0
     t0 = time.perf_counter()
     warm_pts = predict(net, lps, graph)
     t_pred = time.perf_counter() - t0
     t0 = time.perf_counter()
-    warm = HH.ipm_solve_batch(lps, pivot_rule=pivot_rule, warm=warm_pts)
+    warm = HH.ipm_solve_batch(lps, pivot_rule=pivot_rule, warm=warm_pts, gap_tol=gap_tol)
     t_warm = time.perf_counter() - t0
     ci = np.array([r.iterations for r in cold])
     wi = np.array([r.iterations for r in warm])
@@ -296,6 +297,11 @@ def evaluate(net: Network, make_lp, seeds: Sequence[int], graph: Optional[Graph]
         "cold_wall_s": round(t_cold, 4), "warm_wall_s": round(t_warm + t_pred, 4),
         "predict_s": round(t_pred, 4), "warm_solve_s": round(t_warm, 4),
         "iteration_ratio": float(ci.mean() / max(wi.mean(), 1e-9)),
+        "gap_tol": gap_tol,
+        "cold_max_rel_gap": float(max(r.rel_gap for r in cold)),
+        "warm_max_rel_gap": float(max(r.rel_gap for r in warm)),
+        "cold_max_rel_primal": float(max(r.rel_primal for r in cold)),
+        "warm_max_rel_primal": float(max(r.rel_primal for r in warm)),
     }
     log(f"[warmstart] eval {out}")
     return out
