@@ -364,6 +364,18 @@ int hn_gnn_gradients_dev(hn_gnn* g, hn_gnn_graph* gr, const double* val, const d
 int hn_gnn_train_step_dev(hn_gnn* g, hn_gnn_graph* gr, const double* val, const double* rhs, const double* c,
                           const float* tx, const float* ts, const float* ty, float lr, float* loss_dev,
                           void* stream);
+/* hn_ipm_solve_batch with learned starts (no reference counterpart: a
+ * caller that solves many LPs of one family, e.g. the transportation family,
+ * gets its warm starts without a host round trip).  Every LP without its own
+ * x0/y0/s0 whose scaled standard form has exactly the graph's pattern is
+ * started from the network's prediction, computed on the device from the
+ * standard form the solver has packed (several LPs at once on separate
+ * streams); the solver tests the prediction as it tests a caller's warm
+ * start and falls back to the standard start point when it fails
+ * (result.warm_started = 0).  Other LPs start as in hn_ipm_solve_batch.
+ * net and graph must not be used by another call at the same time. */
+int hn_ipm_solve_batch_gnn(const hn_lp* lps, uint32_t count, const hn_ipm_opts* opts, hn_gnn* net,
+                           hn_gnn_graph* graph, hn_ipm_result* results);
 
 /* ---- harness helpers (device buffers, copies, events) ------------------- */
 int hn_dev_alloc(size_t bytes, void** ptr);

@@ -49,7 +49,8 @@ struct Inst {
   long long vm, vn;   // device-only m- and n-vector scratch blocks
   long long om, on;   // b and lambda (m each), c (n) and [x, z] (2n) in the host-visible arrays
   long long dn, w;    // dense N (m*m elements), panel workspace (m*64 elements)
-  int warm;           // 1: x, z, lam hold an accepted warm start
+  int warm;           // 1: x, z, lam hold a warm start accepted on the host; 2: one predicted on the
+                      // device, tested by the kernel before use
 };
 
 // Per-LP vectors.  B, LAM, C, X and Z cross the host link and live in their
@@ -62,7 +63,7 @@ constexpr int kVmIo = 2, kVnIo = 3;  // leading slots held in the host-visible a
 constexpr int kStallIters = 3;
 
 struct Out {
-  int status, iterations, fp64_from, ne_solves, refine, restarted;
+  int status, iterations, fp64_from, ne_solves, refine, restarted, accepted;
   double rp, rd, mu;
 };
 
@@ -575,10 +576,49 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
       for (int i = tid; i < m; i += NT) lam[i] = 0.0;
       __syncthreads();
     };
-    if (!I.warm) cold_start();
+    // A predicted start passes the host's test (warm_start_acceptable) here:
+    // finite and strictly interior, mu below the cold start's, every x_j s_j
+    // at least 1e-3 mu, and a primal residual no larger than the cold start's.
+    auto warm_acceptable = [&]() -> bool {
+      double bn = 1.0, cn = 1.0;
+      for (int i = tid; i < m; i += NT) bn = nan_ignoring_max(bn, fabs(b[i]));
+      for (int j = tid; j < n; j += NT) cn = nan_ignoring_max(cn, fabs(cc[j]));
+      bn = wg_max(bn, red);
+      cn = wg_max(cn, red);
+      double bad = 0.0, mc = 0.0, m0 = 0.0, pmin = HUGE_VAL;
+      for (int j = tid; j < n; j += NT) {
+        if (!(x[j] > 0.0) || !(z[j] > 0.0) || !isfinite(x[j]) || !isfinite(z[j])) bad = 1.0;
+        mc += (bn + 1.0) * fmax(cn + 1.0, fabs(cc[j]));
+        const double pj = x[j] * z[j];
+        m0 += pj;
+        pmin = fmin(pmin, pj);
+      }
+      for (int i = tid; i < m; i += NT)
+        if (!isfinite(lam[i])) bad = 1.0;
+      bad = wg_max(bad, red);
+      mc = wg_sum(mc, red) / (double)n;
+      m0 = wg_sum(m0, red) / (double)n;
+      pmin = wg_min(pmin, red);
+      if (bad > 0.0 || !(m0 < mc) || !(pmin >= 1e-3 * m0)) return false;
+      wg_spmv(c.mrp, c.mci, c.mv, x, tm, m);
+      __syncthreads();
+      double rw = 0.0, rc = 0.0;
+      for (int i = tid; i < m; i += NT) {
+        double sc = 0.0;
+        for (int k = c.mrp[i]; k < c.mrp[i + 1]; ++k) sc += c.mv[k] * (bn + 1.0);
+        rw = fmax(rw, fabs(tm[i] - b[i]));
+        rc = fmax(rc, fabs(sc - b[i]));
+      }
+      rw = wg_max(rw, red);
+      rc = wg_max(rc, red);
+      return rw <= rc;
+    };
+    bool accepted = I.warm == 1;
+    if (I.warm == 2) accepted = warm_acceptable();
+    if (!accepted) cold_start();
     __syncthreads();
     bool done = false, numerr = false, stopped = false;
-    bool warm_live = I.warm != 0, restarted = false;
+    bool warm_live = accepted, restarted = false;
     int stuck = 0;
     int iterations = 0;
     double mu = 0.0, rpn = 0.0, rdn = 0.0;
@@ -759,6 +799,7 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
       o.ne_solves = ne_solves;
       o.refine = refine;
       o.restarted = restarted ? 1 : 0;
+      o.accepted = accepted ? 1 : 0;
       o.rp = rpn;
       o.rd = rdn;
       o.mu = mu;
@@ -849,7 +890,8 @@ int upload_pinned(DevBuf& d, const PinnedBuf& h, long long n, hipStream_t s) {
 
 }  // namespace
 
-int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o, hn_ipm_result* res) {
+int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o, hn_ipm_result* res,
+                         hn_gnn* net = nullptr, hn_gnn_graph* graph = nullptr) {
   const auto t0 = std::chrono::steady_clock::now();
   HN_TRY(enter());
   hipStream_t s = ctx().stream;
@@ -857,7 +899,7 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
   // host preparation, in parallel over LPs
   std::vector<StandardForm> f(count);
   std::vector<int> st(count, HN_OK);
-  std::vector<int> warm_ok(count, 0);
+  std::vector<int> warm_ok(count, 0), predict(count, 0);
   std::vector<std::string> err(count);
   parallel_for(count, [&](uint32_t k) {
     st[k] = build_standard_form(lps[k], f[k]);
@@ -867,6 +909,8 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
     }
     if (lps[k].x0 && lps[k].y0 && lps[k].s0)
       warm_ok[k] = warm_start_acceptable(f[k], lps[k].x0, lps[k].y0, lps[k].s0) ? 1 : 0;
+    else if (net && gnn_pattern_matches(graph, f[k].m, f[k].n, f[k].rp, f[k].ci))
+      predict[k] = 1;  // the network predicts its start on the device
   });
   const auto t_prep = std::chrono::steady_clock::now();
   for (uint32_t k = 0; k < count; ++k) {
@@ -904,8 +948,8 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
     dn += (long long)F.m * F.m;
     I.w = wk;
     wk += (long long)F.m * BK;
-    I.warm = warm_ok[k];
-    any_warm = any_warm || I.warm;
+    I.warm = warm_ok[k] ? 1 : (predict[k] ? 2 : 0);
+    any_warm = any_warm || warm_ok[k];
   }
   BatchWorkspace& w = bws();
   std::lock_guard<std::mutex> lk(w.mtx);
@@ -939,7 +983,7 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
       host_transpose(F, trp + I.tp, tci + I.tz, tv + I.tz);
       std::memcpy(bio + I.om, F.rhs.data(), sizeof(double) * F.m);
       std::memcpy(cio + I.on, F.c.data(), sizeof(double) * F.n);
-      if (I.warm) {
+      if (I.warm == 1) {  // the caller's start (predicted starts are written on the device)
         std::memcpy(xzio + 2 * I.on, lps[k].x0, sizeof(double) * F.n);
         std::memcpy(xzio + 2 * I.on + F.n, lps[k].s0, sizeof(double) * F.n);
         std::memcpy(lio + I.om, lps[k].y0, sizeof(double) * F.m);
@@ -965,6 +1009,17 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
   if (any_warm) {  // cold LPs' entries are overwritten by the kernel's start point
     HN_HIP(hipMemcpyAsync(w.lio.p, w.h_lio.p, sizeof(double) * n_om, hipMemcpyHostToDevice, s));
     HN_HIP(hipMemcpyAsync(w.xzio.p, w.h_xzio.p, sizeof(double) * 2 * n_on, hipMemcpyHostToDevice, s));
+  }
+  {  // learned starts, predicted in place from the packed standard forms
+    std::vector<GnnJob> jobs;
+    for (uint32_t k = 0; k < count; ++k) {
+      if (inst[k].warm != 2) continue;
+      const Inst& I = inst[k];
+      double* xz = w.xzio.as<double>() + 2 * I.on;
+      jobs.push_back({w.mv.as<double>() + I.mz, w.bio.as<double>() + I.om, w.cio.as<double>() + I.on, xz,
+                      w.lio.as<double>() + I.om, xz + I.n});
+    }
+    HN_TRY(gnn_warm_start_many(net, graph, jobs, s));
   }
   HN_TRY(w.vecm.reserve(sizeof(double) * std::max(n_vm, 1LL)));
   HN_TRY(w.vecn.reserve(sizeof(double) * std::max(n_vn, 1LL)));
@@ -1049,7 +1104,7 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
     r.refine_sweeps = out[k].refine;
     r.cg_iterations = 0;
     r.fp64_from_iter = out[k].fp64_from;
-    r.warm_started = warm_ok[k] ? (out[k].restarted ? 2 : 1) : 0;
+    r.warm_started = out[k].accepted ? (out[k].restarted ? 2 : 1) : 0;
     if (r.primal) std::memcpy(r.primal, primal.data(), sizeof(double) * lps[k].n);
     if (r.x && r.cap_n >= F.n) std::memcpy(r.x, x, sizeof(double) * F.n);
     if (r.s && r.cap_n >= F.n) std::memcpy(r.s, z, sizeof(double) * F.n);
@@ -1084,4 +1139,20 @@ extern "C" int hn_ipm_solve_batch(const hn_lp* lps, uint32_t count, const hn_ipm
   HN_CHECK(o.pivot_rule == HN_PIVOT_CLAMP || o.pivot_rule == HN_PIVOT_SKIP, HN_ERR_INVALID,
            "hn_ipm_solve_batch: bad pivot_rule");
   return hn::ipm_solve_batch_impl(lps, count, o, results);
+}
+
+extern "C" int hn_ipm_solve_batch_gnn(const hn_lp* lps, uint32_t count, const hn_ipm_opts* opts, hn_gnn* net,
+                                      hn_gnn_graph* graph, hn_ipm_result* results) {
+  HN_CHECK(count == 0 || (lps && results), HN_ERR_INVALID, "hn_ipm_solve_batch_gnn: null argument");
+  HN_CHECK(net && graph, HN_ERR_INVALID, "hn_ipm_solve_batch_gnn: null network or graph");
+  hn_ipm_opts o;
+  if (opts)
+    o = *opts;
+  else
+    hn_ipm_default_opts(&o);
+  HN_CHECK(o.precision == HN_PREC_F64 || o.precision == HN_PREC_MIXED, HN_ERR_INVALID,
+           "hn_ipm_solve_batch_gnn: bad precision");
+  HN_CHECK(o.pivot_rule == HN_PIVOT_CLAMP || o.pivot_rule == HN_PIVOT_SKIP, HN_ERR_INVALID,
+           "hn_ipm_solve_batch_gnn: bad pivot_rule");
+  return hn::ipm_solve_batch_impl(lps, count, o, results, net, graph);
 }

@@ -28,6 +28,7 @@
 #include "hn_device.hpp"
 #include "hn_internal.hpp"
 
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstdint>
@@ -683,6 +684,18 @@ struct hn_gnn_graph {
   hn::DevBuf lx, ls, ly, dlx, dls, dly;
   hn::DevBuf dhv[2], dhc[2], dav, dac;
   hn::DevBuf part, red, lpart, loss;
+  // host copy of the pattern, to recognise standard forms of this family
+  std::vector<int64_t> h_rp, h_ci;
+  // inference-only workspaces and streams for concurrent warm starts
+  // (created on first use by gnn_warm_start_many, kept until freed)
+  std::vector<hn_gnn_graph*> clones;
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> events;
+  ~hn_gnn_graph() {
+    for (hn_gnn_graph* c : clones) delete c;
+    for (hipStream_t st : streams) (void)hipStreamDestroy(st);
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
+  }
 };
 
 namespace hn {
@@ -940,6 +953,8 @@ extern "C" int hn_gnn_graph_create(int64_t m, int64_t n, const int64_t* row_ptr,
   gr->m = (int)m;
   gr->n = (int)n;
   gr->nnz = nnz;
+  gr->h_rp.assign(row_ptr, row_ptr + m + 1);
+  gr->h_ci.assign(col, col + nnz);
   auto up = [&](DevBuf& d, const std::vector<int>& h) -> int {
     HN_TRY(d.reserve(sizeof(int) * std::max<size_t>(h.size(), 1)));
     if (!h.empty()) HN_HIP(hipMemcpy(d.p, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice));
@@ -1046,6 +1061,111 @@ extern "C" int hn_gnn_train_step_dev(hn_gnn* g, hn_gnn_graph* gr, const double* 
   return check_launch("gnn train step");
 }
 
+namespace hn {
+
+namespace {
+
+constexpr int kInferStreams = 4;  // concurrent warm starts (hardware queues per process)
+
+// A workspace for inference only: the structure of `src` and the forward
+// pass's buffers (no training buffers).
+int clone_for_inference(const hn_gnn_graph* src, hn_gnn_graph** out) {
+  auto* gr = new hn_gnn_graph();
+  gr->m = src->m;
+  gr->n = src->n;
+  gr->nnz = src->nnz;
+  const int m = src->m, n = src->n;
+  const long long nnz = src->nnz;
+  auto dup = [&](DevBuf& d, const DevBuf& from, size_t bytes) -> int {
+    HN_TRY(d.reserve(std::max<size_t>(bytes, 4)));
+    if (bytes) HN_HIP(hipMemcpy(d.p, from.p, bytes, hipMemcpyDeviceToDevice));
+    return HN_OK;
+  };
+  const size_t fm = sizeof(float) * (size_t)m * H, fn = sizeof(float) * (size_t)n * H;
+  int st = dup(gr->rp, src->rp, sizeof(int) * (m + 1));
+  if (st == HN_OK) st = dup(gr->ci, src->ci, sizeof(int) * nnz);
+  if (st == HN_OK) st = dup(gr->trp, src->trp, sizeof(int) * (n + 1));
+  if (st == HN_OK) st = dup(gr->tci, src->tci, sizeof(int) * nnz);
+  if (st == HN_OK) st = dup(gr->tsrc, src->tsrc, sizeof(int) * nnz);
+  if (st == HN_OK) st = dup(gr->slackrow, src->slackrow, sizeof(int) * m);
+  if (st == HN_OK) st = gr->S.reserve(sizeof(float) * 4);
+  if (st == HN_OK) st = gr->spart.reserve(sizeof(double) * 2 * kScaleBlocks);
+  if (st == HN_OK) st = gr->rank.reserve(sizeof(float) * std::max<long long>(nnz, 1));
+  if (st == HN_OK) st = gr->gap.reserve(sizeof(float) * std::max<long long>(nnz, 1));
+  if (st == HN_OK) st = gr->Fv.reserve(sizeof(float) * (size_t)n * 8);
+  if (st == HN_OK) st = gr->Fc.reserve(sizeof(float) * (size_t)m * 4);
+  for (int l = 0; l <= NL && st == HN_OK; ++l) {
+    st = gr->hv[l].reserve(fn);
+    if (st == HN_OK) st = gr->hc[l].reserve(fm);
+    if (st == HN_OK && l < NL) st = gr->av[l].reserve(fn);
+    if (st == HN_OK && l < NL) st = gr->ac[l].reserve(fm);
+  }
+  if (st == HN_OK) st = gr->lx.reserve(sizeof(float) * n);
+  if (st == HN_OK) st = gr->ls.reserve(sizeof(float) * n);
+  if (st == HN_OK) st = gr->ly.reserve(sizeof(float) * m);
+  if (st != HN_OK) {
+    delete gr;
+    return st;
+  }
+  *out = gr;
+  return HN_OK;
+}
+
+int warm_point(hn_gnn_graph* gr, const double* val, const double* c, double* x0, double* y0, double* s0,
+               hipStream_t s) {
+  const int nm = gr->n > gr->m ? gr->n : gr->m;
+  hipLaunchKernelGGL(k_gnn_warm_point, dim3(tgrid(nm)), dim3(256), 0, s, gr->trp.as<int>(), gr->tci.as<int>(),
+                     gr->tsrc.as<int>(), val, c, gr->lx.as<float>(), gr->ls.as<float>(), gr->ly.as<float>(), gr->n,
+                     gr->m, x0, y0, s0);
+  count_launch();
+  return check_launch("gnn warm start");
+}
+
+}  // namespace
+
+bool gnn_pattern_matches(const hn_gnn_graph* g, int64_t m, int64_t n, const std::vector<int64_t>& rp,
+                         const std::vector<int64_t>& ci) {
+  return g && m == g->m && n == g->n && (int64_t)ci.size() == g->nnz && rp == g->h_rp && ci == g->h_ci;
+}
+
+int gnn_warm_start_many(hn_gnn* net, hn_gnn_graph* g, const std::vector<GnnJob>& jobs, hipStream_t s) {
+  if (jobs.empty()) return HN_OK;
+  const int K = (int)std::min<size_t>(kInferStreams, jobs.size());
+  while ((int)g->clones.size() < K - 1) {
+    hn_gnn_graph* c = nullptr;
+    HN_TRY(clone_for_inference(g, &c));
+    g->clones.push_back(c);
+  }
+  while ((int)g->streams.size() < K) {
+    hipStream_t st;
+    HN_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    g->streams.push_back(st);
+  }
+  while ((int)g->events.size() < K + 1) {
+    hipEvent_t e;
+    HN_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    g->events.push_back(e);
+  }
+  // fork from s (the inputs were uploaded on it), one workspace per stream,
+  // join back into s
+  HN_HIP(hipEventRecord(g->events[K], s));
+  for (int q = 0; q < K; ++q) HN_HIP(hipStreamWaitEvent(g->streams[q], g->events[K], 0));
+  for (size_t t = 0; t < jobs.size(); ++t) {
+    const int q = (int)(t % K);
+    hn_gnn_graph* ws = q == 0 ? g : g->clones[q - 1];
+    const GnnJob& j = jobs[t];
+    HN_TRY(forward(net, ws, j.val, j.b, j.c, g->streams[q]));
+    HN_TRY(warm_point(ws, j.val, j.c, j.x0, j.y0, j.s0, g->streams[q]));
+  }
+  for (int q = 0; q < K; ++q) {
+    HN_HIP(hipEventRecord(g->events[q], g->streams[q]));
+    HN_HIP(hipStreamWaitEvent(s, g->events[q], 0));
+  }
+  return HN_OK;
+}
+
+}  // namespace hn
+
 extern "C" int hn_gnn_warm_start_dev(hn_gnn* g, hn_gnn_graph* gr, const double* val, const double* rhs,
                                      const double* c, double* x0, double* y0, double* s0, void* stream) {
   HN_TRY(check_graph_inputs(g, gr));
@@ -1053,10 +1173,5 @@ extern "C" int hn_gnn_warm_start_dev(hn_gnn* g, hn_gnn_graph* gr, const double* 
   HN_TRY(enter());
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx().stream;
   HN_TRY(forward(g, gr, val, rhs, c, s));
-  const int nm = gr->n > gr->m ? gr->n : gr->m;
-  hipLaunchKernelGGL(k_gnn_warm_point, dim3(tgrid(nm)), dim3(256), 0, s, gr->trp.as<int>(), gr->tci.as<int>(),
-                     gr->tsrc.as<int>(), val, c, gr->lx.as<float>(), gr->ls.as<float>(), gr->ly.as<float>(), gr->n,
-                     gr->m, x0, y0, s0);
-  count_launch();
-  return check_launch("gnn warm start");
+  return warm_point(gr, val, c, x0, y0, s0, s);
 }

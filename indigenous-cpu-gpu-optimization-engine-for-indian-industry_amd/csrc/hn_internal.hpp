@@ -162,6 +162,20 @@ int norm_inf_launch(const double* v, int64_t n, double* out, hipStream_t s);
 int axpy_launch(double alpha, const double* x, double* y, int64_t n, hipStream_t s);
 int fill_launch(double* p, double v, int64_t n, hipStream_t s);
 
+// Learned warm start on the device (hn_gnn.hip).  A job reads one scaled
+// standard form (values in the graph's CSR order, b, c) and writes its start
+// point (x0, s0 of length n, y0 of length m), all device pointers.
+struct GnnJob {
+  const double *val, *b, *c;
+  double *x0, *y0, *s0;
+};
+// True when (m, n, rp, ci) is exactly the graph's pattern.
+bool gnn_pattern_matches(const hn_gnn_graph* g, int64_t m, int64_t n, const std::vector<int64_t>& rp,
+                         const std::vector<int64_t>& ci);
+// Runs the jobs on a few streams forked from s (one workspace each) and joins
+// them back into s: work queued on s afterwards sees every start point.
+int gnn_warm_start_many(hn_gnn* net, hn_gnn_graph* g, const std::vector<GnnJob>& jobs, hipStream_t s);
+
 inline unsigned grid_for(int64_t n, int block, int64_t cap = 1 << 20) {
   int64_t g = (n + block - 1) / block;
   if (g < 1) g = 1;

@@ -613,17 +613,23 @@ def ipm_solve(lp, **kw) -> InteriorPointResult:
 
 LIB.hn_ipm_solve_batch.argtypes = [C.POINTER(_HnLp), C.c_uint32, C.POINTER(_HnOpts), C.POINTER(_HnResult)]
 LIB.hn_ipm_solve_batch.restype = C.c_int
+LIB.hn_ipm_solve_batch_gnn.argtypes = [C.POINTER(_HnLp), C.c_uint32, C.POINTER(_HnOpts), C.c_void_p, C.c_void_p,
+                                       C.POINTER(_HnResult)]
+LIB.hn_ipm_solve_batch_gnn.restype = C.c_int
 
 
 def ipm_solve_batch(lps, precision: int = HN_PREC_F64, max_iterations: int = 100, mu_switch: float = 1e-4,
                     want_points: bool = False, pivot_rule: int = HN_PIVOT_CLAMP, mu_target: float = 0.0,
-                    warm=None, verbose: bool = False, gap_tol: float = 0.0):
+                    warm=None, verbose: bool = False, gap_tol: float = 0.0, gnn=None):
     """Solves many LPs in one persistent launch (hn_ipm_solve_batch).
     warm: optional list (one entry per LP, or None) of (x0, y0, s0) warm
     starts of the scaled standard forms.  verbose: the library prints the
     time of each phase of the call on stderr.  gap_tol > 0: an instance
     stops only once its relative gap |c'x - b'y| / (1 + |c'x|) is also at
-    most gap_tol (besides the reference's absolute test).  Returns one
+    most gap_tol (besides the reference's absolute test).  gnn: optional
+    (network handle, graph handle) - LPs without a warm start whose standard
+    form has the graph's pattern are warm-started by the network on the
+    device inside the call (hn_ipm_solve_batch_gnn).  Returns one
     InteriorPointResult per LP (solve_time_ms = whole call)."""
     count = len(lps)
     arr = (_HnLp * count)()
@@ -660,7 +666,10 @@ def ipm_solve_batch(lps, precision: int = HN_PREC_F64, max_iterations: int = 100
         if want_points:
             res[k].x, res[k].y, res[k].s = _dptr(x), _dptr(y), _dptr(s)
             res[k].cap_m, res[k].cap_n = cap_m, cap_n
-    check(LIB.hn_ipm_solve_batch(arr, count, C.byref(opts), res))
+    if gnn is not None:
+        check(LIB.hn_ipm_solve_batch_gnn(arr, count, C.byref(opts), gnn[0], gnn[1], res))
+    else:
+        check(LIB.hn_ipm_solve_batch(arr, count, C.byref(opts), res))
     return [InteriorPointResult(res[k], bufs[k][0][:lps[k].n].copy(), bufs[k][1], bufs[k][2], bufs[k][3])
             for k in range(count)]
 

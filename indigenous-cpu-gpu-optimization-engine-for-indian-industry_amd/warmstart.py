@@ -263,22 +263,36 @@ def predict(net: Network, lps, graph: Optional[Graph] = None, chunk: int = 256):
 
 
 def evaluate(net: Network, make_lp, seeds: Sequence[int], graph: Optional[Graph] = None,
-             pivot_rule: int = HH.HN_PIVOT_SKIP, log=print, gap_tol: float = 0.0) -> dict:
+             pivot_rule: int = HH.HN_PIVOT_SKIP, log=print, gap_tol: float = 0.0, on_device: bool = True) -> dict:
     """Cold vs warm-started batched solves of the held-out seeds: iterations,
     wall time (the warm time includes the network's inference), acceptance
-    of the predictions, and agreement of the two optima.  gap_tol > 0: both
-    solves also stop only at a relative gap <= gap_tol (see hn_ipm_opts)."""
+    of the predictions, and agreement of the two optima.  on_device: the
+    network runs inside the batched call on the packed standard forms
+    (hn_ipm_solve_batch_gnn); else predict() runs first and the points are
+    passed in.  gap_tol > 0: both solves also stop only at a relative gap <=
+    gap_tol (see hn_ipm_opts)."""
     lps = [make_lp(int(s)) for s in seeds]
+    if graph is None:
+        sf = HH.lp_standard_form(lps[0])
+        graph = Graph(sf["m"], sf["n"], sf["rp"], sf["ci"])
     HH.ipm_solve_batch(lps[:4], pivot_rule=pivot_rule, gap_tol=gap_tol)  # warm-up
+    if on_device:
+        HH.ipm_solve_batch(lps[:4], pivot_rule=pivot_rule, gap_tol=gap_tol, gnn=(net.h, graph.h))
     t0 = time.perf_counter()
     cold = HH.ipm_solve_batch(lps, pivot_rule=pivot_rule, gap_tol=gap_tol)
     t_cold = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    warm_pts = predict(net, lps, graph)
-    t_pred = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    warm = HH.ipm_solve_batch(lps, pivot_rule=pivot_rule, warm=warm_pts, gap_tol=gap_tol)
-    t_warm = time.perf_counter() - t0
+    t_pred = 0.0
+    if on_device:
+        t0 = time.perf_counter()
+        warm = HH.ipm_solve_batch(lps, pivot_rule=pivot_rule, gap_tol=gap_tol, gnn=(net.h, graph.h))
+        t_warm = time.perf_counter() - t0
+    else:
+        t0 = time.perf_counter()
+        warm_pts = predict(net, lps, graph)
+        t_pred = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        warm = HH.ipm_solve_batch(lps, pivot_rule=pivot_rule, warm=warm_pts, gap_tol=gap_tol)
+        t_warm = time.perf_counter() - t0
     ci = np.array([r.iterations for r in cold])
     wi = np.array([r.iterations for r in warm])
     acc = np.array([r.warm_started for r in warm])
@@ -295,6 +309,7 @@ def evaluate(net: Network, make_lp, seeds: Sequence[int], graph: Optional[Graph]
         "warm_optimal": int(sum(r.status == "OPTIMAL" for r in warm)),
         "max_rel_objective_diff": float(rel.max()),
         "cold_wall_s": round(t_cold, 4), "warm_wall_s": round(t_warm + t_pred, 4),
+        "inference": "on the device inside the batched call" if on_device else "predict() before the call",
         "predict_s": round(t_pred, 4), "warm_solve_s": round(t_warm, 4),
         "iteration_ratio": float(ci.mean() / max(wi.mean(), 1e-9)),
         "gap_tol": gap_tol,

@@ -92,3 +92,34 @@ def test_trained_warm_start_saves_iterations(ws):
     assert r["accepted"] >= 60
     assert r["max_rel_objective_diff"] <= 1e-6
     assert r["warm_iterations_median"] <= r["cold_iterations_median"] - 3
+
+
+def test_device_warm_start_in_batch_matches_host_prediction(ws):
+    # hn_ipm_solve_batch_gnn predicts each start on the device from the
+    # standard form the solver packed; it must give the same solves as the
+    # host path (predict(), then the points passed in).  An LP of another
+    # size in the same batch has another pattern: it starts cold and solves
+    # exactly as a cold solve does.
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(ws.__file__)), "data", "warmstart_transport_200x300.npy")
+    net = ws.Network(0)
+    net.set_params(np.load(path))
+    lps = [lpgen.transport_lp(s, 200, 300) for s in range(1_000_100, 1_000_124)]
+    odd = lpgen.transport_lp(5, 20, 30)
+    sf = HH.lp_standard_form(lps[0])
+    g = ws.Graph(sf["m"], sf["n"], sf["rp"], sf["ci"])
+    print("[case] device warm start in the batch, 24 + 1 LPs", flush=True)
+    dev = HH.ipm_solve_batch(lps + [odd], pivot_rule=HH.HN_PIVOT_SKIP, gnn=(net.h, g.h))
+    host = HH.ipm_solve_batch(lps, pivot_rule=HH.HN_PIVOT_SKIP, warm=ws.predict(net, lps, g))
+    cold_odd = HH.ipm_solve_batch([odd], pivot_rule=HH.HN_PIVOT_SKIP)[0]
+    accepted = 0
+    for k, (d, h) in enumerate(zip(dev, host)):
+        assert d.status == h.status == "OPTIMAL", k
+        assert d.warm_started == h.warm_started, (k, d.warm_started, h.warm_started)
+        accepted += d.warm_started == 1
+        assert abs(d.iterations - h.iterations) <= 1, (k, d.iterations, h.iterations)
+        assert abs(d.objective_value - h.objective_value) <= 1e-7 * max(1.0, abs(h.objective_value)), k
+    assert accepted >= 20
+    d = dev[-1]
+    assert d.warm_started == 0 and d.status == cold_odd.status == "OPTIMAL"
+    assert d.iterations == cold_odd.iterations and d.objective_value == cold_odd.objective_value
