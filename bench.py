@@ -356,6 +356,31 @@ def run_lp(args) -> dict:
     log(f"warm start: iterations {ev['cold_iterations_mean']:.2f} cold -> {ev['warm_iterations_mean']:.2f} warm, "
         f"wall {ev['cold_wall_s']:.2f} s -> {ev['warm_wall_s']:.2f} s (inference {ev['inference']})")
 
+    # Netlib: the reference's 21 models (tests/golden/netlib/), one GPU solve
+    # each under the reference's rules, against HiGHS's optimum
+    import mps
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "netlib.json")))
+    models = {name: mps.read_mps(os.path.join(ROOT, "tests", "golden", "netlib", name + ".mps.gz"))[0]
+              for name in sorted(fx)}
+    H.ipm_solve(models["afiro"])
+    per, t_all = {}, 0.0
+    for name, mlp in models.items():
+        t0 = time.perf_counter()
+        r = H.ipm_solve(mlp)
+        dt = time.perf_counter() - t0
+        t_all += dt
+        per[name] = {"status": r.status, "iterations": r.iterations, "ms": round(1e3 * dt, 2),
+                     "rel_err_vs_highs": abs(r.objective_value - fx[name]["highs"]) / max(1.0, abs(fx[name]["highs"])),
+                     "reference_ipm": f"{fx[name]['ref_ipm_status']}/{fx[name]['ref_ipm_iterations']}"}
+    out["netlib"] = {"models": len(models), "wall_s": round(t_all, 3),
+                     "within_1e-6_of_highs": sum(v["rel_err_vs_highs"] <= 1e-6 for v in per.values()),
+                     "same_status_and_iterations_as_reference_ipm": sum(
+                         f"{v['status']}/{v['iterations']}" == v["reference_ipm"] for v in per.values()),
+                     "per_model": per}
+    log(f"netlib: 21 models in {t_all:.2f} s, {out['netlib']['within_1e-6_of_highs']} within 1e-6 of HiGHS, "
+        f"{out['netlib']['same_status_and_iterations_as_reference_ipm']} with the reference IPM's status/iterations; "
+        f"dfl001 {per['dfl001']['status']} in {per['dfl001']['ms']:.0f} ms")
+
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O  # baseline only
@@ -401,6 +426,17 @@ def run_lp(args) -> dict:
             "sample": (f"{len(tl)} held-out 200x300 transportation LPs, cold start, {args.cpu_lp_limit:.0f} s "
                        f"limit each; status/iterations " + ", ".join(f"{st}/{it}" for _, st, it in tl))}
         log(f"cpu warm-start-family baseline {out['warm_start']['cpu_baseline']}")
+        # the reference IPM on the Netlib models it solves (fewer than 100 iterations)
+        conv = [n for n in sorted(fx) if fx[n]["ref_ipm_iterations"] < 100 and n != "dfl001"]
+        t0 = time.perf_counter()
+        cpu = {n: solve(models[n], time_limit=args.cpu_lp_limit) for n in conv}
+        dt = time.perf_counter() - t0
+        gpu_t = sum(per[n]["ms"] for n in conv) / 1e3
+        out["netlib"]["cpu_baseline"] = {
+            "value": round(len(conv) / dt, 3), "unit": "LPs/s", "cores": 1, "kind": kind,
+            "sample": (f"{', '.join(conv)}: {dt:.2f} s on the host ({', '.join(cpu[n].status for n in conv)}); "
+                       f"the GPU solver takes {gpu_t:.3f} s on the same models")}
+        log(f"cpu netlib baseline {out['netlib']['cpu_baseline']}")
     return out
 
 
