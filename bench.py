@@ -440,6 +440,68 @@ def run_lp(args) -> dict:
     return out
 
 
+def verify_bytes(lp, with_dual: bool) -> int:
+    """Compulsory bytes of the verifier's two passes: A in CSR and A' in CSR
+    (8-byte values, 4-byte indices and row pointers), the primal, bounds and
+    costs, rhs and senses, the dual once per pass, one flag byte per row and
+    per column."""
+    m, n, z = lp.m, lp.n, lp.nnz
+    rows = z * 12 + (m + 1) * 4 + n * 8 + m * 9 + m + (m * 8 if with_dual else 0)
+    cols = z * 12 + (n + 1) * 4 + n * 32 + n + (m * 8 if with_dual else 0)
+    return rows + cols
+
+
+def run_verify(args) -> dict:
+    """Post-solve verification (SolutionVerifier::verify_detailed) of a
+    perturbed interior point with its dual, on the 2,000 x 5,000 LP and the
+    200,000 x 500,000 power-law LP."""
+    import hypernova_hip as H
+    import lpgen
+    out = {}
+    for name, make in (("2000x5000", lambda: lpgen.random_lp(1, 2000, 5000)),
+                       ("200000x500000", lambda: lpgen.powerlaw_lp(3, 200_000, 500_000))):
+        lp = make()
+        g = np.random.default_rng(5)
+        x = lp.x0.copy()
+        idx = g.choice(lp.n, size=lp.n // 100, replace=False)
+        x[idx] = -g.uniform(1e-9, 1e-3, idx.size)
+        objv = float(np.dot(lp.obj, x))
+        ver = H.SolutionVerifier()
+        sol = H.Solution(x, lp.y0, None, objv)
+        r = ver.verify_detailed(lp, sol)
+        walls, devs = [], []
+        for _ in range(max(3, min(args.steps, 10))):
+            t0 = time.perf_counter()
+            r = ver.verify_detailed(lp, sol)
+            walls.append(time.perf_counter() - t0)
+            devs.append(r.device_ms)
+        wall_ms, dev_ms = 1e3 * float(np.median(walls)), float(np.median(devs))
+        nbytes = verify_bytes(lp, True)
+        out[name] = {"m": lp.m, "n": lp.n, "nnz": lp.nnz, "wall_ms": round(wall_ms, 3), "device_ms": round(dev_ms, 4),
+                     "device_GBps": round(nbytes / (dev_ms * 1e-3) / 1e9, 1), "algorithmic_bytes": nbytes,
+                     "result": {"feasible": r.feasible, "optimal": r.optimal, "n_violated": r.n_violated,
+                                "message": r.message}}
+        log(f"verify {name}: wall {wall_ms:.2f} ms, device {dev_ms:.3f} ms "
+            f"({out[name]['device_GBps']} GB/s of compulsory traffic incl. the transpose build), "
+            f"{r.n_violated} violated, '{r.message}'")
+        if not args.no_cpu_baseline and name == "2000x5000":
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle as O  # baseline only
+            if O.have_reference() and hasattr(O.REF, "ref_verify"):
+                rr = O.ref_verify(lp, x, lp.y0, None, objv)
+                out[name]["cpu_baseline"] = {
+                    "value": round(1e3 * rr.seconds, 2), "unit": "ms per verify", "cores": 1, "kind": "reference",
+                    "sample": ("one SolutionVerifier::verify_detailed call on the same solution "
+                               f"(same result: {rr.feasible == r.feasible and rr.optimal == r.optimal and rr.n_violated == r.n_violated})")}
+                log(f"cpu verify baseline {out[name]['cpu_baseline']}")
+        if name == "200000x500000":
+            out[name]["cpu_baseline"] = {
+                "value": None, "unit": "ms per verify", "cores": 1, "kind": "reference",
+                "sample": ("not run: the reference's dual check scans all nonzeros once per column "
+                           f"(n * nnz = {lp.n * lp.nnz:.2e} steps)")}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -448,6 +510,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-lp", action="store_true")
+    ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--verify-only", action="store_true", help="run only the post-solve verification leg")
     ap.add_argument("--chunks-A", default="", help="fixed chunked layout of A: items,span_log2,threads")
     ap.add_argument("--chunks-At", default="", help="fixed chunked layout of A': items,span_log2,threads")
     ap.add_argument("--batch", type=int, default=4096)
@@ -460,9 +524,14 @@ def main() -> int:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is described in "
                                    "DESIGN.md and not launched", "n_gpus": args.gpus}))
         return 2
+    if args.verify_only:
+        print(json.dumps({"verify": run_verify(args)}), flush=True)
+        return 0
     out = run_spmv(args)
     if not args.no_lp:
         out["lp"] = run_lp(args)
+    if not args.no_verify:
+        out["verify"] = run_verify(args)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -377,6 +377,68 @@ int hn_gnn_train_step_dev(hn_gnn* g, hn_gnn_graph* gr, const double* val, const 
 int hn_ipm_solve_batch_gnn(const hn_lp* lps, uint32_t count, const hn_ipm_opts* opts, hn_gnn* net,
                            hn_gnn_graph* graph, hn_ipm_result* results);
 
+/* ---- post-solve solution verification ------------------------------------
+ * Replaces SolutionVerifier::verify_detailed
+ * (core/validation/solution_verifier.cpp:24-88, declared in
+ * solution_verifier.hpp:23-41), which api/cpp/api.cpp:510,523 runs after
+ * every solve.  The checks, their tolerances and their order are the
+ * reference's; the row activities, reduced costs and objective are summed
+ * in the reference's order without fused multiply-adds, so every value,
+ * flag and index comes out bit-identical.  The reference's dual check scans
+ * all nonzeros once per column (O(n nnz)); here it reads the column of a
+ * transpose built on the device. */
+typedef struct hn_tolerances {   /* numerical::ToleranceConfig fields read by the verifier */
+  double feasibility_tol;        /* 1e-9 (industrial_defaults, tolerance.hpp:60-75) */
+  double optimality_tol;         /* 1e-9 */
+  double complementarity_tol;    /* 1e-6 */
+  double integrality_tol;        /* 1e-6 */
+} hn_tolerances;
+int hn_default_tolerances(hn_tolerances* t);
+
+/* model::Solution (core/model/problem.hpp) as the verifier reads it.  A
+ * dual of another length than lp->m, or reduced costs of another length than
+ * lp->n, are treated as absent, as in the reference. */
+typedef struct hn_solution {
+  const double* primal;          /* n_primal entries */
+  uint64_t n_primal;
+  const double* dual;            /* n_dual entries (may be NULL with n_dual 0) */
+  uint64_t n_dual;
+  const double* reduced_costs;   /* n_reduced_costs entries (may be NULL) */
+  uint64_t n_reduced_costs;
+  double objective_value;
+} hn_solution;
+
+/* var_type codes: model::VarType order (problem.hpp:16-22). */
+#define HN_VAR_CONTINUOUS 0
+#define HN_VAR_INTEGER 1
+#define HN_VAR_BINARY 2
+
+typedef struct hn_verify_result {  /* validation::VerificationResult */
+  int feasible;
+  int optimal;
+  double primal_infeasibility;
+  double dual_infeasibility;
+  double complementarity;
+  double integrality_violation;
+  double objective_discrepancy;
+  /* violated_constraints: column j for a bound violation (listed twice if
+   * both bounds are violated), n + i for row i, in the reference's order.
+   * n_violated is the full count; the first min(n_violated, capacity)
+   * entries go to `violated` when it is not NULL. */
+  uint64_t n_violated;
+  uint64_t* violated;
+  uint64_t violated_capacity;
+  char message[160];               /* the reference's message text */
+  double device_ms;                /* kernel time of the checks (HIP events) */
+} hn_verify_result;
+
+/* var_type: n entries of HN_VAR_* (NULL: all continuous).  lp->x0/y0/s0 are
+ * ignored.  With a primal of the wrong length the result is "Primal
+ * infeasible: Solution vector size mismatch" with optimal = 0 and no other
+ * check (the reference goes on to read past the vector). */
+int hn_verify(const hn_lp* lp, const int8_t* var_type, const hn_solution* sol, const hn_tolerances* tol,
+              hn_verify_result* out);
+
 /* ---- harness helpers (device buffers, copies, events) ------------------- */
 int hn_dev_alloc(size_t bytes, void** ptr);
 int hn_dev_free(void* ptr);

@@ -44,6 +44,7 @@ KERNEL_SOURCES = [
     "hn_ipm.hip",
     "hn_batch.hip",
     "hn_gnn.hip",
+    "hn_verify.hip",
 ]
 
 

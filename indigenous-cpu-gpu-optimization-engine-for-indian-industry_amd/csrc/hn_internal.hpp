@@ -77,6 +77,8 @@ struct HostOpScratch {
   DevBuf a, b, c, red;
 };
 HostOpScratch& host_scratch();
+// Frees the verifier's cached device buffers (hn_finalize).
+void release_verify_scratch();
 
 // ---- device CSR -------------------------------------------------------------
 constexpr int kSpmvBlock = 256;
@@ -143,6 +145,10 @@ int spmv_slab_launch(const hn_csr* a, const V* x, V* y, hipStream_t s);
 int build_chunks(hn_csr* a, int ipt, int span_log2, int threads, hipStream_t s);
 template <typename V>
 int spmv_chunk_launch(const hn_csr* a, const V* x, V* y, hipStream_t s);
+
+// t = A' (t allocated by csr_alloc with A's shape transposed and flags),
+// entries of each row of t in ascending row order of A; no tiling is built.
+int transpose_into(const hn_csr* a, hn_csr* t, hipStream_t s);
 
 // y = A x on device buffers (FP64 or FP32) with the handle's current tiling.
 int spmv_launch(const hn_csr* a, int prec, const void* x, void* y, hipStream_t s);

@@ -141,6 +141,7 @@ extern "C" int hn_finalize(void) {
     s.c.release();
     s.red.release();
   }
+  release_verify_scratch();
   (void)hipStreamDestroy(c.stream);
   c.stream = nullptr;
   c.initialized = false;

@@ -308,6 +308,51 @@ __global__ void k_gather_transpose(const int* __restrict__ rowptr, int nrows, co
 
 using namespace hn;
 
+namespace hn {
+
+int transpose_into(const hn_csr* a, hn_csr* t, hipStream_t s) {
+  DevBuf cnt, perm, scan_tmp;
+  SortScratch scratch;
+  const int64_t nnz = a->nnz, nc = a->ncols;
+  HN_TRY(cnt.reserve(sizeof(int) * std::max<int64_t>(nc, 1)));
+  HN_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int) * std::max<int64_t>(nc, 1), s));
+  if (nnz > 0) {
+    hipLaunchKernelGGL(k_hist, dim3(grid_for(nnz, 256, 16384)), dim3(256), 0, s, a->col.as<int>(),
+                       (long long)nnz, cnt.as<int>());
+    HN_TRY(check_launch("k_hist"));
+  }
+  HN_TRY(exclusive_scan(cnt.as<int>(), nc, t->row_ptr.as<int>(), scan_tmp, s));
+  if (nnz == 0) return HN_OK;
+  // cursor = row pointer copy (reuse cnt)
+  HN_HIP(hipMemcpyAsync(cnt.p, t->row_ptr.p, sizeof(int) * nc, hipMemcpyDeviceToDevice, s));
+  HN_TRY(perm.reserve(sizeof(int) * nnz));
+  hipLaunchKernelGGL(k_scatter_pos, dim3(grid_for(nnz, 256, 16384)), dim3(256), 0, s, a->col.as<int>(),
+                     (long long)nnz, cnt.as<int>(), perm.as<int>());
+  HN_TRY(check_launch("k_scatter_pos"));
+  // segmented_sort synchronises the stream, so the local buffers outlive their use
+  HN_TRY(segmented_sort(t->row_ptr.as<int>(), (int)nc, perm.as<int>(), nnz, scratch, s));
+  const bool f64 = a->flags & HN_F64, f32 = a->flags & HN_F32;
+  const dim3 g(grid_for(nnz, 256, 16384)), bdim(256);
+  if (f64 && f32)
+    hipLaunchKernelGGL((k_gather_transpose<true, true>), g, bdim, 0, s, a->row_ptr.as<int>(), (int)a->nrows,
+                       perm.as<int>(), (long long)nnz, a->val64.as<double>(), a->val32.as<float>(),
+                       t->col.as<int>(), t->val64.as<double>(), t->val32.as<float>());
+  else if (f64)
+    hipLaunchKernelGGL((k_gather_transpose<true, false>), g, bdim, 0, s, a->row_ptr.as<int>(), (int)a->nrows,
+                       perm.as<int>(), (long long)nnz, a->val64.as<double>(), (const float*)nullptr,
+                       t->col.as<int>(), t->val64.as<double>(), (float*)nullptr);
+  else
+    hipLaunchKernelGGL((k_gather_transpose<false, true>), g, bdim, 0, s, a->row_ptr.as<int>(), (int)a->nrows,
+                       perm.as<int>(), (long long)nnz, (const double*)nullptr, a->val32.as<float>(),
+                       t->col.as<int>(), (double*)nullptr, t->val32.as<float>());
+  HN_TRY(check_launch("k_gather_transpose"));
+  // perm is read by the gather: wait before the local buffers are released
+  HN_HIP(hipStreamSynchronize(s));
+  return HN_OK;
+}
+
+}  // namespace hn
+
 extern "C" int hn_csr_transpose(const hn_csr* a, hn_csr** out) {
   HN_CHECK(a && out, HN_ERR_INVALID, "hn_csr_transpose: null argument");
   *out = nullptr;
@@ -315,45 +360,7 @@ extern "C" int hn_csr_transpose(const hn_csr* a, hn_csr** out) {
   hipStream_t s = ctx().stream;
   hn_csr* t = new hn_csr();
   int st = csr_alloc(t, a->ncols, a->nrows, a->nnz, a->flags);
-  DevBuf cnt, perm, scan_tmp;
-  SortScratch scratch;
-  const int64_t nnz = a->nnz, nc = a->ncols;
-  auto run = [&]() -> int {
-    HN_TRY(st);
-    HN_TRY(cnt.reserve(sizeof(int) * std::max<int64_t>(nc, 1)));
-    HN_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int) * std::max<int64_t>(nc, 1), s));
-    if (nnz > 0) {
-      hipLaunchKernelGGL(k_hist, dim3(grid_for(nnz, 256, 16384)), dim3(256), 0, s, a->col.as<int>(),
-                         (long long)nnz, cnt.as<int>());
-      HN_TRY(check_launch("k_hist"));
-    }
-    HN_TRY(exclusive_scan(cnt.as<int>(), nc, t->row_ptr.as<int>(), scan_tmp, s));
-    if (nnz == 0) return HN_OK;
-    // cursor = row pointer copy (reuse cnt)
-    HN_HIP(hipMemcpyAsync(cnt.p, t->row_ptr.p, sizeof(int) * nc, hipMemcpyDeviceToDevice, s));
-    HN_TRY(perm.reserve(sizeof(int) * nnz));
-    hipLaunchKernelGGL(k_scatter_pos, dim3(grid_for(nnz, 256, 16384)), dim3(256), 0, s, a->col.as<int>(),
-                       (long long)nnz, cnt.as<int>(), perm.as<int>());
-    HN_TRY(check_launch("k_scatter_pos"));
-    HN_TRY(segmented_sort(t->row_ptr.as<int>(), (int)nc, perm.as<int>(), nnz, scratch, s));
-    const bool f64 = a->flags & HN_F64, f32 = a->flags & HN_F32;
-    const dim3 g(grid_for(nnz, 256, 16384)), bdim(256);
-    if (f64 && f32)
-      hipLaunchKernelGGL((k_gather_transpose<true, true>), g, bdim, 0, s, a->row_ptr.as<int>(), (int)a->nrows,
-                         perm.as<int>(), (long long)nnz, a->val64.as<double>(), a->val32.as<float>(),
-                         t->col.as<int>(), t->val64.as<double>(), t->val32.as<float>());
-    else if (f64)
-      hipLaunchKernelGGL((k_gather_transpose<true, false>), g, bdim, 0, s, a->row_ptr.as<int>(), (int)a->nrows,
-                         perm.as<int>(), (long long)nnz, a->val64.as<double>(), (const float*)nullptr,
-                         t->col.as<int>(), t->val64.as<double>(), (float*)nullptr);
-    else
-      hipLaunchKernelGGL((k_gather_transpose<false, true>), g, bdim, 0, s, a->row_ptr.as<int>(), (int)a->nrows,
-                         perm.as<int>(), (long long)nnz, (const double*)nullptr, a->val32.as<float>(),
-                         t->col.as<int>(), (double*)nullptr, t->val32.as<float>());
-    HN_TRY(check_launch("k_gather_transpose"));
-    return HN_OK;
-  };
-  st = run();
+  if (st == HN_OK) st = transpose_into(a, t, s);
   if (st == HN_OK) st = build_tiling(t, 8, s);
   if (st == HN_OK) {
     hipError_t e = hipStreamSynchronize(s);

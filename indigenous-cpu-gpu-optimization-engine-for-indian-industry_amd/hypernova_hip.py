@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import ctypes as C
 import enum
+from dataclasses import dataclass
 import os
 from typing import Optional, Sequence
 
@@ -705,3 +706,125 @@ def lp_standard_form(lp) -> dict:
     m, n, z = sm.value, sn.value, nz.value
     return dict(m=m, n=n, rp=rp[:m + 1].copy(), ci=ci[:z].copy(), v=v[:z].copy(), rhs=rhs[:m].copy(),
                 c=c[:n].copy())
+
+
+# ---- post-solve verification (hn_verify) ----------------------------------------
+class _HnTol(C.Structure):
+    _fields_ = [("feasibility_tol", C.c_double), ("optimality_tol", C.c_double),
+                ("complementarity_tol", C.c_double), ("integrality_tol", C.c_double)]
+
+
+class _HnSolution(C.Structure):
+    _fields_ = [("primal", _dp), ("n_primal", C.c_uint64), ("dual", _dp), ("n_dual", C.c_uint64),
+                ("reduced_costs", _dp), ("n_reduced_costs", C.c_uint64), ("objective_value", C.c_double)]
+
+
+class _HnVerifyResult(C.Structure):
+    _fields_ = [("feasible", C.c_int), ("optimal", C.c_int), ("primal_infeasibility", C.c_double),
+                ("dual_infeasibility", C.c_double), ("complementarity", C.c_double),
+                ("integrality_violation", C.c_double), ("objective_discrepancy", C.c_double),
+                ("n_violated", C.c_uint64), ("violated", _u64p), ("violated_capacity", C.c_uint64),
+                ("message", C.c_char * 160), ("device_ms", C.c_double)]
+
+
+LIB.hn_default_tolerances.argtypes = [C.POINTER(_HnTol)]
+LIB.hn_default_tolerances.restype = C.c_int
+LIB.hn_verify.argtypes = [C.POINTER(_HnLp), _i8p, C.POINTER(_HnSolution), C.POINTER(_HnTol),
+                          C.POINTER(_HnVerifyResult)]
+LIB.hn_verify.restype = C.c_int
+
+HN_VAR_CONTINUOUS, HN_VAR_INTEGER, HN_VAR_BINARY = 0, 1, 2
+
+
+@dataclass
+class ToleranceConfig:
+    """The fields of numerical::ToleranceConfig the verifier reads, at their
+    industrial_defaults (core/numerical/tolerance.hpp:60-75)."""
+    feasibility_tol: float = 1e-9
+    optimality_tol: float = 1e-9
+    complementarity_tol: float = 1e-6
+    integrality_tol: float = 1e-6
+
+
+@dataclass
+class Solution:
+    """model::Solution as the verifier reads it (core/model/problem.hpp:172-190);
+    empty dual / reduced_costs are absent, as in the reference."""
+    primal: np.ndarray
+    dual: Optional[np.ndarray] = None
+    reduced_costs: Optional[np.ndarray] = None
+    objective_value: float = 0.0
+
+
+@dataclass
+class VerificationResult:
+    """validation::VerificationResult (core/validation/solution_verifier.hpp:12-21)."""
+    feasible: bool
+    optimal: bool
+    primal_infeasibility: float
+    dual_infeasibility: float
+    complementarity: float
+    integrality_violation: float
+    objective_discrepancy: float
+    violated_constraints: list
+    n_violated: int
+    message: str
+    device_ms: float = 0.0
+
+
+class SolutionVerifier:
+    """SolutionVerifier (core/validation/solution_verifier.hpp:23-41) on the
+    device through hn_verify.  var_type (model::VarType codes, one per
+    column) marks integer columns; None means all continuous."""
+
+    def __init__(self, tol: Optional[ToleranceConfig] = None, max_listed: int = 1 << 16):
+        self.tol = tol or ToleranceConfig()
+        self.max_listed = max_listed
+
+    def verify(self, lp, solution: Solution, var_type=None) -> bool:
+        r = self.verify_detailed(lp, solution, var_type)
+        return r.feasible and r.optimal
+
+    def verify_feasible(self, lp, solution: Solution, var_type=None) -> bool:
+        return self.verify_detailed(lp, solution, var_type).feasible
+
+    def verify_detailed(self, lp, solution: Solution, var_type=None) -> VerificationResult:
+        keep = dict(rp=np.ascontiguousarray(lp.row_ptr, np.int64), ci=np.ascontiguousarray(lp.col, np.int64),
+                    v=np.ascontiguousarray(lp.val, np.float64), sense=np.ascontiguousarray(lp.sense, np.int8),
+                    rhs=np.ascontiguousarray(lp.rhs, np.float64), lb=np.ascontiguousarray(lp.lb, np.float64),
+                    ub=np.ascontiguousarray(lp.ub, np.float64), obj=np.ascontiguousarray(lp.obj, np.float64))
+        P = _HnLp(lp.m, lp.n, keep["rp"].ctypes.data_as(_i64p), keep["ci"].ctypes.data_as(_i64p),
+                  _dptr(keep["v"]), keep["sense"].ctypes.data_as(_i8p), _dptr(keep["rhs"]), _dptr(keep["lb"]),
+                  _dptr(keep["ub"]), _dptr(keep["obj"]), int(lp.maximize), float(getattr(lp, "obj_offset", 0.0)))
+        vec = {}
+        for name in ("primal", "dual", "reduced_costs"):
+            a = getattr(solution, name)
+            vec[name] = None if a is None else np.ascontiguousarray(a, np.float64)
+        S = _HnSolution()
+        for name in ("primal", "dual", "reduced_costs"):
+            a = vec[name]
+            setattr(S, name, _dptr(a) if a is not None and a.size else None)
+        S.n_primal = 0 if vec["primal"] is None else vec["primal"].size
+        S.n_dual = 0 if vec["dual"] is None else vec["dual"].size
+        S.n_reduced_costs = 0 if vec["reduced_costs"] is None else vec["reduced_costs"].size
+        S.objective_value = float(solution.objective_value)
+        t = _HnTol(self.tol.feasibility_tol, self.tol.optimality_tol, self.tol.complementarity_tol,
+                   self.tol.integrality_tol)
+        vt = None if var_type is None else np.ascontiguousarray(var_type, np.int8)
+        listed = np.zeros(max(self.max_listed, 1), np.uint64)
+        out = _HnVerifyResult()
+        out.violated = listed.ctypes.data_as(_u64p)
+        out.violated_capacity = self.max_listed
+        check(LIB.hn_verify(C.byref(P), None if vt is None else vt.ctypes.data_as(_i8p), C.byref(S), C.byref(t),
+                            C.byref(out)))
+        k = min(int(out.n_violated), self.max_listed)
+        return VerificationResult(bool(out.feasible), bool(out.optimal), out.primal_infeasibility,
+                                  out.dual_infeasibility, out.complementarity, out.integrality_violation,
+                                  out.objective_discrepancy, [int(q) for q in listed[:k]], int(out.n_violated),
+                                  out.message.decode(), out.device_ms)
+
+
+def verify(lp, primal, dual=None, reduced_costs=None, objective_value: float = 0.0, var_type=None,
+           tol: Optional[ToleranceConfig] = None) -> VerificationResult:
+    return SolutionVerifier(tol).verify_detailed(lp, Solution(primal, dual, reduced_costs, objective_value),
+                                                 var_type)

@@ -14,6 +14,7 @@
 #include "hn_oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -911,4 +912,200 @@ finish_primal:
   free(nci);
   free(nv);
   return 0;
+}
+
+/* ---- post-solve verification ---------------------------------------------
+ * SolutionVerifier (core/validation/solution_verifier.cpp).  std::max(a, b)
+ * is (a < b) ? b : a and std::min(a, b) is (b < a) ? b : a, which is how a
+ * NaN stays out of every maximum below. */
+static double vmax(double a, double b) { return (a < b) ? b : a; }
+
+static void v_push(or_verify_out* out, int64_t idx) {
+  if (out->violated && out->n_violated < out->violated_cap) out->violated[out->n_violated] = idx;
+  ++out->n_violated;
+}
+
+void or_verify(const or_problem* p, const int8_t* var_type, const double* primal, int64_t n_primal,
+               const double* dual, int64_t n_dual, const double* rc, int64_t n_rc,
+               double objective_value, const double tol[4], or_verify_out* out) {
+  const int64_t m = p->m, n = p->n;
+  const double feas = tol[0], otol = tol[1], ctol = tol[2], itol = tol[3];
+  char pmsg[128] = "";
+  int primal_ok = 1;
+  double max_violation = 0.0;
+  int64_t n_int = 0;
+  out->feasible = 1;
+  out->optimal = 1;
+  out->primal_infeasibility = out->dual_infeasibility = out->complementarity = 0.0;
+  out->integrality_violation = out->objective_discrepancy = 0.0;
+  out->n_violated = 0;
+  out->message[0] = 0;
+  if (var_type)
+    for (int64_t j = 0; j < n; ++j) n_int += (var_type[j] == 1 || var_type[j] == 2);
+
+  /* check_primal_feasibility (:92-191) */
+  if (n_primal != n) {
+    out->feasible = 0;
+    out->optimal = 0;
+    snprintf(out->message, sizeof(out->message), "Primal infeasible: Solution vector size mismatch");
+    return;
+  }
+  {
+    double* col_scale = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    const double ftol = feas * 10.0;
+    int64_t nonfinite = 0;
+    for (int64_t j = 0; j < n; ++j) col_scale[j] = 1.0;
+    for (int64_t k = 0; k < p->rp[m]; ++k) {  /* :108-114 */
+      const int64_t j = p->ci[k];
+      if (j < n) col_scale[j] = vmax(col_scale[j], fabs(p->v[k]));
+    }
+    for (int64_t j = 0; j < n; ++j) {  /* :115-121 */
+      col_scale[j] = vmax(col_scale[j], fabs(p->obj[j]));
+      if (isfinite(p->lb[j])) col_scale[j] = vmax(col_scale[j], fabs(p->lb[j]));
+      if (isfinite(p->ub[j])) col_scale[j] = vmax(col_scale[j], fabs(p->ub[j]));
+    }
+    for (int64_t j = 0; j < n; ++j) {  /* :125-146 */
+      const double val = primal[j];
+      if (!isfinite(val)) {  /* early return: no maximum recorded (:119-123) */
+        nonfinite = 1;
+        snprintf(pmsg, sizeof(pmsg), "Solution contains NaN or Inf values");
+        break;
+      }
+      if (val < p->lb[j] - ftol * col_scale[j]) {
+        max_violation = vmax(max_violation, p->lb[j] - val);
+        v_push(out, j);
+      }
+      if (val > p->ub[j] + ftol * col_scale[j]) {
+        max_violation = vmax(max_violation, val - p->ub[j]);
+        v_push(out, j);
+      }
+    }
+    free(col_scale);
+    if (nonfinite) {
+      primal_ok = 0;
+    } else {
+      for (int64_t i = 0; i < m; ++i) {  /* :148-184 */
+        double activity = 0.0, activity_abs = 0.0;
+        for (int64_t k = p->rp[i]; k < p->rp[i + 1]; ++k) {
+          const double term = p->v[k] * primal[p->ci[k]];
+          activity += term;
+          activity_abs += fabs(term);
+        }
+        const double scale = 1.0 + fabs(p->rhs[i]) + activity_abs;
+        double violation = 0.0, excess;
+        if (p->sense[i] == OR_LE) {
+          excess = activity - p->rhs[i] - ftol * scale;
+          if (excess > 0.0) violation = activity - p->rhs[i];
+        } else if (p->sense[i] == OR_GE) {
+          excess = p->rhs[i] - activity - ftol * scale;
+          if (excess > 0.0) violation = p->rhs[i] - activity;
+        } else {
+          excess = fabs(activity - p->rhs[i]) - ftol * scale;
+          if (excess > 0.0) violation = fabs(activity - p->rhs[i]);
+        }
+        if (excess > 0.0) {
+          max_violation = vmax(max_violation, violation);
+          v_push(out, n + i);
+        }
+      }
+      out->primal_infeasibility = max_violation;
+      if (out->n_violated > 0) {
+        primal_ok = 0;
+        snprintf(pmsg, sizeof(pmsg), "Max primal violation: %f", max_violation);
+      }
+    }
+  }
+  if (!primal_ok) {  /* verify_detailed (:32-37) */
+    out->feasible = 0;
+    snprintf(out->message, sizeof(out->message), "Primal infeasible: %s", pmsg);
+  }
+
+  /* check_dual_feasibility (:193-296): reduced cost c_j - sum_k a_k y_row(k)
+   * over the nonzeros of column j in storage order, accumulated here for all
+   * columns in one pass over the rows (each column's sum sees the same terms
+   * in the same order as the reference's per-column scan). */
+  double dual_inf = 0.0;
+  if (!(n_dual != m || (n_int > 0 && n_rc != n))) {
+    const int use_rc = n_rc == n;
+    double* acc = NULL;
+    if (!use_rc) {
+      acc = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+      for (int64_t i = 0; i < m; ++i)
+        for (int64_t k = p->rp[i]; k < p->rp[i + 1]; ++k) acc[p->ci[k]] += p->v[k] * dual[i];
+    }
+    for (int64_t j = 0; j < n; ++j) {
+      double r = use_rc ? rc[j] : p->obj[j] - acc[j];
+      if (p->maximize) r = -r;
+      const double val = primal[j];
+      double violation;
+      if (val <= p->lb[j] + feas)
+        violation = (r < 0.0) ? r : 0.0;
+      else if (val >= p->ub[j] - feas)
+        violation = (0.0 < r) ? r : 0.0;
+      else
+        violation = fabs(r);
+      violation = fabs(violation) - otol;
+      if (violation > 0.0) dual_inf = vmax(dual_inf, violation);
+    }
+    free(acc);
+  }
+  out->dual_infeasibility = dual_inf;
+
+  /* check_complementarity (:298-347) */
+  double comp = 0.0;
+  if (n_dual == m) {
+    for (int64_t i = 0; i < m; ++i) {
+      double activity = 0.0, activity_abs = 0.0;
+      for (int64_t k = p->rp[i]; k < p->rp[i + 1]; ++k) {
+        const double term = p->v[k] * primal[p->ci[k]];
+        activity += term;
+        activity_abs += fabs(term);
+      }
+      double slack = 0.0;
+      if (p->sense[i] == OR_LE)
+        slack = p->rhs[i] - activity;
+      else if (p->sense[i] == OR_GE)
+        slack = activity - p->rhs[i];
+      double scale = 1.0;  /* std::max({1.0, |rhs|, activity_abs}) */
+      if (scale < fabs(p->rhs[i])) scale = fabs(p->rhs[i]);
+      if (scale < activity_abs) scale = activity_abs;
+      const double dual_scale = vmax(1.0, fabs(dual[i]));
+      comp = vmax(comp, fabs(slack) / scale * (fabs(dual[i]) / dual_scale));
+    }
+  }
+  out->complementarity = comp;
+  if (n_int == 0) {  /* :48-60 */
+    if (dual_inf > otol) {
+      out->optimal = 0;
+      if (out->feasible)
+        snprintf(out->message, sizeof(out->message), "Dual infeasible: Max dual violation: %f", dual_inf);
+    }
+    if (comp > ctol) {
+      out->optimal = 0;
+      if (out->feasible)
+        snprintf(out->message, sizeof(out->message),
+                 "Complementarity violation: Max complementarity violation: %f", comp);
+    }
+  }
+
+  /* check_integrality (:349-371) */
+  double integ = 0.0;
+  if (var_type)
+    for (int64_t j = 0; j < n; ++j)
+      if (var_type[j] == 1 || var_type[j] == 2) integ = vmax(integ, fabs(primal[j] - round(primal[j])));
+  out->integrality_violation = integ;
+  if (integ > itol) {
+    out->feasible = 0;
+    snprintf(out->message, sizeof(out->message), "Integrality violation: Max integrality violation: %f", integ);
+  }
+
+  /* objective (:70-87) */
+  double obj = p->obj_offset;
+  for (int64_t j = 0; j < n; ++j) obj += p->obj[j] * primal[j];
+  out->objective_discrepancy = fabs(obj - objective_value);
+  if (out->objective_discrepancy > 1e-4 * (1.0 + fabs(obj))) {
+    out->optimal = 0;
+    if (out->feasible)
+      snprintf(out->message, sizeof(out->message), "Objective discrepancy: %f", out->objective_discrepancy);
+  }
 }

@@ -124,6 +124,28 @@ int or_standard_form(const or_problem* p, int64_t* sm, int64_t* sn, int64_t* snn
                      int64_t* rp, int64_t* ci, double* v, double* rhs, double* c,
                      double* col_scale, double* offset);
 
+/* ---- post-solve verification ------------------------------------------- */
+typedef struct or_verify_out {  /* validation::VerificationResult (solution_verifier.hpp:12-21) */
+  int feasible, optimal;
+  double primal_infeasibility, dual_infeasibility, complementarity, integrality_violation,
+      objective_discrepancy;
+  int64_t n_violated;           /* length of violated_constraints */
+  int64_t* violated;            /* first min(n_violated, violated_cap) entries (may be NULL) */
+  int64_t violated_cap;
+  char message[160];
+} or_verify_out;
+
+/* SolutionVerifier::verify_detailed (core/validation/solution_verifier.cpp:24-88)
+ * with its checks (:92-371) for a problem without quadratic terms.
+ * var_type: n entries of model::VarType (0 continuous, 1 integer, 2 binary;
+ * NULL = all continuous).  n_dual / n_rc are the lengths of the solution's
+ * dual and reduced_costs vectors.  tol = (feasibility, optimality,
+ * complementarity, integrality).  With n_primal != n only the size-mismatch
+ * result is produced (the reference reads past the vector after it). */
+void or_verify(const or_problem* p, const int8_t* var_type, const double* primal, int64_t n_primal,
+               const double* dual, int64_t n_dual, const double* rc, int64_t n_rc,
+               double objective_value, const double tol[4], or_verify_out* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,13 @@ class _Result(C.Structure):
                 ("rd_norm", C.c_double), ("mu", C.c_double), ("trace", _dp), ("trace_cap", _i64)]
 
 
+class _VerifyOut(C.Structure):
+    _fields_ = [("feasible", C.c_int), ("optimal", C.c_int), ("primal_infeasibility", C.c_double),
+                ("dual_infeasibility", C.c_double), ("complementarity", C.c_double),
+                ("integrality_violation", C.c_double), ("objective_discrepancy", C.c_double),
+                ("n_violated", _i64), ("violated", _i64p), ("violated_cap", _i64), ("message", C.c_char * 160)]
+
+
 def _load_oracle():
     if not os.path.exists(ORACLE_SO):
         build()
@@ -81,6 +88,9 @@ def _load_oracle():
     lib.or_standard_form.argtypes = [C.POINTER(_Problem), _i64p, _i64p, _i64p, _i64p, _i64p, _dp, _dp,
                                      _dp, _dp, _dp]
     lib.or_standard_form.restype = C.c_int
+    lib.or_verify.argtypes = [C.POINTER(_Problem), _i8p, _dp, _i64, _dp, _i64, _dp, _i64, C.c_double, _dp,
+                              C.POINTER(_VerifyOut)]
+    lib.or_verify.restype = None
     return lib
 
 
@@ -101,6 +111,10 @@ def _ref_lib():
     lib.ref_cpu_spmv_t_time.restype = C.c_double
     lib.ref_transpose.argtypes = [_i64, _i64, _i64p, _i64p, _dp, _i64p, _i64p, _dp]
     lib.ref_transpose.restype = _i64
+    if hasattr(lib, "ref_verify"):
+        lib.ref_verify.argtypes = [_i64, _i64, _i64p, _i64p, _dp, _i8p, _dp, _dp, _dp, _dp, C.c_int, C.c_double,
+                                   _i8p, _dp, _i64, _dp, _i64, _dp, _i64, C.c_double, _dp, C.POINTER(C.c_int),
+                                   _dp, _i64p, _i64, _i64p, C.c_char_p, _i64, _dp]
     return lib
 
 
@@ -341,3 +355,79 @@ def ref_transpose(rp, ci, v, ncols: int):
     REF.ref_transpose(rp.size - 1, ncols, _p(rp, _i64p), _p(ci, _i64p), _p(v, _dp), _p(orp, _i64p),
                       _p(oci, _i64p), _p(ov, _dp))
     return orp, oci, ov
+
+
+# ---- post-solve verification ------------------------------------------------
+@dataclass
+class VerifyResult:
+    """validation::VerificationResult (solution_verifier.hpp:12-21)."""
+    feasible: bool
+    optimal: bool
+    primal_infeasibility: float
+    dual_infeasibility: float
+    complementarity: float
+    integrality_violation: float
+    objective_discrepancy: float
+    violated: list
+    n_violated: int
+    message: str
+    seconds: float = 0.0
+
+
+DEFAULT_TOL = (1e-9, 1e-9, 1e-6, 1e-6)  # feasibility, optimality, complementarity, integrality
+
+
+def _vec(a):
+    return None if a is None else _f64(a)
+
+
+def verify(lp, primal, dual=None, reduced_costs=None, objective_value: float = 0.0, var_type=None,
+           tol=DEFAULT_TOL, cap: int = 1 << 20) -> VerifyResult:
+    """The restated SolutionVerifier::verify_detailed.  dual / reduced_costs
+    of None are empty vectors, as in a default model::Solution."""
+    p, keep = _problem(lp)
+    primal, dual, rc = _f64(primal), _vec(dual), _vec(reduced_costs)
+    vt = None if var_type is None else np.ascontiguousarray(var_type, np.int8)
+    t = _f64(tol)
+    viol = np.zeros(max(cap, 1), np.int64)
+    out = _VerifyOut()
+    out.violated = _p(viol, _i64p)
+    out.violated_cap = cap
+    OR.or_verify(C.byref(p), None if vt is None else _p(vt, _i8p), _p(primal, _dp), primal.size,
+                 None if dual is None else _p(dual, _dp), 0 if dual is None else dual.size,
+                 None if rc is None else _p(rc, _dp), 0 if rc is None else rc.size, float(objective_value),
+                 _p(t, _dp), C.byref(out))
+    k = min(out.n_violated, cap)
+    return VerifyResult(bool(out.feasible), bool(out.optimal), out.primal_infeasibility, out.dual_infeasibility,
+                        out.complementarity, out.integrality_violation, out.objective_discrepancy,
+                        viol[:k].tolist(), int(out.n_violated), out.message.decode())
+
+
+def ref_verify(lp, primal, dual=None, reduced_costs=None, objective_value: float = 0.0, var_type=None,
+               tol=DEFAULT_TOL, cap: int = 1 << 20) -> VerifyResult:
+    """The reference's own SolutionVerifier::verify_detailed, from _ref/.  The
+    problem goes through ProblemBuilder::build (rows sorted, duplicates
+    summed), so pass rows in that form to compare with verify()."""
+    if REF is None or not hasattr(REF, "ref_verify"):
+        raise RuntimeError("reference build not present")
+    keep = dict(rp=_i64a(lp.row_ptr), ci=_i64a(lp.col), v=_f64(lp.val), sense=np.ascontiguousarray(lp.sense, np.int8),
+                rhs=_f64(lp.rhs), lb=_f64(lp.lb), ub=_f64(lp.ub), obj=_f64(lp.obj))
+    primal, dual, rc = _f64(primal), _vec(dual), _vec(reduced_costs)
+    vt = None if var_type is None else np.ascontiguousarray(var_type, np.int8)
+    t = _f64(tol)
+    flags = (C.c_int * 2)()
+    vals = np.zeros(5)
+    viol = np.zeros(max(cap, 1), np.int64)
+    nv = _i64()
+    msg = C.create_string_buffer(512)
+    sec = C.c_double()
+    REF.ref_verify(lp.m, lp.n, _p(keep["rp"], _i64p), _p(keep["ci"], _i64p), _p(keep["v"], _dp),
+                   _p(keep["sense"], _i8p), _p(keep["rhs"], _dp), _p(keep["lb"], _dp), _p(keep["ub"], _dp),
+                   _p(keep["obj"], _dp), int(lp.maximize), float(getattr(lp, "obj_offset", 0.0)),
+                   None if vt is None else _p(vt, _i8p), _p(primal, _dp), primal.size,
+                   None if dual is None else _p(dual, _dp), 0 if dual is None else dual.size,
+                   None if rc is None else _p(rc, _dp), 0 if rc is None else rc.size, float(objective_value),
+                   _p(t, _dp), flags, _p(vals, _dp), _p(viol, _i64p), cap, C.byref(nv), msg, 512, C.byref(sec))
+    k = min(nv.value, cap)
+    return VerifyResult(bool(flags[0]), bool(flags[1]), *vals.tolist(), viol[:k].tolist(), int(nv.value),
+                        msg.value.decode(), sec.value)

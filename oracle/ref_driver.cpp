@@ -7,8 +7,11 @@
 // (core/model/problem.hpp:199-292), InteriorPointSolver
 // (core/lp/interior_point.hpp:16-54), ComputeBackendFactory
 // (core/execution/gpu_backend.hpp:53-58) and SparseMatrix
-// (core/numerical/sparse_matrix.hpp:30-89).
+// (core/numerical/sparse_matrix.hpp:30-89) and SolutionVerifier
+// (core/validation/solution_verifier.hpp:23-41).
+#include <chrono>
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <ctime>
@@ -24,6 +27,7 @@
 #include "model/problem.hpp"
 #include "numerical/sparse_matrix.hpp"
 #include "numerical/tolerance.hpp"
+#include "validation/solution_verifier.hpp"
 
 using namespace hypernova;
 
@@ -157,6 +161,64 @@ int64_t ref_transpose(int64_t nrows, int64_t ncols, const int64_t* rp, const int
   if (out_v)
     for (std::size_t k = 0; k < At.nnz(); ++k) out_v[k] = At.values()[k];
   return (int64_t)At.nnz();
+}
+
+// SolutionVerifier::verify_detailed on the problem ProblemBuilder builds
+// from the arguments.  var_type: n model::VarType codes (NULL: continuous);
+// n_dual / n_rc: the lengths given to the solution's dual and reduced_costs
+// (entries are read only when a vector is not NULL).  flags = (feasible,
+// optimal); vals = (primal, dual, complementarity, integrality, objective
+// discrepancy); *seconds = wall time of verify_detailed.
+int ref_verify(int64_t m, int64_t n, const int64_t* rp, const int64_t* ci, const double* v,
+               const int8_t* sense, const double* rhs, const double* lb, const double* ub,
+               const double* obj, int maximize, double obj_offset, const int8_t* var_type,
+               const double* primal, int64_t n_primal, const double* dual, int64_t n_dual,
+               const double* rc, int64_t n_rc, double objective_value, const double* tol, int* flags,
+               double* vals, int64_t* violated, int64_t cap, int64_t* n_violated, char* msg,
+               int64_t msg_cap, double* seconds) {
+  model::ProblemBuilder b("ref");
+  for (int64_t j = 0; j < n; ++j)
+    b.add_variable(lb[j], ub[j], var_type ? static_cast<model::VarType>(var_type[j]) : model::VarType::CONTINUOUS);
+  for (int64_t i = 0; i < m; ++i) {
+    std::vector<std::pair<std::size_t, double>> coeffs;
+    for (int64_t k = rp[i]; k < rp[i + 1]; ++k) coeffs.push_back({(std::size_t)ci[k], v[k]});
+    b.add_constraint(coeffs, static_cast<model::ConstraintSense>(sense[i]), rhs[i]);
+  }
+  std::vector<std::pair<std::size_t, double>> o;
+  for (int64_t j = 0; j < n; ++j) o.push_back({(std::size_t)j, obj[j]});
+  b.set_objective(o, maximize ? model::ObjectiveSense::MAXIMIZE : model::ObjectiveSense::MINIMIZE);
+  model::Problem prob = b.build();
+  prob.obj_offset = obj_offset;
+
+  model::Solution sol;
+  sol.objective_value = objective_value;
+  sol.primal.assign(primal, primal + n_primal);
+  if (dual)
+    sol.dual.assign(dual, dual + n_dual);
+  else
+    sol.dual.assign((std::size_t)n_dual, 0.0);
+  if (rc)
+    sol.reduced_costs.assign(rc, rc + n_rc);
+  else
+    sol.reduced_costs.assign((std::size_t)n_rc, 0.0);
+  numerical::ToleranceConfig t = numerical::ToleranceConfig::industrial_defaults();
+  if (tol) t.feasibility_tol(tol[0]).optimality_tol(tol[1]).complementarity_tol(tol[2]).integrality_tol(tol[3]);
+  validation::SolutionVerifier verifier(t);
+  const auto w0 = std::chrono::steady_clock::now();
+  validation::VerificationResult r = verifier.verify_detailed(prob, sol);
+  const auto w1 = std::chrono::steady_clock::now();
+  if (seconds) *seconds = std::chrono::duration<double>(w1 - w0).count();
+  flags[0] = r.feasible;
+  flags[1] = r.optimal;
+  vals[0] = r.primal_infeasibility;
+  vals[1] = r.dual_infeasibility;
+  vals[2] = r.complementarity;
+  vals[3] = r.integrality_violation;
+  vals[4] = r.objective_discrepancy;
+  *n_violated = (int64_t)r.violated_constraints.size();
+  for (int64_t q = 0; q < cap && q < *n_violated; ++q) violated[q] = (int64_t)r.violated_constraints[q];
+  if (msg && msg_cap > 0) std::snprintf(msg, (size_t)msg_cap, "%s", r.message.c_str());
+  return 0;
 }
 
 }  // extern "C"
