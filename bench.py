@@ -457,6 +457,7 @@ def run_verify(args) -> dict:
     200,000 x 500,000 power-law LP."""
     import hypernova_hip as H
     import lpgen
+    H.init(-1)
     out = {}
     for name, make in (("2000x5000", lambda: lpgen.random_lp(1, 2000, 5000)),
                        ("200000x500000", lambda: lpgen.powerlaw_lp(3, 200_000, 500_000))):

@@ -111,6 +111,13 @@ def _ref_lib():
     lib.ref_cpu_spmv_t_time.restype = C.c_double
     lib.ref_transpose.argtypes = [_i64, _i64, _i64p, _i64p, _dp, _i64p, _i64p, _dp]
     lib.ref_transpose.restype = _i64
+    if hasattr(lib, "ref_crossover"):
+        lib.ref_ipm_solve_crossover.argtypes = [_i64, _i64, _i64p, _i64p, _dp, _i8p, _dp, _dp, _dp, _dp, C.c_int,
+                                                C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), _i64p, _dp, _dp,
+                                                _dp]
+        lib.ref_crossover.argtypes = [_i64, _i64, _i64p, _i64p, _dp, _i8p, _dp, _dp, _dp, _dp, C.c_int, C.c_double,
+                                      C.c_int, C.c_double, _i64, _dp, C.c_int, C.POINTER(C.c_int), _i64p, _dp, _dp,
+                                      _dp]
     if hasattr(lib, "ref_verify"):
         lib.ref_verify.argtypes = [_i64, _i64, _i64p, _i64p, _dp, _i8p, _dp, _dp, _dp, _dp, C.c_int, C.c_double,
                                    _i8p, _dp, _i64, _dp, _i64, _dp, _i64, C.c_double, _dp, C.POINTER(C.c_int),
@@ -216,6 +223,7 @@ class IpmResult:
     mu: float = 0.0
     trace: Optional[np.ndarray] = None
     solve_ms: float = 0.0
+    dual: Optional[np.ndarray] = None
 
 
 def _problem(lp):
@@ -431,3 +439,47 @@ def ref_verify(lp, primal, dual=None, reduced_costs=None, objective_value: float
     k = min(nv.value, cap)
     return VerifyResult(bool(flags[0]), bool(flags[1]), *vals.tolist(), viol[:k].tolist(), int(nv.value),
                         msg.value.decode(), sec.value)
+
+
+# ---- crossover (the reference's simplex polish) -------------------------------
+def _lp_keep(lp):
+    return dict(rp=_i64a(lp.row_ptr), ci=_i64a(lp.col), v=_f64(lp.val), sense=np.ascontiguousarray(lp.sense, np.int8),
+                rhs=_f64(lp.rhs), lb=_f64(lp.lb), ub=_f64(lp.ub), obj=_f64(lp.obj))
+
+
+def _lp_args(lp, keep):
+    return (lp.m, lp.n, _p(keep["rp"], _i64p), _p(keep["ci"], _i64p), _p(keep["v"], _dp), _p(keep["sense"], _i8p),
+            _p(keep["rhs"], _dp), _p(keep["lb"], _dp), _p(keep["ub"], _dp), _p(keep["obj"], _dp), int(lp.maximize),
+            float(getattr(lp, "obj_offset", 0.0)))
+
+
+def ref_ipm_solve_crossover(lp, max_iterations: int = 100, crossover_max_iter: int = 100) -> IpmResult:
+    """The reference's InteriorPointSolver with crossover on (its default)."""
+    if REF is None or not hasattr(REF, "ref_crossover"):
+        raise RuntimeError("reference build not present")
+    keep = _lp_keep(lp)
+    st, it, obj = C.c_int(), _i64(), C.c_double()
+    primal, dual = np.zeros(max(lp.n, 1)), np.zeros(max(lp.m, 1))
+    REF.ref_ipm_solve_crossover(*_lp_args(lp, keep), max_iterations, crossover_max_iter, C.byref(st), C.byref(it),
+                                C.byref(obj), _p(primal, _dp), _p(dual, _dp))
+    return IpmResult(status=STATUS[st.value], iterations=int(it.value), objective=obj.value,
+                     primal=primal[:lp.n].copy(), dual=dual[:lp.m].copy())
+
+
+def ref_crossover(lp, status: str, objective: float, iterations: int, primal,
+                  crossover_max_iter: int = 100) -> IpmResult:
+    """The crossover block of the reference's solve (interior_point.cpp:786-878)
+    from a given barrier endpoint, on the reference's SimplexSolver."""
+    if REF is None or not hasattr(REF, "ref_crossover"):
+        raise RuntimeError("reference build not present")
+    keep = _lp_keep(lp)
+    x = _f64(primal)
+    st, it, obj = C.c_int(), _i64(), C.c_double()
+    out_x, out_y = np.zeros(max(lp.n, 1)), np.zeros(max(lp.m, 1))
+    rc = REF.ref_crossover(*_lp_args(lp, keep), STATUS.index(status), float(objective), int(iterations),
+                           _p(x, _dp), crossover_max_iter, C.byref(st), C.byref(it), C.byref(obj), _p(out_x, _dp),
+                           _p(out_y, _dp))
+    if rc == -2:
+        raise ValueError("problem has duplicate rows (the reference polishes the deduplicated problem)")
+    return IpmResult(status=STATUS[st.value], iterations=int(it.value), objective=obj.value,
+                     primal=out_x[:lp.n].copy(), dual=out_y[:lp.m].copy())

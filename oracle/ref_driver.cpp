@@ -16,6 +16,7 @@
 #include <cstring>
 #include <ctime>
 #include <iostream>
+#include <map>
 #include <limits>
 #include <memory>
 #include <sstream>
@@ -24,6 +25,7 @@
 
 #include "execution/gpu_backend.hpp"
 #include "lp/interior_point.hpp"
+#include "lp/simplex.hpp"
 #include "model/problem.hpp"
 #include "numerical/sparse_matrix.hpp"
 #include "numerical/tolerance.hpp"
@@ -161,6 +163,151 @@ int64_t ref_transpose(int64_t nrows, int64_t ncols, const int64_t* rp, const int
   if (out_v)
     for (std::size_t k = 0; k < At.nnz(); ++k) out_v[k] = At.values()[k];
   return (int64_t)At.nnz();
+}
+
+namespace {
+
+model::Problem build_problem(int64_t m, int64_t n, const int64_t* rp, const int64_t* ci, const double* v,
+                             const int8_t* sense, const double* rhs, const double* lb, const double* ub,
+                             const double* obj, int maximize, double obj_offset) {
+  model::ProblemBuilder b("ref");
+  for (int64_t j = 0; j < n; ++j) b.add_variable(lb[j], ub[j], model::VarType::CONTINUOUS);
+  for (int64_t i = 0; i < m; ++i) {
+    std::vector<std::pair<std::size_t, double>> coeffs;
+    for (int64_t k = rp[i]; k < rp[i + 1]; ++k) coeffs.push_back({(std::size_t)ci[k], v[k]});
+    b.add_constraint(coeffs, static_cast<model::ConstraintSense>(sense[i]), rhs[i]);
+  }
+  std::vector<std::pair<std::size_t, double>> o;
+  for (int64_t j = 0; j < n; ++j) o.push_back({(std::size_t)j, obj[j]});
+  b.set_objective(o, maximize ? model::ObjectiveSense::MAXIMIZE : model::ObjectiveSense::MINIMIZE);
+  model::Problem prob = b.build();
+  prob.obj_offset = obj_offset;
+  return prob;
+}
+
+// True when two rows of the built problem carry the same entries, sense and
+// rhs (the rows remove_duplicate_rows, interior_point.cpp:82-132, would drop).
+bool has_duplicate_rows(const model::Problem& p) {
+  std::map<std::string, int> seen;
+  const auto& A = p.constraint_matrix;
+  for (std::size_t i = 0; i < p.constraints.size(); ++i) {
+    std::ostringstream key;
+    key.precision(17);
+    key << static_cast<int>(p.constraints[i].sense) << ' ' << p.constraints[i].rhs;
+    for (std::size_t k = A.row_ptr()[i]; k < A.row_ptr()[i + 1]; ++k)
+      key << ' ' << A.col_indices()[k] << ':' << A.values()[k];
+    if (!seen.emplace(key.str(), 1).second) return true;
+  }
+  return false;
+}
+
+// Silences the solvers' progress prints on std::cerr for one scope.
+struct QuietCerr {
+  std::ostringstream sink;
+  std::streambuf* old;
+  QuietCerr() : old(std::cerr.rdbuf(sink.rdbuf())) {}
+  ~QuietCerr() { std::cerr.rdbuf(old); }
+};
+
+void copy_out(const std::vector<double>& src, double* dst, int64_t cap) {
+  if (!dst) return;
+  for (int64_t q = 0; q < cap; ++q) dst[q] = q < (int64_t)src.size() ? src[q] : 0.0;
+}
+
+}  // namespace
+
+// The reference's whole solve with crossover on (InteriorPointSolver::solve,
+// interior_point.cpp:402-886): status, objective, iterations, primal (n) and
+// dual (m).
+int ref_ipm_solve_crossover(int64_t m, int64_t n, const int64_t* rp, const int64_t* ci, const double* v,
+                            const int8_t* sense, const double* rhs, const double* lb, const double* ub,
+                            const double* obj, int maximize, double obj_offset, int max_iterations,
+                            int crossover_max_iter, int* status, int64_t* iterations, double* objective,
+                            double* primal, double* dual) {
+  model::Problem prob = build_problem(m, n, rp, ci, v, sense, rhs, lb, ub, obj, maximize, obj_offset);
+  lp::InteriorPointOptions opts;
+  opts.max_iterations = max_iterations;
+  opts.crossover = true;
+  opts.crossover_max_iter = crossover_max_iter;
+  lp::InteriorPointSolver solver(numerical::ToleranceConfig::industrial_defaults(), opts);
+  QuietCerr quiet;
+  lp::InteriorPointResult res = solver.solve(prob);
+  *status = static_cast<int>(res.status);
+  *iterations = (int64_t)res.iterations;
+  *objective = res.objective_value;
+  copy_out(res.primal, primal, n);
+  copy_out(res.dual, dual, m);
+  return 0;
+}
+
+// The crossover block of InteriorPointSolver::solve (interior_point.cpp:786-878)
+// run on the reference's SimplexSolver from a barrier endpoint given by the
+// caller (for example the GPU IPM's): barrier status / objective / iterations /
+// primal in, final ones plus the dual out.  No time limit and no interrupt,
+// as in a default solve.  The reference polishes the problem after
+// remove_duplicate_rows; a problem with duplicate rows returns -2 here.
+int ref_crossover(int64_t m, int64_t n, const int64_t* rp, const int64_t* ci, const double* v,
+                  const int8_t* sense, const double* rhs, const double* lb, const double* ub, const double* obj,
+                  int maximize, double obj_offset, int barrier_status, double barrier_objective,
+                  int64_t barrier_iterations, const double* barrier_primal, int crossover_max_iter, int* status,
+                  int64_t* iterations, double* objective, double* primal, double* dual) {
+  model::Problem cleaned = build_problem(m, n, rp, ci, v, sense, rhs, lb, ub, obj, maximize, obj_offset);
+  if (has_duplicate_rows(cleaned)) return -2;
+  model::ProblemStatus st = static_cast<model::ProblemStatus>(barrier_status);
+  const bool done = st == model::ProblemStatus::OPTIMAL;
+  double objective_value = barrier_objective;
+  int64_t iters = barrier_iterations;
+  std::vector<double> x(barrier_primal, barrier_primal + n), y((std::size_t)m, 0.0);
+  auto adopt = [&](const lp::SimplexResult& sim) {
+    if (sim.status == model::ProblemStatus::OPTIMAL) {
+      st = model::ProblemStatus::OPTIMAL;
+      objective_value = sim.objective_value;
+      x = sim.primal;
+      y = sim.dual;
+      iters = (int64_t)sim.iterations;
+    } else if (!done && sim.status == model::ProblemStatus::INFEASIBLE) {
+      st = model::ProblemStatus::INFEASIBLE;
+    } else if (sim.status == model::ProblemStatus::UNBOUNDED) {
+      st = model::ProblemStatus::UNBOUNDED;
+    }
+  };
+  const numerical::ToleranceConfig tol = numerical::ToleranceConfig::industrial_defaults();
+  lp::SimplexOptions sim_opts;
+  sim_opts.time_limit_seconds = 0.0;
+  std::vector<int> var_status(cleaned.variables.size(), 0);
+  for (std::size_t j = 0; j < cleaned.variables.size(); ++j) {
+    const double l = cleaned.variables[j].lower_bound, u = cleaned.variables[j].upper_bound;
+    const double at_tol = 1e-6 * (1.0 + std::max(std::abs(l), std::abs(u)));
+    if (barrier_primal[j] <= l + at_tol)
+      var_status[j] = -1;
+    else if (barrier_primal[j] >= u - at_tol)
+      var_status[j] = 1;
+  }
+  bool resolved = false;
+  QuietCerr quiet;
+  try {
+    lp::SimplexOptions warm_opts = sim_opts;
+    warm_opts.max_iterations = crossover_max_iter;
+    lp::SimplexSolver polisher(tol, warm_opts);
+    lp::SimplexResult sim = polisher.solve_with_basis(cleaned, var_status);
+    adopt(sim);
+    resolved = sim.status == model::ProblemStatus::OPTIMAL || sim.status == model::ProblemStatus::UNBOUNDED ||
+               (sim.status == model::ProblemStatus::INFEASIBLE && !done);
+  } catch (...) {
+  }
+  if (!resolved) {
+    try {
+      lp::SimplexSolver polisher(tol, sim_opts);
+      adopt(polisher.solve(cleaned));
+    } catch (...) {
+    }
+  }
+  *status = static_cast<int>(st);
+  *iterations = iters;
+  *objective = objective_value;
+  copy_out(x, primal, n);
+  copy_out(y, dual, m);
+  return 0;
 }
 
 // SolutionVerifier::verify_detailed on the problem ProblemBuilder builds

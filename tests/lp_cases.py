@@ -99,3 +99,20 @@ REFERENCE_CASES = {
     "forced_mode_lp": forced_mode_lp,
     "mixed_bounds_lp": mixed_bounds_lp,
 }
+
+
+def crossover_cases():
+    """(name, lp, max_iterations, crossover_max_iter) for the crossover tests:
+    the reference's three crossover tests (test_lp.cpp:327-400: stalled
+    barrier, converged barrier, warm polish capped at one pivot) and small
+    seeded LPs the reference's simplex polishes in well under a second.
+    None has duplicate rows, so the polished problem is the problem itself."""
+    import lpgen
+    out = [("resolves_stall", simple_lp(), 1, 100), ("exact_certificate", simple_lp(), 100, 100),
+           ("cold_fallback", simple_lp(), 1, 1), ("integration", integration_lp(), 100, 100)]
+    for sd in (1, 2, 3):
+        out.append((f"random{sd}_20x50", lpgen.random_lp(sd, 20, 50), 100, 100))
+        out.append((f"random{sd}_60x150", lpgen.random_lp(sd, 60, 150), 100, 100))
+    out.append(("transport5x8", lpgen.transport_lp(1000000, 5, 8), 100, 100))
+    out.append(("transport10x15", lpgen.transport_lp(1000001, 10, 15), 100, 100))
+    return out
