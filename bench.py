@@ -437,6 +437,31 @@ def run_lp(args) -> dict:
             "sample": (f"{', '.join(conv)}: {dt:.2f} s on the host ({', '.join(cpu[n].status for n in conv)}); "
                        f"the GPU solver takes {gpu_t:.3f} s on the same models")}
         log(f"cpu netlib baseline {out['netlib']['cpu_baseline']}")
+        # the reference's presolve and postsolve, which Solver::solve runs around
+        # the IPM (api.cpp:384-412, :176-196): their share of the end-to-end time
+        # beside the GPU solve of the same models
+        if O.REF is not None and hasattr(O.REF, "ref_presolve"):
+            pre_ms = post_ms = 0.0
+            reduced = 0
+            for name in sorted(fx):
+                P = O.RefPresolve(models[name])
+                _, _, _, pms = P.postsolve(np.zeros(P.reduced.n))
+                pre_ms += P.ms
+                post_ms += pms
+                reduced += (P.reduced.m, P.reduced.n) != (models[name].m, models[name].n)
+            P = O.RefPresolve(lp)
+            _, _, _, a_post = P.postsolve(np.zeros(P.reduced.n))
+            a_gpu = single["plain_f64"]["ms"]
+            gpu_netlib = 1e3 * t_all
+            out["presolve"] = {
+                "netlib": {"presolve_ms": round(pre_ms, 2), "postsolve_ms": round(post_ms, 2),
+                           "gpu_solve_ms": round(gpu_netlib, 1), "models_reduced": reduced,
+                           "share": round((pre_ms + post_ms) / (pre_ms + post_ms + gpu_netlib), 4)},
+                "2000x5000": {"presolve_ms": round(P.ms, 2), "postsolve_ms": round(a_post, 3),
+                              "gpu_solve_ms": a_gpu, "rows_cols_removed": [lp.m - P.reduced.m, lp.n - P.reduced.n],
+                              "share": round((P.ms + a_post) / (P.ms + a_post + a_gpu), 4)},
+                "kind": "reference (presolve::Presolver, AGGRESSIVE, 1 thread)"}
+            log(f"presolve share {out['presolve']}")
     return out
 
 

@@ -118,6 +118,14 @@ def _ref_lib():
         lib.ref_crossover.argtypes = [_i64, _i64, _i64p, _i64p, _dp, _i8p, _dp, _dp, _dp, _dp, C.c_int, C.c_double,
                                       C.c_int, C.c_double, _i64, _dp, C.c_int, C.POINTER(C.c_int), _i64p, _dp, _dp,
                                       _dp]
+    if hasattr(lib, "ref_presolve"):
+        lib.ref_presolve.argtypes = [_i64, _i64, _i64p, _i64p, _dp, _i8p, _dp, _dp, _dp, _dp, C.c_int, C.c_double,
+                                     C.c_int, C.POINTER(C.c_void_p), _i64p, _dp]
+        lib.ref_presolve_reduced.argtypes = [C.c_void_p, _i64p, _i64p, _dp, _i8p, _dp, _dp, _dp, _dp,
+                                             C.POINTER(C.c_int), _dp]
+        lib.ref_postsolve.argtypes = [C.c_void_p, _dp, _i64, _dp, _i64, _dp, _dp, _dp, _dp]
+        lib.ref_presolve_free.argtypes = [C.c_void_p]
+        lib.ref_presolve_free.restype = None
     if hasattr(lib, "ref_verify"):
         lib.ref_verify.argtypes = [_i64, _i64, _i64p, _i64p, _dp, _i8p, _dp, _dp, _dp, _dp, C.c_int, C.c_double,
                                    _i8p, _dp, _i64, _dp, _i64, _dp, _i64, C.c_double, _dp, C.POINTER(C.c_int),
@@ -483,3 +491,49 @@ def ref_crossover(lp, status: str, objective: float, iterations: int, primal,
         raise ValueError("problem has duplicate rows (the reference polishes the deduplicated problem)")
     return IpmResult(status=STATUS[st.value], iterations=int(it.value), objective=obj.value,
                      primal=out_x[:lp.n].copy(), dual=out_y[:lp.m].copy())
+
+
+# ---- presolve / postsolve (the reference's Presolver) ----------------------------
+class RefPresolve:
+    """presolve::Presolver run as Solver::solve runs it (level 2 = AGGRESSIVE,
+    the default).  .reduced is the reduced LP, .ms the presolve time;
+    postsolve() maps a reduced (primal, dual) back and recomputes the
+    objective on the original problem."""
+
+    def __init__(self, lp, level: int = 2):
+        if REF is None or not hasattr(REF, "ref_presolve"):
+            raise RuntimeError("reference build not present")
+        import lpgen
+        keep = _lp_keep(lp)
+        self._h = C.c_void_p()
+        dims = np.zeros(3, np.int64)
+        ms = C.c_double()
+        rc = REF.ref_presolve(*_lp_args(lp, keep), level, C.byref(self._h), _p(dims, _i64p), C.byref(ms))
+        if rc != 0:
+            raise ValueError(f"reference presolve returned {rc}")
+        self.ms = ms.value
+        m, n, z = (int(d) for d in dims)
+        rp, ci, v = np.zeros(m + 1, np.int64), np.zeros(max(z, 1), np.int64), np.zeros(max(z, 1))
+        sense, rhs = np.zeros(max(m, 1), np.int8), np.zeros(max(m, 1))
+        lb, ub, obj = np.zeros(max(n, 1)), np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        mx, off = C.c_int(), C.c_double()
+        REF.ref_presolve_reduced(self._h, _p(rp, _i64p), _p(ci, _i64p), _p(v, _dp), _p(sense, _i8p), _p(rhs, _dp),
+                                 _p(lb, _dp), _p(ub, _dp), _p(obj, _dp), C.byref(mx), C.byref(off))
+        self.reduced = lpgen.LP(m=m, n=n, row_ptr=rp, col=ci[:z].copy(), val=v[:z].copy(), rhs=rhs[:m].copy(),
+                                sense=sense[:m].copy(), lb=lb[:n].copy(), ub=ub[:n].copy(), obj=obj[:n].copy(),
+                                maximize=bool(mx.value), obj_offset=off.value)
+        self.m0, self.n0 = lp.m, lp.n
+
+    def postsolve(self, primal, dual=None):
+        x = _f64(primal)
+        y = _f64(dual if dual is not None else np.zeros(self.reduced.m))
+        px, py = np.zeros(max(self.n0, 1)), np.zeros(max(self.m0, 1))
+        obj, ms = C.c_double(), C.c_double()
+        REF.ref_postsolve(self._h, _p(x, _dp), x.size, _p(y, _dp), y.size, _p(px, _dp), _p(py, _dp), C.byref(obj),
+                          C.byref(ms))
+        return px[:self.n0].copy(), py[:self.m0].copy(), obj.value, ms.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) and REF is not None:
+            REF.ref_presolve_free(self._h)
+            self._h = None

@@ -29,6 +29,7 @@
 #include "model/problem.hpp"
 #include "numerical/sparse_matrix.hpp"
 #include "numerical/tolerance.hpp"
+#include "presolve/presolve.hpp"
 #include "validation/solution_verifier.hpp"
 
 using namespace hypernova;
@@ -309,6 +310,89 @@ int ref_crossover(int64_t m, int64_t n, const int64_t* rp, const int64_t* ci, co
   copy_out(y, dual, m);
   return 0;
 }
+
+// ---- presolve / postsolve (presolve::Presolver, core/presolve/presolve.hpp:49-85)
+struct RefPresolve {
+  model::Problem original;
+  presolve::PresolveResult res;
+  presolve::Presolver presolver;
+  RefPresolve(model::Problem p, const presolve::PresolveOptions& o) : original(std::move(p)), presolver(o) {}
+};
+
+// Presolves as Solver::solve does (api/cpp/api.cpp:384-393; level 2 =
+// AGGRESSIVE, the default, adds singleton removal, coefficient strengthening
+// and dual reductions; level 1 = CONSERVATIVE).  dims = (m, n, nnz) of the
+// reduced problem; returns -3 if its matrix is not stored by rows.
+int ref_presolve(int64_t m, int64_t n, const int64_t* rp, const int64_t* ci, const double* v, const int8_t* sense,
+                 const double* rhs, const double* lb, const double* ub, const double* obj, int maximize,
+                 double obj_offset, int level, void** handle, int64_t* dims, double* ms) {
+  presolve::PresolveOptions popts;
+  popts.remove_singletons = level == 2;
+  popts.coefficient_strengthening = level == 2;
+  popts.dual_reductions = level == 2;
+  auto* h = new RefPresolve(build_problem(m, n, rp, ci, v, sense, rhs, lb, ub, obj, maximize, obj_offset), popts);
+  const auto t0 = std::chrono::steady_clock::now();
+  h->res = h->presolver.presolve(h->original);
+  const auto t1 = std::chrono::steady_clock::now();
+  if (ms) *ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  const model::Problem& r = h->res.reduced_problem;
+  if (r.constraint_matrix.order() != model::StorageOrder::CSR) {
+    delete h;
+    return -3;
+  }
+  dims[0] = (int64_t)r.constraints.size();
+  dims[1] = (int64_t)r.variables.size();
+  dims[2] = (int64_t)r.constraint_matrix.nnz();
+  *handle = h;
+  return 0;
+}
+
+// The reduced problem in the hn_lp layout (buffers sized by ref_presolve's dims).
+int ref_presolve_reduced(void* handle, int64_t* rp, int64_t* ci, double* v, int8_t* sense, double* rhs, double* lb,
+                         double* ub, double* obj, int* maximize, double* obj_offset) {
+  const model::Problem& r = static_cast<RefPresolve*>(handle)->res.reduced_problem;
+  const auto& A = r.constraint_matrix;
+  for (std::size_t i = 0; i <= r.constraints.size(); ++i) rp[i] = i < A.row_ptr().size() ? (int64_t)A.row_ptr()[i] : 0;
+  for (std::size_t k = 0; k < A.nnz(); ++k) {
+    ci[k] = (int64_t)A.col_indices()[k];
+    v[k] = A.values()[k];
+  }
+  for (std::size_t i = 0; i < r.constraints.size(); ++i) {
+    sense[i] = static_cast<int8_t>(r.constraints[i].sense);
+    rhs[i] = r.constraints[i].rhs;
+  }
+  for (std::size_t j = 0; j < r.variables.size(); ++j) {
+    lb[j] = r.variables[j].lower_bound;
+    ub[j] = r.variables[j].upper_bound;
+    obj[j] = r.variables[j].objective_coeff;
+  }
+  *maximize = r.obj_sense == model::ObjectiveSense::MAXIMIZE;
+  *obj_offset = r.obj_offset;
+  return 0;
+}
+
+// Presolver::postsolve of a reduced solution (primal n2, dual m2 entries),
+// then the objective as Solver::solve recomputes it (api.cpp:92-106).
+int ref_postsolve(void* handle, const double* primal, int64_t n2, const double* dual, int64_t m2,
+                  double* primal_out, double* dual_out, double* objective, double* ms) {
+  auto* h = static_cast<RefPresolve*>(handle);
+  model::Solution red;
+  red.primal.assign(primal, primal + n2);
+  red.dual.assign(dual, dual + m2);
+  const auto t0 = std::chrono::steady_clock::now();
+  model::Solution full = h->presolver.postsolve(h->original, red, h->res);
+  const auto t1 = std::chrono::steady_clock::now();
+  if (ms) *ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  double o = h->original.obj_offset;
+  for (std::size_t j = 0; j < h->original.variables.size() && j < full.primal.size(); ++j)
+    o += h->original.variables[j].objective_coeff * full.primal[j];
+  *objective = o;
+  copy_out(full.primal, primal_out, (int64_t)h->original.variables.size());
+  copy_out(full.dual, dual_out, (int64_t)h->original.constraints.size());
+  return 0;
+}
+
+void ref_presolve_free(void* handle) { delete static_cast<RefPresolve*>(handle); }
 
 // SolutionVerifier::verify_detailed on the problem ProblemBuilder builds
 // from the arguments.  var_type: n model::VarType codes (NULL: continuous);
