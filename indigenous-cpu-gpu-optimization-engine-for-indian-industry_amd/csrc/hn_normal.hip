@@ -344,7 +344,8 @@ __global__ __launch_bounds__(kWave) void k_ldl_diag(T* __restrict__ A, int m, in
   __shared__ T S[kNB][kNB + 1];
   __shared__ T Xs[kNB][kNB + 1];
   __shared__ T Ts[3 * 256];
-  __shared__ T Lk[kNB];  // the scaled column k of the current step
+  __shared__ T Lk[2][2 * kNB + 16];  // the multipliers of the step, by parity; zeros past row 63
+  __shared__ T Dk[kNB];              // 1 / pivot of each step
   const int j = threadIdx.x;
   // The block's lower triangle, row by row (lane j reads column j of row i:
   // coalesced), mirrored into the upper half; rows past b are the identity.
@@ -369,54 +370,78 @@ __global__ __launch_bounds__(kWave) void k_ldl_diag(T* __restrict__ A, int m, in
     }
   }
   __syncthreads();
-  // Elimination in registers: lane j holds column j (c[i] = S[i][j]); the
-  // k loop is unrolled so every register index is static.  The trailing
-  // block stays symmetric, so lane i's c[k] = S[k][i] = S[i][k] gives lane i
-  // its own multiplier l_i = S[i][k] / d_k, which it keeps in c[k] (lane i
-  // ends holding row i of L); the multipliers reach the other lanes through
-  // one LDS vector per step and the pivot through readlane.
+  // Elimination in registers, one step per trip of a rolled loop (unrolled,
+  // the 64 steps were ~80 KB of straight-line code).  Lane j holds what is
+  // left of column j shifted so that c[0] is the entry in the current pivot
+  // row: at step k, c[t] = A(k + t, j).  A step writes the updated column one
+  // place down (c[t-1] = c[t] - l_{k+t} a_kj), so every register index is
+  // static.  The trailing block stays symmetric, so lane i's own c[0] =
+  // A(k, i) = A(i, k) gives it its multiplier l_i = A(i, k) / d_k; the
+  // multipliers reach the other lanes through an LDS vector
+  // (double-buffered by step parity, zeros past row 63) and are kept as
+  // column k of L in S.  The steps form one dependent chain (pivot, its
+  // reciprocal, multipliers, LDS, next pivot row), so each step updates row
+  // k+1 first and takes the next pivot and its reciprocal from it before
+  // the other 62 rows: the division overlaps their multiply-adds.
   T c[kNB];
 #pragma unroll
   for (int i = 0; i < kNB; ++i) c[i] = S[i][j];
+  Lk[0][kNB + j] = T(0);
+  Lk[1][kNB + j] = T(0);
+  if (j < 16) {
+    Lk[0][2 * kNB + j] = T(0);
+    Lk[1][2 * kNB + j] = T(0);
+  }
   const double d0j = (skip_rel > 0.0 && j < b) ? diag0[k0 + j] : 1.0;
   int ncl = 0;
+  // 1 / pivot of step k (lane k's c[0]) under the pivot rule; a clamped or
+  // skipped pivot is counted when the step is one of the block's (live).
+  // Selects, not branches: the step stays one basic block, so the
+  // scheduler can overlap the division with the multiply-adds.
+  auto pivot_inverse = [&](T cv, int k, bool live) -> T {
+    const T piv = read_lane(cv, k);
+    const bool skipped = skip_rel > 0.0 && !((double)piv > skip_rel * read_lane(d0j, k));
+    const bool clamped = !(skip_rel > 0.0) && piv <= T(tol);
+    const T r = T(1) / (clamped ? T(tol) : piv);
+    ncl += (live && (skipped || clamped)) ? 1 : 0;
+    return skipped ? T(0) : r;
+  };
+  T rinv = pivot_inverse(c[0], 0, true);
+  __syncthreads();  // every lane has read its column of S before S takes L
+  for (int k = 0; k < b; ++k) {
+    const int par = k & 1;
+    const T akj = c[0];  // A(k, j)
+    Dk[k] = rinv;  // every lane stores the same value
+    const T l = (j > k) ? akj * rinv : T(0);
+    Lk[par][j] = l;
+    S[j][k] = l;  // row j of L
+    __syncthreads();
+    const T a = (j > k) ? akj : T(0);
+    const T* lv = &Lk[par][k];
+    c[0] = c[1] - lv[1] * a;  // row k + 1: the next pivot row
+    const T rnext = pivot_inverse(c[0], min(k + 1, kNB - 1), k + 1 < b);
+    // the other rows in four groups of 16, each group's multipliers read
+    // from LDS while the previous group's multiply-adds run
+    T lg[2][16];
 #pragma unroll
-  for (int k = 0; k < kNB; ++k) {
-    if (k < b) {
-      T piv = read_lane(c[k], k);
-      T rinv;
-      if (skip_rel > 0.0) {
-        if (!((double)piv > skip_rel * read_lane(d0j, k))) {
-          rinv = T(0);
-          ++ncl;
-        } else {
-          rinv = T(1) / piv;
-        }
-      } else {
-        if (piv <= T(tol)) {
-          piv = T(tol);
-          ++ncl;
-        }
-        rinv = T(1) / piv;
-      }
-      if (j == 0) dinv[k0 + k] = rinv;
-      const T akj = c[k];  // S[k][j]
-      const T l = (j > k) ? akj * rinv : T(0);
-      Lk[j] = l;
-      __syncthreads();
-      if (j > k) {
+    for (int u = 0; u < 16; ++u) lg[0][u] = lv[2 + u];
 #pragma unroll
-        for (int i = k + 1; i < kNB; ++i) c[i] -= Lk[i] * akj;
-        c[k] = l;
+    for (int g = 0; g < 4; ++g) {
+      if (g < 3) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) lg[(g + 1) & 1][u] = lv[18 + 16 * g + u];
       }
-      __syncthreads();  // Lk is rewritten by the next step
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int t = 2 + 16 * g + u;
+        if (t < kNB) c[t - 1] = c[t] - lg[g & 1][u] * a;
+      }
     }
+    c[kNB - 1] = T(0);
+    rinv = rnext;
   }
-  // S takes L (lane j writes row j: L[j][k] for k < j) for the write-back
-  // and for X below; the entries on and above the diagonal are not read.
-#pragma unroll
-  for (int k = 0; k < kNB; ++k) S[j][k] = c[k];
   __syncthreads();
+  if (j < b) dinv[k0 + j] = Dk[j];
   for (int i = 1; i < b; ++i)  // L below the diagonal, row by row (coalesced)
     if (j < i) A[(long long)(k0 + i) * m + k0 + j] = S[i][j];
   if (j == 0 && ncl) atomicAdd(clamped, ncl);
@@ -438,18 +463,27 @@ __global__ __launch_bounds__(kWave) void k_ldl_diag(T* __restrict__ A, int m, in
     for (int i = 0; i < 16; ++i) Xs[o + i][o + cc] = x[i];
   }
   __syncthreads();
+  // every loop below has a static trip count and is unrolled, so a
+  // thread's 4I independent sums are in flight together instead of one
+  // dependent chain of LDS reads and multiply-adds after another
+#pragma unroll
   for (int I = 1; I < 4; ++I) {
-    for (int e = j; e < I * 256; e += kWave) {
-      const int J = e >> 8, r = (e >> 4) & 15, cc = e & 15;
+#pragma unroll
+    for (int q = 0; q < 4 * I; ++q) {
+      const int e = j + kWave * q;
+      const int J = q >> 2, r = (e >> 4) & 15, cc = e & 15;
       T sx = T(0);
+#pragma unroll
       for (int K = J; K < I; ++K)
 #pragma unroll
         for (int k = 0; k < 16; ++k) sx += S[16 * I + r][16 * K + k] * Xs[16 * K + k][16 * J + cc];
       Ts[e] = sx;
     }
     __syncthreads();
-    for (int e = j; e < I * 256; e += kWave) {
-      const int J = e >> 8, r = (e >> 4) & 15, cc = e & 15;
+#pragma unroll
+    for (int q = 0; q < 4 * I; ++q) {
+      const int e = j + kWave * q;
+      const int J = q >> 2, r = (e >> 4) & 15, cc = e & 15;
       T sx = T(0);
 #pragma unroll
       for (int k = 0; k < 16; ++k) sx += Xs[16 * I + r][16 * I + k] * Ts[(J << 8) + (k << 4) + cc];
