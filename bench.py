@@ -428,14 +428,24 @@ def run_lp(args) -> dict:
         log(f"cpu warm-start-family baseline {out['warm_start']['cpu_baseline']}")
         # the reference IPM on the Netlib models it solves (fewer than 100 iterations)
         conv = [n for n in sorted(fx) if fx[n]["ref_ipm_iterations"] < 100 and n != "dfl001"]
-        t0 = time.perf_counter()
-        cpu = {n: solve(models[n], time_limit=args.cpu_lp_limit) for n in conv}
-        dt = time.perf_counter() - t0
+        cpu, cpu_ms = {}, {}
+        for n in conv:
+            t0 = time.perf_counter()
+            cpu[n] = solve(models[n], time_limit=args.cpu_lp_limit)
+            cpu_ms[n] = 1e3 * (time.perf_counter() - t0)
+        dt = sum(cpu_ms.values()) / 1e3
         gpu_t = sum(per[n]["ms"] for n in conv) / 1e3
+        same = [n for n in conv if f"{cpu[n].status}/{cpu[n].iterations}" == f"{per[n]['status']}/{per[n]['iterations']}"]
         out["netlib"]["cpu_baseline"] = {
             "value": round(len(conv) / dt, 3), "unit": "LPs/s", "cores": 1, "kind": kind,
             "sample": (f"{', '.join(conv)}: {dt:.2f} s on the host ({', '.join(cpu[n].status for n in conv)}); "
-                       f"the GPU solver takes {gpu_t:.3f} s on the same models")}
+                       f"the GPU solver takes {gpu_t:.3f} s on the same models"),
+            "per_model_ms": {n: {"cpu": round(cpu_ms[n], 2), "gpu": per[n]["ms"],
+                                 "cpu_outcome": f"{cpu[n].status}/{cpu[n].iterations}",
+                                 "gpu_outcome": f"{per[n]['status']}/{per[n]['iterations']}"} for n in conv},
+            "same_outcome_models": same,
+            "same_outcome_cpu_s": round(sum(cpu_ms[n] for n in same) / 1e3, 4),
+            "same_outcome_gpu_s": round(sum(per[n]["ms"] for n in same) / 1e3, 4)}
         log(f"cpu netlib baseline {out['netlib']['cpu_baseline']}")
         # the reference's presolve and postsolve, which Solver::solve runs around
         # the IPM (api.cpp:384-412, :176-196): their share of the end-to-end time

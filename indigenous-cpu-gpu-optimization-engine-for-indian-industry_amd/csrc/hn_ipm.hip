@@ -115,11 +115,14 @@ __global__ void k_corrector(const double* __restrict__ dx, const double* __restr
   zs[i] = (sigma_mu - dx[i] * dz[i]) * zinv;
 }
 
+// The update is skipped when the step's direction was not finite (the
+// host finds the flag at its next read of the flags and stops there, as
+// the reference stops before updating, :617-620, :675-678).
 __global__ void k_update(double* __restrict__ x, double* __restrict__ z, const double* __restrict__ dx,
                          const double* __restrict__ dz, const double* __restrict__ S, int n,
                          int* __restrict__ flags) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || flags[F_DIR]) return;
   const double xi = x[i] + S[S_AP] * dx[i];  // :690-693
   const double zi = z[i] + S[S_AD] * dz[i];
   x[i] = xi;
@@ -128,9 +131,9 @@ __global__ void k_update(double* __restrict__ x, double* __restrict__ z, const d
 }
 
 __global__ void k_update_lam(double* __restrict__ lam, const double* __restrict__ dlam,
-                             const double* __restrict__ S, int m) {
+                             const double* __restrict__ S, int m, const int* __restrict__ flags) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < m) lam[i] = lam[i] + S[S_AD] * dlam[i];  // :694-696
+  if (i < m && !flags[F_DIR]) lam[i] = lam[i] + S[S_AD] * dlam[i];  // :694-696
 }
 
 __global__ void k_fill(double* __restrict__ p, double v, int n) {
@@ -829,7 +832,22 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
 
   int64_t iter = 0;
   double mu = 0.0;
+  // The flags of a step's corrector direction and update are read with the
+  // next residuals (one host round trip fewer per iteration): a failed step
+  // ends the solve there, with the iteration count and state the reference
+  // has when it stops inside that step.
+  auto step_failed = [&]() { return sv.hF[F_DIR] != 0 || sv.hF[F_UPD] != 0; };
+  double prev_mu = 0.0, prev_rp = 0.0, prev_rd = 0.0;
   for (iter = 0; iter < o.max_iterations; ++iter) {
+    HN_TRY(residuals());
+    if (iter > 0 && o.verbose)
+      std::fprintf(stderr, "IPM it=%lld mu=%g rp=%g rd=%g ap=%g ad=%g%s\n", (long long)(iter - 1), prev_mu, prev_rp,
+                   prev_rd, hS[S_AP], hS[S_AD], sv.fp64_solve ? "" : " [fp32 NE]");
+    if (step_failed()) {
+      numerical_error = true;
+      res.iterations = iter - 1;
+      break;
+    }
     if (o.interrupt_callback && o.interrupt_callback(o.interrupt_user)) {  // :490-493
       interrupted = true;
       break;
@@ -838,8 +856,10 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
       time_up = true;
       break;
     }
-    HN_TRY(residuals());
     mu = hS[S_XZ] / (double)n;
+    prev_mu = mu;
+    prev_rp = hS[S_RPN];
+    prev_rd = hS[S_RDN];
     if (hS[S_RPN] < o.convergence_tol && hS[S_RDN] < o.convergence_tol && mu < o.complementarity_tol) {
       bool gap_ok = true;
       if (o.gap_tol > 0.0) {  // optional relative-gap bar (not in the reference)
@@ -929,29 +949,26 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
                        sv.z.as<double>(), sv.dz.as<double>(), n, sv.part.as<double>());
     hipLaunchKernelGGL(k_fin_step, dim3(1), dim3(kNT), 0, s, sv.part.as<double>(), np_n, S, (int)S_AP, (int)S_AD);
     HN_TRY(check_launch("corrector"));
-    HN_TRY(sv.pull());
-    if (sv.hF[F_DIR]) {
-      numerical_error = true;
-      break;
-    }
-    // update (:687-708)
+    // update (:687-708); skipped on the device if the corrector's direction
+    // was not finite
     hipLaunchKernelGGL(k_update, sv.g(n), B, 0, s, sv.x.as<double>(), sv.z.as<double>(), sv.dx.as<double>(),
                        sv.dz.as<double>(), S, n, F);
-    hipLaunchKernelGGL(k_update_lam, sv.g(m), B, 0, s, sv.lam.as<double>(), sv.dlam.as<double>(), S, m);
+    hipLaunchKernelGGL(k_update_lam, sv.g(m), B, 0, s, sv.lam.as<double>(), sv.dlam.as<double>(), S, m, F);
     HN_TRY(check_launch("update"));
-    HN_TRY(sv.pull());
-    if (sv.hF[F_UPD]) {
-      numerical_error = true;
-      break;
-    }
-    if (o.verbose)
-      std::fprintf(stderr, "IPM it=%lld mu=%g rp=%g rd=%g ap=%g ad=%g%s\n", (long long)iter, mu, hS[S_RPN],
-                   hS[S_RDN], hS[S_AP], hS[S_AD], sv.fp64_solve ? "" : " [fp32 NE]");
     res.iterations = iter + 1;
   }
 
   if (!done && !interrupted && !stopped) {  // :728-753
     HN_TRY(residuals());
+    if (iter == o.max_iterations) {  // the last step's flags
+      if (iter > 0 && o.verbose)
+        std::fprintf(stderr, "IPM it=%lld mu=%g rp=%g rd=%g ap=%g ad=%g%s\n", (long long)(iter - 1), prev_mu,
+                     prev_rp, prev_rd, hS[S_AP], hS[S_AD], sv.fp64_solve ? "" : " [fp32 NE]");
+      if (step_failed()) {
+        numerical_error = true;
+        res.iterations = iter - 1;
+      }
+    }
     mu = hS[S_XZ] / (double)n;
     if (hS[S_RPN] < 1e-4 && hS[S_RDN] < 1e-4 && mu < 1e-4) done = true;
   } else if (interrupted || stopped) {
