@@ -137,8 +137,8 @@ struct SharedLDL {
   union {
     T X[BK][BK];
     struct {
-      T Ws[BK / 2][64];
-      T Ls[BK / 2][64];
+      T Ws[BK / 2][65];
+      T Ls[BK / 2][65];
     } u;
   };
 };
@@ -268,50 +268,83 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
       }
     }
     __syncthreads();
-    // (3) trailing update of the lower triangle: A_ij -= sum_c W_ic L_jc
+    // (3) trailing update of the lower triangle: A_ij -= sum_c W_ic L_jc,
+    //     64 x 64 tiles (I, J <= I) in steps of 32 columns c.  Each step's W
+    //     and L slices are loaded from memory into registers during the
+    //     step before (they are columns < k0 + b of A, which the update does
+    //     not write), then stored to LDS rows padded to 65, so a store's 32
+    //     lanes of one row fall in different banks.
     const int r0 = k0 + b;
     const int nt = (m - r0 + 63) / 64;
-    for (int tI = 0; tI < nt; ++tI) {
-      for (int tJ = 0; tJ <= tI; ++tJ) {
-        const int i0 = r0 + 64 * tI, j0 = r0 + 64 * tJ;
-        T acc[4][4];
+    const int ncb = (b + BK / 2 - 1) / (BK / 2);
+    const int nsteps = nt * (nt + 1) / 2 * ncb;
+    constexpr int kSlots = 64 * (BK / 2) / NT;  // a thread's (row, column) slots of a step
+    T pw[kSlots], pl[kSlots];
+    auto fetch = [&](int tI, int tJ, int cb) {
+      const int i0 = r0 + 64 * tI, j0 = r0 + 64 * tJ, cw = min(BK / 2, b - cb);
+#pragma unroll
+      for (int u = 0; u < kSlots; ++u) {
+        const int e = tid + NT * u;
+        const int r = e / (BK / 2), c = e % (BK / 2);
+        const int i = i0 + r, j = j0 + r;
+        pw[u] = (i < m && c < cw) ? W[(long long)i * BK + cb + c] : T(0);
+        pl[u] = (j < m && c < cw) ? A[(long long)j * m + k0 + cb + c] : T(0);
+      }
+    };
+    T acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[a][c] = 0;
+    int tI = 0, tJ = 0, cbi = 0;
+    if (nsteps > 0) fetch(0, 0, 0);
+    for (int st = 0; st < nsteps; ++st) {
+#pragma unroll
+      for (int u = 0; u < kSlots; ++u) {
+        const int e = tid + NT * u;
+        const int r = e / (BK / 2), c = e % (BK / 2);
+        sh.u.Ws[c][r] = pw[u];
+        sh.u.Ls[c][r] = pl[u];
+      }
+      __syncthreads();
+      int nI = tI, nJ = tJ, nc = cbi + 1;  // the next step
+      if (nc == ncb) {
+        nc = 0;
+        if (++nJ > nI) {
+          nJ = 0;
+          ++nI;
+        }
+      }
+      if (st + 1 < nsteps) fetch(nI, nJ, nc * (BK / 2));
+      const int cw = min(BK / 2, b - cbi * (BK / 2));
+      for (int c = 0; c < cw; ++c) {
+        T wv[4], lv[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) wv[a] = sh.u.Ws[c][ti + 16 * a];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) lv[a] = sh.u.Ls[c][tj + 16 * a];
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
-          for (int c = 0; c < 4; ++c) acc[a][c] = 0;
-        for (int cb = 0; cb < b; cb += BK / 2) {
-          const int cw = min(BK / 2, b - cb);
-          for (int e = tid; e < 64 * (BK / 2); e += NT) {
-            const int r = e / (BK / 2), c = e % (BK / 2);
-            const int i = i0 + r, j = j0 + r;
-            sh.u.Ws[c][r] = (i < m && c < cw) ? W[(long long)i * BK + cb + c] : T(0);
-            sh.u.Ls[c][r] = (j < m && c < cw) ? A[(long long)j * m + k0 + cb + c] : T(0);
-          }
-          __syncthreads();
-          for (int c = 0; c < cw; ++c) {
-            T wv[4], lv[4];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) wv[a] = sh.u.Ws[c][ti + 16 * a];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) lv[a] = sh.u.Ls[c][tj + 16 * a];
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-              for (int q = 0; q < 4; ++q) acc[a][q] += wv[a] * lv[q];
-          }
-          __syncthreads();
-        }
+          for (int q = 0; q < 4; ++q) acc[a][q] += wv[a] * lv[q];
+      }
+      __syncthreads();
+      if (cbi == ncb - 1) {  // the tile's last step: subtract it from A
+        const int i0 = r0 + 64 * tI, j0 = r0 + 64 * tJ;
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
           const int i = i0 + ti + 16 * a;
-          if (i >= m) continue;
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const int j = j0 + tj + 16 * q;
-            if (j < m && j <= i) A[(long long)i * m + j] -= acc[a][q];
+            if (i < m && j < m && j <= i) A[(long long)i * m + j] -= acc[a][q];
+            acc[a][q] = 0;
           }
         }
       }
+      tI = nI;
+      tJ = nJ;
+      cbi = nc;
     }
     __syncthreads();  // the next block reads what the update wrote
   }
