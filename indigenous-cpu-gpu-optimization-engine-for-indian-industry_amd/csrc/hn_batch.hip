@@ -317,6 +317,9 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
       }
       if (st + 1 < nsteps) fetch(nI, nJ, nc * (BK / 2));
       const int cw = min(BK / 2, b - cbi * (BK / 2));
+      // unrolled so that the LDS reads of the next columns are in flight
+      // during this column's multiply-adds (one wave per SIMD hides nothing)
+#pragma unroll 4
       for (int c = 0; c < cw; ++c) {
         T wv[4], lv[4];
 #pragma unroll
