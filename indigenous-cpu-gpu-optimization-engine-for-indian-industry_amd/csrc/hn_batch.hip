@@ -298,7 +298,20 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
       for (int c = 0; c < 4; ++c) acc[a][c] = 0;
     int tI = 0, tJ = 0, cbi = 0;
     if (nsteps > 0) fetch(0, 0, 0);
+    T aold[4][4];  // the tile's entries of A, read when its first step starts
     for (int st = 0; st < nsteps; ++st) {
+      if (cbi == 0) {
+        const int i0 = r0 + 64 * tI, j0 = r0 + 64 * tJ;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const int i = i0 + ti + 16 * a;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int j = j0 + tj + 16 * q;
+            aold[a][q] = (i < m && j < m && j <= i) ? A[(long long)i * m + j] : T(0);
+          }
+        }
+      }
 #pragma unroll
       for (int u = 0; u < kSlots; ++u) {
         const int e = tid + NT * u;
@@ -340,7 +353,7 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const int j = j0 + tj + 16 * q;
-            if (i < m && j < m && j <= i) A[(long long)i * m + j] -= acc[a][q];
+            if (i < m && j < m && j <= i) A[(long long)i * m + j] = aold[a][q] - acc[a][q];
             acc[a][q] = 0;
           }
         }
