@@ -31,6 +31,7 @@
 #include <cstring>
 #include <numeric>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #pragma clang fp contract(off)
@@ -291,6 +292,22 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
         pl[u] = (j < m && c < cw) ? A[(long long)j * m + k0 + cb + c] : T(0);
       }
     };
+    // Each thread owns 16 entries of a tile: for FP64, the four 16 x 16
+    // blocks of its wave's 32 x 32 quarter in the v_mfma_f64_16x16x4 output
+    // layout (lane l, element e: row 4e + l/16, column l%16 of a block); for
+    // FP32, a 4 x 4 grid at stride 16 (rows ti + 16a, columns tj + 16q).
+    constexpr bool kMfma = std::is_same<T, double>::value;
+    const int ln = tid & (kWave - 1), wq = tid / kWave;
+    const int rq = 32 * (wq >> 1), cq = 32 * (wq & 1);
+    auto entry = [&](int a, int q, int& r, int& c) {  // tile row / column of entry (a, q)
+      if (kMfma) {
+        r = rq + 16 * (a >> 1) + 4 * q + (ln >> 4);  // a: block (row half, column half), q: element
+        c = cq + 16 * (a & 1) + (ln & 15);
+      } else {
+        r = ti + 16 * a;
+        c = tj + 16 * q;
+      }
+    };
     T acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -303,14 +320,14 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
       if (cbi == 0) {
         const int i0 = r0 + 64 * tI, j0 = r0 + 64 * tJ;
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          const int i = i0 + ti + 16 * a;
+        for (int a = 0; a < 4; ++a)
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
-            const int j = j0 + tj + 16 * q;
+            int r, c;
+            entry(a, q, r, c);
+            const int i = i0 + r, j = j0 + c;
             aold[a][q] = (i < m && j < m && j <= i) ? A[(long long)i * m + j] : T(0);
           }
-        }
       }
 #pragma unroll
       for (int u = 0; u < kSlots; ++u) {
@@ -329,34 +346,58 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
         }
       }
       if (st + 1 < nsteps) fetch(nI, nJ, nc * (BK / 2));
-      const int cw = min(BK / 2, b - cbi * (BK / 2));
-      // unrolled so that the LDS reads of the next columns are in flight
-      // during this column's multiply-adds (one wave per SIMD hides nothing)
-#pragma unroll 4
-      for (int c = 0; c < cw; ++c) {
-        T wv[4], lv[4];
+      if constexpr (kMfma) {
+        // 32 columns in steps of 4 (columns past the block's width are zero)
+        typedef double d4 __attribute__((ext_vector_type(4)));
+        d4 dd[4];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) wv[a] = sh.u.Ws[c][ti + 16 * a];
+        for (int a = 0; a < 4; ++a) dd[a] = d4{acc[a][0], acc[a][1], acc[a][2], acc[a][3]};
 #pragma unroll
-        for (int a = 0; a < 4; ++a) lv[a] = sh.u.Ls[c][tj + 16 * a];
+        for (int c0 = 0; c0 < BK / 2; c0 += 4) {
+          const int kk = c0 + (ln >> 4);
+          const double a0 = sh.u.Ws[kk][rq + (ln & 15)];
+          const double a1 = sh.u.Ws[kk][rq + 16 + (ln & 15)];
+          const double b0 = sh.u.Ls[kk][cq + (ln & 15)];
+          const double b1 = sh.u.Ls[kk][cq + 16 + (ln & 15)];
+          dd[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, dd[0], 0, 0, 0);
+          dd[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, dd[1], 0, 0, 0);
+          dd[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, dd[2], 0, 0, 0);
+          dd[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, dd[3], 0, 0, 0);
+        }
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
-          for (int q = 0; q < 4; ++q) acc[a][q] += wv[a] * lv[q];
+          for (int q = 0; q < 4; ++q) acc[a][q] = dd[a][q];
+      } else {
+        const int cw = min(BK / 2, b - cbi * (BK / 2));
+        // unrolled so that the LDS reads of the next columns are in flight
+        // during this column's multiply-adds (one wave per SIMD hides nothing)
+#pragma unroll 4
+        for (int c = 0; c < cw; ++c) {
+          T wv[4], lv[4];
+#pragma unroll
+          for (int a = 0; a < 4; ++a) wv[a] = sh.u.Ws[c][ti + 16 * a];
+#pragma unroll
+          for (int a = 0; a < 4; ++a) lv[a] = sh.u.Ls[c][tj + 16 * a];
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[a][q] += wv[a] * lv[q];
+        }
       }
       __syncthreads();
       if (cbi == ncb - 1) {  // the tile's last step: subtract it from A
         const int i0 = r0 + 64 * tI, j0 = r0 + 64 * tJ;
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          const int i = i0 + ti + 16 * a;
+        for (int a = 0; a < 4; ++a)
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
-            const int j = j0 + tj + 16 * q;
+            int r, c;
+            entry(a, q, r, c);
+            const int i = i0 + r, j = j0 + c;
             if (i < m && j < m && j <= i) A[(long long)i * m + j] = aold[a][q] - acc[a][q];
             acc[a][q] = 0;
           }
-        }
       }
       tI = nI;
       tJ = nJ;
