@@ -138,8 +138,8 @@ struct SharedLDL {
   union {
     T X[BK][BK];
     struct {
-      T Ws[BK / 2][65];
-      T Ls[BK / 2][65];
+      T Wr[BK][65];      // W of the current tile row, column-major (k, row)
+      T Ls[BK / 2][65];  // L slice of the current step, (k, row)
     } u;
   };
 };
@@ -270,26 +270,36 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
     }
     __syncthreads();
     // (3) trailing update of the lower triangle: A_ij -= sum_c W_ic L_jc,
-    //     64 x 64 tiles (I, J <= I) in steps of 32 columns c.  Each step's W
-    //     and L slices are loaded from memory into registers during the
-    //     step before (they are columns < k0 + b of A, which the update does
-    //     not write), then stored to LDS rows padded to 65, so a store's 32
-    //     lanes of one row fall in different banks.
+    //     64 x 64 tiles (I, J <= I) in steps of 32 columns c.  A tile row's
+    //     W stays in LDS for all its tiles; each step's L slice (columns
+    //     < k0 + b of A, which the update does not write) and the next tile
+    //     row's W are loaded from memory into registers during the step
+    //     before, then stored to LDS rows padded to 65, so a store's lanes
+    //     of one row fall in different banks.
     const int r0 = k0 + b;
     const int nt = (m - r0 + 63) / 64;
     const int ncb = (b + BK / 2 - 1) / (BK / 2);
     const int nsteps = nt * (nt + 1) / 2 * ncb;
-    constexpr int kSlots = 64 * (BK / 2) / NT;  // a thread's (row, column) slots of a step
-    T pw[kSlots], pl[kSlots];
-    auto fetch = [&](int tI, int tJ, int cb) {
-      const int i0 = r0 + 64 * tI, j0 = r0 + 64 * tJ, cw = min(BK / 2, b - cb);
+    constexpr int kSlots = 64 * (BK / 2) / NT;  // a thread's (row, column) slots of an L slice
+    constexpr int kRowSlots = 64 * BK / NT;     // ... of a tile row's W
+    T pl[kSlots], pw[kRowSlots];
+    auto fetch_l = [&](int tJ, int cb) {
+      const int j0 = r0 + 64 * tJ, cw = min(BK / 2, b - cb);
 #pragma unroll
       for (int u = 0; u < kSlots; ++u) {
         const int e = tid + NT * u;
         const int r = e / (BK / 2), c = e % (BK / 2);
-        const int i = i0 + r, j = j0 + r;
-        pw[u] = (i < m && c < cw) ? W[(long long)i * BK + cb + c] : T(0);
+        const int j = j0 + r;
         pl[u] = (j < m && c < cw) ? A[(long long)j * m + k0 + cb + c] : T(0);
+      }
+    };
+    auto fetch_w = [&](int tI) {  // W rows of tile row tI, all b columns
+      const int i0 = r0 + 64 * tI;
+#pragma unroll
+      for (int u = 0; u < kRowSlots; ++u) {
+        const int e = tid + NT * u;
+        const int r = e / BK, c = e % BK;
+        pw[u] = (i0 + r < m && c < b) ? W[(long long)(i0 + r) * BK + c] : T(0);
       }
     };
     // Each thread owns 16 entries of a tile: for FP64, the four 16 x 16
@@ -314,7 +324,10 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
 #pragma unroll
       for (int c = 0; c < 4; ++c) acc[a][c] = 0;
     int tI = 0, tJ = 0, cbi = 0;
-    if (nsteps > 0) fetch(0, 0, 0);
+    if (nsteps > 0) {
+      fetch_w(0);
+      fetch_l(0, 0);
+    }
     T aold[4][4];  // the tile's entries of A, read when its first step starts
     for (int st = 0; st < nsteps; ++st) {
       if (cbi == 0) {
@@ -329,11 +342,17 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
             aold[a][q] = (i < m && j < m && j <= i) ? A[(long long)i * m + j] : T(0);
           }
       }
+      if (tJ == 0 && cbi == 0) {  // a new tile row: its W into LDS
+#pragma unroll
+        for (int u = 0; u < kRowSlots; ++u) {
+          const int e = tid + NT * u;
+          sh.u.Wr[e % BK][e / BK] = pw[u];
+        }
+      }
 #pragma unroll
       for (int u = 0; u < kSlots; ++u) {
         const int e = tid + NT * u;
         const int r = e / (BK / 2), c = e % (BK / 2);
-        sh.u.Ws[c][r] = pw[u];
         sh.u.Ls[c][r] = pl[u];
       }
       __syncthreads();
@@ -345,7 +364,10 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
           ++nI;
         }
       }
-      if (st + 1 < nsteps) fetch(nI, nJ, nc * (BK / 2));
+      if (st + 1 < nsteps) {
+        fetch_l(nJ, nc * (BK / 2));
+        if (nJ == 0 && nc == 0) fetch_w(nI);
+      }
       if constexpr (kMfma) {
         // 32 columns in steps of 4 (columns past the block's width are zero)
         typedef double d4 __attribute__((ext_vector_type(4)));
@@ -355,8 +377,8 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
 #pragma unroll
         for (int c0 = 0; c0 < BK / 2; c0 += 4) {
           const int kk = c0 + (ln >> 4);
-          const double a0 = sh.u.Ws[kk][rq + (ln & 15)];
-          const double a1 = sh.u.Ws[kk][rq + 16 + (ln & 15)];
+          const double a0 = sh.u.Wr[cbi * (BK / 2) + kk][rq + (ln & 15)];
+          const double a1 = sh.u.Wr[cbi * (BK / 2) + kk][rq + 16 + (ln & 15)];
           const double b0 = sh.u.Ls[kk][cq + (ln & 15)];
           const double b1 = sh.u.Ls[kk][cq + 16 + (ln & 15)];
           dd[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, dd[0], 0, 0, 0);
@@ -376,7 +398,7 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
         for (int c = 0; c < cw; ++c) {
           T wv[4], lv[4];
 #pragma unroll
-          for (int a = 0; a < 4; ++a) wv[a] = sh.u.Ws[c][ti + 16 * a];
+          for (int a = 0; a < 4; ++a) wv[a] = sh.u.Wr[cbi * (BK / 2) + c][ti + 16 * a];
 #pragma unroll
           for (int a = 0; a < 4; ++a) lv[a] = sh.u.Ls[c][tj + 16 * a];
 #pragma unroll
