@@ -557,8 +557,8 @@ def main() -> int:
     ap.add_argument("--cpu-single-limit", type=float, default=20.0, help="time limit of the reference CPU solve of the 2000x5000 LP")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is described in "
-                                   "DESIGN.md and not launched", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU only: a multi-GPU run (row-block sharding with "
+                                   "an all-gather of x per product) is not launched", "n_gpus": args.gpus}))
         return 2
     if args.verify_only:
         print(json.dumps({"verify": run_verify(args)}), flush=True)

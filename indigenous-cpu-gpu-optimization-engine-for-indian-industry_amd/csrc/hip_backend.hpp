@@ -4,7 +4,7 @@
 // reference's hypernova::execution::IComputeBackend
 // (core/execution/gpu_backend.hpp:18-51): same names, argument types and
 // meaning, so HipBackend can be registered for ComputeBackendType::HIP with
-// a one-line adapter.  HipBackend loads libhn_kernels.so
+// a one-line adapter in the reference's factory.  HipBackend loads libhn_kernels.so
 // at run time with dlopen and binds its C entry points through one table,
 // the way the reference binds its driver (core/execution/cuda_driver.cpp:24-48,
 // 112-145), so a host without a GPU or without the library still links and
