@@ -571,31 +571,49 @@ struct Solver {
 
   // Dense: dlam = N^{-1} rhs then refinement against N (refinement.cpp:7-36):
   // up to 5 corrections, stop when ||rhs - N dlam||_inf <= 1e-10 ||rhs||_inf.
-  int solve_dense(const double* r_in, double* out, bool* converged) {
+  // solve_dense_begin forms the first solution and its residual check on
+  // the device; dense_first_ok reads the check after a pull;
+  // solve_dense_rest runs the corrections when it failed.
+  int solve_dense_begin(const double* r_in, double* out) {
     HN_TRY(dense_apply(dense, r_in, out, false, s));
     ++res.ne_solves;
     const int np = parts_for(m);
+    HN_TRY(apply_N(out, ir_nd.as<double>()));
+    hipLaunchKernelGGL(k_red_resid, dim3(np), dim3(kNT), 0, s, r_in, ir_nd.as<double>(), ir_r.as<double>(), m,
+                       part.as<double>());
+    hipLaunchKernelGGL(k_fin_max2, dim3(1), dim3(kNT), 0, s, part.as<double>(), np, S.as<double>(), S_IRR, S_IRB);
+    return check_launch("dense solve");
+  }
+  bool dense_first_ok() const {
+    const double nb = hS[S_IRB];
+    return nb == 0.0 || nb != nb || hS[S_IRR] / nb <= 1e-10;
+  }
+  int solve_dense_rest(const double* r_in, double* out, bool* converged) {
+    const int np = parts_for(m);
     *converged = false;
-    for (int it = 0; it <= 5; ++it) {
+    for (int it = 1; it <= 5; ++it) {
+      HN_TRY(dense_apply(dense, ir_r.as<double>(), out, true, s));
+      ++res.refine_sweeps;
       HN_TRY(apply_N(out, ir_nd.as<double>()));
       hipLaunchKernelGGL(k_red_resid, dim3(np), dim3(kNT), 0, s, r_in, ir_nd.as<double>(), ir_r.as<double>(), m,
                          part.as<double>());
       hipLaunchKernelGGL(k_fin_max2, dim3(1), dim3(kNT), 0, s, part.as<double>(), np, S.as<double>(), S_IRR, S_IRB);
       HN_TRY(pull());
-      const double nb = hS[S_IRB];
-      if (nb == 0.0 || nb != nb) {
+      if (dense_first_ok()) {
         *converged = true;
         return HN_OK;
       }
-      if (hS[S_IRR] / nb <= 1e-10) {
-        *converged = true;
-        return HN_OK;
-      }
-      if (it == 5) break;
-      HN_TRY(dense_apply(dense, ir_r.as<double>(), out, true, s));
-      ++res.refine_sweeps;
     }
     return HN_OK;
+  }
+  int solve_dense(const double* r_in, double* out, bool* converged) {
+    HN_TRY(solve_dense_begin(r_in, out));
+    HN_TRY(pull());
+    if (dense_first_ok()) {
+      *converged = true;
+      return HN_OK;
+    }
+    return solve_dense_rest(r_in, out, converged);
   }
 
   // PCG on the matrix-free N with a Jacobi preconditioner; the relative
@@ -687,19 +705,40 @@ struct Solver {
     return check_launch("pcg mixed");
   }
 
+  // FP32 inverse could not be refined: switch to FP64 for good and solve again
+  int escalate(const double* r_in, double* out, bool* escalated) {
+    fp64_solve = true;
+    *escalated = true;
+    HN_TRY(ne_numeric(M, Mt, D.as<double>(), o.regularization, rel_reg(), ne, S.as<double>() + S_REG, &dense, false,
+                      s));
+    HN_TRY(dense_factor(dense, o.pivot_rule, s));
+    bool ok = false;
+    return solve_dense(r_in, out, &ok);
+  }
+
   int solve(const double* r_in, double* out, double mu, bool* escalated) {
     *escalated = false;
     if (!use_dense) return fp64_solve ? solve_pcg(r_in, out, mu) : solve_pcg_mixed(r_in, out, mu);
     bool ok = false;
     HN_TRY(solve_dense(r_in, out, &ok));
-    if (!ok && !fp64_solve) {  // FP32 inverse could not be refined: switch to FP64 for good
-      fp64_solve = true;
-      *escalated = true;
-      HN_TRY(ne_numeric(M, Mt, D.as<double>(), o.regularization, rel_reg(), ne, S.as<double>() + S_REG, &dense,
-                        false, s));
-      HN_TRY(dense_factor(dense, o.pivot_rule, s));
-      HN_TRY(solve_dense(r_in, out, &ok));
-    }
+    if (!ok && !fp64_solve) HN_TRY(escalate(r_in, out, escalated));
+    return HN_OK;
+  }
+
+  // The dense solve with its first check left to the caller's next pull
+  // (solve_begin, then the caller's kernels and pull, then solve_settle):
+  // the same kernels as solve() in the same order.  *redo: the first
+  // solution failed its check, the solve went on here, and the caller's
+  // kernels that used it must run again.
+  int solve_begin(const double* r_in, double* out) { return solve_dense_begin(r_in, out); }
+  int solve_settle(const double* r_in, double* out, bool* escalated, bool* redo) {
+    *escalated = false;
+    *redo = false;
+    if (dense_first_ok()) return HN_OK;
+    *redo = true;
+    bool ok = false;
+    HN_TRY(solve_dense_rest(r_in, out, &ok));
+    if (!ok && !fp64_solve) HN_TRY(escalate(r_in, out, escalated));
     return HN_OK;
   }
 };
@@ -909,21 +948,40 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
     hipLaunchKernelGGL(k_rhs, sv.g(m), B, 0, s, sv.rp.as<double>(), sv.mdrd.as<double>(), sv.tm.as<double>(),
                        sv.rhs.as<double>(), m);
     bool esc = false;
-    HN_TRY(sv.solve(sv.rhs.as<double>(), sv.dlam.as<double>(), mu, &esc));
+    // the dense solve's first check is read with the predictor's scalars
+    // (one host round trip fewer); if it failed, the solve is finished and
+    // the predictor's kernels run again
+    const bool deferred = sv.use_dense;
+    if (deferred)
+      HN_TRY(sv.solve_begin(sv.rhs.as<double>(), sv.dlam.as<double>()));
+    else
+      HN_TRY(sv.solve(sv.rhs.as<double>(), sv.dlam.as<double>(), mu, &esc));
+    auto predictor = [&]() -> int {
+      HN_TRY(sv.spmv(sv.Mt, sv.dlam.as<double>(), sv.tn.as<double>()));
+      hipLaunchKernelGGL(k_direction, sv.g(n), B, 0, s, sv.D.as<double>(), sv.tn.as<double>(), sv.rd.as<double>(),
+                         sv.x.as<double>(), (const double*)nullptr, sv.dx.as<double>(), sv.dz.as<double>(), n, F);
+      hipLaunchKernelGGL(k_finite, sv.g(m), B, 0, s, sv.dlam.as<double>(), m, F);
+      // ratio test and mu_aff (:629-643)
+      hipLaunchKernelGGL(k_red_step, dim3(np_n), dim3(kNT), 0, s, sv.x.as<double>(), sv.dx.as<double>(),
+                         sv.z.as<double>(), sv.dz.as<double>(), n, sv.part.as<double>());
+      hipLaunchKernelGGL(k_fin_step, dim3(1), dim3(kNT), 0, s, sv.part.as<double>(), np_n, S, (int)S_AP_A,
+                         (int)S_AD_A);
+      hipLaunchKernelGGL(k_red_muaff, dim3(np_n), dim3(kNT), 0, s, sv.x.as<double>(), sv.dx.as<double>(),
+                         sv.z.as<double>(), sv.dz.as<double>(), n, S, sv.part.as<double>());
+      hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(kNT), 0, s, sv.part.as<double>(), np_n, S, (int)S_MU_AFF);
+      HN_TRY(check_launch("predictor"));
+      return sv.pull();
+    };
+    HN_TRY(predictor());
+    if (deferred) {
+      bool redo = false;
+      HN_TRY(sv.solve_settle(sv.rhs.as<double>(), sv.dlam.as<double>(), &esc, &redo));
+      if (redo) {  // the direction flag of the unrefined solution does not count
+        HN_HIP(hipMemsetAsync(F + F_DIR, 0, sizeof(int), s));
+        HN_TRY(predictor());
+      }
+    }
     if (esc && res.fp64_from_iter < 0) res.fp64_from_iter = iter;
-    HN_TRY(sv.spmv(sv.Mt, sv.dlam.as<double>(), sv.tn.as<double>()));
-    hipLaunchKernelGGL(k_direction, sv.g(n), B, 0, s, sv.D.as<double>(), sv.tn.as<double>(), sv.rd.as<double>(),
-                       sv.x.as<double>(), (const double*)nullptr, sv.dx.as<double>(), sv.dz.as<double>(), n, F);
-    hipLaunchKernelGGL(k_finite, sv.g(m), B, 0, s, sv.dlam.as<double>(), m, F);
-    // ratio test and mu_aff (:629-643)
-    hipLaunchKernelGGL(k_red_step, dim3(np_n), dim3(kNT), 0, s, sv.x.as<double>(), sv.dx.as<double>(),
-                       sv.z.as<double>(), sv.dz.as<double>(), n, sv.part.as<double>());
-    hipLaunchKernelGGL(k_fin_step, dim3(1), dim3(kNT), 0, s, sv.part.as<double>(), np_n, S, (int)S_AP_A, (int)S_AD_A);
-    hipLaunchKernelGGL(k_red_muaff, dim3(np_n), dim3(kNT), 0, s, sv.x.as<double>(), sv.dx.as<double>(),
-                       sv.z.as<double>(), sv.dz.as<double>(), n, S, sv.part.as<double>());
-    hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(kNT), 0, s, sv.part.as<double>(), np_n, S, (int)S_MU_AFF);
-    HN_TRY(check_launch("predictor"));
-    HN_TRY(sv.pull());
     if (sv.hF[F_DIR]) {
       numerical_error = true;
       break;
