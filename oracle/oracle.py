@@ -537,3 +537,63 @@ class RefPresolve:
         if getattr(self, "_h", None) and REF is not None:
             REF.ref_presolve_free(self._h)
             self._h = None
+
+
+def ref_lp_parse(text: str, keep_names: bool = True) -> dict:
+    """model::LPParser::parse_string of the reference (lp_parser.cpp:25-111):
+    {"success", "errors", "lp" (lpgen.LP or None), "var_names", "con_names",
+    "var_type"}.  Test-side only."""
+    if REF is None or not hasattr(REF, "ref_lp_parse"):
+        raise RuntimeError("reference build not present")
+    import lpgen
+    R = REF
+    R.ref_lp_parse.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_void_p), _i64p]
+    R.ref_lp_problem.argtypes = [C.c_void_p, _i64p, _i64p, _dp, _i8p, _dp, _dp, _dp, _dp, _i8p,
+                                 C.POINTER(C.c_int), C.c_char_p, _i64]
+    R.ref_lp_errors.argtypes = [C.c_void_p, C.c_char_p, _i64]
+    R.ref_lp_write.argtypes = [C.c_void_p, C.c_char_p]
+    R.ref_lp_free.argtypes = [C.c_void_p]
+    h = C.c_void_p()
+    dims = np.zeros(5, np.int64)
+    rc = R.ref_lp_parse(text.encode(), 1 if keep_names else 0, C.byref(h), _p(dims, _i64p))
+    if rc != 0:
+        raise ValueError(f"reference LP parse returned {rc}")
+    try:
+        m, n, z, ne, ok = (int(d) for d in dims)
+        buf = C.create_string_buffer(1 << 20)
+        R.ref_lp_errors(h, buf, len(buf))
+        out = {"success": bool(ok), "errors": buf.value.decode().splitlines(), "lp": None}
+        if ok:
+            rp, ci, v = np.zeros(m + 1, np.int64), np.zeros(max(z, 1), np.int64), np.zeros(max(z, 1))
+            sense, rhs = np.zeros(max(m, 1), np.int8), np.zeros(max(m, 1))
+            lb, ub, obj = np.zeros(max(n, 1)), np.zeros(max(n, 1)), np.zeros(max(n, 1))
+            vt = np.zeros(max(n, 1), np.int8)
+            mx = C.c_int()
+            rc = R.ref_lp_problem(h, _p(rp, _i64p), _p(ci, _i64p), _p(v, _dp), _p(sense, _i8p), _p(rhs, _dp),
+                                  _p(lb, _dp), _p(ub, _dp), _p(obj, _dp), _p(vt, _i8p), C.byref(mx), buf, len(buf))
+            if rc != 0:
+                raise ValueError(f"reference LP problem returned {rc}")
+            names = buf.value.decode().split("\n")
+            out["lp"] = lpgen.LP(m=m, n=n, row_ptr=rp, col=ci[:z].copy(), val=v[:z].copy(), rhs=rhs[:m].copy(),
+                                 sense=sense[:m].copy(), lb=lb[:n].copy(), ub=ub[:n].copy(), obj=obj[:n].copy(),
+                                 maximize=bool(mx.value))
+            out["var_names"], out["con_names"] = names[:n], names[n:n + m]
+            out["var_type"] = vt[:n].copy()
+        return out
+    finally:
+        R.ref_lp_free(h)
+
+
+def ref_write_lp(text: str, path: str) -> None:
+    """The reference's write_lp (lp_parser.cpp:603-714) of the problem its
+    LPParser reads from text.  Test-side only."""
+    ref_lp_parse(text)  # sets the argtypes and checks the build
+    h = C.c_void_p()
+    dims = np.zeros(5, np.int64)
+    if REF.ref_lp_parse(text.encode(), 1, C.byref(h), _p(dims, _i64p)) != 0 or not dims[4]:
+        raise ValueError("reference LP parse failed")
+    try:
+        if REF.ref_lp_write(h, path.encode()) != 0:
+            raise OSError("reference write_lp failed")
+    finally:
+        REF.ref_lp_free(h)

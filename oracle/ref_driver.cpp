@@ -26,6 +26,7 @@
 #include "execution/gpu_backend.hpp"
 #include "lp/interior_point.hpp"
 #include "lp/simplex.hpp"
+#include "model/lp_parser.hpp"
 #include "model/problem.hpp"
 #include "numerical/sparse_matrix.hpp"
 #include "numerical/tolerance.hpp"
@@ -451,5 +452,82 @@ int ref_verify(int64_t m, int64_t n, const int64_t* rp, const int64_t* ci, const
   if (msg && msg_cap > 0) std::snprintf(msg, (size_t)msg_cap, "%s", r.message.c_str());
   return 0;
 }
+
+
+// ---- LP-format files (model::LPParser and model::write_lp, core/model/lp_parser.cpp)
+struct RefLpFile {
+  model::LPParseResult res;
+};
+
+// LPParser::parse_string (lp_parser.cpp:25-111).  dims = (m, n, nnz,
+// number of errors, success); returns -3 if the matrix is not stored by rows.
+int ref_lp_parse(const char* text, int keep_names, void** handle, int64_t* dims) {
+  model::LPParser parser;
+  parser.set_keep_names(keep_names != 0);
+  auto* h = new RefLpFile{parser.parse_string(text)};
+  const model::Problem& p = h->res.problem;
+  if (h->res.success && p.constraint_matrix.order() != model::StorageOrder::CSR) {
+    delete h;
+    return -3;
+  }
+  dims[0] = (int64_t)p.constraints.size();
+  dims[1] = (int64_t)p.variables.size();
+  dims[2] = (int64_t)p.constraint_matrix.nnz();
+  dims[3] = (int64_t)h->res.errors.size();
+  dims[4] = h->res.success ? 1 : 0;
+  *handle = h;
+  return 0;
+}
+
+// The parsed problem in the hn_lp layout (buffers sized by ref_lp_parse's
+// dims), the column types (VarType) and the names, each ended by '\n'.
+int ref_lp_problem(void* handle, int64_t* rp, int64_t* ci, double* v, int8_t* sense, double* rhs, double* lb,
+                   double* ub, double* obj, int8_t* vtype, int* maximize, char* names, int64_t names_cap) {
+  const model::Problem& r = static_cast<RefLpFile*>(handle)->res.problem;
+  const auto& A = r.constraint_matrix;
+  for (std::size_t i = 0; i <= r.constraints.size(); ++i) rp[i] = i < A.row_ptr().size() ? (int64_t)A.row_ptr()[i] : 0;
+  for (std::size_t k = 0; k < A.nnz(); ++k) {
+    ci[k] = (int64_t)A.col_indices()[k];
+    v[k] = A.values()[k];
+  }
+  std::string nm;
+  for (std::size_t i = 0; i < r.constraints.size(); ++i) {
+    sense[i] = static_cast<int8_t>(r.constraints[i].sense);
+    rhs[i] = r.constraints[i].rhs;
+  }
+  for (std::size_t j = 0; j < r.variables.size(); ++j) {
+    lb[j] = r.variables[j].lower_bound;
+    ub[j] = r.variables[j].upper_bound;
+    obj[j] = r.variables[j].objective_coeff;
+    vtype[j] = static_cast<int8_t>(r.variables[j].type);
+    nm += r.variables[j].name + "\n";
+  }
+  for (const auto& c : r.constraints) nm += c.name + "\n";
+  *maximize = r.obj_sense == model::ObjectiveSense::MAXIMIZE;
+  if ((int64_t)nm.size() + 1 > names_cap) return -4;
+  std::memcpy(names, nm.c_str(), nm.size() + 1);
+  return 0;
+}
+
+// The parser's errors, each ended by '\n'.
+int ref_lp_errors(void* handle, char* buf, int64_t cap) {
+  std::string all;
+  for (const auto& e : static_cast<RefLpFile*>(handle)->res.errors) all += e + "\n";
+  if ((int64_t)all.size() + 1 > cap) return -4;
+  std::memcpy(buf, all.c_str(), all.size() + 1);
+  return 0;
+}
+
+// write_lp (lp_parser.cpp:603-714) of the parsed problem.
+int ref_lp_write(void* handle, const char* path) {
+  try {
+    model::write_lp(static_cast<RefLpFile*>(handle)->res.problem, path);
+  } catch (...) {
+    return -1;
+  }
+  return 0;
+}
+
+void ref_lp_free(void* handle) { delete static_cast<RefLpFile*>(handle); }
 
 }  // extern "C"
