@@ -163,6 +163,16 @@ int hn_spmm_dev(const hn_csr* a, const double* B, double* C, uint64_t k, void* s
  * out is a device pointer. */
 int hn_dot_dev(const double* a, const double* b, uint64_t n, double* out, void* stream);
 int hn_norm_inf_dev(const double* v, uint64_t n, double* out, void* stream);
+/* hn_dot_dev that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory - code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  The
+ * partial sums live in a fixed 8 KiB device array (1,024 doubles) that each
+ * GPU holds until the process ends; hn_init takes its address.  Same kernels
+ * and grid as hn_dot_dev, so the result is bit-identical.  n up to 2^62
+ * (HN_ERR_INVALID above).  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (different streams, or different host
+ * threads) race and must be serialised by the caller. */
+int hn_dot_dev_noalloc(const double* a, const double* b, uint64_t n, double* out, void* stream);
 int hn_axpy_dev(double alpha, const double* x, double* y, uint64_t n, void* stream);
 
 /* ---- the interior-point solver ------------------------------------------ */

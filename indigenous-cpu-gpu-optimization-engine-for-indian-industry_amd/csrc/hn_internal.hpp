@@ -167,6 +167,9 @@ int dot_launch(const double* a, const double* b, int64_t n, double* out, hipStre
 int norm_inf_launch(const double* v, int64_t n, double* out, hipStream_t s);
 int axpy_launch(double alpha, const double* x, double* y, int64_t n, hipStream_t s);
 int fill_launch(double* p, double v, int64_t n, hipStream_t s);
+// Takes the device address of the allocation-free dot product's scratch
+// array on `device` (current at the call); once per device, from hn_init.
+int vec_noalloc_setup(int device);
 
 // Learned warm start on the device (hn_gnn.hip).  A job reads one scaled
 // standard form (values in the graph's CSR order, b, c) and writes its start

@@ -114,6 +114,7 @@ extern "C" int hn_init(int device, hn_device_info* info) {
     return fail(HN_ERR_NO_DEVICE, std::string("device is ") + p.gcnArchName +
                                       ", this library is built for gfx950 only");
   HN_HIP(hipSetDevice(best));
+  HN_TRY(vec_noalloc_setup(best));
   HN_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   std::memset(&c.info, 0, sizeof(c.info));
   std::snprintf(c.info.name, sizeof(c.info.name), "%s", p.name);

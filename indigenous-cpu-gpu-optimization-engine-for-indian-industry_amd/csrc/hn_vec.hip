@@ -86,6 +86,30 @@ int norm_inf_launch(const double* v, int64_t n, double* out, hipStream_t s) {
   return check_launch("norm_inf");
 }
 
+// ---- allocation-free dot product (hn_dot_dev_noalloc) -----------------------
+// The same two kernels as hn_dot_dev, with the partial sums in a file-scope
+// device array instead of a buffer reserved on first use, so the call never
+// allocates.  Each device holds its own copy of the array until the process
+// ends.  In host code the array's name is not a device address: the address
+// comes from hipGetSymbolAddress, taken once per device with that device
+// current (vec_noalloc_setup, called by hn_init) and kept in g_scratch.
+static __device__ __align__(256) double scratch[kRedBlocks];
+constexpr int kNoAllocDevices = 64;
+static double* g_scratch[kNoAllocDevices] = {};
+// Largest n: the grid is capped at kRedBlocks blocks, so the array serves
+// any length the kernels' signed 64-bit index can reach.
+constexpr uint64_t kNoAllocMaxN = uint64_t(1) << 62;
+
+int vec_noalloc_setup(int device) {
+  HN_CHECK(device >= 0 && device < kNoAllocDevices, HN_ERR_UNSUPPORTED,
+           "allocation-free scratch: device ordinal above 63");
+  if (g_scratch[device]) return HN_OK;
+  void* p = nullptr;
+  HN_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch[device] = static_cast<double*>(p);
+  return HN_OK;
+}
+
 int axpy_launch(double alpha, const double* x, double* y, int64_t n, hipStream_t s) {
   if (n <= 0) return HN_OK;
   hipLaunchKernelGGL(k_axpy, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, alpha, x, y, (long long)n);
@@ -101,6 +125,20 @@ extern "C" int hn_dot_dev(const double* a, const double* b, uint64_t n, double* 
   HN_CHECK(out && (n == 0 || (a && b)), HN_ERR_INVALID, "hn_dot_dev: null argument");
   HN_TRY(enter());
   return dot_launch(a, b, (int64_t)n, out, pick_stream(stream));
+}
+
+extern "C" int hn_dot_dev_noalloc(const double* a, const double* b, uint64_t n, double* out, void* stream) {
+  HN_CHECK(out && (n == 0 || (a && b)), HN_ERR_INVALID, "hn_dot_dev_noalloc: null argument");
+  HN_CHECK(n <= kNoAllocMaxN, HN_ERR_INVALID, "hn_dot_dev_noalloc: n above 2^62");
+  HN_TRY(enter());
+  double* part = g_scratch[ctx().device];
+  HN_CHECK(part, HN_ERR_NOT_INIT, "hn_dot_dev_noalloc: scratch address not set up by hn_init");
+  const int nb = partial_blocks((int64_t)n);
+  hipStream_t s = pick_stream(stream);
+  hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kRedNT), 0, s, a, b, (long long)n, part);
+  hipLaunchKernelGGL((k_finish<false>), dim3(1), dim3(kRedNT), 0, s, (const double*)part, nb, out);
+  count_launch(2);
+  return check_launch("dot (allocation-free)");
 }
 
 extern "C" int hn_norm_inf_dev(const double* v, uint64_t n, double* out, void* stream) {

@@ -79,6 +79,7 @@ def _load_kernels() -> C.CDLL:
         "hn_spmv_dev": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp]),
         "hn_spmm_dev": (C.c_int, [_vp, _vp, _vp, _u64, _vp]),
         "hn_dot_dev": (C.c_int, [_vp, _vp, _u64, _vp, _vp]),
+        "hn_dot_dev_noalloc": (C.c_int, [_vp, _vp, _u64, _vp, _vp]),
         "hn_norm_inf_dev": (C.c_int, [_vp, _u64, _vp, _vp]),
         "hn_axpy_dev": (C.c_int, [C.c_double, _vp, _vp, _u64, _vp]),
         "hn_dev_alloc": (C.c_int, [C.c_size_t, C.POINTER(_vp)]),

@@ -79,3 +79,12 @@ def test_no_gpu_here_is_reported():
     if st == 0:
         pytest.skip("a gfx950 device is visible")
     assert st == 6 and H.last_error()
+
+
+def test_noalloc_dot_rejects_oversized_n_on_the_host():
+    import hypernova_hip as H
+    out = C.c_double()
+    x = (C.c_double * 1)(1.0)
+    st = H.LIB.hn_dot_dev_noalloc(C.cast(x, C.c_void_p), C.cast(x, C.c_void_p), (1 << 62) + 1,
+                                  C.cast(C.byref(out), C.c_void_p), None)
+    assert st == 1 and "2^62" in H.last_error()
