@@ -96,6 +96,15 @@ int hn_csr_upload(const double* values, const uint64_t* col_indices,
 int hn_csr_transpose(const hn_csr* a, hn_csr** out);
 int hn_csr_free(hn_csr* a);
 int hn_csr_dims(const hn_csr* a, uint64_t* nrows, uint64_t* ncols, uint64_t* nnz);
+/* The sparsity pattern of the normal-equation matrix N = A D A' + r I of an
+ * IPM step whose scaling D is positive everywhere, built on the device as
+ * the solver builds it once per LP: the union of the column cliques of A
+ * and the diagonal, each row's columns ascending.  Replaces the symbolic
+ * half of build_normal_equations (core/lp/interior_point.cpp:144-200, whose
+ * from_triplets, core/numerical/sparse_matrix.cpp:8-53, merges equal
+ * entries): bit-exact with that pattern when no entry sums to exactly zero.
+ * *out is a new nrows x nrows handle whose values are zero. */
+int hn_csr_adat_pattern(const hn_csr* a, hn_csr** out);
 /* Copy the device CSR back (row_ptr: nrows+1, col/val: nnz entries). */
 int hn_csr_download(const hn_csr* a, double* values, uint64_t* col_indices,
                     uint64_t* row_ptr);
@@ -174,6 +183,15 @@ int hn_norm_inf_dev(const double* v, uint64_t n, double* out, void* stream);
  * threads) race and must be serialised by the caller. */
 int hn_dot_dev_noalloc(const double* a, const double* b, uint64_t n, double* out, void* stream);
 int hn_axpy_dev(double alpha, const double* x, double* y, uint64_t n, void* stream);
+
+/* The Mehrotra ratio test of both step lengths, with the solver's own
+ * kernels: out[0] = min(1, 0.99 min_{dx_i < 0} -x_i/dx_i) and out[1] the same
+ * for (z, dz) - 0.99 even when no component blocks (max_step,
+ * core/lp/interior_point.cpp:134-142, used at :629-630,687-688).  Minima are
+ * order-free, so both values are bit-identical to the reference's loop.
+ * out is a device pointer to 2 doubles; n < 2^31. */
+int hn_ratio_test_dev(const double* x, const double* dx, const double* z, const double* dz, uint64_t n,
+                      double* out, void* stream);
 
 /* ---- the interior-point solver ------------------------------------------ */
 /* Constraint senses in the order of ConstraintSense (core/model/problem.hpp:24-28). */
@@ -270,9 +288,15 @@ typedef struct hn_ipm_opts {
                                   return it (status HN_STATUS_ITER_LIMIT unless the 1e-8
                                   test was met first); 0 = off */
   double gap_tol;              /* > 0: the stopping test (interior_point.cpp:525-530) also
-                                  requires |c'x - b'y| / (1 + |c'x|) <= gap_tol on the
-                                  scaled form, so a solve meets a relative-gap bar and not
-                                  only the absolute mu bar; 0 = off (the reference's test) */
+                                  requires the relative gap of the caller's LP (rel_gap of
+                                  hn_ipm_result) to be <= gap_tol, so a solve meets a
+                                  relative-gap bar and not only the absolute mu bar; 0 = off
+                                  (the reference's test).  hn_ipm_solve computes rel_gap
+                                  exactly on the host at each iterate that passes the
+                                  absolute test; hn_ipm_solve_batch tests
+                                  |c_s'x_s - b_s'lam| / (1 + |c'x|) in the kernel, which is
+                                  that gap when no column has a finite nonzero bound
+                                  shift (otherwise it differs by sum_j shift_j rd_j) */
 } hn_ipm_opts;
 
 typedef struct hn_ipm_result {
@@ -290,10 +314,24 @@ typedef struct hn_ipm_result {
   int64_t sf_m, sf_n;          /* out: standard-form dimensions */
   /* the reference's stopping quantities on the scaled form (interior_point.cpp:525-530) */
   double rp_norm, rd_norm, mu;
-  /* original-space relative KKT metrics */
-  double rel_primal;           /* ||Ax-b||_inf / (1 + ||b||_inf) over the scaled rows */
-  double rel_dual;             /* ||M'y + s - c||_inf / (1 + ||c||_inf) */
-  double rel_gap;              /* |c'x - b'y| / (1 + |c'x|) */
+  /* relative KKT measures on the caller's own LP (all its rows and columns,
+   * minimisation sense).  The point is mapped back from the scaled standard
+   * form: x by to_original_primal (interior_point.cpp:375-395), row duals
+   * y_i = +-gamma_i lam_r (0 for a dropped duplicate row), one net bound
+   * multiplier z_j = lower - upper per column from the slots' s.  In the
+   * standard-form case (EQ rows, x >= 0) they are exactly
+   * ||Ax - b||_inf / (1 + ||b||_inf), ||A'y + z - c||_inf / (1 + ||c||_inf)
+   * and |c'x - b'y| / (1 + |c'x|); in general the primal one takes row
+   * violations by sense and bound violations, the dual one adds the LE/GE
+   * rows' sign conditions and the gap adds the bound terms of the dual
+   * objective. */
+  double rel_primal;
+  double rel_dual;
+  double rel_gap;
+  /* the same three on the scaled standard form, with its rp, rd, c and rhs */
+  double scaled_rel_primal;
+  double scaled_rel_dual;
+  double scaled_rel_gap;
   int64_t ne_solves;           /* normal-equation solves */
   int64_t refine_sweeps;       /* FP64 refinement sweeps */
   int64_t cg_iterations;       /* PCG iterations */

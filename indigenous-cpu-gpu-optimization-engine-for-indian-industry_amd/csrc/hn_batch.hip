@@ -52,6 +52,7 @@ struct Inst {
   long long dn, w;    // dense N (m*m elements), panel workspace (m*64 elements)
   int warm;           // 1: x, z, lam hold a warm start accepted on the host; 2: one predicted on the
                       // device, tested by the kernel before use
+  double off;         // objective constant of the standard form: c'x of the caller's LP = c_s'x_s + off
 };
 
 // Per-LP vectors.  B, LAM, C, X and Z cross the host link and live in their
@@ -74,7 +75,8 @@ struct Params {
   int mixed;
   int skip;  // HN_PIVOT_SKIP: r = reg only, tiny pivots skipped
   double mu_target;  // > 0: stop at the first iterate with mu <= mu_target
-  double gap_tol;    // > 0: the stop test also needs |c'x - b'y| <= gap_tol (1 + |c'x|)
+  double gap_tol;    // > 0: the stop test also needs |c_s'x_s - b_s'lam| <= gap_tol (1 + |c'x|), c'x the
+                     // caller's objective (the gap of the caller's LP when no column has a bound shift)
 };
 
 // A skipped pivot: its entry of D becomes "infinite", so its column of L and
@@ -764,7 +766,7 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
           for (int i = tid; i < m; i += NT) by += b[i] * lam[i];
           cx = wg_sum(cx, red);
           by = wg_sum(by, red);
-          gap_ok = fabs(cx - by) <= prm.gap_tol * (1.0 + fabs(cx));
+          gap_ok = fabs(cx - by) <= prm.gap_tol * (1.0 + fabs(cx + I.off));
         }
         if (gap_ok) {
           done = true;
@@ -1061,6 +1063,7 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
     I.w = wk;
     wk += (long long)F.m * BK;
     I.warm = warm_ok[k] ? 1 : (predict[k] ? 2 : 0);
+    I.off = F.offset;
     any_warm = any_warm || warm_ok[k];
   }
   BatchWorkspace& w = bws();
@@ -1200,18 +1203,12 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
     r.rp_norm = out[k].rp;
     r.rd_norm = out[k].rd;
     r.mu = out[k].mu;
-    double bn = 0.0, cn = 0.0, ctx_ = 0.0, bty = 0.0;
-    for (int64_t i = 0; i < F.m; ++i) {
-      bn = std::max(bn, std::abs(F.rhs[i]));
-      bty += F.rhs[i] * lam[i];
-    }
-    for (int64_t j = 0; j < F.n; ++j) {
-      cn = std::max(cn, std::abs(F.c[j]));
-      ctx_ += F.c[j] * x[j];
-    }
-    r.rel_primal = out[k].rp / (1.0 + bn);
-    r.rel_dual = out[k].rd / (1.0 + cn);
-    r.rel_gap = std::abs(ctx_ - bty) / (1.0 + std::abs(ctx_));
+    scaled_kkt(F, x, lam, out[k].rp, out[k].rd, r);
+    OriginalKkt kk;
+    original_kkt(lps[k], F, x, lam, z, kk);
+    r.rel_primal = kk.rel_primal;
+    r.rel_dual = kk.rel_dual;
+    r.rel_gap = kk.rel_gap;
     r.ne_solves = out[k].ne_solves;
     r.refine_sweeps = out[k].refine;
     r.cg_iterations = 0;

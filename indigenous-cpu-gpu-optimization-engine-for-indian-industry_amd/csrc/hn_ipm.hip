@@ -753,6 +753,7 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
   res.sf_m = res.sf_n = 0;
   res.rp_norm = res.rd_norm = res.mu = 0.0;
   res.rel_primal = res.rel_dual = res.rel_gap = 0.0;
+  res.scaled_rel_primal = res.scaled_rel_dual = res.scaled_rel_gap = 0.0;
   res.ne_solves = res.refine_sweeps = res.cg_iterations = 0;
   res.fp64_from_iter = -1;
   res.warm_started = 0;
@@ -901,13 +902,15 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
     prev_rd = hS[S_RDN];
     if (hS[S_RPN] < o.convergence_tol && hS[S_RDN] < o.convergence_tol && mu < o.complementarity_tol) {
       bool gap_ok = true;
-      if (o.gap_tol > 0.0) {  // optional relative-gap bar (not in the reference)
-        hipLaunchKernelGGL(k_red_final, dim3(np_n), dim3(kNT), 0, s, sv.c.as<double>(), sv.x.as<double>(),
-                           sv.b.as<double>(), sv.lam.as<double>(), sv.rd.as<double>(), n, m, sv.part.as<double>());
-        hipLaunchKernelGGL(k_fin_final, dim3(1), dim3(kNT), 0, s, sv.part.as<double>(), np_n, S);
-        HN_TRY(check_launch("gap"));
-        HN_TRY(sv.pull());
-        gap_ok = std::abs(hS[S_CTX] - hS[S_BTY]) <= o.gap_tol * (1.0 + std::abs(hS[S_CTX]));
+      if (o.gap_tol > 0.0) {  // optional relative-gap bar on the caller's LP (not in the reference)
+        std::vector<double> gx(n), gz(n), gl(m);
+        HN_HIP(hipMemcpyAsync(gx.data(), sv.x.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        HN_HIP(hipMemcpyAsync(gz.data(), sv.z.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        HN_HIP(hipMemcpyAsync(gl.data(), sv.lam.p, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+        HN_HIP(hipStreamSynchronize(s));
+        OriginalKkt kk;
+        original_kkt(lp, f, gx.data(), gl.data(), gz.data(), kk);
+        gap_ok = kk.rel_gap <= o.gap_tol;
       }
       if (gap_ok) {
         done = true;  // :525-530
@@ -950,8 +953,10 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
     bool esc = false;
     // the dense solve's first check is read with the predictor's scalars
     // (one host round trip fewer); if it failed, the solve is finished and
-    // the predictor's kernels run again
-    const bool deferred = sv.use_dense;
+    // the predictor's kernels run again.  Only for the FP64 factor: an FP32
+    // one almost never passes its first check, so the predictor would run
+    // twice.
+    const bool deferred = sv.use_dense && sv.fp64_solve;
     if (deferred)
       HN_TRY(sv.solve_begin(sv.rhs.as<double>(), sv.dlam.as<double>()));
     else
@@ -1065,12 +1070,12 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
   res.rp_norm = rpn;
   res.rd_norm = hS[S_RDN];
   res.mu = mu;
-  double bn = 0.0, cn = 0.0;
-  for (double v : f.rhs) bn = std::max(bn, std::abs(v));
-  for (double v : f.c) cn = std::max(cn, std::abs(v));
-  res.rel_primal = rpn / (1.0 + bn);
-  res.rel_dual = hS[S_DRN] / (1.0 + cn);
-  res.rel_gap = std::abs(hS[S_CTX] - hS[S_BTY]) / (1.0 + std::abs(hS[S_CTX]));
+  scaled_kkt(f, hx.data(), hl.data(), rpn, hS[S_DRN], res);
+  OriginalKkt kk;
+  original_kkt(lp, f, hx.data(), hl.data(), hz.data(), kk);
+  res.rel_primal = kk.rel_primal;
+  res.rel_dual = kk.rel_dual;
+  res.rel_gap = kk.rel_gap;
   if (res.x && res.cap_n >= n) std::memcpy(res.x, hx.data(), sizeof(double) * n);
   if (res.s && res.cap_n >= n) std::memcpy(res.s, hz.data(), sizeof(double) * n);
   if (res.y && res.cap_m >= m) std::memcpy(res.y, hl.data(), sizeof(double) * m);
@@ -1111,4 +1116,22 @@ extern "C" int hn_ipm_solve(const hn_lp* lp, const hn_ipm_opts* opts, hn_ipm_res
   HN_CHECK(o.pivot_rule == HN_PIVOT_CLAMP || o.pivot_rule == HN_PIVOT_SKIP, HN_ERR_INVALID,
            "hn_ipm_solve: bad pivot_rule");
   return hn::ipm_solve_impl(*lp, o, *res);
+}
+
+// The ratio test as a device operation, on the solver's own kernels.
+extern "C" int hn_ratio_test_dev(const double* x, const double* dx, const double* z, const double* dz, uint64_t n,
+                                 double* out, void* stream) {
+  HN_CHECK(out && (n == 0 || (x && dx && z && dz)), HN_ERR_INVALID, "hn_ratio_test_dev: null argument");
+  HN_CHECK(n < (uint64_t)INT_MAX, HN_ERR_UNSUPPORTED, "hn_ratio_test_dev: n >= 2^31");
+  HN_TRY(hn::enter());
+  static std::mutex mtx;
+  static hn::DevBuf* part = new hn::DevBuf();  // partial minima, kept until the process ends
+  std::lock_guard<std::mutex> lk(mtx);
+  HN_TRY(part->reserve(sizeof(double) * 2 * hn::kMaxParts));
+  hipStream_t s = hn::pick_stream(stream);
+  const int np = hn::parts_for((int64_t)n);
+  hipLaunchKernelGGL(hn::k_red_step, dim3(np), dim3(hn::kNT), 0, s, x, dx, z, dz, (int)n, part->as<double>());
+  hipLaunchKernelGGL(hn::k_fin_step, dim3(1), dim3(hn::kNT), 0, s, (const double*)part->as<double>(), np, out, 0, 1);
+  hn::count_launch(2);
+  return hn::check_launch("ratio test");
 }

@@ -757,3 +757,20 @@ int ne_jacobi_diag(const hn_csr* M, const double* D, double* diag, hipStream_t s
 }
 
 }  // namespace hn
+
+extern "C" int hn_csr_adat_pattern(const hn_csr* a, hn_csr** out) {
+  HN_CHECK(a && out, HN_ERR_INVALID, "hn_csr_adat_pattern: null argument");
+  *out = nullptr;
+  HN_TRY(hn::enter());
+  hipStream_t s = hn::ctx().stream;
+  hn_csr t;
+  HN_TRY(hn::csr_alloc(&t, a->ncols, a->nrows, a->nnz, a->flags));
+  HN_TRY(hn::transpose_into(a, &t, s));
+  hn::NormalEq ne;
+  HN_TRY(hn::ne_symbolic(a, &t, ne, s));
+  HN_HIP(hipMemsetAsync(ne.N->val64.p, 0, sizeof(double) * std::max<int64_t>(ne.N->nnz, 1), s));
+  HN_HIP(hipStreamSynchronize(s));
+  *out = ne.N;
+  ne.N = nullptr;
+  return HN_OK;
+}

@@ -181,6 +181,10 @@ int build_standard_form(const hn_lp& lp, StandardForm& f) {
   for (int64_t j = 0; j < n; ++j)
     for (int s = 0; s < f.vars[j].nslots; ++s) f.c[f.vars[j].slot[s]] = f.vars[j].sign[s] > 0 ? cbase[j] : -cbase[j];
   t.clear();
+  f.row_gamma.assign(norig, 1.0);
+  f.row_slack.assign(norig, -1);
+  f.ub_row.assign(n, -1);
+  f.ub_slack.assign(n, -1);
   int64_t next_slack = nslots;
   for (int64_t r = 0; r < norig; ++r) {
     const int64_t i = f.kept_rows[r];
@@ -201,13 +205,19 @@ int build_standard_form(const hn_lp& lp, StandardForm& f) {
         }
       }
     }
-    if (lp.sense[i] == HN_LE || lp.sense[i] == HN_GE) t.push_back({r, next_slack++, gamma});
+    f.row_gamma[r] = gamma;
+    if (lp.sense[i] == HN_LE || lp.sense[i] == HN_GE) {
+      f.row_slack[r] = next_slack;
+      t.push_back({r, next_slack++, gamma});
+    }
     f.rhs[r] = flip ? (-gamma * lp.rhs[i] + const_i) : (gamma * lp.rhs[i] - const_i);
   }
   int64_t next_ub = norig;
   for (int64_t j = 0; j < n; ++j) {
     const double lb = lp.lb[j], ub = lp.ub[j];
     if (std::isfinite(lb) && std::isfinite(ub) && std::abs(ub - lb) > kFeasTol) {
+      f.ub_row[j] = next_ub;
+      f.ub_slack[j] = next_slack;
       t.push_back({next_ub, f.vars[j].slot[0], 1.0});
       t.push_back({next_ub, next_slack++, 1.0});
       f.rhs[next_ub] = ub - lb;
@@ -246,6 +256,85 @@ void to_original_primal(const StandardForm& f, const double* x, double* primal) 
     }
     primal[j] = val;
   }
+}
+
+void original_kkt(const hn_lp& lp, const StandardForm& f, const double* x, const double* lam, const double* s,
+                  OriginalKkt& out) {
+  out = OriginalKkt();
+  const int64_t m0 = lp.m, n = lp.n;
+  std::vector<double> primal(n), y(m0, 0.0), aty(n, 0.0);
+  to_original_primal(f, x, primal.data());
+  const int64_t norig = (int64_t)f.kept_rows.size();
+  for (int64_t r = 0; r < norig; ++r) {
+    const int64_t i = f.kept_rows[r];
+    const double yi = f.row_gamma[r] * lam[r];
+    y[i] = lp.sense[i] == HN_GE ? -yi : yi;
+  }
+  double bn = 0.0, cn = 0.0, pres = 0.0, dres = 0.0, pobj = 0.0, dobj = 0.0;
+  for (int64_t i = 0; i < m0; ++i) {
+    double ax = 0.0;
+    for (int64_t k = lp.row_ptr[i]; k < lp.row_ptr[i + 1]; ++k) {
+      ax += lp.val[k] * primal[lp.col[k]];
+      aty[lp.col[k]] += lp.val[k] * y[i];
+    }
+    const double d = ax - lp.rhs[i];
+    const double v = lp.sense[i] == HN_EQ ? std::abs(d) : lp.sense[i] == HN_LE ? std::max(0.0, d) : std::max(0.0, -d);
+    pres = std::max(pres, v);
+    bn = std::max(bn, std::abs(lp.rhs[i]));
+    dobj += lp.rhs[i] * y[i];
+  }
+  for (int64_t r = 0; r < norig; ++r) {  // row slacks: (+-y_i) + v_i = 0
+    const int64_t w = f.row_slack[r];
+    if (w < 0) continue;
+    const int64_t i = f.kept_rows[r];
+    const double v = s[w] / f.col_scale[w];
+    dres = std::max(dres, std::abs((lp.sense[i] == HN_GE ? -y[i] : y[i]) + v));
+  }
+  for (int64_t j = 0; j < n; ++j) {
+    const double cj = lp.maximize ? -lp.obj[j] : lp.obj[j];
+    const double lb = lp.lb[j], ub = lp.ub[j], xj = primal[j];
+    cn = std::max(cn, std::abs(lp.obj[j]));
+    pobj += cj * xj;
+    pres = std::max(pres, std::max(std::max(0.0, lb - xj), std::max(0.0, xj - ub)));
+    const VarMap& vm = f.vars[j];
+    if (vm.fixed) {  // its bound multiplier is free: c_j - (A'y)_j, no residual
+      dobj += vm.fixed_value * (cj - aty[j]);
+      continue;
+    }
+    double lower = 0.0, upper = 0.0;
+    if (vm.nslots == 1) {
+      const int64_t k = vm.slot[0];
+      const double mult = s[k] / f.col_scale[k];
+      if (vm.sign[0] > 0) lower = mult; else upper = mult;
+      if (f.ub_slack[j] >= 0) upper = s[f.ub_slack[j]] / f.col_scale[f.ub_slack[j]];
+    }
+    if (lower != 0.0) dobj += lb * lower;
+    if (upper != 0.0) dobj -= ub * upper;
+    dres = std::max(dres, std::abs(aty[j] + (lower - upper) - cj));
+  }
+  out.primal_res = pres;
+  out.dual_res = dres;
+  out.primal_obj = pobj;
+  out.dual_obj = dobj;
+  out.rel_primal = pres / (1.0 + bn);
+  out.rel_dual = dres / (1.0 + cn);
+  out.rel_gap = std::abs(pobj - dobj) / (1.0 + std::abs(pobj));
+}
+
+void scaled_kkt(const StandardForm& f, const double* x, const double* lam, double rp_norm, double rd_norm,
+                hn_ipm_result& res) {
+  double bn = 0.0, cn = 0.0, cx = 0.0, by = 0.0;
+  for (int64_t i = 0; i < f.m; ++i) {
+    bn = std::max(bn, std::abs(f.rhs[i]));
+    by += f.rhs[i] * lam[i];
+  }
+  for (int64_t j = 0; j < f.n; ++j) {
+    cn = std::max(cn, std::abs(f.c[j]));
+    cx += f.c[j] * x[j];
+  }
+  res.scaled_rel_primal = rp_norm / (1.0 + bn);
+  res.scaled_rel_dual = rd_norm / (1.0 + cn);
+  res.scaled_rel_gap = std::abs(cx - by) / (1.0 + std::abs(cx));
 }
 
 // The warm-start acceptance test (no counterpart in the reference, whose

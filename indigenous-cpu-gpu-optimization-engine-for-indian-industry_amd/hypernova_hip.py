@@ -61,6 +61,7 @@ def _load_kernels() -> C.CDLL:
         "hn_reset_execution_stats": (None, []),
         "hn_csr_upload": (C.c_int, [_dp, _u64p, _u64p, _u64, _u64, _u64, C.c_int, C.POINTER(_vp)]),
         "hn_csr_transpose": (C.c_int, [_vp, C.POINTER(_vp)]),
+        "hn_csr_adat_pattern": (C.c_int, [_vp, C.POINTER(_vp)]),
         "hn_csr_free": (C.c_int, [_vp]),
         "hn_csr_dims": (C.c_int, [_vp, _u64p, _u64p, _u64p]),
         "hn_csr_download": (C.c_int, [_vp, _dp, _u64p, _u64p]),
@@ -81,6 +82,7 @@ def _load_kernels() -> C.CDLL:
         "hn_dot_dev": (C.c_int, [_vp, _vp, _u64, _vp, _vp]),
         "hn_dot_dev_noalloc": (C.c_int, [_vp, _vp, _u64, _vp, _vp]),
         "hn_norm_inf_dev": (C.c_int, [_vp, _u64, _vp, _vp]),
+        "hn_ratio_test_dev": (C.c_int, [_vp, _vp, _vp, _vp, _u64, _vp, _vp]),
         "hn_axpy_dev": (C.c_int, [C.c_double, _vp, _vp, _u64, _vp]),
         "hn_dev_alloc": (C.c_int, [C.c_size_t, C.POINTER(_vp)]),
         "hn_dev_free": (C.c_int, [_vp]),
@@ -217,6 +219,12 @@ class CsrMatrix:
     def transpose(self) -> "CsrMatrix":
         h = _vp()
         check(LIB.hn_csr_transpose(self.h, C.byref(h)))
+        return CsrMatrix(h)
+
+    def adat_pattern(self) -> "CsrMatrix":
+        """Pattern of A D A' (D > 0) built on the device (hn_csr_adat_pattern)."""
+        h = _vp()
+        check(LIB.hn_csr_adat_pattern(self.h, C.byref(h)))
         return CsrMatrix(h)
 
     def download(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
@@ -491,7 +499,8 @@ class _HnResult(C.Structure):
                 ("cap_m", C.c_int64), ("cap_n", C.c_int64), ("sf_m", C.c_int64), ("sf_n", C.c_int64),
                 ("rp_norm", C.c_double), ("rd_norm", C.c_double), ("mu", C.c_double),
                 ("rel_primal", C.c_double), ("rel_dual", C.c_double), ("rel_gap", C.c_double),
-                ("ne_solves", C.c_int64), ("refine_sweeps", C.c_int64), ("cg_iterations", C.c_int64),
+                ("scaled_rel_primal", C.c_double), ("scaled_rel_dual", C.c_double),
+                ("scaled_rel_gap", C.c_double), ("ne_solves", C.c_int64), ("refine_sweeps", C.c_int64), ("cg_iterations", C.c_int64),
                 ("fp64_from_iter", C.c_int64), ("warm_started", C.c_int)]
 
 
@@ -522,7 +531,7 @@ class InteriorPointOptions:
         self.warm_start = None          # (x0, y0, s0) of the scaled standard form
         self.pivot_rule = HN_PIVOT_CLAMP  # HN_PIVOT_SKIP: Wright's skipped pivots, r = regularization
         self.mu_target = 0.0            # > 0: return the first iterate with mu <= mu_target
-        self.gap_tol = 0.0              # > 0: also stop only once |c'x - b'y| / (1 + |c'x|) <= gap_tol
+        self.gap_tol = 0.0              # > 0: also stop only once the caller's LP has rel_gap <= gap_tol
         self.verbose = False
         for k, v in kw.items():
             if not hasattr(self, k):
@@ -539,7 +548,10 @@ class InteriorPointResult:
         self.primal = primal
         self.x, self.y, self.s = x[:r.sf_n].copy(), y[:r.sf_m].copy(), s[:r.sf_n].copy()
         self.rp_norm, self.rd_norm, self.mu = r.rp_norm, r.rd_norm, r.mu
+        # relative KKT measures on the caller's LP, and the same on the scaled standard form
         self.rel_primal, self.rel_dual, self.rel_gap = r.rel_primal, r.rel_dual, r.rel_gap
+        self.scaled_rel_primal, self.scaled_rel_dual = r.scaled_rel_primal, r.scaled_rel_dual
+        self.scaled_rel_gap = r.scaled_rel_gap
         self.ne_solves, self.refine_sweeps = int(r.ne_solves), int(r.refine_sweeps)
         self.cg_iterations, self.fp64_from_iter = int(r.cg_iterations), int(r.fp64_from_iter)
         self.warm_started = bool(r.warm_started)

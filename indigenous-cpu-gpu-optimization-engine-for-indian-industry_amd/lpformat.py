@@ -131,9 +131,13 @@ class LPParser:
         self.keep_names = bool(keep)
 
     def parse_file(self, path: str) -> LPParseResult:
+        # The reference reads the file's bytes unchanged into a string and
+        # parses that (lp_parser.cpp:13-23): no newline translation, no
+        # decoding.  Latin-1 maps every byte to one character, so a bare CR
+        # stays inside its line and any byte value reads without error.
         try:
-            with open(path, "r") as fh:
-                text = fh.read()
+            with open(path, "rb") as fh:
+                text = fh.read().decode("latin-1")
         except OSError:
             return LPParseResult(errors=["Cannot open file: " + path])
         return self.parse_string(text)

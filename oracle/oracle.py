@@ -555,14 +555,16 @@ def ref_lp_parse(text: str, keep_names: bool = True) -> dict:
     R.ref_lp_free.argtypes = [C.c_void_p]
     h = C.c_void_p()
     dims = np.zeros(5, np.int64)
-    rc = R.ref_lp_parse(text.encode(), 1 if keep_names else 0, C.byref(h), _p(dims, _i64p))
+    data = text if isinstance(text, bytes) else text.encode()
+    rc = R.ref_lp_parse(data, 1 if keep_names else 0, C.byref(h), _p(dims, _i64p))
     if rc != 0:
         raise ValueError(f"reference LP parse returned {rc}")
     try:
         m, n, z, ne, ok = (int(d) for d in dims)
         buf = C.create_string_buffer(1 << 20)
         R.ref_lp_errors(h, buf, len(buf))
-        out = {"success": bool(ok), "errors": buf.value.decode().splitlines(), "lp": None}
+        enc = "latin-1" if isinstance(text, bytes) else "utf-8"
+        out = {"success": bool(ok), "errors": buf.value.decode(enc).splitlines(), "lp": None}
         if ok:
             rp, ci, v = np.zeros(m + 1, np.int64), np.zeros(max(z, 1), np.int64), np.zeros(max(z, 1))
             sense, rhs = np.zeros(max(m, 1), np.int8), np.zeros(max(m, 1))
@@ -573,7 +575,7 @@ def ref_lp_parse(text: str, keep_names: bool = True) -> dict:
                                   _p(lb, _dp), _p(ub, _dp), _p(obj, _dp), _p(vt, _i8p), C.byref(mx), buf, len(buf))
             if rc != 0:
                 raise ValueError(f"reference LP problem returned {rc}")
-            names = buf.value.decode().split("\n")
+            names = buf.value.decode(enc).split("\n")
             out["lp"] = lpgen.LP(m=m, n=n, row_ptr=rp, col=ci[:z].copy(), val=v[:z].copy(), rhs=rhs[:m].copy(),
                                  sense=sense[:m].copy(), lb=lb[:n].copy(), ub=ub[:n].copy(), obj=obj[:n].copy(),
                                  maximize=bool(mx.value))

@@ -116,3 +116,41 @@ def crossover_cases():
     out.append(("transport5x8", lpgen.transport_lp(1000000, 5, 8), 100, 100))
     out.append(("transport10x15", lpgen.transport_lp(1000001, 10, 15), 100, 100))
     return out
+
+
+def general_bounds_lp(seed=0, m=24, n=60, free=False, maximize=False):
+    """A feasible, bounded LP with every column and row kind of the standard
+    form: per column j by j % 5 lb-only, ub-only, boxed, free (or, with
+    free=False, lb-only again: the reference's IPM stalls on split free
+    columns), fixed; rows LE, GE, EQ in turn, plus one duplicated row; an
+    objective offset; maximize=True states the same LP as max -c'x.
+    Feasible at an interior x*, bounded because c is built from a dual point
+    that meets every sign condition (y <= 0 on LE rows, >= 0 on GE rows,
+    positive lower/upper multipliers, zero reduced cost on free columns)."""
+    rng = np.random.default_rng(seed)
+    A = np.where(rng.random((m, n)) < 0.25, rng.standard_normal((m, n)), 0.0)
+    for i in range(m):
+        A[i, rng.integers(0, n)] = 1.0 + rng.random()
+    A[:, 3::5] += np.where(rng.random((m, len(range(3, n, 5)))) < 0.3, 1.0, 0.0)  # free columns not empty
+    xs = rng.uniform(-2, 2, n)
+    kind = np.arange(n) % 5
+    if not free:
+        kind[kind == 3] = 0
+    lb = np.where((kind == 0) | (kind == 2), xs - rng.uniform(0.5, 1.5, n), -INF)
+    ub = np.where((kind == 1) | (kind == 2), xs + rng.uniform(0.5, 1.5, n), INF)
+    lb[kind == 4] = ub[kind == 4] = xs[kind == 4]
+    sense = np.array([(LE, GE, EQ)[i % 3] for i in range(m)], np.int8)
+    ax = A @ xs
+    rhs = np.where(sense == LE, ax + rng.uniform(0.5, 2, m), np.where(sense == GE, ax - rng.uniform(0.5, 2, m), ax))
+    y = rng.uniform(0.2, 2, m) * np.where(sense == LE, -1.0, 1.0)
+    y[sense == EQ] = rng.standard_normal(int((sense == EQ).sum()))
+    lower = np.where((kind == 0) | (kind == 2), rng.uniform(0.1, 2, n), 0.0)
+    upper = np.where((kind == 1), rng.uniform(0.1, 2, n), 0.0)
+    c = A.T @ y + lower - upper
+    c[kind == 4] = rng.standard_normal(int((kind == 4).sum()))
+    rows = [[(j, A[i, j]) for j in range(n) if A[i, j] != 0.0] for i in range(m)]
+    rows.append(list(rows[1]))  # an exact duplicate of a GE row
+    lp = from_rows(n, rows, list(sense) + [sense[1]], list(rhs) + [rhs[1]], lb, ub, -c if maximize else c,
+                   maximize=maximize)
+    lp.obj_offset = 1.5
+    return lp

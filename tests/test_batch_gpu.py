@@ -109,3 +109,31 @@ def test_batch_gap_tol_meets_relative_bar(hip, pivot_rule, precision):
         assert r.rel_gap <= 1e-8 and r.rel_primal <= 1e-8 and r.rel_dual <= 1e-8, (k, r.rel_gap, r.rel_primal)
         assert r.iterations >= p.iterations  # the extra bar can only add iterations
         assert rel(r.objective_value, _highs_objective(lp)) <= 3e-8, k
+
+
+def test_batch_whole_size_range_against_highs(hip):
+    """Instances spread over the whole range of m (up to 2,000), checked
+    against HiGHS's optimum (tests/golden/highs_objectives.json, written by
+    make_highs_fixtures.py) with the relative-gap bar on top of the
+    reference's test, under both pivot rules."""
+    import json
+    import os
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "highs_objectives.json")))
+    ids = sorted(int(k.split("_")[1]) for k in gold if k.startswith("batch_"))
+    lps = [lpgen.batch_lp(i) for i in ids]
+    assert max(lp.m for lp in lps) >= 1990
+    for rule in (hip.HN_PIVOT_SKIP, hip.HN_PIVOT_CLAMP):
+        print(f"[case] batch vs HiGHS, {len(lps)} LPs m={[lp.m for lp in lps]} pivot_rule={rule}", flush=True)
+        out = hip.ipm_solve_batch(lps, pivot_rule=rule, gap_tol=1e-8)
+        for i, lp, r in zip(ids, lps, out):
+            h = gold[f"batch_{i}"]["objective"]
+            print(f"  {i} m={lp.m} {r.status} it={r.iterations} obj rel err {rel(r.objective_value, h):.1e} "
+                  f"kkt {r.rel_primal:.1e} {r.rel_dual:.1e} {r.rel_gap:.1e}", flush=True)
+            assert r.status == "OPTIMAL", (i, r.status)
+            if rule == hip.HN_PIVOT_CLAMP and r.iterations >= 100:
+                # the reference's rule stalls on some instances and exits
+                # through its relaxed post-loop test (interior_point.cpp:745-752)
+                assert rel(r.objective_value, h) <= 1e-5, i
+                continue
+            assert rel(r.objective_value, h) <= 2e-8, (i, r.objective_value, h)
+            assert r.rel_primal <= 1e-8 and r.rel_dual <= 1e-8 and r.rel_gap <= 1e-8, i

@@ -278,3 +278,22 @@ def test_round_trip_of_a_generated_lp(tmp_path):
     np.testing.assert_allclose(back.val, lp.val, rtol=5e-6)
     np.testing.assert_allclose(back.rhs, lp.rhs, rtol=5e-6)
     np.testing.assert_allclose(back.obj, lp.obj, rtol=5e-6)
+
+
+@need_ref
+@pytest.mark.parametrize("data", [
+    b"Maximize\r obj: x + y\rSubject To\r c1: x + y <= 4\rEnd\r",            # bare CR separators
+    b"\\ caf\xe9 au lait\nMaximize\n obj: x\nSubject To\n c1: x <= 1\nEnd\n",  # a Latin-1 byte
+    b"Minimize\r\n obj: 2 x\r\nSubject To\r\n c1: x >= 1\r\nEnd\r\n",          # CR LF
+], ids=["cr_only", "latin1_comment", "crlf"])
+def test_parse_file_matches_reference_bytes(tmp_path, data):
+    # parse_file reads the file's bytes as the reference does (lp_parser.cpp:12-23)
+    path = tmp_path / "m.lp"
+    path.write_bytes(data)
+    ref = oracle.ref_lp_parse(data)
+    r = lpformat.LPParser().parse_file(str(path))
+    assert r.success == ref["success"]
+    assert r.errors == ref["errors"]
+    if ref["success"]:
+        _same_lp(r.problem, ref["lp"])
+        assert r.var_names == ref["var_names"] and r.con_names == ref["con_names"]
