@@ -62,6 +62,28 @@ def measured_traffic(kernel: str):
     return None
 
 
+def host_info() -> dict:
+    """The host the CPU baselines ran on: usable cores and the CPU model."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": usable, "cpu_count": os.cpu_count(), "cpu_model": model}
+
+
+def all_core_threads() -> int:
+    # the GPU box gives one GPU's job a 16-core share of its host
+    return max(1, min(16, host_info()["nproc"]))
+
+
 def spmv_bytes(nrows: int, ncols: int, nnz: int, sv: int) -> int:
     """Algorithmic bytes of one CSR product: values + 4-byte column indices,
     4-byte row pointers, x once and y once."""
@@ -91,7 +113,28 @@ def cpu_baseline(case, target_s: float) -> dict:
         kind = "port"
         sample = f"{reps} full FP64 A x products (restated CPU loop) on the 2M x 5M matrix"
     return {"value": round((ab + tb) / (ta + tt) / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": kind,
-            "sample": sample, "ms_per_step": round(1e3 * (ta + tt), 3)}
+            "sample": sample, "ms_per_step": round(1e3 * (ta + tt), 3), "host": host_info()}
+
+
+def cpu_all_core_spmv(case, target_s: float) -> dict:
+    """This repository's own CPU loop (sequential sum per row, as the
+    reference's) with the rows split over every usable host core, on A and on
+    the host transpose.  Labelled: not the reference's kernel."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # baseline only
+    T = all_core_threads()
+    trp, tci, tv = O.csr_transpose(case.row_ptr, case.col, case.val, case.ncols)
+    t1 = O.port_spmv_mt_seconds(case.row_ptr, case.col, case.val, case.x, 1, T)
+    reps = max(1, int(target_s / 2 / max(t1, 1e-3)))
+    ta = O.port_spmv_mt_seconds(case.row_ptr, case.col, case.val, case.x, reps, T) / reps
+    tt = O.port_spmv_mt_seconds(trp, tci, tv, case.w, reps, T) / reps
+    ab = spmv_bytes(case.nrows, case.ncols, case.nnz, 8)
+    tb = spmv_bytes(case.ncols, case.nrows, case.nnz, 8)
+    return {"value": round((ab + tb) / (ta + tt) / 1e9, 3), "unit": "GB/s", "cores": T, "kind": "port",
+            "sample": (f"{reps} FP64 products each of A x and A' w (host CSR transpose) on the 2M x 5M matrix, "
+                       f"rows split over {T} threads by nonzero count; this repository's loop, not the "
+                       f"reference's kernel"),
+            "ms_per_step": round(1e3 * (ta + tt), 3), "host": host_info()}
 
 
 def run_spmv(args) -> dict:
@@ -250,6 +293,9 @@ def run_spmv(args) -> dict:
         t = time.time()
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
         log(f"cpu baseline {out['cpu_baseline']} in {time.time() - t:.1f}s")
+        t = time.time()
+        out["cpu_baseline_all_cores"] = cpu_all_core_spmv(case, args.cpu_seconds / 2)
+        log(f"cpu all-core baseline {out['cpu_baseline_all_cores']} in {time.time() - t:.1f}s")
     return out
 
 
@@ -301,6 +347,26 @@ def run_lp(args) -> dict:
         batch[rname] = b
         runs[rname] = res
         log(f"batch {rname}: {count} LPs in {wall:.2f}s -> {count / wall:.1f} LPs/s {status}")
+    # mixed precision in the batch: FP32 factor of N with FP64 refinement of
+    # every Newton step, FP64 once mu < mu_switch, same stop bar
+    for rule, rname in ((H.HN_PIVOT_SKIP, "skip_rule_mixed"),):
+        t0 = time.perf_counter()
+        res = H.ipm_solve_batch(lps, precision=H.HN_PREC_MIXED, pivot_rule=rule, gap_tol=KKT_TOL)
+        wall = time.perf_counter() - t0
+        status = {}
+        for r in res:
+            status[r.status] = status.get(r.status, 0) + 1
+        its = [r.iterations for r in res]
+        b = {"wall_s": round(wall, 3), "lps_per_s": round(count / wall, 2), "status": status,
+             "iterations_mean": round(float(np.mean(its)), 2), "iterations_max": int(max(its)),
+             "precision": "mixed (fp32 factor, fp64 refinement)",
+             "fp64_from_iter_mean": round(float(np.mean([r.fp64_from_iter for r in res])), 2),
+             "at_iteration_limit": sum(r.iterations >= 100 for r in res)}
+        b.update(_kkt_max(res))
+        batch[rname] = b
+        log(f"batch {rname}: {count} LPs in {wall:.2f}s -> {count / wall:.1f} LPs/s {status}")
+    batch["mixed_speedup_vs_f64_batch"] = round(batch["skip_rule_mixed"]["lps_per_s"] /
+                                                batch["skip_rule"]["lps_per_s"], 3)
     res = runs["reference_rule"]
     # plain FP64 single-LP GPU solver, sequential, on a strided sample
     stride = max(1, count // args.seq_sample)
@@ -338,6 +404,13 @@ def run_lp(args) -> dict:
                         "rel_gap": r.rel_gap, "fp64_from_iter": r.fp64_from_iter}
         log(f"single {name}: {r.status} it={r.iterations} {1e3 * dt:.1f} ms")
     out["single"] = single
+
+    # the large power-law LP's generator (column counts on [2, 200], rows
+    # scaled by 10^U(-2,2), seed 3): at reduced sizes, which the dense normal
+    # equations solve to the 1e-8 bar, plain FP64 against mixed precision;
+    # at the full 200,000 x 500,000 size, where only the matrix-free
+    # Jacobi-PCG path exists, what plain and mixed reach under a time limit
+    out["large"] = run_large(args, H, lpgen)
 
     # learned warm start: the 200 x 300 transportation family, the committed
     # trained network on 1,000 held-out seeds, cold vs warm batched solves
@@ -388,14 +461,18 @@ def run_lp(args) -> dict:
             solve, kind = O.ref_ipm_solve, "reference"
         else:
             solve, kind = O.ipm_solve, "port"
+        # times are the reference's own solve_time_ms (steady_clock around
+        # InteriorPointSolver::solve, interior_point.cpp:883-884)
+        def secs(r, wall):
+            return r.solve_ms * 1e-3 if r.solve_ms > 0 else wall
+
         done, spent, limited = 0, 0.0, 0
         for i in idx:
             if spent >= args.cpu_seconds:
                 break
             t0 = time.perf_counter()
-            r = solve(lps[i], time_limit=args.cpu_lp_limit)
-            dt = time.perf_counter() - t0
-            spent += dt
+            r = solve(lps[i], time_limit=args.cpu_lp_limit, trace_cap=0)
+            spent += secs(r, time.perf_counter() - t0)
             if r.status == "OPTIMAL":
                 done += 1
             else:
@@ -404,12 +481,34 @@ def run_lp(args) -> dict:
         out["cpu_baseline"] = {
             "value": round(done / spent, 3), "unit": "LPs/s", "cores": 1, "kind": kind,
             "sample": (f"first {seen} instances of the strided batch sample, {args.cpu_lp_limit:.0f} s limit per "
-                       f"instance; {limited} stopped at the limit and count as unsolved with their time charged")}
+                       f"instance; {limited} stopped at the limit and count as unsolved with their time charged"),
+            "host": host_info()}
         log(f"cpu lp baseline {out['cpu_baseline']}")
+        # the same solver on independent instances over every usable core
+        # (this repository's harness around the reference's solver)
+        from concurrent.futures import ThreadPoolExecutor
+        T = all_core_threads()
+        sub = idx[:2 * T]
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(T) as ex:
+            par = list(ex.map(lambda i: solve(lps[i], time_limit=args.cpu_lp_limit, trace_cap=0), sub))
+        wall = time.perf_counter() - t0
+        pdone = sum(r.status == "OPTIMAL" for r in par)
+        out["cpu_baseline_all_cores"] = {
+            "value": round(pdone / wall, 3), "unit": "LPs/s", "cores": T, "kind": kind,
+            "sample": (f"first {len(sub)} instances of the strided batch sample solved concurrently by the "
+                       f"reference's InteriorPointSolver, one instance per thread on {T} threads, "
+                       f"{args.cpu_lp_limit:.0f} s limit each; {len(sub) - pdone} stopped at the limit and count as "
+                       f"unsolved; wall time of the whole set (this repository's harness)"),
+            "host": host_info()}
+        log(f"cpu all-core lp baseline {out['cpu_baseline_all_cores']}")
+        if kind == "reference" and not args.no_extrapolation:
+            out["cpu_extrapolation"] = ref_extrapolation(O)
+            log(f"cpu extrapolation {out['cpu_extrapolation']}")
         # the reference IPM on the 2,000 x 5,000 LP, bounded
         t0 = time.perf_counter()
-        r = solve(lp, time_limit=args.cpu_single_limit)
-        dt = time.perf_counter() - t0
+        r = solve(lp, time_limit=args.cpu_single_limit, trace_cap=0)
+        dt = secs(r, time.perf_counter() - t0)
         out["single"]["cpu_baseline"] = {
             "value": round(1.0 / dt, 4) if r.status == "OPTIMAL" else 0.0, "unit": "LPs/s", "cores": 1,
             "kind": kind, "sample": (f"one solve of the 2000x5000 LP (seed 1) under a {args.cpu_single_limit:.0f} s "
@@ -419,8 +518,8 @@ def run_lp(args) -> dict:
         tl = []
         for sd in range(1_000_000, 1_000_002):
             t0 = time.perf_counter()
-            r = solve(lpgen.transport_lp(sd, 200, 300), time_limit=args.cpu_lp_limit)
-            tl.append((time.perf_counter() - t0, r.status, r.iterations))
+            r = solve(lpgen.transport_lp(sd, 200, 300), time_limit=args.cpu_lp_limit, trace_cap=0)
+            tl.append((secs(r, time.perf_counter() - t0), r.status, r.iterations))
         out["warm_start"]["cpu_baseline"] = {
             "value": round(len(tl) / sum(t for t, _, _ in tl), 4), "unit": "LPs/s", "cores": 1, "kind": kind,
             "sample": (f"{len(tl)} held-out 200x300 transportation LPs, cold start, {args.cpu_lp_limit:.0f} s "
@@ -431,8 +530,8 @@ def run_lp(args) -> dict:
         cpu, cpu_ms = {}, {}
         for n in conv:
             t0 = time.perf_counter()
-            cpu[n] = solve(models[n], time_limit=args.cpu_lp_limit)
-            cpu_ms[n] = 1e3 * (time.perf_counter() - t0)
+            cpu[n] = solve(models[n], time_limit=args.cpu_lp_limit, trace_cap=0)
+            cpu_ms[n] = 1e3 * secs(cpu[n], time.perf_counter() - t0)
         dt = sum(cpu_ms.values()) / 1e3
         gpu_t = sum(per[n]["ms"] for n in conv) / 1e3
         same = [n for n in conv if f"{cpu[n].status}/{cpu[n].iterations}" == f"{per[n]['status']}/{per[n]['iterations']}"]
@@ -472,6 +571,72 @@ def run_lp(args) -> dict:
                               "share": round((P.ms + a_post) / (P.ms + a_post + a_gpu), 4)},
                 "kind": "reference (presolve::Presolver, AGGRESSIVE, 1 thread)"}
             log(f"presolve share {out['presolve']}")
+    return out
+
+
+def run_large(args, H, lpgen) -> dict:
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "highs_objectives.json")))
+    out = {"generator": "powerlaw_lp(seed 3, n = 2.5 m, alpha 2.179, column counts 2..200)"}
+    for m in (2000, 5000):
+        lp = lpgen.powerlaw_lp(3, m, int(2.5 * m), 2.179, 2, 200)
+        h = gold[f"powerlaw_{m}x{int(2.5 * m)}"]["objective"]
+        H.ipm_solve(lp, pivot_rule=H.HN_PIVOT_SKIP, gap_tol=KKT_TOL)  # warm-up
+        leg = {"stop": "reference test and relative gap <= 1e-8, skip pivot rule (dense normal equations)"}
+        for name, prec in (("plain_f64", H.HN_PREC_F64), ("mixed", H.HN_PREC_MIXED)):
+            t0 = time.perf_counter()
+            r = H.ipm_solve(lp, precision=prec, pivot_rule=H.HN_PIVOT_SKIP, gap_tol=KKT_TOL)
+            dt = time.perf_counter() - t0
+            leg[name] = {"ms": round(1e3 * dt, 2), "status": r.status, "iterations": r.iterations,
+                         "rel_err_vs_highs": abs(r.objective_value - h) / abs(h), "rel_primal": r.rel_primal,
+                         "rel_dual": r.rel_dual, "rel_gap": r.rel_gap, "fp64_from_iter": r.fp64_from_iter}
+        leg["mixed_speedup"] = round(leg["plain_f64"]["ms"] / leg["mixed"]["ms"], 3)
+        out[f"{m}x{int(2.5 * m)}"] = leg
+        log(f"large-LP generator at {m}: plain {leg['plain_f64']['ms']:.0f} ms, mixed {leg['mixed']['ms']:.0f} ms")
+    if args.large_limit > 0:
+        lp = lpgen.powerlaw_lp(3, 200_000, 500_000, 2.179, 2, 200)
+        leg = {"nnz": lp.nnz, "time_limit_s": args.large_limit,
+               "solver": "matrix-free Jacobi-PCG (dense normal equations would need 200,000^2 entries)"}
+        for name, prec in (("plain_f64", H.HN_PREC_F64), ("mixed", H.HN_PREC_MIXED)):
+            t0 = time.perf_counter()
+            r = H.ipm_solve(lp, precision=prec, time_limit_seconds=args.large_limit, gap_tol=KKT_TOL)
+            dt = time.perf_counter() - t0
+            leg[name] = {"s": round(dt, 2), "status": r.status, "iterations": r.iterations, "rp_norm": r.rp_norm,
+                         "rd_norm": r.rd_norm, "mu": r.mu, "rel_primal": r.rel_primal, "rel_dual": r.rel_dual,
+                         "rel_gap": r.rel_gap, "cg_iterations": r.cg_iterations, "ne_solves": r.ne_solves}
+            log(f"large LP 200000x500000 {name}: {r.status} after {r.iterations} iterations, {dt:.1f} s, "
+                f"rp {r.rp_norm:.2e} mu {r.mu:.2e}")
+        out["200000x500000"] = leg
+    return out
+
+
+def ref_extrapolation(O) -> dict:
+    """The reference's solve time at the full sizes of the single-LP (2,000 x
+    5,000) and large-LP (200,000 x 500,000) workloads, which it cannot reach
+    in a bench's time: its own solve_time_ms at m = 250, 500, 1,000 of the
+    same generators (n = 2.5 m), solved concurrently one per thread, then a
+    least-squares fit t = a m^b in log-log space.  Labelled as an
+    extrapolation everywhere; never a measurement."""
+    import lpgen
+    from concurrent.futures import ThreadPoolExecutor
+    gens = {"single_lp_2000x5000": (lambda m: lpgen.random_lp(1, m, int(2.5 * m)), 2000),
+            "large_lp_200000x500000": (lambda m: lpgen.powerlaw_lp(3, m, int(2.5 * m), 2.179, 2, 200), 200_000)}
+    ms = (250, 500, 1000)
+    jobs = [(name, m, make(m)) for name, (make, _) in gens.items() for m in ms]
+    with ThreadPoolExecutor(len(jobs)) as ex:
+        res = list(ex.map(lambda j: O.ref_ipm_solve(j[2], time_limit=120.0, trace_cap=0), jobs))
+    out = {}
+    for name, (_, target) in gens.items():
+        pts = [(m, r) for (nm, m, _), r in zip(jobs, res) if nm == name]
+        t = np.array([r.solve_ms * 1e-3 for _, r in pts])
+        b, loga = np.polyfit(np.log([m for m, _ in pts]), np.log(t), 1)
+        out[name] = {
+            "measured": {str(m): {"s": round(r.solve_ms * 1e-3, 3), "status": r.status, "iterations": r.iterations}
+                         for m, r in pts},
+            "fit": f"t = {np.exp(loga):.3e} * m^{b:.2f} s",
+            "extrapolated_s": float(f"{np.exp(loga) * target ** b:.3g}"),
+            "at_m": target,
+            "label": ("extrapolation, not a measurement: power-law fit of the reference's own solve_time_ms "
+                      "(interior_point.cpp:883-884) at m = 250, 500, 1000 of the same generator, 1 thread each")}
     return out
 
 
@@ -555,6 +720,10 @@ def main() -> int:
     ap.add_argument("--cpu-lp-limit", type=float, default=8.0)
     ap.add_argument("--ws-test", type=int, default=1000, help="held-out transportation LPs of the warm-start leg")
     ap.add_argument("--cpu-single-limit", type=float, default=20.0, help="time limit of the reference CPU solve of the 2000x5000 LP")
+    ap.add_argument("--large-limit", type=float, default=15.0,
+                    help="time limit (s) of each full-size 200000x500000 solve (0 skips them)")
+    ap.add_argument("--no-extrapolation", action="store_true",
+                    help="skip the reference solves at m = 250/500/1000 behind the labelled extrapolations")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only: a multi-GPU run (row-block sharding with "

@@ -1066,6 +1066,19 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
     I.off = F.offset;
     any_warm = any_warm || warm_ok[k];
   }
+  // every instance's slices lie inside the arenas sized above (checked on
+  // the host before anything is launched)
+  for (uint32_t k = 0; k < count; ++k) {
+    const Inst& I = inst[k];
+    const long long m = f[k].m, n = f[k].n, nz = (long long)f[k].ci.size();
+    const bool inside = I.mp >= 0 && I.mp + m + 1 <= n_mp && I.mz >= 0 && I.mz + nz <= n_mz && I.tp >= 0 &&
+                        I.tp + n + 1 <= n_tp && I.tz == I.mz && I.vm >= 0 &&
+                        I.vm + (long long)(VM_COUNT - kVmIo) * m <= n_vm && I.vn >= 0 &&
+                        I.vn + (long long)(VN_COUNT - kVnIo) * n <= n_vn && I.om >= 0 && I.om + m <= n_om &&
+                        I.on >= 0 && I.on + n <= n_on && I.dn >= 0 && I.dn + m * m <= dn && I.w >= 0 &&
+                        I.w + m * BK <= wk;
+    HN_CHECK(inside, HN_ERR_INVALID, "batch LP " + std::to_string(k) + ": a slice lies outside its arena");
+  }
   BatchWorkspace& w = bws();
   std::lock_guard<std::mutex> lk(w.mtx);
   HN_TRY(w.h_mrp.reserve(sizeof(int) * n_mp));

@@ -54,6 +54,8 @@ enum Flag { F_DIR = 0, F_UPD, F_COUNT };
 
 constexpr int kNT = 256;
 constexpr int kMaxParts = 512;
+// nonzeros of M from which the matrix-free solver autotunes its SpMV layouts
+constexpr size_t kAutotuneNnz = size_t(1) << 20;
 
 int parts_for(int64_t n) {
   return (int)std::min<int64_t>(kMaxParts, std::max<int64_t>(1, (n + kNT * 8 - 1) / (kNT * 8)));
@@ -517,6 +519,13 @@ struct Solver {
     if (use_dense) {
       HN_TRY(ne_symbolic(M, Mt, ne, s));
     } else {
+      // the matrix-free path spends its time in products with M and M':
+      // on large matrices both get the SpMV layout that is fastest on them
+      // (a few ms of trial products, once per solve)
+      if (f.v.size() >= kAutotuneNnz) {
+        HN_TRY(hn_csr_autotune(M, 3, nullptr));
+        HN_TRY(hn_csr_autotune(Mt, 3, nullptr));
+      }
       for (DevBuf* d : {&pcg_r, &pcg_z, &pcg_p, &pcg_q, &pcg_dinv, &pcg_diag}) HN_TRY(d->reserve(bm));
       HN_TRY(pcg_tn.reserve(bn));
       if (o.precision == HN_PREC_MIXED) {

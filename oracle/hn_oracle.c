@@ -18,6 +18,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#include <pthread.h>
 
 /* ---- sparse kernels ------------------------------------------------------ */
 void or_spmv(int64_t nrows, const int64_t* rp, const int64_t* ci, const double* v,
@@ -1108,4 +1109,50 @@ void or_verify(const or_problem* p, const int8_t* var_type, const double* primal
     if (out->feasible)
       snprintf(out->message, sizeof(out->message), "Objective discrepancy: %f", out->objective_discrepancy);
   }
+}
+
+/* ---- all-core SpMV timing (a baseline of this repository, not the
+ * reference's kernel): rows cut into `threads` ranges of equal nonzero
+ * count, each summed left to right like or_spmv. ------------------------- */
+typedef struct {
+  const int64_t *rp, *ci;
+  const double *v, *x;
+  double* y;
+  int64_t r0, r1;
+  int repeats;
+} or_spmv_part;
+
+static void* or_spmv_worker(void* arg) {
+  const or_spmv_part* p = (const or_spmv_part*)arg;
+  for (int r = 0; r < p->repeats; ++r)
+    for (int64_t i = p->r0; i < p->r1; ++i) {
+      double s = 0.0;
+      for (int64_t k = p->rp[i]; k < p->rp[i + 1]; ++k) s += p->v[k] * p->x[p->ci[k]];
+      p->y[i] = s;
+    }
+  return NULL;
+}
+
+double or_spmv_mt_seconds(int64_t nrows, const int64_t* rp, const int64_t* ci, const double* v, const double* x,
+                          double* y, int repeats, int threads) {
+  if (threads < 1) threads = 1;
+  if (threads > 256) threads = 256;
+  or_spmv_part parts[256];
+  pthread_t tid[256];
+  const int64_t nnz = rp[nrows];
+  int64_t row = 0;
+  for (int t = 0; t < threads; ++t) {
+    const int64_t target = (t + 1 == threads) ? nnz : nnz / threads * (t + 1);
+    int64_t e = row;
+    while (e < nrows && rp[e] < target) ++e;
+    if (t + 1 == threads) e = nrows;
+    parts[t] = (or_spmv_part){rp, ci, v, x, y, row, e, repeats};
+    row = e;
+  }
+  struct timespec a, b;
+  clock_gettime(CLOCK_MONOTONIC, &a);
+  for (int t = 0; t < threads; ++t) pthread_create(&tid[t], NULL, or_spmv_worker, &parts[t]);
+  for (int t = 0; t < threads; ++t) pthread_join(tid[t], NULL);
+  clock_gettime(CLOCK_MONOTONIC, &b);
+  return (double)(b.tv_sec - a.tv_sec) + 1e-9 * (double)(b.tv_nsec - a.tv_nsec);
 }

@@ -146,6 +146,12 @@ void or_verify(const or_problem* p, const int8_t* var_type, const double* primal
                const double* dual, int64_t n_dual, const double* rc, int64_t n_rc,
                double objective_value, const double tol[4], or_verify_out* out);
 
+/* Wall seconds of `repeats` products y = A x on `threads` host threads
+ * (rows split by nonzero count); a baseline of this repository, not the
+ * reference's CPU kernel. */
+double or_spmv_mt_seconds(int64_t nrows, const int64_t* rp, const int64_t* ci, const double* v, const double* x,
+                          double* y, int repeats, int threads);
+
 #ifdef __cplusplus
 }
 #endif

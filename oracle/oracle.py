@@ -311,11 +311,14 @@ def ref_ipm_solve(lp, max_iterations: int = 100, time_limit: float = 0.0, interr
     trace = np.zeros(5 * trace_cap)
     tl = _i64()
     ms = C.c_double()
+    # trace_cap 0: no trace, which also leaves the process's stderr and
+    # environment alone, so several threads may solve at once
     REF.ref_ipm_solve(lp.m, lp.n, _p(keep["rp"], _i64p), _p(keep["ci"], _i64p), _p(keep["v"], _dp),
                       _p(keep["sense"], _i8p), _p(keep["rhs"], _dp), _p(keep["lb"], _dp), _p(keep["ub"], _dp),
                       _p(keep["obj"], _dp), int(lp.maximize), float(getattr(lp, "obj_offset", 0.0)),
                       max_iterations, time_limit, int(interrupt), C.byref(st), C.byref(it), C.byref(obj),
-                      _p(primal, _dp), _p(trace, _dp), trace_cap, C.byref(tl), C.byref(ms))
+                      _p(primal, _dp), _p(trace, _dp) if trace_cap > 0 else None, trace_cap, C.byref(tl),
+                      C.byref(ms))
     return IpmResult(status=STATUS[st.value], iterations=int(it.value), objective=obj.value,
                      primal=primal[:lp.n].copy(), trace=trace[:5 * tl.value].reshape(-1, 5).copy(),
                      solve_ms=ms.value)
@@ -360,6 +363,17 @@ def port_spmv_seconds(rp, ci, v, x, repeats: int) -> float:
     for _ in range(repeats):
         OR.or_spmv(rp.size - 1, _p(rp, _i64p), _p(ci, _i64p), _p(v, _dp), _p(x, _dp), _p(y, _dp))
     return time.perf_counter() - t
+
+
+def port_spmv_mt_seconds(rp, ci, v, x, repeats: int, threads: int) -> float:
+    """Wall seconds of `repeats` products on `threads` host threads (this
+    repository's row-split loop, not the reference's kernel)."""
+    rp, ci, v, x = _i64a(rp), _i64a(ci), _f64(v), _f64(x)
+    y = np.empty(rp.size - 1)
+    OR.or_spmv_mt_seconds.argtypes = [_i64, _i64p, _i64p, _dp, _dp, _dp, C.c_int, C.c_int]
+    OR.or_spmv_mt_seconds.restype = C.c_double
+    return OR.or_spmv_mt_seconds(rp.size - 1, _p(rp, _i64p), _p(ci, _i64p), _p(v, _dp), _p(x, _dp), _p(y, _dp),
+                                 repeats, threads)
 
 
 def ref_transpose(rp, ci, v, ncols: int):
