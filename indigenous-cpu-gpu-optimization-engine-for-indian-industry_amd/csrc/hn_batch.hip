@@ -304,21 +304,19 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
         pw[u] = (i0 + r < m && c < b) ? W[(long long)(i0 + r) * BK + c] : T(0);
       }
     };
-    // Each thread owns 16 entries of a tile: for FP64, the four 16 x 16
-    // blocks of its wave's 32 x 32 quarter in the v_mfma_f64_16x16x4 output
-    // layout (lane l, element e: row 4e + l/16, column l%16 of a block); for
-    // FP32, a 4 x 4 grid at stride 16 (rows ti + 16a, columns tj + 16q).
-    constexpr bool kMfma = std::is_same<T, double>::value;
+    // Each thread owns 16 entries of a tile: the four 16 x 16 blocks of its
+    // wave's 32 x 32 quarter in the output layout of v_mfma_f64_16x16x4
+    // (lane l, element e: row 4e + l/16, column l%16 of a block) or of
+    // v_mfma_f32_16x16x4f32 (row 4(l/16) + e, column l%16); both layouts
+    // were read back from the device.  Operands: lane l holds row l%16 of
+    // A and column l%16 of B at k = l/16.
+    constexpr bool kF64 = std::is_same<T, double>::value;
     const int ln = tid & (kWave - 1), wq = tid / kWave;
     const int rq = 32 * (wq >> 1), cq = 32 * (wq & 1);
     auto entry = [&](int a, int q, int& r, int& c) {  // tile row / column of entry (a, q)
-      if (kMfma) {
-        r = rq + 16 * (a >> 1) + 4 * q + (ln >> 4);  // a: block (row half, column half), q: element
-        c = cq + 16 * (a & 1) + (ln & 15);
-      } else {
-        r = ti + 16 * a;
-        c = tj + 16 * q;
-      }
+      // a: block (row half, column half), q: element
+      r = rq + 16 * (a >> 1) + (kF64 ? 4 * q + (ln >> 4) : 4 * (ln >> 4) + q);
+      c = cq + 16 * (a & 1) + (ln & 15);
     };
     T acc[4][4];
 #pragma unroll
@@ -370,44 +368,35 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
         fetch_l(nJ, nc * (BK / 2));
         if (nJ == 0 && nc == 0) fetch_w(nI);
       }
-      if constexpr (kMfma) {
+      {
         // 32 columns in steps of 4 (columns past the block's width are zero)
-        typedef double d4 __attribute__((ext_vector_type(4)));
-        d4 dd[4];
+        typedef T t4 __attribute__((ext_vector_type(4)));
+        t4 dd[4];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) dd[a] = d4{acc[a][0], acc[a][1], acc[a][2], acc[a][3]};
+        for (int a = 0; a < 4; ++a) dd[a] = t4{acc[a][0], acc[a][1], acc[a][2], acc[a][3]};
 #pragma unroll
         for (int c0 = 0; c0 < BK / 2; c0 += 4) {
           const int kk = c0 + (ln >> 4);
-          const double a0 = sh.u.Wr[cbi * (BK / 2) + kk][rq + (ln & 15)];
-          const double a1 = sh.u.Wr[cbi * (BK / 2) + kk][rq + 16 + (ln & 15)];
-          const double b0 = sh.u.Ls[kk][cq + (ln & 15)];
-          const double b1 = sh.u.Ls[kk][cq + 16 + (ln & 15)];
-          dd[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, dd[0], 0, 0, 0);
-          dd[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, dd[1], 0, 0, 0);
-          dd[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, dd[2], 0, 0, 0);
-          dd[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, dd[3], 0, 0, 0);
+          const T a0 = sh.u.Wr[cbi * (BK / 2) + kk][rq + (ln & 15)];
+          const T a1 = sh.u.Wr[cbi * (BK / 2) + kk][rq + 16 + (ln & 15)];
+          const T b0 = sh.u.Ls[kk][cq + (ln & 15)];
+          const T b1 = sh.u.Ls[kk][cq + 16 + (ln & 15)];
+          if constexpr (kF64) {
+            dd[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, dd[0], 0, 0, 0);
+            dd[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, dd[1], 0, 0, 0);
+            dd[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, dd[2], 0, 0, 0);
+            dd[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, dd[3], 0, 0, 0);
+          } else {
+            dd[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, dd[0], 0, 0, 0);
+            dd[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, dd[1], 0, 0, 0);
+            dd[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, dd[2], 0, 0, 0);
+            dd[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, dd[3], 0, 0, 0);
+          }
         }
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
           for (int q = 0; q < 4; ++q) acc[a][q] = dd[a][q];
-      } else {
-        const int cw = min(BK / 2, b - cbi * (BK / 2));
-        // unrolled so that the LDS reads of the next columns are in flight
-        // during this column's multiply-adds (one wave per SIMD hides nothing)
-#pragma unroll 4
-        for (int c = 0; c < cw; ++c) {
-          T wv[4], lv[4];
-#pragma unroll
-          for (int a = 0; a < 4; ++a) wv[a] = sh.u.Wr[cbi * (BK / 2) + c][ti + 16 * a];
-#pragma unroll
-          for (int a = 0; a < 4; ++a) lv[a] = sh.u.Ls[c][tj + 16 * a];
-#pragma unroll
-          for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) acc[a][q] += wv[a] * lv[q];
-        }
       }
       __syncthreads();
       if (cbi == ncb - 1) {  // the tile's last step: subtract it from A
