@@ -297,6 +297,21 @@ typedef struct hn_ipm_opts {
                                   |c_s'x_s - b_s'lam| / (1 + |c'x|) in the kernel, which is
                                   that gap when no column has a finite nonzero bound
                                   shift (otherwise it differs by sum_j shift_j rd_j) */
+  double endgame_mu;           /* > 0 (not in the reference; hn_ipm_solve_batch only): at
+                                  the first iterate with mu <= endgame_mu and ||rp||, ||rd||
+                                  < 1e-4, and again each time mu has halved or 4 more
+                                  iterations have passed, project
+                                  onto the estimated optimal face B = {j : x_j >= s_j}:
+                                  x' = x_B + D_B M'w with N_B w = b - M x_B (x'_N = 0),
+                                  y' = y + d with N_B d = M D_B (c - M'y) (least squares
+                                  for s'_B = 0), s' = c - M'y', N_B = M D_B M' + r I,
+                                  D_B = x_B, factored under the reference's pivot rule.  The projection ends the solve (OPTIMAL,
+                                  result.projected = 1) when x' >= -tol and the point
+                                  with x', s' clamped at 0 meets the stop test (rp,
+                                  rd = max(-s', 0), mu) and, with gap_tol, the gap;
+                                  otherwise the iterates continue unchanged.  It finishes
+                                  degenerate LPs (transportation problems) whose dual
+                                  steps stall near the optimum; 0 = off */
 } hn_ipm_opts;
 
 typedef struct hn_ipm_result {
@@ -341,6 +356,7 @@ typedef struct hn_ipm_result {
                                   while a residual stayed above its own for three
                                   iterations, and the solve restarted from the
                                   standard point */
+  int projected;               /* 1 when the endgame_mu projection ended the solve */
 } hn_ipm_result;
 
 int hn_ipm_default_opts(hn_ipm_opts* o);

@@ -65,7 +65,7 @@ constexpr int kVmIo = 2, kVnIo = 3;  // leading slots held in the host-visible a
 constexpr int kStallIters = 3;
 
 struct Out {
-  int status, iterations, fp64_from, ne_solves, refine, restarted, accepted;
+  int status, iterations, fp64_from, ne_solves, refine, restarted, accepted, projected;
   double rp, rd, mu;
 };
 
@@ -77,6 +77,8 @@ struct Params {
   double mu_target;  // > 0: stop at the first iterate with mu <= mu_target
   double gap_tol;    // > 0: the stop test also needs |c_s'x_s - b_s'lam| <= gap_tol (1 + |c'x|), c'x the
                      // caller's objective (the gap of the caller's LP when no column has a bound shift)
+  double endgame_mu; // > 0: optimal-face projection once mu <= endgame_mu (then after each halving of
+                     // mu or every 4 iterations)
 };
 
 // A skipped pivot: its entry of D becomes "infinite", so its column of L and
@@ -745,6 +747,98 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
       rdn = wg_max(bb, red);
       mu = wg_sum(s, red) / (double)n;
     };
+    // Optimal-face projection (not in the reference; opt-in by endgame_mu).
+    // B = {j : x_j >= z_j} estimates the support of the optimal primal face.
+    // With D_B = x_B (0 off B) and N_B = M D_B M' + r I factored once:
+    //   x' = x_B + D_B M' w,   N_B w = b - M x_B        (x'_N = 0, M x' = b)
+    //   y' = lam + d,           N_B d = M D_B (c - M'lam)   (weighted least
+    //        squares for z'_B = c_B - M_B'y' = 0)
+    //   z' = c - M'y'
+    // x' and z' are clamped at 0 (each by at most conv_tol) and the point is
+    // accepted when it meets the reference's stop test (||M x' - b||,
+    // ||rd|| = ||max(-z', 0)|| and mu below their tolerances) and, with
+    // gap_tol, the relative gap; the solve then ends there.  Otherwise the
+    // iterates are left unchanged.
+    auto project = [&]() -> bool {
+      for (int j = tid; j < n; j += NT) {
+        D[j] = (x[j] >= z[j]) ? x[j] : 0.0;
+        t[j] = (x[j] >= z[j]) ? x[j] : 0.0;
+      }
+      __syncthreads();
+      // N_B is singular when M_B has fewer than m independent columns
+      // (degenerate LPs); both right-hand sides lie in its range, and the
+      // reference's regularisation (clamped pivots, r >= 1e-12 max diag)
+      // keeps the solutions' null-space parts at rounding size - a skipped
+      // pivot would instead fix them at an arbitrary value
+      double reg = 0.0;
+      wg_factor<double>(c, A64, W64, sh64, red, prm.reg, false, &reg);
+      wg_spmv(c.mrp, c.mci, c.mv, t, tm, m);
+      __syncthreads();
+      for (int i = tid; i < m; i += NT) rhs[i] = b[i] - tm[i];
+      __syncthreads();
+      (void)wg_solve<double>(c, A64, rhs, dlam, reg, red, &refine);
+      wg_spmv(c.trp, c.tci, c.tv, dlam, t, n);
+      __syncthreads();
+      // x'_B below zero by no more than the tolerance (degenerate basic
+      // components) is clamped to 0; the residual below is that of the
+      // clamped point
+      double bad = 0.0, xneg = 0.0;
+      for (int j = tid; j < n; j += NT) {
+        const double v = (D[j] > 0.0) ? x[j] + D[j] * t[j] : 0.0;
+        dx[j] = fmax(v, 0.0);
+        xneg = nan_ignoring_max(xneg, -v);
+        if (!isfinite(v)) bad = 1.0;
+      }
+      wg_spmv(c.trp, c.tci, c.tv, lam, t, n);
+      __syncthreads();
+      for (int j = tid; j < n; j += NT) zs[j] = D[j] * (cc[j] - t[j]);
+      __syncthreads();
+      wg_spmv(c.mrp, c.mci, c.mv, zs, rhs2, m);
+      __syncthreads();
+      (void)wg_solve<double>(c, A64, rhs2, mdrd, reg, red, &refine);
+      for (int i = tid; i < m; i += NT) mdrd[i] = lam[i] + mdrd[i];  // y'
+      __syncthreads();
+      wg_spmv(c.trp, c.tci, c.tv, mdrd, t, n);
+      wg_spmv(c.mrp, c.mci, c.mv, dx, tm, m);
+      __syncthreads();
+      double zneg = 0.0, cx = 0.0, by = 0.0, rp2 = 0.0, xz = 0.0;
+      for (int j = tid; j < n; j += NT) {
+        const double v = cc[j] - t[j];
+        dz[j] = v;
+        if (!isfinite(v)) bad = 1.0;
+        zneg = nan_ignoring_max(zneg, -v);
+        cx += cc[j] * dx[j];
+        xz += dx[j] * fmax(v, 0.0);
+      }
+      for (int i = tid; i < m; i += NT) {
+        rp2 = nan_ignoring_max(rp2, fabs(tm[i] - b[i]));
+        by += b[i] * mdrd[i];
+        if (!isfinite(mdrd[i])) bad = 1.0;
+      }
+      bad = wg_max(bad, red);
+      xneg = wg_max(xneg, red);
+      zneg = wg_max(zneg, red);
+      rp2 = wg_max(rp2, red);
+      cx = wg_sum(cx, red);
+      by = wg_sum(by, red);
+      xz = wg_sum(xz, red) / (double)n;
+      // the reference's stop test on the projected point (rd = max(-z', 0))
+      bool ok = bad == 0.0 && xneg < prm.conv_tol && zneg < prm.conv_tol && rp2 < prm.conv_tol &&
+                xz < prm.comp_tol;
+      if (ok && prm.gap_tol > 0.0) ok = fabs(cx - by) <= prm.gap_tol * (1.0 + fabs(cx + I.off));
+      if (ok) {
+        for (int j = tid; j < n; j += NT) {
+          x[j] = dx[j];
+          z[j] = fmax(dz[j], 0.0);
+        }
+        for (int i = tid; i < m; i += NT) lam[i] = mdrd[i];
+        __syncthreads();
+      }
+      ++ne_solves;
+      return ok;
+    };
+    double endgame_next = prm.endgame_mu;
+    int projected = 0, endgame_iter = 0;
     for (int iter = 0; iter < prm.max_iterations; ++iter) {
       residuals();
       if (rpn < prm.conv_tol && rdn < prm.conv_tol && mu < prm.comp_tol) {
@@ -765,6 +859,17 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
       if (prm.mu_target > 0.0 && mu <= prm.mu_target) {
         stopped = true;
         break;
+      }
+      if (endgame_next > 0.0 && rpn < 1e-4 && rdn < 1e-4 &&
+          (mu <= endgame_next || (mu <= prm.endgame_mu && iter >= endgame_iter + 4))) {
+        if (project()) {  // passed the stop test: the solve ends at the projected point
+          residuals();
+          projected = 1;
+          done = true;
+          break;
+        }
+        endgame_next = 0.5 * mu;  // next try once mu has halved, or after 4 iterations
+        endgame_iter = iter;
       }
       // a warm start that lets mu fall below its tolerance while the
       // residuals stay above theirs for kStallIters iterations has stalled:
@@ -903,6 +1008,7 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
       o.refine = refine;
       o.restarted = restarted ? 1 : 0;
       o.accepted = accepted ? 1 : 0;
+      o.projected = projected;
       o.rp = rpn;
       o.rd = rdn;
       o.mu = mu;
@@ -1159,6 +1265,7 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
   prm.skip = o.pivot_rule == HN_PIVOT_SKIP;
   prm.mu_target = o.mu_target;
   prm.gap_tol = o.gap_tol;
+  prm.endgame_mu = o.endgame_mu;
   const int grid = (int)std::min<uint32_t>(count, 2u * (uint32_t)std::max(1, ctx().info.compute_units));
   HN_HIP(hipEventRecord(w.ev[0], s));
   hipLaunchKernelGGL(k_batch_ipm, dim3(grid), dim3(NT), 0, s, w.inst.as<Inst>(), w.order.as<int>(), (int)count,
@@ -1216,6 +1323,7 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
     r.cg_iterations = 0;
     r.fp64_from_iter = out[k].fp64_from;
     r.warm_started = out[k].accepted ? (out[k].restarted ? 2 : 1) : 0;
+    r.projected = out[k].projected;
     if (r.primal) std::memcpy(r.primal, primal.data(), sizeof(double) * lps[k].n);
     if (r.x && r.cap_n >= F.n) std::memcpy(r.x, x, sizeof(double) * F.n);
     if (r.s && r.cap_n >= F.n) std::memcpy(r.s, z, sizeof(double) * F.n);

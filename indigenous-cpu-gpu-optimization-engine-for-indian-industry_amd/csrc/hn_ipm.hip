@@ -766,6 +766,7 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
   res.ne_solves = res.refine_sweeps = res.cg_iterations = 0;
   res.fp64_from_iter = -1;
   res.warm_started = 0;
+  res.projected = 0;
   auto finish = [&]() { res.solve_ms = 1e3 * clock.s(); };
 
   if (lp.n == 0) {  // interior_point.cpp:406-410

@@ -490,7 +490,7 @@ class _HnOpts(C.Structure):
                 ("interrupt_user", C.c_void_p), ("solver", C.c_int), ("precision", C.c_int),
                 ("mu_switch", C.c_double), ("dense_max_m", C.c_int64), ("pcg_max_iter", C.c_int),
                 ("x0", _dp), ("y0", _dp), ("s0", _dp), ("verbose", C.c_int), ("pivot_rule", C.c_int),
-                ("mu_target", C.c_double), ("gap_tol", C.c_double)]
+                ("mu_target", C.c_double), ("gap_tol", C.c_double), ("endgame_mu", C.c_double)]
 
 
 class _HnResult(C.Structure):
@@ -501,7 +501,7 @@ class _HnResult(C.Structure):
                 ("rel_primal", C.c_double), ("rel_dual", C.c_double), ("rel_gap", C.c_double),
                 ("scaled_rel_primal", C.c_double), ("scaled_rel_dual", C.c_double),
                 ("scaled_rel_gap", C.c_double), ("ne_solves", C.c_int64), ("refine_sweeps", C.c_int64), ("cg_iterations", C.c_int64),
-                ("fp64_from_iter", C.c_int64), ("warm_started", C.c_int)]
+                ("fp64_from_iter", C.c_int64), ("warm_started", C.c_int), ("projected", C.c_int)]
 
 
 LIB.hn_ipm_default_opts.argtypes = [C.POINTER(_HnOpts)]
@@ -555,6 +555,7 @@ class InteriorPointResult:
         self.ne_solves, self.refine_sweeps = int(r.ne_solves), int(r.refine_sweeps)
         self.cg_iterations, self.fp64_from_iter = int(r.cg_iterations), int(r.fp64_from_iter)
         self.warm_started = bool(r.warm_started)
+        self.projected = bool(r.projected)
 
     def __repr__(self):
         return (f"InteriorPointResult({self.status}, obj={self.objective_value!r}, it={self.iterations}, "
@@ -634,7 +635,7 @@ LIB.hn_ipm_solve_batch_gnn.restype = C.c_int
 
 def ipm_solve_batch(lps, precision: int = HN_PREC_F64, max_iterations: int = 100, mu_switch: float = 1e-4,
                     want_points: bool = False, pivot_rule: int = HN_PIVOT_CLAMP, mu_target: float = 0.0,
-                    warm=None, verbose: bool = False, gap_tol: float = 0.0, gnn=None):
+                    warm=None, verbose: bool = False, gap_tol: float = 0.0, gnn=None, endgame_mu: float = 0.0):
     """Solves many LPs in one persistent launch (hn_ipm_solve_batch).
     warm: optional list (one entry per LP, or None) of (x0, y0, s0) warm
     starts of the scaled standard forms.  verbose: the library prints the
@@ -668,6 +669,7 @@ def ipm_solve_batch(lps, precision: int = HN_PREC_F64, max_iterations: int = 100
     opts.pivot_rule = pivot_rule
     opts.mu_target = mu_target
     opts.gap_tol = gap_tol
+    opts.endgame_mu = endgame_mu
     opts.verbose = int(verbose)
     res = (_HnResult * count)()
     bufs = []

@@ -263,35 +263,38 @@ def predict(net: Network, lps, graph: Optional[Graph] = None, chunk: int = 256):
 
 
 def evaluate(net: Network, make_lp, seeds: Sequence[int], graph: Optional[Graph] = None,
-             pivot_rule: int = HH.HN_PIVOT_SKIP, log=print, gap_tol: float = 0.0, on_device: bool = True) -> dict:
+             pivot_rule: int = HH.HN_PIVOT_SKIP, log=print, gap_tol: float = 0.0, on_device: bool = True,
+             endgame_mu: float = 0.0) -> dict:
     """Cold vs warm-started batched solves of the held-out seeds: iterations,
     wall time (the warm time includes the network's inference), acceptance
     of the predictions, and agreement of the two optima.  on_device: the
     network runs inside the batched call on the packed standard forms
     (hn_ipm_solve_batch_gnn); else predict() runs first and the points are
     passed in.  gap_tol > 0: both solves also stop only at a relative gap <=
-    gap_tol (see hn_ipm_opts)."""
+    gap_tol (see hn_ipm_opts).  endgame_mu > 0: both solves may end by the
+    optimal-face projection (hn_ipm_opts.endgame_mu)."""
     lps = [make_lp(int(s)) for s in seeds]
     if graph is None:
         sf = HH.lp_standard_form(lps[0])
         graph = Graph(sf["m"], sf["n"], sf["rp"], sf["ci"])
-    HH.ipm_solve_batch(lps[:4], pivot_rule=pivot_rule, gap_tol=gap_tol)  # warm-up
+    kw = dict(pivot_rule=pivot_rule, gap_tol=gap_tol, endgame_mu=endgame_mu)
+    HH.ipm_solve_batch(lps[:4], **kw)  # warm-up
     if on_device:
-        HH.ipm_solve_batch(lps[:4], pivot_rule=pivot_rule, gap_tol=gap_tol, gnn=(net.h, graph.h))
+        HH.ipm_solve_batch(lps[:4], gnn=(net.h, graph.h), **kw)
     t0 = time.perf_counter()
-    cold = HH.ipm_solve_batch(lps, pivot_rule=pivot_rule, gap_tol=gap_tol)
+    cold = HH.ipm_solve_batch(lps, **kw)
     t_cold = time.perf_counter() - t0
     t_pred = 0.0
     if on_device:
         t0 = time.perf_counter()
-        warm = HH.ipm_solve_batch(lps, pivot_rule=pivot_rule, gap_tol=gap_tol, gnn=(net.h, graph.h))
+        warm = HH.ipm_solve_batch(lps, gnn=(net.h, graph.h), **kw)
         t_warm = time.perf_counter() - t0
     else:
         t0 = time.perf_counter()
         warm_pts = predict(net, lps, graph)
         t_pred = time.perf_counter() - t0
         t0 = time.perf_counter()
-        warm = HH.ipm_solve_batch(lps, pivot_rule=pivot_rule, warm=warm_pts, gap_tol=gap_tol)
+        warm = HH.ipm_solve_batch(lps, warm=warm_pts, **kw)
         t_warm = time.perf_counter() - t0
     ci = np.array([r.iterations for r in cold])
     wi = np.array([r.iterations for r in warm])
@@ -317,6 +320,11 @@ def evaluate(net: Network, make_lp, seeds: Sequence[int], graph: Optional[Graph]
         "warm_max_rel_gap": float(max(r.rel_gap for r in warm)),
         "cold_max_rel_primal": float(max(r.rel_primal for r in cold)),
         "warm_max_rel_primal": float(max(r.rel_primal for r in warm)),
+        "endgame_mu": endgame_mu,
+        "cold_projected": int(sum(r.projected for r in cold)),
+        "warm_projected": int(sum(r.projected for r in warm)),
+        "cold_kkt_1e-8": int(sum(max(r.rel_primal, r.rel_dual, r.rel_gap) <= 1e-8 for r in cold)),
+        "warm_kkt_1e-8": int(sum(max(r.rel_primal, r.rel_dual, r.rel_gap) <= 1e-8 for r in warm)),
     }
     log(f"[warmstart] eval {out}")
     return out

@@ -6,7 +6,8 @@ Cases:
   * batch instances of the 4,096-LP workload (seed 1000+i) spread over the
     whole range of m, the largest ones near m = 2,000;
   * the large power-law LP's generator at reduced sizes (m x 2.5m, column
-    counts on [2, 200], seed 3).
+    counts on [2, 200], seed 3);
+  * the first 12 held-out 200 x 300 transportation LPs (seeds 1,000,000+).
 Run:  python tests/golden/make_highs_fixtures.py
 """
 import json
@@ -65,5 +66,26 @@ def main():
         json.dump(data, open(OUT, "w"), indent=1)
 
 
+def transport_main():
+    """HiGHS optima of the first held-out transportation LPs (200 x 300)."""
+    data = json.load(open(OUT))
+    for seed in range(1_000_000, 1_000_012):
+        key = f"transport_{seed}"
+        if key in data:
+            continue
+        lp = lpgen.transport_lp(seed)
+        A = sp.csr_matrix((lp.val, lp.col, lp.row_ptr), shape=(lp.m, lp.n))
+        le = lp.sense == lpgen.LE
+        t = time.time()
+        r = linprog(lp.obj, A_ub=A[le], b_ub=lp.rhs[le], A_eq=A[~le], b_eq=lp.rhs[~le], bounds=(0, None),
+                    method="highs", options={"primal_feasibility_tolerance": 1e-10,
+                                             "dual_feasibility_tolerance": 1e-10})
+        assert r.status == 0, r.message
+        data[key] = {"m": int(lp.m), "n": int(lp.n), "objective": float(r.fun), "seconds": round(time.time() - t, 2)}
+        print(key, data[key], flush=True)
+        json.dump(data, open(OUT, "w"), indent=1)
+
+
 if __name__ == "__main__":
     main()
+    transport_main()

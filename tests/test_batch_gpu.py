@@ -137,3 +137,31 @@ def test_batch_whole_size_range_against_highs(hip):
                 continue
             assert rel(r.objective_value, h) <= 2e-8, (i, r.objective_value, h)
             assert r.rel_primal <= 1e-8 and r.rel_dual <= 1e-8 and r.rel_gap <= 1e-8, i
+
+
+def test_batch_endgame_projection_transport(hip):
+    """endgame_mu (not in the reference): the optimal-face projection ends
+    degenerate transportation LPs, whose dual steps stall near a relative
+    gap of 1e-7, at the 1e-8 bar on every measure of the caller's LP, with
+    HiGHS's optimum (tests/golden/highs_objectives.json)."""
+    import json
+    import os
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "highs_objectives.json")))
+    seeds = sorted(int(k.split("_")[1]) for k in gold if k.startswith("transport_"))
+    lps = [lpgen.transport_lp(s) for s in seeds]
+    print(f"[case] endgame projection on {len(lps)} transportation LPs", flush=True)
+    plain = hip.ipm_solve_batch(lps, pivot_rule=hip.HN_PIVOT_SKIP, gap_tol=1e-8)
+    out = hip.ipm_solve_batch(lps, pivot_rule=hip.HN_PIVOT_SKIP, gap_tol=1e-8, endgame_mu=1e-6)
+    ended = 0
+    for s, lp, r, p in zip(seeds, lps, out, plain):
+        h = gold[f"transport_{s}"]["objective"]
+        print(f"  {s}: {r.status} it={r.iterations} projected={r.projected} obj err {rel(r.objective_value, h):.1e} "
+              f"kkt {r.rel_primal:.1e} {r.rel_dual:.1e} {r.rel_gap:.1e}; without: it={p.iterations} "
+              f"gap {p.rel_gap:.1e}", flush=True)
+        assert r.status == "OPTIMAL"
+        if r.projected:
+            ended += 1
+            assert r.rel_primal <= 1e-8 and r.rel_dual <= 1e-8 and r.rel_gap <= 1e-8
+            assert rel(r.objective_value, h) <= 1e-8
+            assert r.iterations <= p.iterations
+    assert ended >= len(lps) - 1
