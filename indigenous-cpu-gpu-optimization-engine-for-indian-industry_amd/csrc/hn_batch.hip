@@ -870,6 +870,7 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
         }
         endgame_next = 0.5 * mu;  // next try once mu has halved, or after 4 iterations
         endgame_iter = iter;
+        residuals();  // the attempt overwrote M x (tm) and M'lam (t), which the step reads
       }
       // a warm start that lets mu fall below its tolerance while the
       // residuals stay above theirs for kStallIters iterations has stalled:
