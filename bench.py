@@ -618,6 +618,28 @@ def run_large(args, H, lpgen) -> dict:
     return out
 
 
+def run_batch_bucket(args) -> dict:
+    """The batched solver on the instances of the 4,096-LP workload whose m
+    lies in [lo, hi): wall time, iterations, and the multiply-adds of the
+    LDL' factorisations it ran (sum over instances of iterations * m^3 / 3,
+    one factorisation per iteration), for counter runs per size bucket."""
+    import hypernova_hip as H
+    import lpgen
+    lo, hi = (int(t) for t in args.batch_range.split(","))
+    H.init(-1)
+    lps = [lp for lp in (lpgen.batch_lp(i) for i in range(args.batch)) if lo <= lp.m < hi]
+    H.ipm_solve_batch(lps[:8], pivot_rule=H.HN_PIVOT_SKIP, gap_tol=KKT_TOL)
+    t0 = time.perf_counter()
+    res = H.ipm_solve_batch(lps, pivot_rule=H.HN_PIVOT_SKIP, gap_tol=KKT_TOL)
+    wall = time.perf_counter() - t0
+    fma = sum(r.iterations * lp.m ** 3 / 3.0 for lp, r in zip(lps, res))
+    return {"batch_bucket": f"m in [{lo}, {hi})", "lps": len(lps), "wall_s": round(wall, 4),
+            "lps_per_s": round(len(lps) / wall, 2), "iterations_mean": round(float(np.mean([r.iterations for r in res])), 2),
+            "m_mean": round(float(np.mean([lp.m for lp in lps])), 1), "factor_fma": fma,
+            "factor_gflops_wall": round(2 * fma / wall / 1e9, 1),
+            "kkt_1e-8": sum(max(r.rel_primal, r.rel_dual, r.rel_gap) <= KKT_TOL for r in res)}
+
+
 def ref_extrapolation(O) -> dict:
     """The reference's solve time at the full sizes of the single-LP (2,000 x
     5,000) and large-LP (200,000 x 500,000) workloads, which it cannot reach
@@ -725,6 +747,7 @@ def main() -> int:
     ap.add_argument("--chunks-A", default="", help="fixed chunked layout of A: items,span_log2,threads")
     ap.add_argument("--chunks-At", default="", help="fixed chunked layout of A': items,span_log2,threads")
     ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--batch-range", default="", help="lo,hi: only the batched solver on instances with lo <= m < hi")
     ap.add_argument("--seq-sample", type=int, default=64)
     ap.add_argument("--cpu-lp-limit", type=float, default=8.0)
     ap.add_argument("--ws-test", type=int, default=1000, help="held-out transportation LPs of the warm-start leg")
@@ -738,6 +761,9 @@ def main() -> int:
         print(json.dumps({"error": "this build runs on one GPU only: a multi-GPU run (row-block sharding with "
                                    "an all-gather of x per product) is not launched", "n_gpus": args.gpus}))
         return 2
+    if args.batch_range:
+        print(json.dumps(run_batch_bucket(args)), flush=True)
+        return 0
     if args.verify_only:
         print(json.dumps({"verify": run_verify(args)}), flush=True)
         return 0

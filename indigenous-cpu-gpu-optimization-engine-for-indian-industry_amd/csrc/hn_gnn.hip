@@ -224,64 +224,29 @@ __global__ __launch_bounds__(256) void k_gnn_embed(const float* __restrict__ F, 
 }
 
 // ---- message passing --------------------------------------------------------
-// out[r][f] (+)= sum_{k in row r} w_k in[col_k][f], w_k = val[src_k] / deg,
-// deg = the row's own length (self_deg) or the length of the neighbour's row
-// in the other CSR (orp).  One wave per row, lane = feature.
-__global__ __launch_bounds__(256) void k_gnn_spmm(const int* __restrict__ rp, const int* __restrict__ col,
-                                                  const int* __restrict__ src, const double* __restrict__ val,
-                                                  const int* __restrict__ orp, int self_deg, int rows,
-                                                  const float* __restrict__ in, float* __restrict__ out,
-                                                  int accumulate) {
-  const int r = (blockIdx.x * 256 + threadIdx.x) / kWave;
-  const int f = threadIdx.x & (kWave - 1);
+// The aggregations are sparse products out (+)= W in over the pattern of M
+// (rows of M: constraints gather from variables) or of M' (rows of M':
+// variables gather from constraints), run by the library's SpMM
+// (spmm_rows_launch).  Their weights depend on the LP's values and are
+// formed here, once per pass, in the order the products read them:
+//   self  w_k = v_k / |own row|         (the mean over a node's neighbours)
+//   other w_k = v_k / |neighbour's row| (the transpose of the other side's
+//                                        mean, for the backward pass)
+// with v_k = (float) val of the entry (through src for M', whose entry q is
+// M's entry src[q]); self weights multiply by the row's float reciprocal,
+// other weights divide by the neighbour's degree.
+__global__ __launch_bounds__(256) void k_gnn_weights(const int* __restrict__ rp, const int* __restrict__ col,
+                                                     const int* __restrict__ src, const double* __restrict__ val,
+                                                     const int* __restrict__ orp, int self_deg, int rows,
+                                                     float* __restrict__ w) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
   if (r >= rows) return;
   const int k0 = rp[r], k1 = rp[r + 1];
   const float inv_self = k1 > k0 ? 1.f / (float)(k1 - k0) : 0.f;
-  float acc = 0.f;
   for (int k = k0; k < k1; ++k) {
     const int cc = col[k];
     const float v = (float)val[src ? src[k] : k];
-    const float w = self_deg ? v * inv_self : v / (float)(orp[cc + 1] - orp[cc]);
-    acc += w * in[(long long)cc * H + f];
-  }
-  float* o = out + (long long)r * H + f;
-  *o = accumulate ? *o + acc : acc;
-}
-
-// The same product for long rows: one workgroup per row, wave w takes the
-// entries w, w + 4, ... and the four partial rows are added in a fixed order.
-__global__ __launch_bounds__(256) void k_gnn_spmm_long(const int* __restrict__ rp, const int* __restrict__ col,
-                                                       const int* __restrict__ src, const double* __restrict__ val,
-                                                       const int* __restrict__ orp, int self_deg,
-                                                       const float* __restrict__ in, float* __restrict__ out,
-                                                       int accumulate) {
-  __shared__ float part[4][H];
-  const int r = blockIdx.x;
-  const int w = threadIdx.x / kWave, f = threadIdx.x & (kWave - 1);
-  const int k0 = rp[r], k1 = rp[r + 1];
-  const float inv_self = k1 > k0 ? 1.f / (float)(k1 - k0) : 0.f;
-  float a0 = 0.f, a1 = 0.f;
-  int k = k0 + w;
-  for (; k + 4 < k1; k += 8) {
-    const int c0 = col[k], c1 = col[k + 4];
-    const float v0 = (float)val[src ? src[k] : k], v1 = (float)val[src ? src[k + 4] : k + 4];
-    const float w0 = self_deg ? v0 * inv_self : v0 / (float)(orp[c0 + 1] - orp[c0]);
-    const float w1 = self_deg ? v1 * inv_self : v1 / (float)(orp[c1 + 1] - orp[c1]);
-    a0 += w0 * in[(long long)c0 * H + f];
-    a1 += w1 * in[(long long)c1 * H + f];
-  }
-  for (; k < k1; k += 4) {
-    const int c0 = col[k];
-    const float v0 = (float)val[src ? src[k] : k];
-    const float w0 = self_deg ? v0 * inv_self : v0 / (float)(orp[c0 + 1] - orp[c0]);
-    a0 += w0 * in[(long long)c0 * H + f];
-  }
-  part[w][f] = a0 + a1;
-  __syncthreads();
-  if (w == 0) {
-    const float acc = ((part[0][f] + part[1][f]) + part[2][f]) + part[3][f];
-    float* o = out + (long long)r * H + f;
-    *o = accumulate ? *o + acc : acc;
+    w[k] = self_deg ? v * inv_self : v / (float)(orp[cc + 1] - orp[cc]);
   }
 }
 
@@ -684,6 +649,7 @@ struct hn_gnn_graph {
   hn::DevBuf lx, ls, ly, dlx, dls, dly;
   hn::DevBuf dhv[2], dhc[2], dav, dac;
   hn::DevBuf part, red, lpart, loss;
+  hn::DevBuf wgt;  // aggregation weights, 4 x nnz floats
   // host copy of the pattern, to recognise standard forms of this family
   std::vector<int64_t> h_rp, h_ci;
   // inference-only workspaces and streams for concurrent warm starts
@@ -732,16 +698,20 @@ std::vector<float> init_params(uint64_t seed) {
   return p;
 }
 
-// Row-wise aggregation: a workgroup per row when the rows are long (mean
-// length >= 32), else a wave per row.
-void spmm(hn_gnn_graph* gr, const int* rp, const int* col, const int* src, const double* val, const int* orp,
-          int self_deg, int rows, const float* in, float* out, int accumulate, hipStream_t s) {
-  if (gr->nnz >= 32LL * rows)
-    hipLaunchKernelGGL(k_gnn_spmm_long, dim3(rows), dim3(256), 0, s, rp, col, src, val, orp, self_deg, in, out,
-                       accumulate);
-  else
-    hipLaunchKernelGGL(k_gnn_spmm, dim3(wgrid(rows)), dim3(256), 0, s, rp, col, src, val, orp, self_deg, rows, in,
-                       out, accumulate);
+// Aggregation weights into gr->wgt: [0, nnz) self weights over the rows of
+// M, [nnz, 2 nnz) self weights over the rows of M' (forward), [2 nnz, 3 nnz)
+// and [3 nnz, 4 nnz) the other-degree weights of the same two patterns
+// (backward).
+void weights(hn_gnn_graph* gr, const double* val, bool backward_pair, hipStream_t s) {
+  const int m = gr->m, n = gr->n;
+  const long long nnz = gr->nnz;
+  float* w = gr->wgt.as<float>() + (backward_pair ? 2 * nnz : 0);
+  const int* rp = gr->rp.as<int>();
+  const int* trp = gr->trp.as<int>();
+  hipLaunchKernelGGL(k_gnn_weights, dim3(tgrid(m)), dim3(256), 0, s, rp, gr->ci.as<int>(), (const int*)nullptr, val,
+                     trp, backward_pair ? 0 : 1, m, w);
+  hipLaunchKernelGGL(k_gnn_weights, dim3(tgrid(n)), dim3(256), 0, s, trp, gr->tci.as<int>(), gr->tsrc.as<int>(), val,
+                     rp, backward_pair ? 0 : 1, n, w + nnz);
 }
 
 int forward(hn_gnn* g, hn_gnn_graph* gr, const double* val, const double* b, const double* c, hipStream_t s) {
@@ -766,11 +736,15 @@ int forward(hn_gnn* g, hn_gnn_graph* gr, const double* val, const double* b, con
                      P + Lo.bev, n, gr->hv[0].as<float>());
   hipLaunchKernelGGL(k_gnn_embed, dim3(wgrid(m)), dim3(256), 0, s, gr->Fc.as<float>(), 4, FC, P + Lo.wec,
                      P + Lo.bec, m, gr->hc[0].as<float>());
+  weights(gr, val, false, s);
+  const float* wr = gr->wgt.as<float>();
+  const float* wc = wr + gr->nnz;
   for (int l = 0; l < NL; ++l) {
-    spmm(gr, rp, ci, nullptr, val, trp, 1, m, gr->hv[l].as<float>(), gr->ac[l].as<float>(), 0, s);
+    HN_TRY(spmm_rows_launch<float>(rp, ci, wr, m, gr->nnz, gr->hv[l].as<float>(), gr->ac[l].as<float>(), H, false, s));
     hipLaunchKernelGGL(k_gnn_gemm_fwd, dim3((m + 63) / 64), dim3(256), 0, s, gr->hc[l].as<float>(),
                        gr->ac[l].as<float>(), P + Lo.wc[l], P + Lo.bc[l], m, gr->hc[l + 1].as<float>());
-    spmm(gr, trp, tci, tsrc, val, trp, 1, n, gr->hc[l + 1].as<float>(), gr->av[l].as<float>(), 0, s);
+    HN_TRY(spmm_rows_launch<float>(trp, tci, wc, n, gr->nnz, gr->hc[l + 1].as<float>(), gr->av[l].as<float>(), H,
+                                   false, s));
     hipLaunchKernelGGL(k_gnn_gemm_fwd, dim3((n + 63) / 64), dim3(256), 0, s, gr->hv[l].as<float>(),
                        gr->av[l].as<float>(), P + Lo.wv[l], P + Lo.bv[l], n, gr->hv[l + 1].as<float>());
   }
@@ -778,7 +752,7 @@ int forward(hn_gnn* g, hn_gnn_graph* gr, const double* val, const double* b, con
                      P + Lo.ws, P + Lo.bs, S, 0, gr->lx.as<float>(), gr->ls.as<float>());
   hipLaunchKernelGGL(k_gnn_heads, dim3(wgrid(m)), dim3(256), 0, s, gr->hc[NL].as<float>(), m, P + Lo.wy, P + Lo.by,
                      (const float*)nullptr, (const float*)nullptr, S, 1, gr->ly.as<float>(), (float*)nullptr);
-  count_launch(8 + 4 * NL);
+  count_launch(10 + 2 * NL);  // the aggregations count themselves
   return check_launch("gnn forward");
 }
 
@@ -800,6 +774,9 @@ int backward(hn_gnn* g, hn_gnn_graph* gr, const double* val, const float* tx, co
   const int* trp = gr->trp.as<int>();
   const int* tci = gr->tci.as<int>();
   const int* tsrc = gr->tsrc.as<int>();
+  weights(gr, val, true, s);
+  const float* wro = gr->wgt.as<float>() + 2 * gr->nnz;
+  const float* wco = wro + gr->nnz;
   // loss and head-output gradients
   hipLaunchKernelGGL(k_gnn_loss_v, dim3(tgrid(n)), dim3(256), 0, s, gr->lx.as<float>(), gr->ls.as<float>(), tx, ts,
                      n, gr->dlx.as<float>(), gr->dls.as<float>(), gr->lpart.as<float>());
@@ -832,7 +809,8 @@ int backward(hn_gnn* g, hn_gnn_graph* gr, const double* val, const float* tx, co
     hipLaunchKernelGGL(k_gnn_gemm_dx, dim3((n + 63) / 64), dim3(256), 0, s, gr->dhv[cur].as<float>(),
                        gr->hv[l + 1].as<float>(), P + Lo.wv[l], n, gr->dhv[nxt].as<float>(), gr->dav.as<float>());
     // av[l] = Avc hc[l+1]  ->  dhc[l+1] += Avc' dav  (rows of M, weight M_ij / deg_j)
-    spmm(gr, rp, ci, nullptr, val, trp, 0, m, gr->dav.as<float>(), gr->dhc[cur].as<float>(), 1, s);
+    HN_TRY(spmm_rows_launch<float>(rp, ci, wro, m, gr->nnz, gr->dav.as<float>(), gr->dhc[cur].as<float>(), H, true,
+                                   s));
     // constraint update: hc[l+1] = relu([hc[l], ac[l]] Wc + bc)
     hipLaunchKernelGGL(k_gnn_gemm_dw, dim3(cgrid(m)), dim3(256), 0, s, gr->hc[l].as<float>(), gr->ac[l].as<float>(),
                        gr->dhc[cur].as<float>(), gr->hc[l + 1].as<float>(), m, gr->part.as<float>());
@@ -840,7 +818,8 @@ int backward(hn_gnn* g, hn_gnn_graph* gr, const double* val, const float* tx, co
     hipLaunchKernelGGL(k_gnn_gemm_dx, dim3((m + 63) / 64), dim3(256), 0, s, gr->dhc[cur].as<float>(),
                        gr->hc[l + 1].as<float>(), P + Lo.wc[l], m, gr->dhc[nxt].as<float>(), gr->dac.as<float>());
     // ac[l] = Acv hv[l]  ->  dhv[l] += Acv' dac  (rows of M', weight M_ij / deg_i)
-    spmm(gr, trp, tci, tsrc, val, rp, 0, n, gr->dac.as<float>(), gr->dhv[nxt].as<float>(), 1, s);
+    HN_TRY(spmm_rows_launch<float>(trp, tci, wco, n, gr->nnz, gr->dac.as<float>(), gr->dhv[nxt].as<float>(), H, true,
+                                   s));
     cur = nxt;
   }
   // embeddings
@@ -850,7 +829,7 @@ int backward(hn_gnn* g, hn_gnn_graph* gr, const double* val, const float* tx, co
   hipLaunchKernelGGL(k_gnn_embed_dw, dim3(cgrid(m)), dim3(256), 0, s, gr->Fc.as<float>(), 4, FC,
                      gr->dhc[cur].as<float>(), gr->hc[0].as<float>(), m, gr->part.as<float>());
   reduce_into(g, gr, (int)cgrid(m), FC * H + H, Lo.wec, s);
-  count_launch(13 + 8 * NL);
+  count_launch(15 + 6 * NL);
   return check_launch("gnn backward");
 }
 
@@ -978,6 +957,7 @@ extern "C" int hn_gnn_graph_create(int64_t m, int64_t n, const int64_t* row_ptr,
   }
   if (st == HN_OK) st = gr->rank.reserve(sizeof(float) * std::max<long long>(nnz, 1));
   if (st == HN_OK) st = gr->gap.reserve(sizeof(float) * std::max<long long>(nnz, 1));
+  if (st == HN_OK) st = gr->wgt.reserve(sizeof(float) * 4 * std::max<long long>(nnz, 1));
   if (st == HN_OK) st = gr->Fv.reserve(sizeof(float) * (size_t)n * 8);
   if (st == HN_OK) st = gr->Fc.reserve(sizeof(float) * (size_t)m * 4);
   for (int l = 0; l <= NL && st == HN_OK; ++l) {
@@ -1092,6 +1072,7 @@ int clone_for_inference(const hn_gnn_graph* src, hn_gnn_graph** out) {
   if (st == HN_OK) st = gr->spart.reserve(sizeof(double) * 2 * kScaleBlocks);
   if (st == HN_OK) st = gr->rank.reserve(sizeof(float) * std::max<long long>(nnz, 1));
   if (st == HN_OK) st = gr->gap.reserve(sizeof(float) * std::max<long long>(nnz, 1));
+  if (st == HN_OK) st = gr->wgt.reserve(sizeof(float) * 4 * std::max<long long>(nnz, 1));
   if (st == HN_OK) st = gr->Fv.reserve(sizeof(float) * (size_t)n * 8);
   if (st == HN_OK) st = gr->Fc.reserve(sizeof(float) * (size_t)m * 4);
   for (int l = 0; l <= NL && st == HN_OK; ++l) {

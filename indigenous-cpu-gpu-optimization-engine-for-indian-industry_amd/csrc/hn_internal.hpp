@@ -162,6 +162,13 @@ int segmented_sort(const int* ptr, int nseg, int* keys, int64_t total, SortScrat
 // out[0..n] = exclusive scan of in[0..n-1], out[n] = total (int range).
 int exclusive_scan(const int* in, int64_t n, int* out, DevBuf& tmp, hipStream_t s);
 
+// C (+)= A B for a CSR given by device arrays (B row-major ncols x k, C
+// row-major nrows x k, FP64 or FP32): a wave per row, or a workgroup per row
+// when the mean row holds 32 or more nonzeros; sums in a fixed order.
+template <typename V>
+int spmm_rows_launch(const int* rowptr, const int* col, const V* val, int64_t nrows, int64_t nnz, const V* B, V* C,
+                     int k, bool accumulate, hipStream_t s);
+
 // Two-pass deterministic FP64 reductions into a device scalar.
 int dot_launch(const double* a, const double* b, int64_t n, double* out, hipStream_t s);
 int norm_inf_launch(const double* v, int64_t n, double* out, hipStream_t s);

@@ -257,15 +257,16 @@ int fill_launch(double* p, double v, int64_t n, hipStream_t s) {
   return check_launch("k_fill");
 }
 
-// ---- SpMM: C = A * B, B row-major (ncols x k) --------------------------------
+// ---- SpMM: C (+)= A * B, B row-major (ncols x k) -----------------------------
 // One wave per row; lanes cover 64 consecutive columns of B/C, so every
-// nonzero reads one contiguous strip of a B row.
+// nonzero reads one contiguous strip of a B row.  Each entry of C is summed
+// over the row's nonzeros in order.
 template <typename V>
 __global__ __launch_bounds__(256) void k_spmm_rows(const int* __restrict__ rowptr,
                                                    const int* __restrict__ col,
                                                    const V* __restrict__ val,
                                                    const V* __restrict__ B, V* __restrict__ C,
-                                                   int nrows, int k) {
+                                                   int nrows, int k, int accumulate) {
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   if (wave >= nrows) return;
@@ -275,10 +276,71 @@ __global__ __launch_bounds__(256) void k_spmm_rows(const int* __restrict__ rowpt
     V acc = 0;
     if (c < k) {
       for (int e = beg; e < end; ++e) acc += val[e] * B[(long long)col[e] * k + c];
-      C[(long long)wave * k + c] = acc;
+      V* o = C + (long long)wave * k + c;
+      *o = accumulate ? *o + acc : acc;
     }
   }
 }
+
+// The same product for long rows: one workgroup per row; wave w takes the
+// nonzeros w, w + 4, w + 8, ... into two running sums (w + 8t and
+// w + 8t + 4), and the four waves' partial rows are added in a fixed order,
+// so the result is deterministic.
+template <typename V>
+__global__ __launch_bounds__(256) void k_spmm_rows_long(const int* __restrict__ rowptr,
+                                                        const int* __restrict__ col,
+                                                        const V* __restrict__ val,
+                                                        const V* __restrict__ B, V* __restrict__ C, int k,
+                                                        int accumulate) {
+  __shared__ V part[4][kWave];
+  const int r = blockIdx.x;
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const int k0 = rowptr[r], k1 = rowptr[r + 1];
+  for (int c0 = 0; c0 < k; c0 += kWave) {
+    const int c = c0 + lane;
+    V a0 = 0, a1 = 0;
+    if (c < k) {
+      int e = k0 + w;
+      for (; e + 4 < k1; e += 8) {
+        const int j0 = col[e], j1 = col[e + 4];
+        a0 += val[e] * B[(long long)j0 * k + c];
+        a1 += val[e + 4] * B[(long long)j1 * k + c];
+      }
+      for (; e < k1; e += 4) a0 += val[e] * B[(long long)col[e] * k + c];
+    }
+    part[w][lane] = a0 + a1;
+    __syncthreads();
+    if (w == 0 && c < k) {
+      const V acc = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+      V* o = C + (long long)r * k + c;
+      *o = accumulate ? *o + acc : acc;
+    }
+    __syncthreads();
+  }
+}
+
+template <typename V>
+int spmm_rows_launch(const int* rowptr, const int* col, const V* val, int64_t nrows, int64_t nnz, const V* B, V* C,
+                     int k, bool accumulate, hipStream_t s) {
+  if (nrows <= 0 || k <= 0) return HN_OK;
+  HN_CHECK(nrows * (int64_t)kWave < (int64_t)INT_MAX * 4, HN_ERR_UNSUPPORTED, "spmm: too many rows");
+  if (nnz >= 32 * nrows) {
+    HN_CHECK(nrows < INT_MAX, HN_ERR_UNSUPPORTED, "spmm: too many rows");
+    hipLaunchKernelGGL((k_spmm_rows_long<V>), dim3((unsigned)nrows), dim3(256), 0, s, rowptr, col, val, B, C, k,
+                       accumulate ? 1 : 0);
+  } else {
+    const int64_t threads = nrows * (int64_t)kWave;
+    hipLaunchKernelGGL((k_spmm_rows<V>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, rowptr, col, val,
+                       B, C, (int)nrows, k, accumulate ? 1 : 0);
+  }
+  count_launch();
+  return check_launch("spmm");
+}
+
+template int spmm_rows_launch<double>(const int*, const int*, const double*, int64_t, int64_t, const double*,
+                                      double*, int, bool, hipStream_t);
+template int spmm_rows_launch<float>(const int*, const int*, const float*, int64_t, int64_t, const float*, float*,
+                                     int, bool, hipStream_t);
 
 }  // namespace hn
 
@@ -551,15 +613,10 @@ extern "C" int hn_spmv(const hn_csr* a, const double* x, double* y) {
 
 namespace hn {
 static int spmm_launch(const hn_csr* a, const double* B, double* C, int64_t k, hipStream_t s) {
-  if (a->nrows == 0 || k == 0) return HN_OK;
   HN_CHECK(a->flags & HN_F64, HN_ERR_INVALID, "spmm: matrix has no FP64 values");
-  HN_CHECK(k < INT_MAX && a->nrows * (int64_t)kWave < (int64_t)INT_MAX * 4, HN_ERR_UNSUPPORTED, "spmm: shape too large");
-  const int64_t threads = a->nrows * kWave;
-  hipLaunchKernelGGL((k_spmm_rows<double>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s,
-                     a->row_ptr.as<int>(), a->col.as<int>(), a->val64.as<double>(), B, C, (int)a->nrows, (int)k);
-  HN_TRY(check_launch("k_spmm_rows"));
-  count_launch();
-  return HN_OK;
+  HN_CHECK(k < INT_MAX, HN_ERR_UNSUPPORTED, "spmm: shape too large");
+  return spmm_rows_launch<double>(a->row_ptr.as<int>(), a->col.as<int>(), a->val64.as<double>(), a->nrows, a->nnz,
+                                  B, C, (int)k, false, s);
 }
 }  // namespace hn
 
