@@ -99,15 +99,17 @@ struct Tiling {
   DevBuf slab_val64;      // double[nnz] (if HN_F64)
   DevBuf slab_val32;      // float[nnz]  (if HN_F32)
   // Chunked stream (used when chunk_ipt > 0, ahead of both layouts above):
-  // row blocks of at most 8*chunk_threads rows, each block's nonzeros in
+  // row blocks of at most chunk_rows rows, each block's nonzeros in
   // column-slab order cut into chunks of at most chunk_threads*chunk_ipt
   // nonzeros spanning at most 2^chunk_span columns, each chunk stored sorted
   // by row.
   int chunk_ipt = 0, chunk_threads = 256, chunk_span = 0, chunk_nb = 0, chunk_n = 0;
+  int chunk_cb = 19;      // column bits of the packed word (the row takes the rest)
+  int chunk_rows = 0;     // most rows of a block (LDS sums per workgroup)
   DevBuf chunk_rb;        // int[nb + 1]: first row of each block
   DevBuf chunk_bc;        // int[nb + 1]: first chunk of each block
   DevBuf chunk_rec;       // int4[n]: (offset, count, first column, 0); offsets are multiples of chunk_ipt
-  DevBuf chunk_pk;        // uint32[padded]: row within the block << 19 | column - first column
+  DevBuf chunk_pk;        // uint32[padded]: row within the block << chunk_cb | column - first column
   DevBuf chunk_val64;     // double[padded] (if HN_F64)
   DevBuf chunk_val32;     // float[padded]  (if HN_F32)
 };

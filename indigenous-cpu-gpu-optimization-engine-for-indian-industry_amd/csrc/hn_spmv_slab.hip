@@ -48,10 +48,14 @@ constexpr int kRowsPerThread = kSlabRows / kNT;
 static_assert(kSlabRows % kNT == 0, "block rows must split evenly over the threads");
 constexpr int kSlabColBits = 21;  // column within a slab; the row within the block takes the upper 11 bits
 static_assert(kSlabRows <= (1 << (32 - kSlabColBits)), "row within a block must fit the packed word");
-// chunked stream: up to 8192 rows per block (13 bits), columns relative to
-// the chunk's first column in the low 19 bits
+// chunked stream: the packed word holds the row within the block in its
+// upper bits and the column relative to the chunk's first column in the low
+// cb bits; cb = 19 (8,192 rows) unless the blocks must be taller to be
+// resident at once, down to kChunkMinColBits (rows up to kChunkMaxRows)
 constexpr int kChunkColBits = 19;
-constexpr int kChunkMaxRows = 1 << (32 - kChunkColBits);
+constexpr int kChunkMinColBits = 17;
+constexpr int kChunkMaxRows = 80 * kNT;  // 20,480 rows: k_slab_build<80>
+static_assert(kChunkMaxRows <= (1 << (32 - kChunkMinColBits)), "rows per block must fit the packed word");
 constexpr int kChunkMaxItems = 8192;  // nonzeros per chunk (13-bit position in the pack sort key)
 
 // Exclusive prefix of v over the workgroup (thread order); *total receives
@@ -223,11 +227,11 @@ __global__ void k_chunk_cut(const int* __restrict__ seg, int nb, int K, int slab
 
 // One workgroup per chunk: sorts the chunk's nonzeros by (row, stream
 // position) in LDS (bitonic) and writes them at the chunk's padded offset
-// as (row << 19 | column - first column) and value.  src = (stream offset,
+// as (row << cb | column - first column) and value.  src = (stream offset,
 // count, first column, block); dst_off = padded output offset.  The stream
-// words are (row << 19 | column within the slab).
+// words are (row << cb | column within the slab).
 __global__ __launch_bounds__(1024) void k_chunk_pack(const int4* __restrict__ src, const int* __restrict__ dst_off,
-                                                     const int* __restrict__ seg, int K, int slab_log2,
+                                                     const int* __restrict__ seg, int K, int slab_log2, int cb,
                                                      const unsigned* __restrict__ spk,
                                                      const double* __restrict__ sv64, const float* __restrict__ sv32,
                                                      unsigned* __restrict__ pk, double* __restrict__ v64,
@@ -240,7 +244,7 @@ __global__ __launch_bounds__(1024) void k_chunk_pack(const int4* __restrict__ sr
   int P = 1;
   while (P < n) P <<= 1;
   for (int i = tid; i < P; i += nt)
-    key[i] = (i < n) ? (((spk[off + i] >> kChunkColBits) << 13) | (unsigned)i) : 0xFFFFFFFFu;
+    key[i] = (i < n) ? (((spk[off + i] >> cb) << 13) | (unsigned)i) : 0xFFFFFFFFu;
   __syncthreads();
   for (int k = 2; k <= P; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
@@ -272,7 +276,7 @@ __global__ __launch_bounds__(1024) void k_chunk_pack(const int4* __restrict__ sr
         hi = mid - 1;
     }
     const int colg = (lo << slab_log2) | (int)(spk[p] & in_slab);
-    pk[out + i] = ((kk >> 13) << kChunkColBits) | (unsigned)(colg - base);
+    pk[out + i] = ((kk >> 13) << cb) | (unsigned)(colg - base);
     if (v64) v64[out + i] = sv64[p];
     if (v32) v32[out + i] = sv32[p];
   }
@@ -316,17 +320,117 @@ __device__ __forceinline__ void load_vals(const float* p, float (&v)[IPT]) {
   }
 }
 
+// Wave-wide data movement on the DPP lanes of the VALU (no LDS round trip):
+// row_shr:k within each row of 16 lanes, row_bcast:15/31 across rows,
+// wave_shr:1 / wave_shl:1 across the wave.  Lanes without a source read 0.
+constexpr int kDppRowShr = 0x110;
+constexpr int kDppWaveShl1 = 0x130;
+constexpr int kDppWaveShr1 = 0x138;
+constexpr int kDppRowBcast15 = 0x142;
+constexpr int kDppRowBcast31 = 0x143;
+
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ int dpp(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, true);
+}
+
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ double dpp(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = dpp<CTRL, ROWS>((int)(b & 0xffffffffll)), hi = dpp<CTRL, ROWS>((int)(b >> 32));
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ float dpp(float v) {
+  return __builtin_bit_cast(float, dpp<CTRL, ROWS>(__builtin_bit_cast(int, v)));
+}
+
+__device__ __forceinline__ double readlane(double v, int l) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ float readlane(float v, int l) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+
+// One step of a segmented scan: (f, v) absorbs the pair (fp, vp) that comes
+// before it; a set flag marks the start of a segment.
+template <typename V>
+__device__ __forceinline__ void seg_absorb(int& f, V& v, int fp, V vp) {
+  if (!f) v = vp + v;
+  f |= fp;
+}
+
+template <int CTRL, int ROWS, typename V>
+__device__ __forceinline__ void seg_level(int& f, V& v) {
+  const int fp = dpp<CTRL, ROWS>(f);
+  const V vp = dpp<CTRL, ROWS>(v);
+  seg_absorb(f, v, fp, vp);
+}
+
+// Inclusive segmented scan within each row of 16 lanes.
+template <typename V>
+__device__ __forceinline__ void seg_scan_rows(int& f, V& v) {
+  seg_level<kDppRowShr + 1, 0xf>(f, v);
+  seg_level<kDppRowShr + 2, 0xf>(f, v);
+  seg_level<kDppRowShr + 4, 0xf>(f, v);
+  seg_level<kDppRowShr + 8, 0xf>(f, v);
+}
+
+// Inclusive segmented scan over the wave: rows, then each row absorbs the
+// row before it (lane 15 into row 1, lane 47 into row 3; then lane 31 into
+// rows 2 and 3).  A fixed order, so the sums are deterministic.
+template <typename V>
+__device__ __forceinline__ void seg_scan_wave(int& f, V& v) {
+  seg_scan_rows(f, v);
+  seg_level<kDppRowBcast15, 0xa>(f, v);
+  seg_level<kDppRowBcast31, 0xc>(f, v);
+}
+
+// A register copy the compiler cannot fold away: the copy of chunk c+1's
+// stream into the current registers happens here, after its loads have
+// arrived, so the registers that the next loads write are free again (a
+// folded copy would sit at the loop's end, behind those next loads, and
+// wait for them).
+__device__ __forceinline__ unsigned vmov(unsigned v) {
+  unsigned o;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(o) : "v"(v));
+  return o;
+}
+
+__device__ __forceinline__ float vmov(float v) {
+  return __builtin_bit_cast(float, vmov(__builtin_bit_cast(unsigned, v)));
+}
+
+__device__ __forceinline__ double vmov(double v) {
+  double o;
+  asm volatile("v_mov_b64 %0, %1" : "=v"(o) : "v"(v));
+  return o;
+}
+
 // pk and val are not __restrict__: the compiler would then treat their loads
 // as invariant and move them past the fence that keeps the next chunk's
-// stream ahead of the wait for the gathers.
+// loads ahead of the chunk's reduction.
+//
+// Pipeline per chunk c: wait for chunk c's gathers and chunk c+1's stream
+// (all issued one step earlier); form chunk c's products and rows; copy
+// chunk c+1's stream, which has arrived, into the current registers; issue
+// chunk c+1's gathers and chunk c+2's stream; then reduce chunk c (scans,
+// the barrier, LDS sums) while those loads are in flight, so the next
+// chunk's gathers overlap the barrier.
 template <typename V, int IPT, int NT>
 __global__ __launch_bounds__(NT) void k_spmv_chunk(const int* __restrict__ rb, const int* __restrict__ bc,
                                                    const int4* __restrict__ rec, const unsigned* pk, const V* val,
-                                                   const V* __restrict__ x, V* __restrict__ y) {
+                                                   const V* __restrict__ x, V* __restrict__ y, int cb) {
   constexpr int NW = NT / kWave;
-  constexpr unsigned kColMask = (1u << kChunkColBits) - 1u;
+  static_assert(NW <= 16, "the cross-wave pass scans one row of 16 lanes");
+  const unsigned col_mask = (1u << cb) - 1u;
   extern __shared__ __align__(16) unsigned char hn_chunk_lds[];
-  V* ys = reinterpret_cast<V*>(hn_chunk_lds);  // the block's row sums (8*NT rows at most)
+  V* ys = reinterpret_cast<V*>(hn_chunk_lds);  // the block's row sums
   // per wave and chunk parity: inclusive tail value at lane 63, whether a
   // run starts in the wave, the first row of lane 0 and the last of lane 63
   __shared__ V wv[2][NW];
@@ -338,59 +442,62 @@ __global__ __launch_bounds__(NT) void k_spmv_chunk(const int* __restrict__ rb, c
   for (int i = tid; i < R; i += NT) ys[i] = V(0);
   const int c_beg = bc[b], c_end = bc[b + 1];
   const int k0 = tid * IPT;
-  // Two register sets, A and B: a chunk's words and values are loaded into
-  // one while the other chunk's gathers are in flight (the loop is unrolled
-  // by two, so no register copy makes the compiler wait for the loads).
   // Every load is issued unconditionally, so the compiler can wait for the
-  // gathers alone: a thread past the end of a chunk reads the chunk's first
-  // words instead, and words past the end inside a thread are zero padding,
-  // so every gather stays inside the chunk's columns; such values are never
-  // summed.
-  unsigned wdA[IPT], wdB[IPT];
-  V vA[IPT], vB[IPT];
-  int nA = 0, baseA = 0, nB = 0, baseB = 0;
-  if (c_beg < c_end) {
-    const int4 r = rec[c_beg];
-    nA = r.y;
-    baseA = r.z;
-    const int kk = (k0 < nA) ? k0 : 0;
-    load_words<IPT>(pk + r.x + kk, wdA);
-    load_vals<IPT>(val + r.x + kk, vA);
-  }
-  // one chunk: gathers with (wd, v), next chunk's stream into (wd2, v2)
-  auto step = [&](const unsigned (&wd)[IPT], const V (&v)[IPT], int n, int base, unsigned (&wd2)[IPT],
-                  V (&v2)[IPT], int& n2, int& base2, int c) {
-    const int par = (c - c_beg) & 1;
-    const int nv = min(IPT, max(0, n - k0));  // valid nonzeros of this thread (a prefix)
-    V xv[IPT];
-    {
-      const V* xb = x + base;
-#pragma unroll
-      for (int u = 0; u < IPT; ++u) xv[u] = xb[wd[u] & kColMask];
-    }
-    __builtin_amdgcn_sched_barrier(0);  // gathers first, then the next chunk's stream
-    {
-      const bool more = c + 1 < c_end;
-      const int4 r2 = rec[more ? c + 1 : c];
-      n2 = more ? r2.y : 0;
-      base2 = r2.z;
-      const int kk = (k0 < r2.y) ? k0 : 0;
-      load_words<IPT>(pk + r2.x + kk, wd2);
-      load_vals<IPT>(val + r2.x + kk, v2);
-    }
-    // keep the stream loads here, ahead of the wait for the gathers (an
-    // empty asm with a memory clobber stops the optimiser from sinking them;
-    // it reads no register, so it waits for nothing)
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_sched_barrier(0);
-    int row[IPT];
-    V sum[IPT];  // sum[u]: sum of the thread's nonzeros of row[u] up to u
+  // ones it needs alone: a chunk index past the end reads the block's last
+  // chunk again (n = 0; its lines are still in L2), a thread past the end of
+  // a chunk reads the chunk's first words, and words past the end inside a
+  // thread are zero padding, so every gather stays inside the chunk's
+  // columns; such values are never summed.
+  unsigned wdC[IPT], wdN[IPT];  // current chunk (arrived), next chunk (in flight)
+  V vC[IPT], vN[IPT], xv[IPT];
+  int nC = 0, baseC = 0, nN = 0, baseN = 0;
+  auto load_next = [&](int c) {
+    const bool ok = c < c_end;
+    const int4 r = rec[ok ? c : c_end - 1];
+    nN = ok ? r.y : 0;
+    baseN = r.z;
+    const int kk = (k0 < r.y) ? k0 : 0;
+    load_words<IPT>(pk + r.x + kk, wdN);
+    load_vals<IPT>(val + r.x + kk, vN);
+  };
+  auto advance = [&]() {  // next -> current, then the current chunk's gathers
 #pragma unroll
     for (int u = 0; u < IPT; ++u) {
-      row[u] = (u < nv) ? (int)(wd[u] >> kChunkColBits) : -1;
-      const V p = v[u] * xv[u];
-      sum[u] = (u > 0 && row[u] == row[u - 1]) ? sum[u - 1] + p : p;
+      wdC[u] = vmov(wdN[u]);
+      vC[u] = vmov(vN[u]);
     }
+    nC = nN;
+    baseC = baseN;
+    const V* xb = x + baseC;
+#pragma unroll
+    for (int u = 0; u < IPT; ++u) xv[u] = xb[wdC[u] & col_mask];
+  };
+  if (c_beg < c_end) {
+    load_next(c_beg);
+    advance();
+    load_next(c_beg + 1);
+  }
+  for (int c = c_beg; c < c_end; ++c) {
+    const int par = (c - c_beg) & 1;
+    const int nv = min(IPT, max(0, nC - k0));  // valid nonzeros of this thread (a prefix)
+    int row[IPT];
+    V p[IPT];
+#pragma unroll
+    for (int u = 0; u < IPT; ++u) {
+      row[u] = (u < nv) ? (int)(wdC[u] >> cb) : -1;
+      p[u] = vC[u] * xv[u];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    advance();
+    load_next(c + 2);
+    // keep these loads here, ahead of the reduction (an empty asm with a
+    // memory clobber stops the optimiser from sinking them; it reads no
+    // register, so it waits for nothing)
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    V sum[IPT];  // sum[u]: sum of the thread's nonzeros of row[u] up to u
+#pragma unroll
+    for (int u = 0; u < IPT; ++u) sum[u] = (u > 0 && row[u] == row[u - 1]) ? sum[u - 1] + p[u] : p[u];
     int first = -1, last = -1;
     V tail = V(0);
     if (nv > 0) first = row[0];
@@ -402,21 +509,13 @@ __global__ __launch_bounds__(NT) void k_spmv_chunk(const int* __restrict__ rb, c
       }
     const bool single = nv > 0 && first == last;
     // does this thread's first run continue the previous thread's last run?
-    const int prev_last = __shfl_up(last, 1, kWave);
+    const int prev_last = dpp<kDppWaveShr1>(last);
     bool cont = lane > 0 && nv > 0 && prev_last == first;
     // segmented scan of the tail runs; lane 0 of a wave assumes it continues
     // the previous wave when single (its carry is added below or not)
     int f = (nv == 0 || !single) ? 1 : ((lane == 0) ? 0 : (cont ? 0 : 1));
     V sv = tail;
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-      const V ov = __shfl_up(sv, off, kWave);
-      const int of = __shfl_up(f, off, kWave);
-      if (lane >= off) {
-        if (!f) sv = ov + sv;
-        f |= of;
-      }
-    }
+    seg_scan_wave(f, sv);
     if (lane == kWave - 1) {
       wv[par][w] = sv;
       wf[par][w] = f;
@@ -424,18 +523,24 @@ __global__ __launch_bounds__(NT) void k_spmv_chunk(const int* __restrict__ rb, c
     }
     if (lane == 0) wfirst[par][w] = first;
     __syncthreads();
-    // tail value accumulated through the end of wave w-1
-    V carry = V(0);
-    for (int q = 0; q < w; ++q) {
-      const bool cq = q > 0 && wfirst[par][q] >= 0 && wlast[par][q - 1] == wfirst[par][q];
-      carry = wf[par][q] ? wv[par][q] : ((cq ? carry : V(0)) + wv[par][q]);
+    // tail value accumulated through the end of wave w-1: a segmented scan
+    // over the waves' records, lane q holding wave q's (a segment starts at
+    // wave q unless its first row continues wave q-1's last)
+    int cf = 0;
+    V cv = V(0);
+    if (lane < NW) {
+      const bool cq = lane > 0 && wfirst[par][lane] >= 0 && wlast[par][lane - 1] == wfirst[par][lane];
+      cf = (wf[par][lane] || !cq) ? 1 : 0;
+      cv = wv[par][lane];
     }
+    seg_scan_rows(cf, cv);
+    const V carry = (w > 0) ? readlane(cv, w - 1) : V(0);
     if (lane == 0) cont = w > 0 && nv > 0 && wlast[par][w - 1] == first;
     const bool wave_cont = w > 0 && wfirst[par][w] >= 0 && wlast[par][w - 1] == wfirst[par][w];
     const V S = f ? sv : ((wave_cont ? carry : V(0)) + sv);  // inclusive block-wide tail value
-    V S_prev = __shfl_up(S, 1, kWave);
+    V S_prev = dpp<kDppWaveShr1>(S);
     if (lane == 0) S_prev = carry;
-    int next_first = __shfl_down(first, 1, kWave);
+    int next_first = dpp<kDppWaveShl1>(first);
     if (lane == kWave - 1) next_first = (w + 1 < NW) ? wfirst[par][w + 1] : -1;
     if (nv > 0) {
       bool first_run = true;
@@ -450,10 +555,6 @@ __global__ __launch_bounds__(NT) void k_spmv_chunk(const int* __restrict__ rb, c
       }
       if (next_first != last) ys[last] += S;
     }
-  };
-  for (int c = c_beg; c < c_end; c += 2) {
-    step(wdA, vA, nA, baseA, wdB, vB, nB, baseB, c);
-    if (c + 1 < c_end) step(wdB, vB, nB, baseB, wdA, vA, nA, baseA, c + 1);
   }
   __syncthreads();  // blocks without chunks still zeroed ys
   for (int i = tid; i < R; i += NT) y[r0 + i] = ys[i];
@@ -489,7 +590,7 @@ std::vector<int> row_blocks(const std::vector<int>& rp, int64_t nrows, int64_t n
 // column) into pk/sv64/sv32 as (row within the block << shift | column
 // within the slab), with seg[b*K + s] the start of block b's slab s and
 // seg[nb*K] = nnz.  drb receives rb.  Blocks hold at most 2048 rows
-// (rows_per_thread 8) or 8192 (32).
+// (rows_per_thread 8), 8192 (32) or 20,480 (80).
 int slab_stream(const hn_csr* a, const std::vector<int>& rb, int slab_log2, long long K, int shift,
                 int rows_per_thread, DevBuf& drb, DevBuf& seg, DevBuf& pk, DevBuf& sv64, DevBuf& sv32,
                 hipStream_t s) {
@@ -506,12 +607,16 @@ int slab_stream(const hn_csr* a, const std::vector<int>& rb, int slab_log2, long
   const float* v32 = (a->flags & HN_F32) ? a->val32.as<float>() : nullptr;
   double* o64 = (a->flags & HN_F64) ? sv64.as<double>() : nullptr;
   float* o32 = (a->flags & HN_F32) ? sv32.as<float>() : nullptr;
+#define HN_SLAB_BUILD(RPT)                                                                                      \
+  hipLaunchKernelGGL(k_slab_build<RPT>, dim3((unsigned)nb), dim3(kNT), 0, s, a->row_ptr.as<int>(), a->col.as<int>(), \
+                     v64, v32, drb.as<int>(), (int)K, slab_log2, shift, seg.as<int>(), pk.as<unsigned>(), o64, o32)
   if (rows_per_thread == 8)
-    hipLaunchKernelGGL(k_slab_build<8>, dim3((unsigned)nb), dim3(kNT), 0, s, a->row_ptr.as<int>(), a->col.as<int>(),
-                       v64, v32, drb.as<int>(), (int)K, slab_log2, shift, seg.as<int>(), pk.as<unsigned>(), o64, o32);
+    HN_SLAB_BUILD(8);
+  else if (rows_per_thread == 32)
+    HN_SLAB_BUILD(32);
   else
-    hipLaunchKernelGGL(k_slab_build<32>, dim3((unsigned)nb), dim3(kNT), 0, s, a->row_ptr.as<int>(), a->col.as<int>(),
-                       v64, v32, drb.as<int>(), (int)K, slab_log2, shift, seg.as<int>(), pk.as<unsigned>(), o64, o32);
+    HN_SLAB_BUILD(80);
+#undef HN_SLAB_BUILD
   HN_TRY(check_launch("k_slab_build"));
   HN_HIP(hipStreamSynchronize(s));  // rb and last live in the caller's frame
   return HN_OK;
@@ -607,17 +712,37 @@ int build_chunks(hn_csr* a, int ipt, int span_log2, int threads, hipStream_t s) 
   }
   std::vector<int> rp;
   HN_TRY(download_row_ptr(a, rp, s));
-  // rows per block: 8 per thread, summed in LDS
-  const int max_rows = 8 * threads;
-  static_assert(8 * 1024 <= kChunkMaxRows, "rows per block must fit the packed word");
+  // Rows per block, summed in LDS: 8 per thread, or more when the blocks
+  // would otherwise not all be resident at once (a matrix with many short
+  // rows, such as the transpose of a wide one): then the fewest blocks per CU
+  // whose LDS still fits, so that every block walks the columns in the same
+  // sweep.
   const size_t vb = (a->flags & HN_F64) ? sizeof(double) : sizeof(float);
-  const size_t lds = vb * max_rows;
-  int per_cu = 0;
-  HN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, (const void*)((a->flags & HN_F64) ? chunk_kernel_for<double>(ipt, threads)
-                                                 : chunk_kernel_for<float>(ipt, threads)),
-      threads, lds));
+  const void* kern = (a->flags & HN_F64) ? chunk_kernel_for<double>(ipt, threads) : chunk_kernel_for<float>(ipt, threads);
+  const int cus = std::max(1, ctx().info.compute_units);
+  auto resident = [&](int p) { return std::max<int64_t>(1, (int64_t)(0.97 * p * cus)); };
+  int max_rows = 8 * threads, per_cu = 0;
+  HN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, vb * max_rows));
   HN_CHECK(per_cu > 0, HN_ERR_UNSUPPORTED, "chunked layout: kernel does not fit a CU");
+  if (resident(per_cu) * max_rows < a->nrows) {
+    int lds_cap = 0;
+    HN_HIP(hipDeviceGetAttribute(&lds_cap, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx().info.device));
+    for (int p = per_cu; p >= 1; --p) {
+      const int64_t rows = ((a->nrows + resident(p) - 1) / resident(p) + kNT - 1) / kNT * kNT;
+      if (rows > kChunkMaxRows || (int64_t)vb * rows > (int64_t)lds_cap) continue;
+      int q = 0;
+      HN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&q, kern, threads, vb * rows));
+      if (q >= p) {
+        max_rows = (int)rows;
+        per_cu = p;
+        break;
+      }
+    }  // none fits: 8 rows per thread, in more than one round of blocks
+  }
+  int cb = kChunkColBits;
+  while (cb > kChunkMinColBits && (int64_t)max_rows > (int64_t(1) << (32 - cb))) --cb;
+  HN_CHECK((int64_t)max_rows <= (int64_t(1) << (32 - cb)), HN_ERR_UNSUPPORTED, "chunked layout: rows exceed the packed word");
+  span_log2 = std::min(span_log2, cb);
   const std::vector<int> rb = row_blocks(rp, a->nrows, a->nnz, per_cu, max_rows);
   const long long nb = (long long)rb.size() - 1;
   // slabs only steer the cut: 2^12 columns, or the span if narrower
@@ -627,8 +752,8 @@ int build_chunks(hn_csr* a, int ipt, int span_log2, int threads, hipStream_t s) 
   const int CH = threads * ipt;
   HN_CHECK(nb * K < INT_MAX, HN_ERR_UNSUPPORTED, "chunked layout: too many blocks x slabs");
   DevBuf drb, seg, spk, sv64, sv32;
-  HN_TRY(slab_stream(a, rb, slab_log2, K, kChunkColBits, max_rows > kSlabRows ? 32 : 8, drb, seg, spk, sv64, sv32,
-                     s));
+  HN_TRY(slab_stream(a, rb, slab_log2, K, cb, max_rows <= kSlabRows ? 8 : (max_rows <= 32 * kNT ? 32 : 80), drb, seg,
+                     spk, sv64, sv32, s));
   // Slots per block: each chunk is full (CH), ends a span window (a new
   // window starts at least span_slabs slabs further) or ends the block.
   std::vector<int> cs(nb + 1, 0);
@@ -688,7 +813,7 @@ int build_chunks(hn_csr* a, int ipt, int span_log2, int threads, hipStream_t s) 
     HN_HIP(hipMemcpyAsync(dsrc.p, src.data(), sizeof(int4) * nch, hipMemcpyHostToDevice, s));
     HN_HIP(hipMemcpyAsync(ddst.p, dst.data(), sizeof(int) * nch, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_chunk_pack, dim3((unsigned)nch), dim3(1024), 0, s, dsrc.as<int4>(), ddst.as<int>(),
-                       seg.as<int>(), (int)K, slab_log2, spk.as<unsigned>(),
+                       seg.as<int>(), (int)K, slab_log2, cb, spk.as<unsigned>(),
                        (a->flags & HN_F64) ? sv64.as<double>() : nullptr,
                        (a->flags & HN_F32) ? sv32.as<float>() : nullptr, tl.chunk_pk.as<unsigned>(),
                        (a->flags & HN_F64) ? tl.chunk_val64.as<double>() : nullptr,
@@ -699,6 +824,8 @@ int build_chunks(hn_csr* a, int ipt, int span_log2, int threads, hipStream_t s) 
   tl.chunk_ipt = ipt;
   tl.chunk_threads = threads;
   tl.chunk_span = span_log2;
+  tl.chunk_cb = cb;
+  tl.chunk_rows = max_rows;
   tl.chunk_nb = (int)nb;
   tl.chunk_n = (int)nch;
   return HN_OK;
@@ -712,10 +839,11 @@ int spmv_chunk_launch(const hn_csr* a, const V* x, V* y, hipStream_t s) {
   const int* bc = tl.chunk_bc.as<int>();
   const int4* rec = tl.chunk_rec.as<int4>();
   const unsigned* pk = tl.chunk_pk.as<unsigned>();
-  const size_t lds = sizeof(V) * 8 * tl.chunk_threads;
+  const size_t lds = sizeof(V) * tl.chunk_rows;
   const dim3 g(tl.chunk_nb), t(tl.chunk_threads);
+  const int cb = tl.chunk_cb;
 #define HN_CHUNK_LAUNCH(IPT, NT) \
-  hipLaunchKernelGGL((k_spmv_chunk<V, IPT, NT>), g, t, lds, s, rb, bc, rec, pk, val, x, y)
+  hipLaunchKernelGGL((k_spmv_chunk<V, IPT, NT>), g, t, lds, s, rb, bc, rec, pk, val, x, y, cb)
   if (tl.chunk_ipt == 4) {
     if (tl.chunk_threads == 256)
       HN_CHUNK_LAUNCH(4, 256);

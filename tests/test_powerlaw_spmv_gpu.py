@@ -89,9 +89,33 @@ def test_powerlaw_full_size_benchmarked_matrix(hip):
     A = hip.CsrMatrix.upload(c.val, ci, rp, c.ncols, hip.HN_F64 | hip.HN_F32)
     At = A.transpose()
     A.set_chunks(4, 19, 1024)
-    At.set_chunks(4, 19, 512)
     e1 = _err(A.spmv(c.x), O.spmv(rp, ci, c.val, c.x))
-    e2 = _err(At.spmv(c.w), O.spmv_t(rp, ci, c.val, c.w, c.ncols))
     f1 = _f32_ok(hip, A, c.val, rp, ci, c.nrows, c.x)
-    print(f"  A x {e1:.2e}  A' w {e2:.2e}  FP32 A x {f1:.2e}", flush=True)
-    assert e1 <= 1e-12 and e2 <= 1e-12 and f1 <= 1e-5
+    print(f"  A x {e1:.2e}  FP32 A x {f1:.2e}", flush=True)
+    assert e1 <= 1e-12 and f1 <= 1e-5
+    # The transpose has 5,000,000 short rows: for every block to be resident
+    # at once its blocks hold more than 8 rows per thread (about 10,000 rows
+    # at 512 threads, 20,000 at 1024), so the packed word gives the row 14 or
+    # 15 bits and the chunk span shrinks to 2^18 or 2^17 columns.
+    zref = O.spmv_t(rp, ci, c.val, c.w, c.ncols)
+    trp, trow, tval = _t_pattern(rp, ci, c.val, c.ncols)
+    narrowed = 0
+    for layout in ((4, 19, 512), (4, 19, 1024), (8, 19, 256)):
+        At.set_chunks(*layout)
+        got = At.chunks()
+        e2 = _err(At.spmv(c.w), zref)
+        g2 = _f32_ok(hip, At, tval, trp, trow, c.ncols, c.w)
+        print(f"  A' w {layout} -> {got}: {e2:.2e}  FP32 {g2:.2e}", flush=True)
+        assert got[0] == layout[0] and got[2] == layout[2] and 17 <= got[1] <= layout[1]
+        narrowed += got[1] < layout[1]
+        assert e2 <= 1e-12 and g2 <= 1e-5
+    assert narrowed >= 2  # the tall blocks were exercised
+
+
+def _t_pattern(rp, ci, val, ncols):
+    # CSR of the transpose (row pointer, column indices, values), each row
+    # in the original row order
+    order = np.argsort(ci, kind="stable")
+    trp = np.concatenate(([0], np.cumsum(np.bincount(ci, minlength=ncols)))).astype(np.int64)
+    rows = np.repeat(np.arange(len(rp) - 1, dtype=np.int64), np.diff(rp))
+    return trp, rows[order], val[order]
