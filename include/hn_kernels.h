@@ -105,6 +105,44 @@ int hn_csr_dims(const hn_csr* a, uint64_t* nrows, uint64_t* ncols, uint64_t* nnz
  * entries): bit-exact with that pattern when no entry sums to exactly zero.
  * *out is a new nrows x nrows handle whose values are zero. */
 int hn_csr_adat_pattern(const hn_csr* a, hn_csr** out);
+
+/* ---- sparse LDL' on the exact minimum-degree ordering ------------------- */
+typedef struct hn_sldl hn_sldl;
+/* SparseCholesky::analyze (core/numerical/factorization.cpp:1240-1245):
+ * exact minimum-degree ordering with the lowest index breaking ties, the
+ * row patterns of L, the elimination tree and the postorder
+ * (compute_md_order_and_pattern :103-191, build_elimination_graph :71-97,
+ * compute_elimination_tree / compute_postorder :1327-1362) - the same
+ * permutation and patterns as the reference, computed on the host (no device
+ * call; usable before hn_init).  The pattern is the n x n CSR (row_ptr,
+ * col); values are not needed.  work_limit > 0 stops the ordering once it
+ * has merged that many adjacency entries (the fill of dense-filling graphs
+ * grows as n^3): then *out is still created and hn_sldl_sizes reports
+ * aborted = 1. */
+int hn_sldl_analyze(uint64_t n, const uint64_t* row_ptr, const uint64_t* col, double work_limit,
+                    hn_sldl** out);
+/* n, entries of L (diagonal included), number of dependency levels (the
+ * device factor runs one level at a time), aborted. */
+int hn_sldl_sizes(const hn_sldl* h, uint64_t* n, uint64_t* l_nnz, uint64_t* levels, int* aborted);
+/* SparseCholesky::perm / L_row_ptr / L_col_indices (factorization.hpp:
+ * 190-193) and the symbolic parent / postorder; any pointer may be NULL. */
+int hn_sldl_symbolic(const hn_sldl* h, uint64_t* perm, uint64_t* l_row_ptr, uint64_t* l_col,
+                     uint64_t* parent, uint64_t* postorder);
+/* SparseCholesky::factorize / refactorize (factorization.cpp:1230-1238,
+ * 1248-1325) on the device: LDL' of P A P' for the device matrix a (FP64
+ * values, the analysed pattern, ascending columns per row), bit-identical
+ * to the reference's L and D; pivots <= singular_tol are replaced by
+ * singular_tol (the reference's 1e-10) and counted in *clamped (may be
+ * NULL). */
+int hn_sldl_factor(hn_sldl* h, const hn_csr* a, double singular_tol, int* clamped);
+/* SparseCholesky::L_values / D_values (factorization.hpp:189-190) copied to
+ * the host (l_nnz and n entries; either may be NULL). */
+int hn_sldl_factors(const hn_sldl* h, double* l_values, double* d_values);
+/* SparseCholesky::solve (factorization.cpp:1364-1396): x = A^{-1} b on
+ * device buffers (b, x of n entries, not aliased), bit-identical to the
+ * reference's; enqueued on the stream (NULL: the library stream). */
+int hn_sldl_solve_dev(const hn_sldl* h, const double* b, double* x, void* stream);
+int hn_sldl_free(hn_sldl* h);
 /* Copy the device CSR back (row_ptr: nrows+1, col/val: nnz entries). */
 int hn_csr_download(const hn_csr* a, double* values, uint64_t* col_indices,
                     uint64_t* row_ptr);
@@ -237,9 +275,17 @@ typedef struct hn_lp {
 } hn_lp;
 
 /* Normal-equation solver of each Newton step. */
-#define HN_NE_AUTO 0   /* dense below dense_max_m rows, else PCG             */
+#define HN_NE_AUTO 0   /* dense up to dense_max_m rows, else sparse or PCG   */
 #define HN_NE_DENSE 1  /* dense inverse of M D M' + FP64 iterative refinement */
 #define HN_NE_PCG 2    /* Jacobi-preconditioned CG on M D M' (matrix-free)   */
+#define HN_NE_SPARSE 3 /* the reference's factorisation: sparse LDL' of      */
+                       /* M D M' + r I on the exact minimum-degree ordering  */
+                       /* (hn_sldl_*), FP64 iterative refinement; FP64 and   */
+                       /* HN_PIVOT_CLAMP only.  HN_NE_AUTO (FP64, clamp)     */
+                       /* takes it above dense_max_m when N has at most 2e7  */
+                       /* candidate entries and the ordering stays within a  */
+                       /* work budget (3e8 merged entries, 1e8 entries of    */
+                       /* L), else PCG                                       */
 /* Arithmetic of the normal-equation solve. */
 #define HN_PREC_F64 1   /* plain FP64 everywhere                              */
 #define HN_PREC_MIXED 2 /* FP32 bulk, FP64 residuals/refinement, FP64 below mu_switch */
@@ -357,6 +403,7 @@ typedef struct hn_ipm_result {
                                   iterations, and the solve restarted from the
                                   standard point */
   int projected;               /* 1 when the endgame_mu projection ended the solve */
+  int64_t sparse_l_nnz;        /* entries of the sparse LDL' factor (0: not used) */
 } hn_ipm_result;
 
 int hn_ipm_default_opts(hn_ipm_opts* o);

@@ -45,6 +45,8 @@ KERNEL_SOURCES = [
     "hn_batch.hip",
     "hn_gnn.hip",
     "hn_verify.hip",
+    "hn_order.cpp",
+    "hn_sparse_ldl.hip",
 ]
 
 

@@ -19,6 +19,7 @@
 #include "hn_device.hpp"
 #include "hn_internal.hpp"
 #include "hn_ipm_internal.hpp"
+#include "hn_sldl.hpp"
 #include "hn_stdform.hpp"
 
 #include <algorithm>
@@ -56,6 +57,21 @@ constexpr int kNT = 256;
 constexpr int kMaxParts = 512;
 // nonzeros of M from which the matrix-free solver autotunes its SpMV layouts
 constexpr size_t kAutotuneNnz = size_t(1) << 20;
+// HN_NE_AUTO above dense_max_m: the minimum-degree ordering may merge this
+// many adjacency entries (well under a second on the host) and the factor
+// hold this many entries before the solve falls back to PCG ...
+constexpr double kSparseAutoWork = 3e8;
+constexpr double kSparseAutoLnnz = 1e8;
+// ... and it is tried only when M's columns give at most this many candidate
+// entries of N (the power-law LP of 200,000 rows gives 9e7)
+constexpr double kSparseAutoCand = 2e7;
+// ... and when the factor's critical path (sequential merge steps of one row
+// per dependency level, summed over the levels) is short enough: a step is a
+// dependent memory access (25fv47, 5.0e6 steps, takes 0.57 s per iteration
+// with its solves), so factors with long dense rows take the PCG path
+constexpr double kSparseAutoCrit = 2e5;
+// singular_tol of ToleranceConfig::industrial_defaults (tolerance.hpp:70)
+constexpr double kSingularTol = 1e-10;
 
 int parts_for(int64_t n) {
   return (int)std::min<int64_t>(kMaxParts, std::max<int64_t>(1, (n + kNT * 8 - 1) / (kNT * 8)));
@@ -474,12 +490,15 @@ struct Solver {
   NormalEq ne;
   DenseNE dense;
   bool use_dense = true;
+  bool use_sparse = false;   // sparse LDL' on the exact minimum-degree ordering
+  hn_sldl* sl = nullptr;
   bool fp64_solve = true;  // current normal-equation precision (dense)
   double reg_now = 0.0;
   bool dense_ready = false;
 
   Solver(const hn_ipm_opts& oo, hn_ipm_result& rr, hipStream_t ss) : o(oo), res(rr), s(ss) {}
   ~Solver() {
+    sldl_destroy(sl);
     delete M;
     delete Mt;
     if (hS) (void)hipHostFree(hS);
@@ -516,9 +535,46 @@ struct Solver {
     HN_HIP(hipMemcpyAsync(b.p, f.rhs.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
     HN_HIP(hipMemcpyAsync(c.p, f.c.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
     use_dense = o.solver == HN_NE_DENSE || (o.solver == HN_NE_AUTO && f.m <= o.dense_max_m);
+    // candidate entries of N (sum over columns of M of their squared counts)
+    // bound its pattern and the ordering's first pass
+    double cand = 0.0;
+    if (!use_dense && o.solver == HN_NE_AUTO) {
+      std::vector<int64_t> cc(n, 0);
+      for (const int64_t j : f.ci) ++cc[j];
+      for (const int64_t k : cc) cand += (double)k * (double)k;
+    }
+    if (!use_dense && (o.solver == HN_NE_SPARSE ||
+                       (o.solver == HN_NE_AUTO && o.pivot_rule == HN_PIVOT_CLAMP && o.precision == HN_PREC_F64 &&
+                        cand <= kSparseAutoCand))) {
+      // the reference's factorisation: symbolic analysis once per solve
+      // (interior_point.cpp:549-576; D > 0 on every iterate keeps N's
+      // pattern), numeric LDL' per iteration.  Under AUTO the ordering gets a
+      // work budget and a factor that fills beyond it falls back to PCG.
+      HN_CHECK(o.solver != HN_NE_SPARSE || (o.pivot_rule == HN_PIVOT_CLAMP && o.precision == HN_PREC_F64),
+               HN_ERR_UNSUPPORTED, "hn_ipm_solve: the sparse LDL' solver takes the reference pivot rule in FP64");
+      HN_TRY(ne_symbolic(M, Mt, ne, s));
+      std::vector<int> nrp(m + 1), nci((size_t)std::max<int64_t>(ne.N->nnz, 1));
+      HN_HIP(hipMemcpyAsync(nrp.data(), ne.N->row_ptr.p, sizeof(int) * (m + 1), hipMemcpyDeviceToHost, s));
+      if (ne.N->nnz)
+        HN_HIP(hipMemcpyAsync(nci.data(), ne.N->col.p, sizeof(int) * ne.N->nnz, hipMemcpyDeviceToHost, s));
+      HN_HIP(hipStreamSynchronize(s));
+      std::vector<int64_t> rp64n(nrp.begin(), nrp.end()), ci64n(nci.begin(), nci.begin() + ne.N->nnz);
+      const bool auto_pick = o.solver == HN_NE_AUTO;
+      HN_TRY(sldl_create(m, rp64n.data(), ci64n.data(), auto_pick ? kSparseAutoWork : 0.0,
+                         auto_pick ? kSparseAutoLnnz : 0.0, &sl));
+      if (!sl->S.aborted && auto_pick && sl->S.crit_steps > kSparseAutoCrit) sl->S.aborted = true;
+      if (!sl->S.aborted) {
+        use_sparse = true;
+        res.sparse_l_nnz = sl->S.Lrp[m];
+      } else {
+        sldl_destroy(sl);
+        sl = nullptr;
+        HN_CHECK(auto_pick, HN_ERR_UNSUPPORTED, "hn_ipm_solve: sparse LDL' ordering stopped");
+      }
+    }
     if (use_dense) {
       HN_TRY(ne_symbolic(M, Mt, ne, s));
-    } else {
+    } else if (!use_sparse) {
       // the matrix-free path spends its time in products with M and M':
       // on large matrices both get the SpMV layout that is fastest on them
       // (a few ms of trial products, once per solve)
@@ -545,7 +601,12 @@ struct Solver {
 
   // Factor (dense) or prepare (PCG) the normal equations for the current D.
   int prepare_ne() {
-    if (use_dense) {
+    if (use_sparse) {
+      HN_TRY(ne_numeric(M, Mt, D.as<double>(), o.regularization, rel_reg(), ne, S.as<double>() + S_REG, nullptr,
+                        false, s));
+      HN_TRY(sldl_factor(sl, ne.N, kSingularTol, s));
+      dense_ready = true;
+    } else if (use_dense) {
       HN_TRY(ne_numeric(M, Mt, D.as<double>(), o.regularization, rel_reg(), ne, S.as<double>() + S_REG, &dense,
                         !fp64_solve, s));
       HN_TRY(dense_factor(dense, o.pivot_rule, s));
@@ -560,9 +621,15 @@ struct Solver {
     return HN_OK;
   }
 
+  // out (+)= N^{-1} r by the dense inverse or the sparse factor
+  int ne_apply(const double* r, double* out, bool accumulate, hipStream_t st) {
+    if (use_sparse) return sldl_solve(sl, r, out, accumulate, st);
+    return dense_apply(dense, r, out, accumulate, st);
+  }
+
   // N d = v by the matrix-free operator (FP64).
   int apply_N(const double* v, double* out) {
-    if (use_dense) return spmv(ne.N, v, out);
+    if (use_dense || use_sparse) return spmv(ne.N, v, out);
     HN_TRY(spmv(Mt, v, pcg_tn.as<double>()));
     hipLaunchKernelGGL((k_mul<double>), g(n), dim3(256), 0, s, pcg_tn.as<double>(), D.as<double>(),
                        pcg_tn.as<double>(), n);
@@ -584,7 +651,7 @@ struct Solver {
   // the device; dense_first_ok reads the check after a pull;
   // solve_dense_rest runs the corrections when it failed.
   int solve_dense_begin(const double* r_in, double* out) {
-    HN_TRY(dense_apply(dense, r_in, out, false, s));
+    HN_TRY(ne_apply(r_in, out, false, s));
     ++res.ne_solves;
     const int np = parts_for(m);
     HN_TRY(apply_N(out, ir_nd.as<double>()));
@@ -601,7 +668,7 @@ struct Solver {
     const int np = parts_for(m);
     *converged = false;
     for (int it = 1; it <= 5; ++it) {
-      HN_TRY(dense_apply(dense, ir_r.as<double>(), out, true, s));
+      HN_TRY(ne_apply(ir_r.as<double>(), out, true, s));
       ++res.refine_sweeps;
       HN_TRY(apply_N(out, ir_nd.as<double>()));
       hipLaunchKernelGGL(k_red_resid, dim3(np), dim3(kNT), 0, s, r_in, ir_nd.as<double>(), ir_r.as<double>(), m,
@@ -727,7 +794,7 @@ struct Solver {
 
   int solve(const double* r_in, double* out, double mu, bool* escalated) {
     *escalated = false;
-    if (!use_dense) return fp64_solve ? solve_pcg(r_in, out, mu) : solve_pcg_mixed(r_in, out, mu);
+    if (!use_dense && !use_sparse) return fp64_solve ? solve_pcg(r_in, out, mu) : solve_pcg_mixed(r_in, out, mu);
     bool ok = false;
     HN_TRY(solve_dense(r_in, out, &ok));
     if (!ok && !fp64_solve) HN_TRY(escalate(r_in, out, escalated));
@@ -767,6 +834,7 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
   res.fp64_from_iter = -1;
   res.warm_started = 0;
   res.projected = 0;
+  res.sparse_l_nnz = 0;
   auto finish = [&]() { res.solve_ms = 1e3 * clock.s(); };
 
   if (lp.n == 0) {  // interior_point.cpp:406-410
@@ -966,7 +1034,7 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
     // the predictor's kernels run again.  Only for the FP64 factor: an FP32
     // one almost never passes its first check, so the predictor would run
     // twice.
-    const bool deferred = sv.use_dense && sv.fp64_solve;
+    const bool deferred = (sv.use_dense || sv.use_sparse) && sv.fp64_solve;
     if (deferred)
       HN_TRY(sv.solve_begin(sv.rhs.as<double>(), sv.dlam.as<double>()));
     else
@@ -1121,7 +1189,7 @@ extern "C" int hn_ipm_solve(const hn_lp* lp, const hn_ipm_opts* opts, hn_ipm_res
   else
     hn_ipm_default_opts(&o);
   HN_CHECK(o.precision == HN_PREC_F64 || o.precision == HN_PREC_MIXED, HN_ERR_INVALID, "hn_ipm_solve: bad precision");
-  HN_CHECK(o.solver >= HN_NE_AUTO && o.solver <= HN_NE_PCG, HN_ERR_INVALID, "hn_ipm_solve: bad solver");
+  HN_CHECK(o.solver >= HN_NE_AUTO && o.solver <= HN_NE_SPARSE, HN_ERR_INVALID, "hn_ipm_solve: bad solver");
   HN_CHECK(o.max_iterations >= 0, HN_ERR_INVALID, "hn_ipm_solve: negative max_iterations");
   HN_CHECK(o.pivot_rule == HN_PIVOT_CLAMP || o.pivot_rule == HN_PIVOT_SKIP, HN_ERR_INVALID,
            "hn_ipm_solve: bad pivot_rule");

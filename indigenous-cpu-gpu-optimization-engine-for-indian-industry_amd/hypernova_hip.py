@@ -62,6 +62,13 @@ def _load_kernels() -> C.CDLL:
         "hn_csr_upload": (C.c_int, [_dp, _u64p, _u64p, _u64, _u64, _u64, C.c_int, C.POINTER(_vp)]),
         "hn_csr_transpose": (C.c_int, [_vp, C.POINTER(_vp)]),
         "hn_csr_adat_pattern": (C.c_int, [_vp, C.POINTER(_vp)]),
+        "hn_sldl_analyze": (C.c_int, [_u64, _u64p, _u64p, C.c_double, C.POINTER(_vp)]),
+        "hn_sldl_sizes": (C.c_int, [_vp, _u64p, _u64p, _u64p, C.POINTER(C.c_int)]),
+        "hn_sldl_symbolic": (C.c_int, [_vp, _u64p, _u64p, _u64p, _u64p, _u64p]),
+        "hn_sldl_factor": (C.c_int, [_vp, _vp, C.c_double, C.POINTER(C.c_int)]),
+        "hn_sldl_factors": (C.c_int, [_vp, _dp, _dp]),
+        "hn_sldl_solve_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
+        "hn_sldl_free": (C.c_int, [_vp]),
         "hn_csr_free": (C.c_int, [_vp]),
         "hn_csr_dims": (C.c_int, [_vp, _u64p, _u64p, _u64p]),
         "hn_csr_download": (C.c_int, [_vp, _dp, _u64p, _u64p]),
@@ -193,6 +200,60 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+class SparseLDL:
+    """SparseCholesky (core/numerical/factorization.hpp:162-205): the exact
+    minimum-degree analysis runs on the host at construction
+    (hn_sldl_analyze, no device needed), the numeric LDL' and the solves on
+    the device (hn_sldl_factor, hn_sldl_solve_dev)."""
+
+    def __init__(self, row_ptr, col, work_limit: float = 0.0):
+        rp = np.ascontiguousarray(row_ptr, np.uint64)
+        ci = np.ascontiguousarray(col, np.uint64)
+        self.h = _vp()
+        check(LIB.hn_sldl_analyze(rp.size - 1, rp.ctypes.data_as(_u64p), ci.ctypes.data_as(_u64p),
+                                  float(work_limit), C.byref(self.h)))
+
+    def sizes(self) -> tuple[int, int, int, bool]:
+        """(n, entries of L, dependency levels, aborted)"""
+        n, nnz, lv = _u64(), _u64(), _u64()
+        ab = C.c_int()
+        check(LIB.hn_sldl_sizes(self.h, C.byref(n), C.byref(nnz), C.byref(lv), C.byref(ab)))
+        return int(n.value), int(nnz.value), int(lv.value), bool(ab.value)
+
+    def symbolic(self) -> dict:
+        n, nnz, _, _ = self.sizes()
+        out = {k: np.zeros(n, np.uint64) for k in ("perm", "parent", "postorder")}
+        out["l_row_ptr"] = np.zeros(n + 1, np.uint64)
+        out["l_col"] = np.zeros(max(nnz, 1), np.uint64)
+        check(LIB.hn_sldl_symbolic(self.h, *(out[k].ctypes.data_as(_u64p)
+                                             for k in ("perm", "l_row_ptr", "l_col", "parent", "postorder"))))
+        out["l_col"] = out["l_col"][:nnz]
+        return {k: v.astype(np.int64) for k, v in out.items()}
+
+    def factor(self, a: "CsrMatrix", singular_tol: float = 1e-10) -> int:
+        """Numeric LDL' of the device matrix a; returns the number of clamped pivots."""
+        c = C.c_int()
+        check(LIB.hn_sldl_factor(self.h, a.h, singular_tol, C.byref(c)))
+        return int(c.value)
+
+    def factors(self) -> tuple[np.ndarray, np.ndarray]:
+        n, nnz, _, _ = self.sizes()
+        L, D = np.zeros(max(nnz, 1)), np.zeros(max(n, 1))
+        check(LIB.hn_sldl_factors(self.h, _dptr(L), _dptr(D)))
+        return L[:nnz], D[:n]
+
+    def solve(self, b) -> np.ndarray:
+        db = DeviceArray.from_numpy(np.ascontiguousarray(b, np.float64))
+        dx = DeviceArray(db.n)
+        check(LIB.hn_sldl_solve_dev(self.h, db.ptr, dx.ptr, None))
+        return dx.to_numpy()
+
+    def __del__(self):
+        if getattr(self, "h", None) and LIB is not None:
+            LIB.hn_sldl_free(self.h)
+            self.h = None
 
 
 class CsrMatrix:
@@ -466,7 +527,7 @@ def gpu_visible() -> bool:
 # ---------------------------------------------------------------------------
 # Interior-point solver (hn_ipm_solve)
 # ---------------------------------------------------------------------------
-HN_NE_AUTO, HN_NE_DENSE, HN_NE_PCG = 0, 1, 2
+HN_NE_AUTO, HN_NE_DENSE, HN_NE_PCG, HN_NE_SPARSE = 0, 1, 2, 3
 HN_PREC_F64, HN_PREC_MIXED = 1, 2
 HN_PIVOT_CLAMP, HN_PIVOT_SKIP = 0, 1
 STATUS = ["UNKNOWN", "OPTIMAL", "INFEASIBLE", "UNBOUNDED", "SUBOPTIMAL", "TIME_LIMIT",
@@ -501,7 +562,8 @@ class _HnResult(C.Structure):
                 ("rel_primal", C.c_double), ("rel_dual", C.c_double), ("rel_gap", C.c_double),
                 ("scaled_rel_primal", C.c_double), ("scaled_rel_dual", C.c_double),
                 ("scaled_rel_gap", C.c_double), ("ne_solves", C.c_int64), ("refine_sweeps", C.c_int64), ("cg_iterations", C.c_int64),
-                ("fp64_from_iter", C.c_int64), ("warm_started", C.c_int), ("projected", C.c_int)]
+                ("fp64_from_iter", C.c_int64), ("warm_started", C.c_int), ("projected", C.c_int),
+                ("sparse_l_nnz", C.c_int64)]
 
 
 LIB.hn_ipm_default_opts.argtypes = [C.POINTER(_HnOpts)]
@@ -556,6 +618,7 @@ class InteriorPointResult:
         self.cg_iterations, self.fp64_from_iter = int(r.cg_iterations), int(r.fp64_from_iter)
         self.warm_started = bool(r.warm_started)
         self.projected = bool(r.projected)
+        self.sparse_l_nnz = int(r.sparse_l_nnz)
 
     def __repr__(self):
         return (f"InteriorPointResult({self.status}, obj={self.objective_value!r}, it={self.iterations}, "

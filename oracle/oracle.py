@@ -111,6 +111,13 @@ def _ref_lib():
     lib.ref_cpu_spmv_t_time.restype = C.c_double
     lib.ref_transpose.argtypes = [_i64, _i64, _i64p, _i64p, _dp, _i64p, _i64p, _dp]
     lib.ref_transpose.restype = _i64
+    if hasattr(lib, "ref_chol_create"):
+        lib.ref_chol_create.argtypes = [_i64, _u64p, _u64p, _dp, _i64p, C.POINTER(C.c_int)]
+        lib.ref_chol_create.restype = C.c_void_p
+        lib.ref_chol_get.argtypes = [C.c_void_p, _u64p, _u64p, _u64p, _dp, _dp]
+        lib.ref_chol_solve.argtypes = [C.c_void_p, _dp, _dp]
+        lib.ref_chol_free.argtypes = [C.c_void_p]
+        lib.ref_chol_free.restype = None
     if hasattr(lib, "ref_crossover"):
         lib.ref_ipm_solve_crossover.argtypes = [_i64, _i64, _i64p, _i64p, _dp, _i8p, _dp, _dp, _dp, _dp, C.c_int,
                                                 C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), _i64p, _dp, _dp,
@@ -214,6 +221,163 @@ def adat_dense(rp, ci, v, D, m: int, n: int, reg: float = 1e-8):
     N = np.empty(m * m)
     dyn = OR.or_adat_dense(m, n, _p(rp, _i64p), _p(ci, _i64p), _p(v, _dp), _p(D, _dp), reg, _p(N, _dp))
     return N.reshape(m, m), dyn
+
+
+# ---- sparse LDL' on the minimum-degree ordering (pure Python, small n) ------
+def md_order(n: int, rp, ci):
+    """compute_md_order_and_pattern (core/numerical/factorization.cpp:103-191)
+    over build_elimination_graph (:71-97), restated literally: adjacency lists
+    keep self loops and dead vertices, deg counts live neighbours, the next
+    vertex is the live one of least degree (lowest index on ties), its live
+    neighbour pairs gain edges.  Returns (perm, l_row_ptr, l_col)."""
+    adj = [[] for _ in range(n)]
+    for i in range(n):
+        seen = set()
+        for k in range(int(rp[i]), int(rp[i + 1])):
+            j = int(ci[k])
+            if j >= n or j in seen:
+                continue
+            seen.add(j)
+            adj[i].append(j)
+    for i in range(n):
+        for j in list(adj[i]):
+            if j != i:
+                adj[j].append(i)
+    adj = [sorted(set(a)) for a in adj]
+    live = [True] * n
+    deg = [sum(1 for w in adj[v] if w != v) for v in range(n)]
+    perm, inv = [n] * n, [n] * n
+    pattern = []
+    for r in range(n):
+        best, best_deg = n, n + 1
+        for v in range(n):
+            if live[v] and (deg[v] < best_deg or (deg[v] == best_deg and v < best)):
+                best, best_deg = v, deg[v]
+        perm[r], inv[best] = best, r
+        pattern.append(sorted({inv[w] for w in adj[best] if w != best and inv[w] < r} | {r}))
+        live[best] = False
+        for w in adj[best]:
+            if live[w]:
+                deg[w] -= 1
+        nb = list(adj[best])
+        for a in nb:
+            if not live[a] or a == best:
+                continue
+            for b in nb:
+                if not live[b] or b <= a:
+                    continue
+                if b not in adj[a]:
+                    adj[a].append(b)
+                    adj[b].append(a)
+                    deg[a] += 1
+                    deg[b] += 1
+    lrp = np.zeros(n + 1, np.int64)
+    for i in range(n):
+        lrp[i + 1] = lrp[i] + len(pattern[i])
+    lci = np.array([c for row in pattern for c in row], np.int64)
+    return np.array(perm, np.int64), lrp, lci
+
+
+def etree_postorder(lrp, lci):
+    """compute_elimination_tree / compute_postorder (factorization.cpp:
+    1327-1362): parent[j] = least row i > j with an entry (i, j), else n; the
+    postorder of the recursive depth-first search over the row patterns."""
+    n = len(lrp) - 1
+    parent = [n] * n
+    for i in range(n):
+        for k in range(int(lrp[i]), int(lrp[i + 1])):
+            j = int(lci[k])
+            if j < i:
+                parent[j] = min(parent[j], i)
+    visited, post = [False] * n, []
+    import sys
+    sys.setrecursionlimit(max(1000, 4 * n + 100))
+
+    def dfs(u):
+        visited[u] = True
+        for k in range(int(lrp[u]), int(lrp[u + 1])):
+            v = int(lci[k])
+            if v < u and not visited[v]:
+                dfs(v)
+        post.append(u)
+
+    for i in range(n):
+        if not visited[i]:
+            dfs(i)
+    return np.array(parent, np.int64), np.array(post, np.int64)
+
+
+def sparse_ldl(rp, ci, v, perm, lrp, lci, singular_tol: float = 1e-10):
+    """SparseCholesky::numeric_factorization (factorization.cpp:1248-1325) on
+    B = P A P' (permute_symmetric :196-234; duplicates summed), restated:
+    returns (L values, D, singular)."""
+    n = len(perm)
+    inv = np.empty(n, np.int64)
+    inv[np.asarray(perm)] = np.arange(n)
+    rows = [[] for _ in range(n)]
+    for i in range(n):
+        for k in range(int(rp[i]), int(rp[i + 1])):
+            j = int(ci[k])
+            if j < n:
+                rows[int(inv[i])].append((int(inv[j]), float(v[k])))
+    B = [dict() for _ in range(n)]
+    for r in range(n):
+        for c, val in sorted(rows[r]):  # pairs sorted as std::sort sorts them, equal columns summed in that order
+            B[r][c] = B[r][c] + val if c in B[r] else val
+    L = np.zeros(int(lrp[n]))
+    D = np.zeros(n)
+    singular = False
+    for i in range(n):
+        diag = B[i].get(i, 0.0)
+        for k in range(int(lrp[i]), int(lrp[i + 1])):
+            j = int(lci[k])
+            if j < i:
+                s = B[i].get(j, 0.0)
+                kj, ki = int(lrp[j]), int(lrp[i])
+                while kj < int(lrp[j + 1]) and ki < k:
+                    if lci[kj] == lci[ki]:
+                        s -= L[kj] * L[ki] * D[int(lci[kj])]
+                        kj += 1
+                        ki += 1
+                    elif lci[kj] < lci[ki]:
+                        kj += 1
+                    else:
+                        ki += 1
+                L[k] = s / D[j]
+            elif j == i:
+                s = 0.0
+                for kk in range(int(lrp[i]), k):
+                    s += L[kk] * L[kk] * D[int(lci[kk])]
+                val = diag - s
+                if val <= singular_tol:
+                    val, singular = singular_tol, True
+                D[i] = val
+                L[k] = 1.0
+    return L, D, singular
+
+
+def sparse_ldl_solve(perm, lrp, lci, L, D, b):
+    """SparseCholesky::solve (factorization.cpp:1364-1396), restated."""
+    n = len(perm)
+    x = [float(b[int(perm[i])]) for i in range(n)]
+    for i in range(n):
+        s = x[i]
+        for k in range(int(lrp[i]), int(lrp[i + 1])):
+            j = int(lci[k])
+            if j < i:
+                s -= L[k] * x[j]
+        x[i] = s
+    for i in range(n):
+        if D[i] != 0.0:
+            x[i] /= D[i]
+    for i in range(n - 1, -1, -1):
+        for k in range(int(lrp[i]), int(lrp[i + 1])):
+            j = int(lci[k])
+            if j < i:
+                x[j] -= L[k] * x[i]
+    out = np.empty(n)
+    out[np.asarray(perm)] = x
+    return out
 
 
 # ---- IPM ------------------------------------------------------------------
@@ -385,6 +549,37 @@ def ref_transpose(rp, ci, v, ncols: int):
     REF.ref_transpose(rp.size - 1, ncols, _p(rp, _i64p), _p(ci, _i64p), _p(v, _dp), _p(orp, _i64p),
                       _p(oci, _i64p), _p(ov, _dp))
     return orp, oci, ov
+
+
+def ref_sparse_cholesky(rp, ci, v, b=None) -> dict:
+    """The reference's SparseCholesky (analyze + factorize, industrial
+    tolerances) on the n x n CSR, from _ref/: perm, l_row_ptr, l_col, L, D,
+    singular and, when b is given, x = A^{-1} b by its solve."""
+    if REF is None or not hasattr(REF, "ref_chol_create"):
+        raise RuntimeError("reference build not present")
+    rp = np.ascontiguousarray(rp, np.uint64)
+    ci = np.ascontiguousarray(ci, np.uint64)
+    v = _f64(v)
+    n = rp.size - 1
+    nnz, sing = _i64(), C.c_int()
+    h = REF.ref_chol_create(n, _p(rp, _u64p), _p(ci, _u64p), _p(v, _dp), C.byref(nnz), C.byref(sing))
+    try:
+        out = dict(perm=np.zeros(n, np.uint64), l_row_ptr=np.zeros(n + 1, np.uint64),
+                   l_col=np.zeros(max(nnz.value, 1), np.uint64), L=np.zeros(max(nnz.value, 1)), D=np.zeros(max(n, 1)))
+        REF.ref_chol_get(h, _p(out["perm"], _u64p), _p(out["l_row_ptr"], _u64p), _p(out["l_col"], _u64p),
+                         _p(out["L"], _dp), _p(out["D"], _dp))
+        if b is not None:
+            bb = _f64(b)
+            x = np.empty(n)
+            REF.ref_chol_solve(h, _p(bb, _dp), _p(x, _dp))
+            out["x"] = x
+    finally:
+        REF.ref_chol_free(h)
+    out["l_col"], out["L"], out["D"] = out["l_col"][:nnz.value], out["L"][:nnz.value], out["D"][:n]
+    for k in ("perm", "l_row_ptr", "l_col"):
+        out[k] = out[k].astype(np.int64)
+    out["singular"] = bool(sing.value)
+    return out
 
 
 # ---- post-solve verification ------------------------------------------------

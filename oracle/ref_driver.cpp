@@ -7,8 +7,9 @@
 // (core/model/problem.hpp:199-292), InteriorPointSolver
 // (core/lp/interior_point.hpp:16-54), ComputeBackendFactory
 // (core/execution/gpu_backend.hpp:53-58) and SparseMatrix
-// (core/numerical/sparse_matrix.hpp:30-89) and SolutionVerifier
-// (core/validation/solution_verifier.hpp:23-41).
+// (core/numerical/sparse_matrix.hpp:30-89), SolutionVerifier
+// (core/validation/solution_verifier.hpp:23-41) and SparseCholesky
+// (core/numerical/factorization.hpp:162-205).
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
@@ -28,6 +29,7 @@
 #include "lp/simplex.hpp"
 #include "model/lp_parser.hpp"
 #include "model/problem.hpp"
+#include "numerical/factorization.hpp"
 #include "numerical/sparse_matrix.hpp"
 #include "numerical/tolerance.hpp"
 #include "presolve/presolve.hpp"
@@ -529,5 +531,45 @@ int ref_lp_write(void* handle, const char* path) {
 }
 
 void ref_lp_free(void* handle) { delete static_cast<RefLpFile*>(handle); }
+
+// SparseCholesky (analyze + factorize with the industrial tolerances) of the
+// n x n CSR given as is (rows sorted, no duplicates expected).  *l_nnz and
+// *singular receive the factor size and the singular flag.
+void* ref_chol_create(int64_t n, const uint64_t* rp, const uint64_t* ci, const double* v, int64_t* l_nnz,
+                      int* singular) {
+  numerical::SparseMatrix A((std::size_t)n, (std::size_t)n);
+  A.mutable_row_ptr().assign(rp, rp + n + 1);
+  A.mutable_col_indices().assign(ci, ci + rp[n]);
+  A.mutable_values().assign(v, v + rp[n]);
+  auto* c = new numerical::SparseCholesky(A, numerical::ToleranceConfig::industrial_defaults());
+  *l_nnz = (int64_t)c->L_row_ptr()[n];
+  *singular = c->stats().singular ? 1 : 0;
+  return c;
+}
+
+// perm (n), L_row_ptr (n+1), L_col_indices and L_values (l_nnz), D (n);
+// any pointer may be NULL.
+int ref_chol_get(void* h, uint64_t* perm, uint64_t* lrp, uint64_t* lci, double* lv, double* d) {
+  auto* c = static_cast<numerical::SparseCholesky*>(h);
+  const std::size_t n = c->perm().size();
+  if (perm) for (std::size_t i = 0; i < n; ++i) perm[i] = c->perm()[i];
+  if (lrp) for (std::size_t i = 0; i <= n; ++i) lrp[i] = c->L_row_ptr()[i];
+  const std::size_t nnz = c->L_row_ptr()[n];
+  if (lci) for (std::size_t k = 0; k < nnz; ++k) lci[k] = c->L_col_indices()[k];
+  if (lv) for (std::size_t k = 0; k < nnz; ++k) lv[k] = c->L_values()[k];
+  if (d) for (std::size_t i = 0; i < n; ++i) d[i] = c->D_values()[i];
+  return 0;
+}
+
+// x = A^{-1} b by SparseCholesky::solve.
+int ref_chol_solve(void* h, const double* b, double* x) {
+  auto* c = static_cast<numerical::SparseCholesky*>(h);
+  std::vector<double> t(b, b + c->perm().size());
+  c->solve(t);
+  std::memcpy(x, t.data(), sizeof(double) * t.size());
+  return 0;
+}
+
+void ref_chol_free(void* h) { delete static_cast<numerical::SparseCholesky*>(h); }
 
 }  // extern "C"
