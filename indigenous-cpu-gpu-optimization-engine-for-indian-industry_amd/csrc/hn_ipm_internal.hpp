@@ -13,13 +13,15 @@ struct NormalEq {
   ~NormalEq() { delete N; }
 };
 
-// Dense inverse workspace: after dense_factor, A holds -N^{-1}.
+// Dense factor workspace: after dense_factor, G = L^{-1} and dinv = D^{-1},
+// so N^{-1} = G' D^{-1} G.
 struct DenseNE {
   int m = 0;
   bool f32 = false;
-  DevBuf A;              // N, then L (lower), then -N^{-1} (full)
+  DevBuf A;              // N, then L (lower)
   DevBuf G;              // L^{-1} (lower triangular; 64 x 64 diagonal blocks stored in full)
   DevBuf dinv;           // 1 / pivot (0 for a skipped pivot)
+  DevBuf u, part;        // apply workspace: D^{-1} G r (m), per-block partial sums of G' u (m per block)
   DevBuf wt, lt;         // panel W = L D and L, k-major (64 x m)
   DevBuf clamped;        // number of clamped or skipped pivots
   DevBuf diag0;  // diagonal of N (regularisation included) before elimination
@@ -38,9 +40,10 @@ int ne_symbolic(const hn_csr* M, const hn_csr* Mt, NormalEq& ne, hipStream_t s);
 int ne_numeric(const hn_csr* M, const hn_csr* Mt, const double* D, double reg, double rel_reg, NormalEq& ne,
                double* reg_out, DenseNE* dense, bool f32, hipStream_t s);
 int dense_reserve(DenseNE& d, int m, bool f32);
-// Inverts the dense N in place (d.A becomes -N^{-1}) under the pivot rule
+// Factors the dense N (into G = L^{-1} and D^{-1}) under the pivot rule
 // HN_PIVOT_CLAMP or HN_PIVOT_SKIP.
 int dense_factor(DenseNE& d, int pivot_rule, hipStream_t s);
+// y = N^{-1} r (accumulate: y += N^{-1} r) as G' (D^{-1} (G r)).
 int dense_apply(const DenseNE& d, const double* r, double* y, bool accumulate, hipStream_t s);
 int ne_jacobi_diag(const hn_csr* M, const double* D, double* diag, hipStream_t s);
 

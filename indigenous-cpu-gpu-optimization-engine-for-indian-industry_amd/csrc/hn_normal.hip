@@ -15,11 +15,11 @@
 //   * numeric   - one thread per row accumulates its row in column order
 //     (deterministic, no atomics), then r is added on the diagonal;
 //   * dense     - for moderate m the matrix is scattered into a dense
-//     row-major buffer, factored N = L D L' by 64-wide blocks (pivots in
-//     natural order, clamped like the reference or skipped), and -N^{-1} =
-//     -L^{-T} D^{-1} L^{-1} is formed over it from the triangular factor with
-//     LDS-staged 64x64 tiles.  Solves are then one dense GEMV; FP64
-//     refinement against the sparse N corrects them;
+//     row-major buffer and factored N = L D L' by 64-wide blocks (pivots in
+//     natural order, clamped like the reference or skipped), forming
+//     G = L^{-1} alongside.  A solve applies G, D^{-1} and G' (two
+//     triangular matrix-vector passes, the bytes of one dense GEMV); FP64
+//     refinement against the sparse N corrects it;
 //   * PCG       - matrix-free products M (D (M' p)) + r p and the Jacobi
 //     diagonal sum_j m_ij^2 D_j + r, for LPs too large to densify.
 #include "hn_device.hpp"
@@ -286,12 +286,12 @@ int ne_numeric(const hn_csr* M, const hn_csr* Mt, const double* D, double reg, d
 // dense N^{-1} through a blocked LDL' factorisation
 // ---------------------------------------------------------------------------
 // N = L D L' (right-looking, 64-wide blocks, natural pivot order - the pivots
-// are the reference's LDL' pivots and are clamped or skipped like them), then
-// G = L^{-1} (block rows from the inverted diagonal blocks) and finally
-// -N^{-1} = -G' D^{-1} G written over N, so every solve stays one GEMV.
-// Forming the inverse from the triangular factor keeps it as accurate as
-// triangular solves; a Gauss-Jordan sweep of N itself loses several more
-// digits once N's condition passes ~1e10, which the skip rule reaches.
+// are the reference's LDL' pivots and are clamped or skipped like them), and
+// G = L^{-1} (block rows from the inverted diagonal blocks), so that
+// N^{-1} = G' D^{-1} G.  L^{-1} formed from the triangular factor is as
+// accurate as triangular solves; a Gauss-Jordan sweep of N itself loses
+// several more digits once N's condition passes ~1e10, which the skip rule
+// reaches.
 constexpr int kNB = 64;
 
 // acc[a][c] += sum_k As[k][ty + 16 a] * Bs[k][tx + 16 c], k < kw.
@@ -614,49 +614,6 @@ __global__ __launch_bounds__(256) void k_ldl_update(T* __restrict__ A, T* __rest
   }
 }
 
-// (4) A[I, J] = A[J, I] = -sum_{K >= I} G[K, I]' diag(dinv_K) G[K, J] for the
-// lower tiles I >= J: the full symmetric -N^{-1}.
-template <typename T>
-__global__ __launch_bounds__(256) void k_inv_syrk(const T* __restrict__ G, const T* __restrict__ dinv, int m,
-                                                  int nb, T* __restrict__ A) {
-  const int I = blockIdx.y, J = blockIdx.x;
-  if (J > I) return;
-  __shared__ T As[kNB][kNB];
-  __shared__ T Bs[kNB][kNB];
-  const int i0 = I * kNB, j0 = J * kNB;
-  const int tid = threadIdx.x;
-  const int tx = tid & 15, ty = tid >> 4;
-  T acc[4][4];
-  tile_zero(acc);
-  for (int K = I; K < nb; ++K) {
-    const int kb = K * kNB;
-    const int kw = min(kNB, m - kb);
-    __syncthreads();
-    for (int e = tid; e < kNB * kNB; e += 256) {
-      const int k = e / kNB, q = e % kNB;
-      const bool ok = k < kw;
-      As[k][q] = (ok && i0 + q < m) ? G[(long long)(kb + k) * m + i0 + q] : T(0);
-      Bs[k][q] = (ok && j0 + q < m) ? dinv[kb + k] * G[(long long)(kb + k) * m + j0 + q] : T(0);
-    }
-    __syncthreads();
-    tile_acc(acc, As, Bs, kw);
-  }
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const int i = i0 + ty + 16 * a;
-    if (i >= m) continue;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int j = j0 + tx + 16 * c;
-      // on a diagonal tile (i, j) and (j, i) are formed by two threads with
-      // different roundings: only the lower one is written, and mirrored
-      if (j >= m || j > i) continue;
-      A[(long long)i * m + j] = -acc[a][c];
-      A[(long long)j * m + i] = -acc[a][c];
-    }
-  }
-}
-
 template <typename T>
 static int dense_invert_typed(DenseNE& d, int m, double tol, double skip_rel, hipStream_t s) {
   T* A = d.A.as<T>();
@@ -677,9 +634,8 @@ static int dense_invert_typed(DenseNE& d, int m, double tol, double skip_rel, hi
     if (nt > 0)
       hipLaunchKernelGGL((k_ldl_update<T>), dim3(K + 1 + nt, nt), dim3(256), 0, s, A, G, m, k0, b, K, Wt, Lt);
   }
-  hipLaunchKernelGGL((k_inv_syrk<T>), dim3(nb, nb), dim3(256), 0, s, G, dinv, m, nb, A);
-  count_launch(3 * nb + 1);
-  return check_launch("dense invert");
+  count_launch(3 * nb);
+  return check_launch("dense factor");
 }
 
 int dense_reserve(DenseNE& d, int m, bool f32) {
@@ -691,12 +647,14 @@ int dense_reserve(DenseNE& d, int m, bool f32) {
   HN_TRY(d.lt.reserve(es * (size_t)kNB * m));
   HN_TRY(d.clamped.reserve(sizeof(int)));
   HN_TRY(d.diag0.reserve(sizeof(double) * (size_t)m));
+  HN_TRY(d.u.reserve(sizeof(double) * (size_t)m));
+  HN_TRY(d.part.reserve(sizeof(double) * (size_t)m * ((m + kNB - 1) / kNB)));
   d.m = m;
   return HN_OK;
 }
 
-// In-place inverse of the dense N written by ne_numeric: afterwards d.A
-// holds -N^{-1}.
+// Factors the dense N written by ne_numeric: afterwards d.G holds L^{-1}
+// and d.dinv the inverse pivots.
 int dense_factor(DenseNE& d, int pivot_rule, hipStream_t s) {
   const double tol = 1e-10;  // singular_tol of ToleranceConfig::industrial_defaults
   HN_HIP(hipMemsetAsync(d.clamped.p, 0, sizeof(int), s));
@@ -705,30 +663,74 @@ int dense_factor(DenseNE& d, int pivot_rule, hipStream_t s) {
                : dense_invert_typed<double>(d, d.m, tol, skip ? kSkipRel64 : 0.0, s);
 }
 
-// y = -A r  (A holds -N^{-1}); one wave per row, FP64 accumulation.
+// Applying N^{-1} = G' D^{-1} G: the same bytes as one GEMV with the
+// explicit inverse (each half of G once), and a pivot of D that was clamped
+// to 1e-10 scales its own component only, as in the reference's triangular
+// solves (an explicit inverse would spread its 1e10 through every entry).
+//
+// (1) u = D^{-1} (G r): a wave per row k, over the columns up to the end of
+// k's 64-row block (G is lower block-triangular; the upper part of each
+// diagonal block is zero).
 template <typename T>
-__global__ __launch_bounds__(256) void k_dense_gemv(const T* __restrict__ A, int m, const double* __restrict__ r,
-                                                    double* __restrict__ y, int accumulate) {
+__global__ __launch_bounds__(256) void k_gapply_lower(const T* __restrict__ G, const T* __restrict__ dinv, int m,
+                                                      const double* __restrict__ r, double* __restrict__ u) {
   const int row = (blockIdx.x * 256 + threadIdx.x) / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   if (row >= m) return;
-  const T* a = A + (long long)row * m;
+  const int len = min(m, (row / kNB + 1) * kNB);
+  const T* g = G + (long long)row * m;
   double s = 0.0;
-  for (int j = lane; j < m; j += kWave) s += (double)a[j] * r[j];
+  for (int j = lane; j < len; j += kWave) s += (double)g[j] * r[j];
   s = wave_sum(s);
-  if (lane == 0) y[row] = accumulate ? y[row] - s : -s;
+  if (lane == 0) u[row] = (double)dinv[row] * s;
+}
+
+// (2) part[K][i] = sum over the rows k of block K of G[k][i] u[k], for the
+// columns i below the block's end (the rest of those rows is zero).
+template <typename T>
+__global__ __launch_bounds__(256) void k_gapply_upper_part(const T* __restrict__ G, int m,
+                                                           const double* __restrict__ u, double* __restrict__ part) {
+  const int K = blockIdx.y;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int kb = K * kNB, ke = min(m, kb + kNB);
+  if (i >= ke) return;
+  double s = 0.0;
+  for (int k = kb; k < ke; ++k) s += (double)G[(long long)k * m + i] * u[k];
+  part[(long long)K * m + i] = s;
+}
+
+// (3) y_i (+)= sum over the blocks K from i's own up of part[K][i], in block
+// order (a fixed order: the result is deterministic).
+__global__ void k_gapply_sum(const double* __restrict__ part, int m, int nb, double* __restrict__ y,
+                             int accumulate) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  double s = 0.0;
+  for (int K = i / kNB; K < nb; ++K) s += part[(long long)K * m + i];
+  y[i] = accumulate ? y[i] + s : s;
 }
 
 int dense_apply(const DenseNE& d, const double* r, double* y, bool accumulate, hipStream_t s) {
   const int m = d.m;
+  const int nb = (m + kNB - 1) / kNB;
   const long long threads = (long long)m * kWave;
   const dim3 g((unsigned)((threads + 255) / 256));
-  if (d.f32)
-    hipLaunchKernelGGL((k_dense_gemv<float>), g, dim3(256), 0, s, d.A.as<float>(), m, r, y, (int)accumulate);
-  else
-    hipLaunchKernelGGL((k_dense_gemv<double>), g, dim3(256), 0, s, d.A.as<double>(), m, r, y, (int)accumulate);
-  count_launch();
-  return check_launch("k_dense_gemv");
+  const dim3 gp((unsigned)((m + 255) / 256), (unsigned)nb);
+  if (d.f32) {
+    hipLaunchKernelGGL((k_gapply_lower<float>), g, dim3(256), 0, s, d.G.as<float>(), d.dinv.as<float>(), m, r,
+                       d.u.as<double>());
+    hipLaunchKernelGGL((k_gapply_upper_part<float>), gp, dim3(256), 0, s, d.G.as<float>(), m, d.u.as<double>(),
+                       d.part.as<double>());
+  } else {
+    hipLaunchKernelGGL((k_gapply_lower<double>), g, dim3(256), 0, s, d.G.as<double>(), d.dinv.as<double>(), m, r,
+                       d.u.as<double>());
+    hipLaunchKernelGGL((k_gapply_upper_part<double>), gp, dim3(256), 0, s, d.G.as<double>(), m, d.u.as<double>(),
+                       d.part.as<double>());
+  }
+  hipLaunchKernelGGL(k_gapply_sum, dim3(grid_for(m, 256)), dim3(256), 0, s, d.part.as<double>(), m, nb, y,
+                     (int)accumulate);
+  count_launch(3);
+  return check_launch("dense apply");
 }
 
 // ---------------------------------------------------------------------------
