@@ -73,6 +73,15 @@ int hn_mem_info(size_t* free_bytes, size_t* total_bytes);
  * (gpu_backend.hpp:48-50): number of hot-path kernel launches so far. */
 uint64_t hn_kernel_executions(void);
 void hn_reset_execution_stats(void);
+/* No counterpart in the reference, whose tests reach their CPU fallbacks by
+ * disabling the GPU: for callers that must exercise a per-call fallback
+ * (AutoBackend::spmv retries a failed device call on the CPU,
+ * gpu_backend.cpp:688-694) with a device present.  The next `count` entry
+ * points that use the device fail with `code` (an HN_ERR_* value) before
+ * touching it, and hn_last_error() reports an injected failure; an entry
+ * point that calls another one inside the library consumes one for each.
+ * count 0 clears the injection. */
+int hn_inject_failures(uint32_t count, int code);
 
 /* ---- device-resident CSR matrices --------------------------------------- */
 typedef struct hn_csr hn_csr;

@@ -22,7 +22,8 @@ namespace hn::backend {
   X(int, hn_spmm, (const void*, const double*, double*, uint64_t))                                   \
   X(int, hn_dot, (const double*, const double*, uint64_t, double*))                                  \
   X(int, hn_axpy, (double, const double*, double*, uint64_t))                                        \
-  X(int, hn_norm_inf, (const double*, uint64_t, double*))
+  X(int, hn_norm_inf, (const double*, uint64_t, double*))                                            \
+  X(int, hn_inject_failures, (uint32_t, int))
 
 struct KernelApi {
 #define HN_DECL(ret, name, args) ret(*name) args = nullptr;
@@ -246,6 +247,10 @@ bool HipBackend::was_gpu_kernel_executed() const { return available_ && api_->hn
 std::size_t HipBackend::gpu_kernel_executions() const { return available_ ? api_->hn_kernel_executions() : 0; }
 void HipBackend::reset_execution_stats() {
   if (available_) api_->hn_reset_execution_stats();
+}
+
+void HipBackend::inject_failures(unsigned count) {
+  if (available_) api_->hn_inject_failures(count, 2 /* HN_ERR_HIP */);
 }
 
 std::unique_ptr<IComputeBackend> create_hip_backend(std::string library) {

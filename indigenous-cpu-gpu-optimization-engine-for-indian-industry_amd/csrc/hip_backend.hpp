@@ -84,6 +84,9 @@ class HipBackend final : public IComputeBackend {
   void reset_execution_stats() override;
 
   const std::string& last_error() const { return last_error_; }
+  // Test hook (hn_inject_failures): the next `count` library calls fail as a
+  // device failure would, so a caller's CPU fallback can be exercised.
+  void inject_failures(unsigned count);
 
  private:
   struct Resident;

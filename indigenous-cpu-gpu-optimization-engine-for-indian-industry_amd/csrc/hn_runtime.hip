@@ -39,7 +39,16 @@ Context& ctx() {
   return c;
 }
 
+// Failures forced from the host (hn_inject_failures): the next g_inject
+// calls of enter() fail with g_inject_code before touching the device.
+namespace {
+std::atomic<int64_t> g_inject{0};
+std::atomic<int> g_inject_code{HN_ERR_HIP};
+}  // namespace
+
 int enter() {
+  if (g_inject.load(std::memory_order_relaxed) > 0 && g_inject.fetch_sub(1) > 0)
+    return fail(g_inject_code.load(), "injected failure (hn_inject_failures)");
   Context& c = ctx();
   if (!c.initialized) return fail(HN_ERR_NOT_INIT, "hn_init() has not been called");
   int cur = -1;
@@ -166,6 +175,13 @@ extern "C" int hn_mem_info(size_t* free_bytes, size_t* total_bytes) {
 
 extern "C" uint64_t hn_kernel_executions(void) { return ctx().launches.load(); }
 extern "C" void hn_reset_execution_stats(void) { ctx().launches.store(0); }
+
+extern "C" int hn_inject_failures(uint32_t count, int code) {
+  HN_CHECK(code != HN_OK, HN_ERR_INVALID, "hn_inject_failures: the injected code must be an error");
+  g_inject_code.store(code);
+  g_inject.store((int64_t)count);
+  return HN_OK;
+}
 
 extern "C" int hn_dev_alloc(size_t bytes, void** ptr) {
   HN_CHECK(ptr, HN_ERR_INVALID, "hn_dev_alloc: null out pointer");

@@ -62,6 +62,7 @@ def _load_kernels() -> C.CDLL:
         "hn_csr_upload": (C.c_int, [_dp, _u64p, _u64p, _u64, _u64, _u64, C.c_int, C.POINTER(_vp)]),
         "hn_csr_transpose": (C.c_int, [_vp, C.POINTER(_vp)]),
         "hn_csr_adat_pattern": (C.c_int, [_vp, C.POINTER(_vp)]),
+        "hn_inject_failures": (C.c_int, [C.c_uint32, C.c_int]),
         "hn_sldl_analyze": (C.c_int, [_u64, _u64p, _u64p, C.c_double, C.POINTER(_vp)]),
         "hn_sldl_sizes": (C.c_int, [_vp, _u64p, _u64p, _u64p, C.POINTER(C.c_int)]),
         "hn_sldl_symbolic": (C.c_int, [_vp, _u64p, _u64p, _u64p, _u64p, _u64p]),
@@ -127,6 +128,13 @@ def last_error() -> str:
 def check(status: int) -> None:
     if status != HN_OK:
         raise HnError(status, last_error())
+
+
+def inject_failures(count: int, code: int = 2) -> None:
+    """hn_inject_failures: the next `count` library calls that use the device
+    fail with `code` (default HN_ERR_HIP) before touching it; 0 clears.  For
+    exercising a caller's CPU fallback with a device present."""
+    check(LIB.hn_inject_failures(count, code))
 
 
 def _dptr(a: np.ndarray) -> _dp:

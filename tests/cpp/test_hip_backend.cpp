@@ -182,6 +182,38 @@ int main(int argc, char** argv) {
     be->spmv(p.values, p.col_indices, p.row_ptr, p.x, y2);
     expect_close(y2, cpu_spmv(p), 1e-10);
   });
+  run("HipBackendTest.InjectedFailureFallsBackToCpu", [&] {
+    // AutoBackend::spmv (gpu_backend.cpp:688-694): a device call that throws
+    // is retried on the CPU for that call only.  The failure is forced from
+    // the host; the next call runs on the device again.
+    auto* hb = dynamic_cast<HipBackend*>(be.get());
+    EXPECT(hb != nullptr);
+    const Problem p = make_problem(800, 600, 0.01, 31);
+    auto auto_spmv = [&](std::vector<double>& y) -> bool {  // true: the device answered
+      try {
+        be->spmv(p.values, p.col_indices, p.row_ptr, p.x, y);
+        return true;
+      } catch (const std::exception&) {
+        y = cpu_spmv(p);
+        return false;
+      }
+    };
+    std::vector<double> y1, y2;
+    hb->inject_failures(1);
+    EXPECT(!auto_spmv(y1));
+    expect_close(y1, cpu_spmv(p), 0.0);
+    EXPECT(auto_spmv(y2));
+    expect_close(y2, cpu_spmv(p), 1e-10);
+    bool threw = false;
+    hb->inject_failures(1);
+    try {
+      std::vector<double> y;
+      be->spmv(p.values, p.col_indices, p.row_ptr, p.x, y);
+    } catch (const std::runtime_error& e) {
+      threw = std::strstr(e.what(), "injected failure") != nullptr;
+    }
+    EXPECT(threw);
+  });
   run("HipBackendTest.VectorOps", [&] {  // CPUBackend semantics, gpu_backend.cpp:124-136
     std::vector<double> a(20000), b(20000);
     for (std::size_t i = 0; i < a.size(); ++i) {
