@@ -3,6 +3,10 @@
 
 #include <dlfcn.h>
 
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <stdexcept>
 
@@ -23,7 +27,13 @@ namespace hn::backend {
   X(int, hn_dot, (const double*, const double*, uint64_t, double*))                                  \
   X(int, hn_axpy, (double, const double*, double*, uint64_t))                                        \
   X(int, hn_norm_inf, (const double*, uint64_t, double*))                                            \
-  X(int, hn_inject_failures, (uint32_t, int))
+  X(int, hn_inject_failures, (uint32_t, int))                                                        \
+  X(int, hn_dev_alloc, (std::size_t, void**))                                                        \
+  X(int, hn_dev_free, (void*))                                                                       \
+  X(int, hn_memcpy_h2d, (void*, const void*, std::size_t))                                           \
+  X(int, hn_memcpy_d2h, (void*, const void*, std::size_t))                                           \
+  X(int, hn_spmv_dev, (const void*, int, const void*, void*, void*))                                 \
+  X(int, hn_stream_sync, (void*))
 
 struct KernelApi {
 #define HN_DECL(ret, name, args) ret(*name) args = nullptr;
@@ -264,6 +274,286 @@ std::string hip_backend_probe_error(std::string library) {
   HipBackend b(std::move(library));
   b.initialize();
   return b.is_available() ? std::string() : b.last_error();
+}
+
+// ---- calibration ---------------------------------------------------------------
+void HipBackend::calibrate() {
+  using clock = std::chrono::steady_clock;
+  auto ms_since = [](clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(clock::now() - t0).count();
+  };
+  measured_spmv_gbps_ = measured_h2d_gbps_ = measured_d2h_gbps_ = 0.0;
+  if (available_) {
+    // the reference's synthetic problem: 4096 rows x 32 nonzeros, column
+    // (i + j) mod 4096, values 1, x = 1
+    const std::size_t nrows = 4096, per_row = 32, nnz = nrows * per_row;
+    std::vector<std::size_t> rp(nrows + 1), ci(nnz);
+    std::vector<double> val(nnz, 1.0), x(nrows, 1.0);
+    for (std::size_t i = 0; i < nrows; ++i) {
+      rp[i] = i * per_row;
+      for (std::size_t j = 0; j < per_row; ++j) ci[i * per_row + j] = (i + j) % nrows;
+    }
+    rp[nrows] = nnz;
+    // device-only rate (the model charges the vector transfers separately):
+    // matrix and vectors resident, the product timed with the stream drained
+    void* a = nullptr;
+    void *dx = nullptr, *dy = nullptr;
+    if (api_->hn_csr_upload(val.data(), reinterpret_cast<const uint64_t*>(ci.data()),
+                            reinterpret_cast<const uint64_t*>(rp.data()), nrows, nrows, nnz, 1, &a) == 0 &&
+        api_->hn_dev_alloc(nrows * sizeof(double), &dx) == 0 && api_->hn_dev_alloc(nrows * sizeof(double), &dy) == 0 &&
+        api_->hn_memcpy_h2d(dx, x.data(), nrows * sizeof(double)) == 0 &&
+        api_->hn_spmv_dev(a, 1, dx, dy, nullptr) == 0 && api_->hn_stream_sync(nullptr) == 0) {
+      constexpr int kReps = 5;
+      const auto t0 = clock::now();
+      bool ok = true;
+      for (int r = 0; r < kReps && ok; ++r) ok = api_->hn_spmv_dev(a, 1, dx, dy, nullptr) == 0;
+      ok = ok && api_->hn_stream_sync(nullptr) == 0;
+      const double ms = ms_since(t0);
+      if (ok && ms > 0.0)
+        measured_spmv_gbps_ = (double)nnz * (sizeof(double) + sizeof(std::size_t)) * kReps / (ms / 1e3) / 1e9;
+    } else {
+      char buf[512];
+      api_->hn_last_error(buf, sizeof buf);
+      last_error_ = std::string("calibration SpMV failed: ") + buf;
+    }
+    if (dx) api_->hn_dev_free(dx);
+    if (dy) api_->hn_dev_free(dy);
+    if (a) api_->hn_csr_free(a);
+    void* d = nullptr;
+    const std::size_t bytes = nnz * sizeof(double);
+    if (api_->hn_dev_alloc(bytes, &d) == 0) {
+      auto t0 = clock::now();
+      if (api_->hn_memcpy_h2d(d, val.data(), bytes) == 0) {
+        const double ms = ms_since(t0);
+        if (ms > 0.0) measured_h2d_gbps_ = (double)bytes / (ms / 1e3) / 1e9;
+      }
+      std::vector<double> back(nnz);
+      t0 = clock::now();
+      if (api_->hn_memcpy_d2h(back.data(), d, bytes) == 0) {
+        const double ms = ms_since(t0);
+        if (ms > 0.0) measured_d2h_gbps_ = (double)bytes / (ms / 1e3) / 1e9;
+      }
+      api_->hn_dev_free(d);
+    }
+  }
+  if (measured_h2d_gbps_ <= 0.0) measured_h2d_gbps_ = 6.0;
+  if (measured_d2h_gbps_ <= 0.0) measured_d2h_gbps_ = 6.0;
+  if (measured_spmv_gbps_ <= 0.0) measured_spmv_gbps_ = 50.0;
+}
+
+// ---- CPU backend ------------------------------------------------------------------
+void CpuBackend::spmv(const std::vector<double>& values, const std::vector<std::size_t>& col_indices,
+                      const std::vector<std::size_t>& row_ptr, const std::vector<double>& x, std::vector<double>& y) {
+  const std::size_t nrows = row_ptr.size() >= 1 ? row_ptr.size() - 1 : 0;
+  y.assign(nrows, 0.0);
+  for (std::size_t i = 0; i < nrows; ++i) {
+    double sum = 0.0;
+    for (std::size_t k = row_ptr[i]; k < row_ptr[i + 1]; ++k) sum += values[k] * x[col_indices[k]];
+    y[i] = sum;
+  }
+}
+
+void CpuBackend::spmm(const std::vector<double>& values, const std::vector<std::size_t>& col_indices,
+                      const std::vector<std::size_t>& row_ptr, const std::vector<double>& B, std::vector<double>& C,
+                      std::size_t ncols_B) {
+  const std::size_t nrows = row_ptr.size() >= 1 ? row_ptr.size() - 1 : 0;
+  C.assign(nrows * ncols_B, 0.0);
+  for (std::size_t i = 0; i < nrows; ++i)
+    for (std::size_t k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
+      const std::size_t j = col_indices[k];
+      const double a = values[k];
+      for (std::size_t c = 0; c < ncols_B; ++c) C[i * ncols_B + c] += a * B[j * ncols_B + c];
+    }
+}
+
+double CpuBackend::dot(const std::vector<double>& a, const std::vector<double>& b) {
+  double sum = 0.0;
+  for (std::size_t i = 0; i < a.size() && i < b.size(); ++i) sum += a[i] * b[i];
+  return sum;
+}
+
+void CpuBackend::axpy(double alpha, const std::vector<double>& x, std::vector<double>& y) {
+  for (std::size_t i = 0; i < x.size() && i < y.size(); ++i) y[i] += alpha * x[i];
+}
+
+double CpuBackend::norm_inf(const std::vector<double>& v) {
+  double mx = 0.0;
+  for (const double e : v) mx = std::max(mx, std::abs(e));
+  return mx;
+}
+
+// ---- cost model ---------------------------------------------------------------------
+namespace {
+struct ModelParams {
+  double h2d_gbps = 6.0;
+  double d2h_gbps = 6.0;
+  double device_spmv_gbps = 0.0;  // set by a calibrated backend
+  double cpu_gbps = 8.0;          // sustained single-thread SpMV stream rate
+  double launch_overhead_ms = 0.01;
+  double per_call_overhead_ms = 0.02;
+};
+ModelParams& model() {
+  static ModelParams p;
+  return p;
+}
+std::mutex& model_mutex() {
+  static std::mutex m;
+  return m;
+}
+}  // namespace
+
+void HipCostModel::set_device_params(double device_spmv_gbps, double h2d_gbps, double d2h_gbps) {
+  std::lock_guard<std::mutex> lk(model_mutex());
+  if (device_spmv_gbps > 0.0) model().device_spmv_gbps = device_spmv_gbps;
+  if (h2d_gbps > 0.0) model().h2d_gbps = h2d_gbps;
+  if (d2h_gbps > 0.0) model().d2h_gbps = d2h_gbps;
+}
+
+void HipCostModel::reset_device_params() {
+  std::lock_guard<std::mutex> lk(model_mutex());
+  model() = ModelParams();
+}
+
+double HipCostModel::estimate_cpu_spmv(std::size_t nnz, std::size_t nrows) {
+  const double bytes = (double)nnz * (sizeof(double) + sizeof(std::size_t) + sizeof(double)) +
+                       (double)nrows * (sizeof(double) + sizeof(std::size_t));
+  std::lock_guard<std::mutex> lk(model_mutex());
+  return bytes / (model().cpu_gbps * 1e9) * 1e3;
+}
+
+double HipCostModel::estimate_gpu_spmv(std::size_t nnz, std::size_t nrows, std::size_t row_ptr_size,
+                                       int batch_size) {
+  std::lock_guard<std::mutex> lk(model_mutex());
+  const ModelParams& p = model();
+  const double matrix_bytes = (double)nnz * (sizeof(double) + sizeof(std::size_t)) +
+                              (double)row_ptr_size * sizeof(std::size_t);
+  const double vector_bytes = (double)nrows * (sizeof(double) + sizeof(std::size_t));
+  const double upload_ms = matrix_bytes / (p.h2d_gbps * 1e9) * 1e3 +
+                           (vector_bytes / (p.h2d_gbps * 1e9) + vector_bytes / (p.d2h_gbps * 1e9)) * 1e3 * batch_size;
+  const double kernel_ms =
+      (double)nnz * (2.0 * sizeof(double) + sizeof(std::size_t)) / (p.device_spmv_gbps * 1e9) * 1e3 * batch_size;
+  return upload_ms + kernel_ms + p.launch_overhead_ms + p.per_call_overhead_ms;
+}
+
+HipCostModel::Decision HipCostModel::decide(const std::vector<double>& values,
+                                            const std::vector<std::size_t>& col_indices,
+                                            const std::vector<std::size_t>& row_ptr, std::size_t nrows,
+                                            std::size_t ncols, int batch_size, bool force_gpu) {
+  (void)col_indices;
+  (void)ncols;
+  Decision d;
+  const std::size_t nnz = values.size();
+  double rate;
+  {
+    std::lock_guard<std::mutex> lk(model_mutex());
+    rate = model().device_spmv_gbps;
+  }
+  if (rate <= 0.0 && !force_gpu) {
+    d.reason = "no calibrated GPU device; CPU dispatch";
+    d.estimated_cpu_time_ms = estimate_cpu_spmv(nnz, nrows) * batch_size;
+    return d;
+  }
+  d.estimated_cpu_time_ms = estimate_cpu_spmv(nnz, nrows) * batch_size;
+  d.estimated_gpu_time_ms = estimate_gpu_spmv(nnz, nrows, row_ptr.size(), batch_size);
+  const bool big_enough = nnz >= 50000 && nrows >= 2000;
+  const double speedup = d.estimated_gpu_time_ms > 0.0 ? d.estimated_cpu_time_ms / d.estimated_gpu_time_ms : 0.0;
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%.2f", speedup);
+  if (force_gpu || (big_enough && speedup >= 1.5)) {
+    d.backend = ComputeBackendType::HIP;
+    d.reason = force_gpu ? "GPU execution forced by options"
+                         : std::string("GPU estimated ~") + buf + "x faster (nnz=" + std::to_string(nnz) +
+                               ", matrix resident, batch=" + std::to_string(batch_size) + ")";
+  } else if (!big_enough) {
+    d.reason = "problem too small to amortize GPU launch/transfer overhead";
+  } else {
+    d.reason = std::string("GPU estimate not faster enough than CPU (~") + buf + "x)";
+  }
+  return d;
+}
+
+// ---- AUTO backend -------------------------------------------------------------------
+AutoHipBackend::AutoHipBackend(bool force_gpu, std::string library)
+    : force_gpu_(force_gpu), library_(std::move(library)) {}
+
+std::string AutoHipBackend::name() const {
+  std::lock_guard<std::mutex> lk(mutex_);
+  return name_;
+}
+
+void AutoHipBackend::initialize() {
+  std::lock_guard<std::mutex> lk(mutex_);
+  if (initialized_) return;
+  initialized_ = true;
+  cpu_ = std::make_unique<CpuBackend>();
+  auto h = std::make_unique<HipBackend>(library_);
+  h->initialize();
+  if (h->is_available()) {
+    h->calibrate();
+    HipCostModel::set_device_params(h->measured_spmv_gbps(), h->measured_h2d_gbps(), h->measured_d2h_gbps());
+    hip_ = std::move(h);
+  }
+  name_ = hip_ ? "Auto(" + hip_->name() + " | CPU fallback)" : "Auto(CPU)";
+}
+
+void AutoHipBackend::finalize() {
+  std::lock_guard<std::mutex> lk(mutex_);
+  if (hip_) hip_->finalize();
+}
+
+void AutoHipBackend::spmv(const std::vector<double>& values, const std::vector<std::size_t>& col_indices,
+                          const std::vector<std::size_t>& row_ptr, const std::vector<double>& x,
+                          std::vector<double>& y) {
+  const std::size_t nrows = row_ptr.size() >= 1 ? row_ptr.size() - 1 : 0;
+  const auto d = HipCostModel::decide(values, col_indices, row_ptr, nrows, 0, 1, force_gpu_);
+  if (d.backend == ComputeBackendType::HIP && hip_ && hip_->is_available()) {
+    try {
+      hip_->spmv(values, col_indices, row_ptr, x, y);
+      return;
+    } catch (const std::exception&) {
+      // a device failure: this call runs on the CPU
+    }
+  }
+  cpu_->spmv(values, col_indices, row_ptr, x, y);
+}
+
+void AutoHipBackend::spmm(const std::vector<double>& values, const std::vector<std::size_t>& col_indices,
+                          const std::vector<std::size_t>& row_ptr, const std::vector<double>& B,
+                          std::vector<double>& C, std::size_t ncols_B) {
+  const std::size_t nrows = row_ptr.size() >= 1 ? row_ptr.size() - 1 : 0;
+  const auto d = HipCostModel::decide(values, col_indices, row_ptr, nrows, ncols_B, (int)ncols_B, force_gpu_);
+  if (d.backend == ComputeBackendType::HIP && hip_ && hip_->is_available()) {
+    try {
+      hip_->spmm(values, col_indices, row_ptr, B, C, ncols_B);
+      return;
+    } catch (const std::exception&) {
+      // a device failure: this call runs on the CPU
+    }
+  }
+  cpu_->spmm(values, col_indices, row_ptr, B, C, ncols_B);
+}
+
+double AutoHipBackend::dot(const std::vector<double>& a, const std::vector<double>& b) {
+  if (hip_ && a.size() >= 10000) return hip_->dot(a, b);
+  return cpu_->dot(a, b);
+}
+
+void AutoHipBackend::axpy(double alpha, const std::vector<double>& x, std::vector<double>& y) {
+  if (hip_ && x.size() >= 10000)
+    hip_->axpy(alpha, x, y);
+  else
+    cpu_->axpy(alpha, x, y);
+}
+
+double AutoHipBackend::norm_inf(const std::vector<double>& v) {
+  if (hip_ && v.size() >= 10000) return hip_->norm_inf(v);
+  return cpu_->norm_inf(v);
+}
+
+std::unique_ptr<IComputeBackend> create_auto_backend(bool force_gpu, std::string library) {
+  auto b = std::make_unique<AutoHipBackend>(force_gpu, std::move(library));
+  b->initialize();
+  return b;
 }
 
 }  // namespace hn::backend

@@ -110,6 +110,45 @@ int main(int argc, char** argv) {
     }
     EXPECT(threw);
   });
+  run("HipCostModelTest.UnsetDeviceMeansCPU", [] {  // test_gpu_backend.cpp:133-141
+    HipCostModel::reset_device_params();
+    const Problem p = make_problem(5000, 5000, 0.01, 123);
+    const auto d = HipCostModel::decide(p.values, p.col_indices, p.row_ptr, p.nrows, p.ncols);
+    EXPECT(d.backend == ComputeBackendType::CPU);
+    EXPECT(!d.reason.empty());
+    EXPECT(d.estimated_cpu_time_ms >= 0.0);
+  });
+  run("HipCostModelTest.LargeBatchFavorsGPUWhenCalibrated", [] {  // :143-153
+    const Problem p = make_problem(6000, 4000, 0.05, 321);
+    HipCostModel::set_device_params(700.0, 8.0, 8.0);
+    const auto d = HipCostModel::decide(p.values, p.col_indices, p.row_ptr, p.nrows, p.ncols, 64);
+    EXPECT(d.backend == ComputeBackendType::HIP);
+    HipCostModel::reset_device_params();
+  });
+  run("HipCostModelTest.SmallProblemStaysOnCPU", [] {  // :155-161
+    Problem p;  // make_diagonal(100)
+    p.nrows = p.ncols = 100;
+    p.row_ptr.push_back(0);
+    for (std::size_t i = 0; i < 100; ++i) {
+      p.values.push_back(1.0 + i);
+      p.col_indices.push_back(i);
+      p.row_ptr.push_back(i + 1);
+    }
+    HipCostModel::set_device_params(700.0, 8.0, 8.0);
+    const auto d = HipCostModel::decide(p.values, p.col_indices, p.row_ptr, p.nrows, p.ncols);
+    EXPECT(d.backend == ComputeBackendType::CPU);
+    HipCostModel::reset_device_params();
+  });
+  run("AutoHipBackendTest.FallsBackToCPUWithoutGPU", [] {  // :163-175
+    auto b = create_auto_backend(false, "/nonexistent/libhn_kernels.so");
+    EXPECT(b && b->is_available());
+    EXPECT(b->name() == "Auto(CPU)");
+    const Problem p = make_problem(64, 48, 0.3, 5);
+    std::vector<double> y;
+    b->spmv(p.values, p.col_indices, p.row_ptr, p.x, y);
+    expect_close(y, cpu_spmv(p), 1e-12);
+    b->finalize();
+  });
   if (!gpu) return g_failures ? 1 : 0;
 
   auto be = create_hip_backend();
@@ -213,6 +252,41 @@ int main(int argc, char** argv) {
       threw = std::strstr(e.what(), "injected failure") != nullptr;
     }
     EXPECT(threw);
+  });
+  run("AutoHipBackendTest.CalibratedDispatch", [&] {
+    // AutoBackend over the HIP backend: calibration feeds the cost model; a
+    // large SpMV runs on the device, a small one on the CPU, both equal to
+    // the CPU product; an injected device failure falls back to the CPU for
+    // that call (gpu_backend.cpp:623-746)
+    HipCostModel::reset_device_params();
+    auto b = create_auto_backend();
+    auto* ab = dynamic_cast<AutoHipBackend*>(b.get());
+    EXPECT(ab && ab->hip());
+    if (!ab || !ab->hip()) return;
+    std::printf("  %s: spmv %.1f GB/s, h2d %.1f GB/s, d2h %.1f GB/s\n", b->name().c_str(),
+                ab->hip()->measured_spmv_gbps(), ab->hip()->measured_h2d_gbps(), ab->hip()->measured_d2h_gbps());
+    EXPECT(b->name().rfind("Auto(", 0) == 0 && b->name().find("CPU fallback") != std::string::npos);
+    EXPECT(ab->hip()->measured_spmv_gbps() > 0.0);
+    const Problem big = make_problem(20000, 20000, 0.002, 99);  // ~800k nonzeros
+    const Problem small = make_problem(300, 300, 0.02, 98);
+    const auto db = HipCostModel::decide(big.values, big.col_indices, big.row_ptr, big.nrows, big.ncols);
+    const auto ds = HipCostModel::decide(small.values, small.col_indices, small.row_ptr, small.nrows, small.ncols);
+    std::printf("  big: %s\n  small: %s\n", db.reason.c_str(), ds.reason.c_str());
+    EXPECT(ds.backend == ComputeBackendType::CPU);
+    std::vector<double> y;
+    b->reset_execution_stats();
+    b->spmv(small.values, small.col_indices, small.row_ptr, small.x, y);
+    expect_close(y, cpu_spmv(small), 0.0);
+    EXPECT(b->gpu_kernel_executions() == 0);
+    b->spmv(big.values, big.col_indices, big.row_ptr, big.x, y);
+    expect_close(y, cpu_spmv(big), 1e-10);
+    EXPECT((db.backend == ComputeBackendType::HIP) == (b->gpu_kernel_executions() > 0));
+    ab->set_force_gpu(true);
+    ab->hip()->inject_failures(1);
+    b->spmv(big.values, big.col_indices, big.row_ptr, big.x, y);
+    expect_close(y, cpu_spmv(big), 0.0);  // the CPU answered
+    b->finalize();
+    HipCostModel::reset_device_params();
   });
   run("HipBackendTest.VectorOps", [&] {  // CPUBackend semantics, gpu_backend.cpp:124-136
     std::vector<double> a(20000), b(20000);
