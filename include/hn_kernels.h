@@ -367,7 +367,20 @@ typedef struct hn_ipm_opts {
                                   otherwise the iterates continue unchanged.  It finishes
                                   degenerate LPs (transportation problems) whose dual
                                   steps stall near the optimum; 0 = off */
+  int loop;                    /* hn_ipm_solve, dense normal equations in FP64 (not in the
+                                  reference, which runs on the host): HN_LOOP_GRAPH (default)
+                                  keeps each iteration on the device between two reads of
+                                  its scalars and replays it as a captured HIP graph;
+                                  HN_LOOP_DEVICE the same without graph capture (for tools
+                                  that cannot follow graph launches); HN_LOOP_HOST decides
+                                  sigma and the refinement sweeps on the host, with three
+                                  reads per iteration.  The iterates are the same in all
+                                  three. */
 } hn_ipm_opts;
+
+#define HN_LOOP_GRAPH 0
+#define HN_LOOP_DEVICE 1
+#define HN_LOOP_HOST 2
 
 typedef struct hn_ipm_result {
   int status;                  /* HN_STATUS_* */
@@ -413,6 +426,11 @@ typedef struct hn_ipm_result {
                                   standard point */
   int projected;               /* 1 when the endgame_mu projection ended the solve */
   int64_t sparse_l_nnz;        /* entries of the sparse LDL' factor (0: not used) */
+  int64_t device_steps;        /* iterations run on the device between two reads
+                                  (hn_ipm_opts.loop), graph replays included */
+  int64_t host_redos;          /* of those, iterations whose solve needed more
+                                  refinement sweeps than the device step holds: the step
+                                  was undone and run again on the host */
 } hn_ipm_result;
 
 int hn_ipm_default_opts(hn_ipm_opts* o);

@@ -1326,6 +1326,7 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
     r.warm_started = out[k].accepted ? (out[k].restarted ? 2 : 1) : 0;
     r.projected = out[k].projected;
     r.sparse_l_nnz = 0;
+    r.device_steps = r.host_redos = 0;
     if (r.primal) std::memcpy(r.primal, primal.data(), sizeof(double) * lps[k].n);
     if (r.x && r.cap_n >= F.n) std::memcpy(r.x, x, sizeof(double) * F.n);
     if (r.s && r.cap_n >= F.n) std::memcpy(r.s, z, sizeof(double) * F.n);

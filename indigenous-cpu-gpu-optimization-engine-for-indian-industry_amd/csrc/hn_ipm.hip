@@ -49,9 +49,12 @@ enum Slot {
   S_IRB,       // ||rhs||_inf
   S_CTX, S_BTY, S_DRN,  // c'x, b'y, ||M'y + s - c||_inf (final metrics)
   S_PCG_RZ, S_PCG_PQ, S_PCG_RN, S_PCG_RZ_OLD,
+  S_SWEEPS,    // refinement sweeps run by device-driven iterations
   S_COUNT
 };
-enum Flag { F_DIR = 0, F_UPD, F_COUNT };
+// F_MORE: a device-driven step's solve still failed its refinement test
+// after the sweeps the step holds (the host would have run more)
+enum Flag { F_DIR = 0, F_UPD, F_MORE, F_COUNT };
 
 constexpr int kNT = 256;
 constexpr int kMaxParts = 512;
@@ -72,6 +75,11 @@ constexpr double kSparseAutoCand = 2e7;
 constexpr double kSparseAutoCrit = 2e5;
 // singular_tol of ToleranceConfig::industrial_defaults (tolerance.hpp:70)
 constexpr double kSingularTol = 1e-10;
+// refinement sweeps per solve of a device-driven step once a solve of the
+// LP has needed one (the refinement's own limit); until then the step holds
+// none, and a solve that needs one has its step undone and run again on the
+// host path
+constexpr int kMaxSweeps = 5;
 
 int parts_for(int64_t n) {
   return (int)std::min<int64_t>(kMaxParts, std::max<int64_t>(1, (n + kNT * 8 - 1) / (kNT * 8)));
@@ -83,10 +91,14 @@ __global__ void k_sub(const double* __restrict__ a, const double* __restrict__ b
   if (i < n) o[i] = a[i] - b[i];
 }
 
-__global__ void k_rd(const double* __restrict__ t, const double* __restrict__ z, const double* __restrict__ c,
-                     double* __restrict__ rd, int n) {
+// rp = M x - b over m and rd = M' lam + z - c over n in one launch
+// (interior_point.cpp:512-515)
+__global__ void k_resid(const double* __restrict__ mx, const double* __restrict__ b, double* __restrict__ rp, int m,
+                        const double* __restrict__ t, const double* __restrict__ z, const double* __restrict__ c,
+                        double* __restrict__ rd, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) rd[i] = t[i] + z[i] - c[i];  // interior_point.cpp:515
+  if (i < m) rp[i] = mx[i] - b[i];
+  if (i < n) rd[i] = t[i] + z[i] - c[i];
 }
 
 __global__ void k_scaling(const double* __restrict__ x, const double* __restrict__ z, const double* __restrict__ rd,
@@ -105,11 +117,14 @@ __global__ void k_rhs(const double* __restrict__ rp, const double* __restrict__ 
   if (i < m) rhs[i] = -rp[i] - mdrd[i] + mx[i];  // :598-600
 }
 
+// dx, dz over n from t = M' dlam, and the finiteness test of dx, dz and
+// dlam (over m) in one launch
 __global__ void k_direction(const double* __restrict__ D, const double* __restrict__ t,
                             const double* __restrict__ rd, const double* __restrict__ x,
                             const double* __restrict__ zs, double* __restrict__ dx, double* __restrict__ dz, int n,
-                            int* __restrict__ flags) {
+                            const double* __restrict__ dlam, int m, int* __restrict__ flags) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m && !isfinite(dlam[i])) flags[F_DIR] = 1;
   if (i >= n) return;
   const double s = t[i] + rd[i];
   double a = D[i] * s - x[i];     // :610 / :668
@@ -120,11 +135,6 @@ __global__ void k_direction(const double* __restrict__ D, const double* __restri
   if (!isfinite(a) || !isfinite(b)) flags[F_DIR] = 1;
 }
 
-__global__ void k_finite(const double* __restrict__ v, int n, int* __restrict__ flags) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && !isfinite(v[i])) flags[F_DIR] = 1;
-}
-
 __global__ void k_corrector(const double* __restrict__ dx, const double* __restrict__ dz,
                             const double* __restrict__ z, double sigma_mu, double* __restrict__ zs, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -133,14 +143,55 @@ __global__ void k_corrector(const double* __restrict__ dx, const double* __restr
   zs[i] = (sigma_mu - dx[i] * dz[i]) * zinv;
 }
 
+// sigma * mu of the corrector on the device (interior_point.cpp:638-643):
+// sigma = clamp((mu_aff / mu)^3, 0.1, 0.9), or 0.5 when mu_aff is not a
+// finite non-negative number or mu is not positive.  The cube is formed in
+// double-double (exact square, then an error-compensated product), so it is
+// the correctly rounded cube except in rare near-tie cases, as the host's
+// std::pow is.
+__device__ __forceinline__ double cube_rn(double q) {
+  const double p = q * q;
+  const double e = __builtin_fma(q, q, -p);
+  const double r = p * q;
+  const double e2 = __builtin_fma(p, q, -r);
+  return r + (e * q + e2);
+}
+
+__device__ __forceinline__ double sigma_mu_dev(const double* __restrict__ S, int n) {
+  const double mu = S[S_XZ] / (double)n;
+  const double mu_aff = S[S_MU_AFF] / (double)n;
+  double sigma;
+  if (!isfinite(mu_aff) || mu_aff < 0.0 || !(mu > 0.0)) {
+    sigma = 0.5;
+  } else {
+    const double c = cube_rn(mu_aff / fmax(mu, 1e-300));
+    sigma = (c < 0.1) ? 0.1 : ((0.9 < c) ? 0.9 : c);
+  }
+  return sigma * mu;
+}
+
+// k_corrector with sigma * mu formed from S (each thread forms the same value)
+__global__ void k_corrector_s(const double* __restrict__ dx, const double* __restrict__ dz,
+                              const double* __restrict__ z, const double* __restrict__ S, double* __restrict__ zs,
+                              int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double sigma_mu = sigma_mu_dev(S, n);
+  const double zinv = 1.0 / z[i];
+  zs[i] = (sigma_mu - dx[i] * dz[i]) * zinv;
+}
+
 // The update is skipped when the step's direction was not finite (the
 // host finds the flag at its next read of the flags and stops there, as
 // the reference stops before updating, :617-620, :675-678).
 __global__ void k_update(double* __restrict__ x, double* __restrict__ z, const double* __restrict__ dx,
                          const double* __restrict__ dz, const double* __restrict__ S, int n,
+                         double* __restrict__ lam, const double* __restrict__ dlam, int m,
                          int* __restrict__ flags) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || flags[F_DIR]) return;
+  if (flags[F_DIR]) return;
+  if (i < m) lam[i] = lam[i] + S[S_AD] * dlam[i];  // :694-696
+  if (i >= n) return;
   const double xi = x[i] + S[S_AP] * dx[i];  // :690-693
   const double zi = z[i] + S[S_AD] * dz[i];
   x[i] = xi;
@@ -148,10 +199,38 @@ __global__ void k_update(double* __restrict__ x, double* __restrict__ z, const d
   if (xi < 0.0 || zi < 0.0 || !isfinite(xi) || !isfinite(zi)) flags[F_UPD] = 1;  // :698-708
 }
 
-__global__ void k_update_lam(double* __restrict__ lam, const double* __restrict__ dlam,
-                             const double* __restrict__ S, int m, const int* __restrict__ flags) {
+__global__ void k_clear_flag(int* __restrict__ flags, int f) {
+  if (threadIdx.x == 0) flags[f] = 0;
+}
+
+// A device-driven step keeps the point it started from (x, z, lam), so that
+// a step whose solve needs more refinement sweeps than it holds can be
+// undone and run again on the host path.
+__global__ void k_keep_point(const double* __restrict__ x, const double* __restrict__ z,
+                             const double* __restrict__ lam, double* __restrict__ xk, double* __restrict__ zk,
+                             double* __restrict__ lk, int n, int m) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < m && !flags[F_DIR]) lam[i] = lam[i] + S[S_AD] * dlam[i];  // :694-696
+  if (i < n) {
+    xk[i] = x[i];
+    zk[i] = z[i];
+  }
+  if (i < m) lk[i] = lam[i];
+}
+
+__global__ void k_undo_step(double* __restrict__ x, double* __restrict__ z, double* __restrict__ lam,
+                            const double* __restrict__ xk, const double* __restrict__ zk,
+                            const double* __restrict__ lk, int n, int m, int* __restrict__ flags,
+                            double* __restrict__ S, double sweeps) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    x[i] = xk[i];
+    z[i] = zk[i];
+  }
+  if (i < m) lam[i] = lk[i];
+  if (i == 0) {
+    for (int f = 0; f < F_COUNT; ++f) flags[f] = 0;
+    S[S_SWEEPS] = sweeps;
+  }
 }
 
 __global__ void k_fill(double* __restrict__ p, double v, int n) {
@@ -263,9 +342,19 @@ __global__ __launch_bounds__(kNT) void k_fin_sum(const double* __restrict__ part
 }
 
 // refinement residual: r = rhs - Nd, ||r||_inf and ||rhs||_inf
+// (irr, irb given: a refinement check that is skipped once the solution has
+// met the test, like the correction before it; see ne_refined)
+__device__ __forceinline__ bool refined_dev(const double* irr, const double* irb) {
+  if (!irr) return false;
+  const double nb = *irb;
+  return nb == 0.0 || nb != nb || *irr / nb <= 1e-10;
+}
+
 __global__ __launch_bounds__(kNT) void k_red_resid(const double* __restrict__ rhs, const double* __restrict__ nd,
-                                                   double* __restrict__ r, int m, double* __restrict__ part) {
+                                                   double* __restrict__ r, int m, double* __restrict__ part,
+                                                   const double* irr, const double* irb) {
   __shared__ double red[kNT / kWave];
+  if (refined_dev(irr, irb)) return;
   double a = 0.0, b = 0.0;
   for (int i = blockIdx.x * kNT + threadIdx.x; i < m; i += gridDim.x * kNT) {
     const double ri = rhs[i] - nd[i];
@@ -281,9 +370,12 @@ __global__ __launch_bounds__(kNT) void k_red_resid(const double* __restrict__ rh
   }
 }
 
+// guarded: skipped once the previous check has passed; more (optional): set
+// when this check fails the refinement test
 __global__ __launch_bounds__(kNT) void k_fin_max2(const double* __restrict__ part, int np, double* __restrict__ S,
-                                                  int slot_a, int slot_b) {
+                                                  int slot_a, int slot_b, int guarded, int* __restrict__ more) {
   __shared__ double red[kNT / kWave];
+  if (guarded && refined_dev(S + slot_a, S + slot_b)) return;
   double a = 0.0, b = 0.0;
   for (int i = threadIdx.x; i < np; i += kNT) {
     a = nan_ignoring_max(a, part[i]);
@@ -294,6 +386,7 @@ __global__ __launch_bounds__(kNT) void k_fin_max2(const double* __restrict__ par
   if (threadIdx.x == 0) {
     S[slot_a] = a;
     S[slot_b] = b;
+    if (more && !(b == 0.0 || b != b || a / b <= 1e-10)) *more = 1;
   }
 }
 
@@ -485,6 +578,7 @@ struct Solver {
   // FP32 copies for the bulk PCG of mixed precision
   DevBuf f_r, f_z, f_p, f_q, f_x, f_dinv, f_tn, f_D, f_rhs;
   DevBuf S, flags, part;
+  DevBuf keep_x, keep_z, keep_lam;  // the point a device-driven step started from
   double* hS = nullptr;  // pinned mirror of S
   int* hF = nullptr;
   NormalEq ne;
@@ -656,8 +750,8 @@ struct Solver {
     const int np = parts_for(m);
     HN_TRY(apply_N(out, ir_nd.as<double>()));
     hipLaunchKernelGGL(k_red_resid, dim3(np), dim3(kNT), 0, s, r_in, ir_nd.as<double>(), ir_r.as<double>(), m,
-                       part.as<double>());
-    hipLaunchKernelGGL(k_fin_max2, dim3(1), dim3(kNT), 0, s, part.as<double>(), np, S.as<double>(), S_IRR, S_IRB);
+                       part.as<double>(), nullptr, nullptr);
+    hipLaunchKernelGGL(k_fin_max2, dim3(1), dim3(kNT), 0, s, part.as<double>(), np, S.as<double>(), S_IRR, S_IRB, 0, nullptr);
     return check_launch("dense solve");
   }
   bool dense_first_ok() const {
@@ -672,8 +766,8 @@ struct Solver {
       ++res.refine_sweeps;
       HN_TRY(apply_N(out, ir_nd.as<double>()));
       hipLaunchKernelGGL(k_red_resid, dim3(np), dim3(kNT), 0, s, r_in, ir_nd.as<double>(), ir_r.as<double>(), m,
-                         part.as<double>());
-      hipLaunchKernelGGL(k_fin_max2, dim3(1), dim3(kNT), 0, s, part.as<double>(), np, S.as<double>(), S_IRR, S_IRB);
+                         part.as<double>(), nullptr, nullptr);
+      hipLaunchKernelGGL(k_fin_max2, dim3(1), dim3(kNT), 0, s, part.as<double>(), np, S.as<double>(), S_IRR, S_IRB, 0, nullptr);
       HN_TRY(pull());
       if (dense_first_ok()) {
         *converged = true;
@@ -690,6 +784,34 @@ struct Solver {
       return HN_OK;
     }
     return solve_dense_rest(r_in, out, converged);
+  }
+
+  // The dense solve with its refinement on the device: the first solution
+  // and its check, then `sweeps` correction sweeps enqueued ahead, each
+  // doing nothing once a check has passed (ne_refined).  The same kernels in
+  // the same order as solve_dense, with no host round trip, so a whole
+  // iteration can be captured in a graph.  When the last check still fails
+  // and sweeps < kMaxSweeps (the host would run another sweep), F_MORE is set.
+  int solve_dense_dev(const double* r_in, double* out, int sweeps) {
+    const int np = parts_for(m);
+    double* Sd = S.as<double>();
+    int* more = flags.as<int>() + F_MORE;
+    HN_TRY(dense_apply(dense, r_in, out, false, s));
+    HN_TRY(apply_N(out, ir_nd.as<double>()));
+    hipLaunchKernelGGL(k_red_resid, dim3(np), dim3(kNT), 0, s, r_in, ir_nd.as<double>(), ir_r.as<double>(), m,
+                       part.as<double>(), nullptr, nullptr);
+    hipLaunchKernelGGL(k_fin_max2, dim3(1), dim3(kNT), 0, s, part.as<double>(), np, Sd, S_IRR, S_IRB, 0,
+                       sweeps == 0 ? more : nullptr);
+    for (int it = 1; it <= sweeps; ++it) {
+      HN_TRY(dense_apply_refine(dense, ir_r.as<double>(), out, Sd + S_IRR, Sd + S_IRB, Sd + S_SWEEPS, s));
+      HN_TRY(apply_N(out, ir_nd.as<double>()));
+      hipLaunchKernelGGL(k_red_resid, dim3(np), dim3(kNT), 0, s, r_in, ir_nd.as<double>(), ir_r.as<double>(), m,
+                         part.as<double>(), Sd + S_IRR, Sd + S_IRB);
+      hipLaunchKernelGGL(k_fin_max2, dim3(1), dim3(kNT), 0, s, part.as<double>(), np, Sd, S_IRR, S_IRB, 1,
+                         (it == sweeps && it < kMaxSweeps) ? more : nullptr);
+    }
+    count_launch(2 + 4 * sweeps);
+    return check_launch("dense solve (device refinement)");
   }
 
   // PCG on the matrix-free N with a Jacobi preconditioner; the relative
@@ -740,8 +862,8 @@ struct Solver {
       if (sweep > 0) {
         HN_TRY(apply_N(out, ir_nd.as<double>()));
         hipLaunchKernelGGL(k_red_resid, dim3(np), dim3(kNT), 0, s, r_in, ir_nd.as<double>(), ir_r.as<double>(), m,
-                           part.as<double>());
-        hipLaunchKernelGGL(k_fin_max2, dim3(1), dim3(kNT), 0, s, part.as<double>(), np, S.as<double>(), S_IRR, S_IRB);
+                           part.as<double>(), nullptr, nullptr);
+        hipLaunchKernelGGL(k_fin_max2, dim3(1), dim3(kNT), 0, s, part.as<double>(), np, S.as<double>(), S_IRR, S_IRB, 0, nullptr);
         HN_TRY(pull());
         r0 = hS[S_IRB];
         if (!(hS[S_IRR] > rel * r0)) break;
@@ -835,6 +957,7 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
   res.warm_started = 0;
   res.projected = 0;
   res.sparse_l_nnz = 0;
+  res.device_steps = res.host_redos = 0;
   auto finish = [&]() { res.solve_ms = 1e3 * clock.s(); };
 
   if (lp.n == 0) {  // interior_point.cpp:406-410
@@ -935,17 +1058,128 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
   int stuck = 0;
   auto deadline = [&]() { return o.time_limit_seconds > 0.0 && clock.s() >= o.time_limit_seconds; };
   // residuals, mu and their norms on the device; host copies in hS
-  auto residuals = [&]() -> int {
+  const dim3 g_nm = sv.g(std::max(n, m));
+  auto residual_kernels = [&]() -> int {  // :508-522
     HN_TRY(sv.spmv(sv.M, sv.x.as<double>(), sv.tm.as<double>()));
-    hipLaunchKernelGGL(k_sub, sv.g(m), B, 0, s, sv.tm.as<double>(), sv.b.as<double>(), sv.rp.as<double>(), m);
     HN_TRY(sv.spmv(sv.Mt, sv.lam.as<double>(), sv.tn.as<double>()));
-    hipLaunchKernelGGL(k_rd, sv.g(n), B, 0, s, sv.tn.as<double>(), sv.z.as<double>(), sv.c.as<double>(),
-                       sv.rd.as<double>(), n);
+    hipLaunchKernelGGL(k_resid, g_nm, B, 0, s, sv.tm.as<double>(), sv.b.as<double>(), sv.rp.as<double>(), m,
+                       sv.tn.as<double>(), sv.z.as<double>(), sv.c.as<double>(), sv.rd.as<double>(), n);
     hipLaunchKernelGGL(k_red_start, dim3(np_n), dim3(kNT), 0, s, sv.x.as<double>(), sv.z.as<double>(),
                        sv.rp.as<double>(), sv.rd.as<double>(), n, m, sv.part.as<double>());
     hipLaunchKernelGGL(k_fin_start, dim3(1), dim3(kNT), 0, s, sv.part.as<double>(), np_n, S);
-    HN_TRY(check_launch("residuals"));
+    count_launch(3);
+    return check_launch("residuals");
+  };
+  auto residuals = [&]() -> int {
+    HN_TRY(residual_kernels());
     return sv.pull();
+  };
+  // direction (:606-620 / :664-678) from dlam, and its ratio test (:629-631)
+  auto direction = [&](const double* zs, int slot_p, int slot_d) -> int {
+    HN_TRY(sv.spmv(sv.Mt, sv.dlam.as<double>(), sv.tn.as<double>()));
+    hipLaunchKernelGGL(k_direction, g_nm, B, 0, s, sv.D.as<double>(), sv.tn.as<double>(), sv.rd.as<double>(),
+                       sv.x.as<double>(), zs, sv.dx.as<double>(), sv.dz.as<double>(), n, sv.dlam.as<double>(), m, F);
+    hipLaunchKernelGGL(k_red_step, dim3(np_n), dim3(kNT), 0, s, sv.x.as<double>(), sv.dx.as<double>(),
+                       sv.z.as<double>(), sv.dz.as<double>(), n, sv.part.as<double>());
+    hipLaunchKernelGGL(k_fin_step, dim3(1), dim3(kNT), 0, s, sv.part.as<double>(), np_n, S, slot_p, slot_d);
+    count_launch(3);
+    return check_launch("direction");
+  };
+  auto update = [&]() -> int {  // :687-708; skipped on the device if the direction was not finite
+    hipLaunchKernelGGL(k_update, g_nm, B, 0, s, sv.x.as<double>(), sv.z.as<double>(), sv.dx.as<double>(),
+                       sv.dz.as<double>(), S, n, sv.lam.as<double>(), sv.dlam.as<double>(), m, F);
+    count_launch(1);
+    return check_launch("update");
+  };
+
+  // ---- device-driven iterations (dense normal equations in FP64) ----------
+  // Everything between two reads of the residuals runs on the device: the
+  // scaling, N and its factor, the predictor with its ratio test and mu_aff,
+  // sigma (formed by k_corrector_s), the corrector, both solves with their
+  // refinement (solve_dense_dev), the update, and the next point's
+  // residuals.  A predictor direction that is not finite sets the flag the
+  // update checks, so the step leaves the point unchanged and the host finds
+  // the flag with the residuals, as it finds the corrector's.  The step
+  // holds no refinement sweep until a solve of this LP has needed one (the
+  // FP64 factor rarely leaves one to do); a solve that needs more sweeps
+  // than the step holds sets F_MORE, and the host then puts back the point
+  // the step started from, runs the iteration on the host path, and has
+  // later steps hold all kMaxSweeps (guarded) sweeps.  So the iterates are
+  // those of the host path in every case.  With HN_LOOP_GRAPH the first such iteration runs directly (it
+  // sizes every workspace), the next is captured in a graph and later ones
+  // replay it: an iteration costs one graph launch and one read of the
+  // scalars.  The time limit is checked between iterations only (the host
+  // path also checks it after factoring N).
+  const bool use_graph = o.loop == HN_LOOP_GRAPH;
+  struct StepGraph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void reset() {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      if (g) (void)hipGraphDestroy(g);
+      exec = nullptr;
+      g = nullptr;
+    }
+    ~StepGraph() { reset(); }
+  } graph;
+  bool stepped = false;
+  int dev_sweeps = 0;
+  uint64_t step_launches = 0;
+  auto device_step_body = [&]() -> int {
+    hipLaunchKernelGGL(k_keep_point, g_nm, B, 0, s, sv.x.as<double>(), sv.z.as<double>(), sv.lam.as<double>(),
+                       sv.keep_x.as<double>(), sv.keep_z.as<double>(), sv.keep_lam.as<double>(), n, m);
+    hipLaunchKernelGGL(k_scaling, sv.g(n), B, 0, s, sv.x.as<double>(), sv.z.as<double>(), sv.rd.as<double>(),
+                       sv.D.as<double>(), sv.Drd.as<double>(), n);
+    count_launch(2);
+    HN_TRY(sv.prepare_ne());
+    // predictor (:584-643)
+    HN_TRY(sv.spmv(sv.M, sv.Drd.as<double>(), sv.mdrd.as<double>()));
+    hipLaunchKernelGGL(k_rhs, sv.g(m), B, 0, s, sv.rp.as<double>(), sv.mdrd.as<double>(), sv.tm.as<double>(),
+                       sv.rhs.as<double>(), m);
+    HN_TRY(sv.solve_dense_dev(sv.rhs.as<double>(), sv.dlam.as<double>(), dev_sweeps));
+    HN_TRY(direction(nullptr, S_AP_A, S_AD_A));
+    hipLaunchKernelGGL(k_red_muaff, dim3(np_n), dim3(kNT), 0, s, sv.x.as<double>(), sv.dx.as<double>(),
+                       sv.z.as<double>(), sv.dz.as<double>(), n, S, sv.part.as<double>());
+    hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(kNT), 0, s, sv.part.as<double>(), np_n, S, (int)S_MU_AFF);
+    // sigma and corrector (:638-685)
+    hipLaunchKernelGGL(k_corrector_s, sv.g(n), B, 0, s, sv.dx.as<double>(), sv.dz.as<double>(), sv.z.as<double>(),
+                       (const double*)S, sv.zs.as<double>(), n);
+    HN_TRY(sv.spmv(sv.M, sv.zs.as<double>(), sv.mz.as<double>()));
+    hipLaunchKernelGGL(k_sub, sv.g(m), B, 0, s, sv.rhs.as<double>(), sv.mz.as<double>(), sv.rhs2.as<double>(), m);
+    count_launch(5);
+    HN_TRY(sv.solve_dense_dev(sv.rhs2.as<double>(), sv.dlam.as<double>(), dev_sweeps));
+    HN_TRY(direction(sv.zs.as<double>(), S_AP, S_AD));
+    HN_TRY(update());
+    HN_TRY(check_launch("device step"));
+    return residual_kernels();
+  };
+  auto device_step = [&]() -> int {
+    if (graph.exec) {
+      HN_HIP(hipGraphLaunch(graph.exec, s));
+      count_launch(step_launches);
+      return HN_OK;
+    }
+    if (!stepped || !use_graph) {
+      if (!stepped) {
+        HN_TRY(sv.keep_x.reserve(sizeof(double) * std::max(n, 1)));
+        HN_TRY(sv.keep_z.reserve(sizeof(double) * std::max(n, 1)));
+        HN_TRY(sv.keep_lam.reserve(sizeof(double) * std::max(m, 1)));
+      }
+      stepped = true;
+      return device_step_body();
+    }
+    const uint64_t l0 = ctx().launches.load();
+    HN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int st = device_step_body();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (g) graph.g = g;
+    HN_TRY(st);
+    HN_HIP(e);
+    HN_HIP(hipGraphInstantiate(&graph.exec, graph.g, nullptr, nullptr, 0));
+    step_launches = ctx().launches.load() - l0;
+    HN_HIP(hipGraphLaunch(graph.exec, s));
+    return HN_OK;
   };
 
   int64_t iter = 0;
@@ -956,8 +1190,10 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
   // has when it stops inside that step.
   auto step_failed = [&]() { return sv.hF[F_DIR] != 0 || sv.hF[F_UPD] != 0; };
   double prev_mu = 0.0, prev_rp = 0.0, prev_rd = 0.0;
+  bool fresh = false;  // hS already holds this point's residuals (read after a device step)
   for (iter = 0; iter < o.max_iterations; ++iter) {
-    HN_TRY(residuals());
+    if (!fresh) HN_TRY(residuals());
+    fresh = false;
     if (iter > 0 && o.verbose)
       std::fprintf(stderr, "IPM it=%lld mu=%g rp=%g rd=%g ap=%g ad=%g%s\n", (long long)(iter - 1), prev_mu, prev_rp,
                    prev_rd, hS[S_AP], hS[S_AD], sv.fp64_solve ? "" : " [fp32 NE]");
@@ -1017,6 +1253,28 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
       sv.fp64_solve = true;
       if (res.fp64_from_iter < 0) res.fp64_from_iter = iter;
     }
+    if (o.loop != HN_LOOP_HOST && sv.use_dense && sv.fp64_solve) {
+      const double sweeps_before = hS[S_SWEEPS];
+      HN_TRY(device_step());
+      HN_TRY(sv.pull());
+      ++res.device_steps;
+      if (!sv.hF[F_MORE]) {
+        res.ne_solves += 2;
+        res.iterations = iter + 1;
+        fresh = true;
+        continue;
+      }
+      // a solve needed more refinement sweeps than the step holds: back to
+      // the point this iteration started from, and the host path below
+      ++res.host_redos;
+      dev_sweeps = kMaxSweeps;
+      graph.reset();  // the stream is idle: pull() synchronised it
+      hipLaunchKernelGGL(k_undo_step, g_nm, B, 0, s, sv.x.as<double>(), sv.z.as<double>(), sv.lam.as<double>(),
+                         sv.keep_x.as<double>(), sv.keep_z.as<double>(), sv.keep_lam.as<double>(), n, m, F, S,
+                         sweeps_before);
+      HN_TRY(check_launch("undo step"));
+      HN_TRY(residuals());
+    }
     hipLaunchKernelGGL(k_scaling, sv.g(n), B, 0, s, sv.x.as<double>(), sv.z.as<double>(), sv.rd.as<double>(),
                        sv.D.as<double>(), sv.Drd.as<double>(), n);
     HN_TRY(sv.prepare_ne());
@@ -1040,15 +1298,8 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
     else
       HN_TRY(sv.solve(sv.rhs.as<double>(), sv.dlam.as<double>(), mu, &esc));
     auto predictor = [&]() -> int {
-      HN_TRY(sv.spmv(sv.Mt, sv.dlam.as<double>(), sv.tn.as<double>()));
-      hipLaunchKernelGGL(k_direction, sv.g(n), B, 0, s, sv.D.as<double>(), sv.tn.as<double>(), sv.rd.as<double>(),
-                         sv.x.as<double>(), (const double*)nullptr, sv.dx.as<double>(), sv.dz.as<double>(), n, F);
-      hipLaunchKernelGGL(k_finite, sv.g(m), B, 0, s, sv.dlam.as<double>(), m, F);
-      // ratio test and mu_aff (:629-643)
-      hipLaunchKernelGGL(k_red_step, dim3(np_n), dim3(kNT), 0, s, sv.x.as<double>(), sv.dx.as<double>(),
-                         sv.z.as<double>(), sv.dz.as<double>(), n, sv.part.as<double>());
-      hipLaunchKernelGGL(k_fin_step, dim3(1), dim3(kNT), 0, s, sv.part.as<double>(), np_n, S, (int)S_AP_A,
-                         (int)S_AD_A);
+      // direction, ratio test and mu_aff (:606-643)
+      HN_TRY(direction(nullptr, S_AP_A, S_AD_A));
       hipLaunchKernelGGL(k_red_muaff, dim3(np_n), dim3(kNT), 0, s, sv.x.as<double>(), sv.dx.as<double>(),
                          sv.z.as<double>(), sv.dz.as<double>(), n, S, sv.part.as<double>());
       hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(kNT), 0, s, sv.part.as<double>(), np_n, S, (int)S_MU_AFF);
@@ -1060,7 +1311,7 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
       bool redo = false;
       HN_TRY(sv.solve_settle(sv.rhs.as<double>(), sv.dlam.as<double>(), &esc, &redo));
       if (redo) {  // the direction flag of the unrefined solution does not count
-        HN_HIP(hipMemsetAsync(F + F_DIR, 0, sizeof(int), s));
+        hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, s, F, (int)F_DIR);
         HN_TRY(predictor());
       }
     }
@@ -1082,25 +1333,14 @@ int ipm_solve_impl(const hn_lp& lp, const hn_ipm_opts& o, hn_ipm_result& res) {
     hipLaunchKernelGGL(k_sub, sv.g(m), B, 0, s, sv.rhs.as<double>(), sv.mz.as<double>(), sv.rhs2.as<double>(), m);
     HN_TRY(sv.solve(sv.rhs2.as<double>(), sv.dlam.as<double>(), mu, &esc));
     if (esc && res.fp64_from_iter < 0) res.fp64_from_iter = iter;
-    HN_TRY(sv.spmv(sv.Mt, sv.dlam.as<double>(), sv.tn.as<double>()));
-    hipLaunchKernelGGL(k_direction, sv.g(n), B, 0, s, sv.D.as<double>(), sv.tn.as<double>(), sv.rd.as<double>(),
-                       sv.x.as<double>(), sv.zs.as<double>(), sv.dx.as<double>(), sv.dz.as<double>(), n, F);
-    hipLaunchKernelGGL(k_finite, sv.g(m), B, 0, s, sv.dlam.as<double>(), m, F);
-    hipLaunchKernelGGL(k_red_step, dim3(np_n), dim3(kNT), 0, s, sv.x.as<double>(), sv.dx.as<double>(),
-                       sv.z.as<double>(), sv.dz.as<double>(), n, sv.part.as<double>());
-    hipLaunchKernelGGL(k_fin_step, dim3(1), dim3(kNT), 0, s, sv.part.as<double>(), np_n, S, (int)S_AP, (int)S_AD);
-    HN_TRY(check_launch("corrector"));
-    // update (:687-708); skipped on the device if the corrector's direction
-    // was not finite
-    hipLaunchKernelGGL(k_update, sv.g(n), B, 0, s, sv.x.as<double>(), sv.z.as<double>(), sv.dx.as<double>(),
-                       sv.dz.as<double>(), S, n, F);
-    hipLaunchKernelGGL(k_update_lam, sv.g(m), B, 0, s, sv.lam.as<double>(), sv.dlam.as<double>(), S, m, F);
-    HN_TRY(check_launch("update"));
+    HN_TRY(direction(sv.zs.as<double>(), S_AP, S_AD));
+    HN_TRY(update());
     res.iterations = iter + 1;
   }
 
+  res.refine_sweeps += (int64_t)hS[S_SWEEPS];
   if (!done && !interrupted && !stopped) {  // :728-753
-    HN_TRY(residuals());
+    if (!fresh) HN_TRY(residuals());
     if (iter == o.max_iterations) {  // the last step's flags
       if (iter > 0 && o.verbose)
         std::fprintf(stderr, "IPM it=%lld mu=%g rp=%g rd=%g ap=%g ad=%g%s\n", (long long)(iter - 1), prev_mu,
@@ -1193,6 +1433,7 @@ extern "C" int hn_ipm_solve(const hn_lp* lp, const hn_ipm_opts* opts, hn_ipm_res
   HN_CHECK(o.max_iterations >= 0, HN_ERR_INVALID, "hn_ipm_solve: negative max_iterations");
   HN_CHECK(o.pivot_rule == HN_PIVOT_CLAMP || o.pivot_rule == HN_PIVOT_SKIP, HN_ERR_INVALID,
            "hn_ipm_solve: bad pivot_rule");
+  HN_CHECK(o.loop >= HN_LOOP_GRAPH && o.loop <= HN_LOOP_HOST, HN_ERR_INVALID, "hn_ipm_solve: bad loop");
   return hn::ipm_solve_impl(*lp, o, *res);
 }
 

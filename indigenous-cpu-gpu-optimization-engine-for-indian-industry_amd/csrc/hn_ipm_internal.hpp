@@ -45,6 +45,11 @@ int dense_reserve(DenseNE& d, int m, bool f32);
 int dense_factor(DenseNE& d, int pivot_rule, hipStream_t s);
 // y = N^{-1} r (accumulate: y += N^{-1} r) as G' (D^{-1} (G r)).
 int dense_apply(const DenseNE& d, const double* r, double* y, bool accumulate, hipStream_t s);
+// One refinement correction y += N^{-1} r that does nothing when the last
+// check (*irr = ||rhs - N y||_inf, *irb = ||rhs||_inf, device scalars) has
+// already met the refinement test; *sweeps counts the corrections that ran.
+int dense_apply_refine(const DenseNE& d, const double* r, double* y, const double* irr, const double* irb,
+                       double* sweeps, hipStream_t s);
 int ne_jacobi_diag(const hn_csr* M, const double* D, double* diag, hipStream_t s);
 
 }  // namespace hn

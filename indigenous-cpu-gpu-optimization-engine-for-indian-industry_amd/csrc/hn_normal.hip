@@ -614,6 +614,10 @@ __global__ __launch_bounds__(256) void k_ldl_update(T* __restrict__ A, T* __rest
   }
 }
 
+__global__ void k_zero_int(int* p) {
+  if (threadIdx.x == 0) *p = 0;
+}
+
 template <typename T>
 static int dense_invert_typed(DenseNE& d, int m, double tol, double skip_rel, hipStream_t s) {
   T* A = d.A.as<T>();
@@ -657,7 +661,8 @@ int dense_reserve(DenseNE& d, int m, bool f32) {
 // and d.dinv the inverse pivots.
 int dense_factor(DenseNE& d, int pivot_rule, hipStream_t s) {
   const double tol = 1e-10;  // singular_tol of ToleranceConfig::industrial_defaults
-  HN_HIP(hipMemsetAsync(d.clamped.p, 0, sizeof(int), s));
+  // a kernel, not a memset: this factorisation is captured in graphs
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, d.clamped.as<int>());
   const bool skip = pivot_rule == HN_PIVOT_SKIP;
   return d.f32 ? dense_invert_typed<float>(d, d.m, tol, skip ? kSkipRel32 : 0.0, s)
                : dense_invert_typed<double>(d, d.m, tol, skip ? kSkipRel64 : 0.0, s);
@@ -668,12 +673,24 @@ int dense_factor(DenseNE& d, int pivot_rule, hipStream_t s) {
 // to 1e-10 scales its own component only, as in the reference's triangular
 // solves (an explicit inverse would spread its 1e10 through every entry).
 //
+// Refinement guard: with irr/irb (||rhs - N x||_inf and ||rhs||_inf of the
+// last check) given, a correction sweep does nothing once the solution has
+// met the refinement test (refinement.cpp:7-36: rel <= 1e-10, or a zero or
+// NaN right-hand side), so a fixed number of sweeps can be enqueued ahead.
+__device__ __forceinline__ bool ne_refined(const double* irr, const double* irb) {
+  if (!irr) return false;
+  const double nb = *irb;
+  return nb == 0.0 || nb != nb || *irr / nb <= 1e-10;
+}
+
 // (1) u = D^{-1} (G r): a wave per row k, over the columns up to the end of
 // k's 64-row block (G is lower block-triangular; the upper part of each
 // diagonal block is zero).
 template <typename T>
 __global__ __launch_bounds__(256) void k_gapply_lower(const T* __restrict__ G, const T* __restrict__ dinv, int m,
-                                                      const double* __restrict__ r, double* __restrict__ u) {
+                                                      const double* __restrict__ r, double* __restrict__ u,
+                                                      const double* irr, const double* irb) {
+  if (ne_refined(irr, irb)) return;
   const int row = (blockIdx.x * 256 + threadIdx.x) / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   if (row >= m) return;
@@ -689,7 +706,9 @@ __global__ __launch_bounds__(256) void k_gapply_lower(const T* __restrict__ G, c
 // columns i below the block's end (the rest of those rows is zero).
 template <typename T>
 __global__ __launch_bounds__(256) void k_gapply_upper_part(const T* __restrict__ G, int m,
-                                                           const double* __restrict__ u, double* __restrict__ part) {
+                                                           const double* __restrict__ u, double* __restrict__ part,
+                                                           const double* irr, const double* irb) {
+  if (ne_refined(irr, irb)) return;
   const int K = blockIdx.y;
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int kb = K * kNB, ke = min(m, kb + kNB);
@@ -702,15 +721,18 @@ __global__ __launch_bounds__(256) void k_gapply_upper_part(const T* __restrict__
 // (3) y_i (+)= sum over the blocks K from i's own up of part[K][i], in block
 // order (a fixed order: the result is deterministic).
 __global__ void k_gapply_sum(const double* __restrict__ part, int m, int nb, double* __restrict__ y,
-                             int accumulate) {
+                             int accumulate, const double* irr, const double* irb, double* sweeps) {
+  if (ne_refined(irr, irb)) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sweeps && i == 0) *sweeps += 1.0;  // a correction sweep that ran
   if (i >= m) return;
   double s = 0.0;
   for (int K = i / kNB; K < nb; ++K) s += part[(long long)K * m + i];
   y[i] = accumulate ? y[i] + s : s;
 }
 
-int dense_apply(const DenseNE& d, const double* r, double* y, bool accumulate, hipStream_t s) {
+static int dense_apply_impl(const DenseNE& d, const double* r, double* y, bool accumulate, const double* irr,
+                            const double* irb, double* sweeps, hipStream_t s) {
   const int m = d.m;
   const int nb = (m + kNB - 1) / kNB;
   const long long threads = (long long)m * kWave;
@@ -718,19 +740,28 @@ int dense_apply(const DenseNE& d, const double* r, double* y, bool accumulate, h
   const dim3 gp((unsigned)((m + 255) / 256), (unsigned)nb);
   if (d.f32) {
     hipLaunchKernelGGL((k_gapply_lower<float>), g, dim3(256), 0, s, d.G.as<float>(), d.dinv.as<float>(), m, r,
-                       d.u.as<double>());
+                       d.u.as<double>(), irr, irb);
     hipLaunchKernelGGL((k_gapply_upper_part<float>), gp, dim3(256), 0, s, d.G.as<float>(), m, d.u.as<double>(),
-                       d.part.as<double>());
+                       d.part.as<double>(), irr, irb);
   } else {
     hipLaunchKernelGGL((k_gapply_lower<double>), g, dim3(256), 0, s, d.G.as<double>(), d.dinv.as<double>(), m, r,
-                       d.u.as<double>());
+                       d.u.as<double>(), irr, irb);
     hipLaunchKernelGGL((k_gapply_upper_part<double>), gp, dim3(256), 0, s, d.G.as<double>(), m, d.u.as<double>(),
-                       d.part.as<double>());
+                       d.part.as<double>(), irr, irb);
   }
   hipLaunchKernelGGL(k_gapply_sum, dim3(grid_for(m, 256)), dim3(256), 0, s, d.part.as<double>(), m, nb, y,
-                     (int)accumulate);
+                     (int)accumulate, irr, irb, sweeps);
   count_launch(3);
   return check_launch("dense apply");
+}
+
+int dense_apply(const DenseNE& d, const double* r, double* y, bool accumulate, hipStream_t s) {
+  return dense_apply_impl(d, r, y, accumulate, nullptr, nullptr, nullptr, s);
+}
+
+int dense_apply_refine(const DenseNE& d, const double* r, double* y, const double* irr, const double* irb,
+                       double* sweeps, hipStream_t s) {
+  return dense_apply_impl(d, r, y, true, irr, irb, sweeps, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -538,6 +538,7 @@ def gpu_visible() -> bool:
 HN_NE_AUTO, HN_NE_DENSE, HN_NE_PCG, HN_NE_SPARSE = 0, 1, 2, 3
 HN_PREC_F64, HN_PREC_MIXED = 1, 2
 HN_PIVOT_CLAMP, HN_PIVOT_SKIP = 0, 1
+HN_LOOP_GRAPH, HN_LOOP_DEVICE, HN_LOOP_HOST = 0, 1, 2
 STATUS = ["UNKNOWN", "OPTIMAL", "INFEASIBLE", "UNBOUNDED", "SUBOPTIMAL", "TIME_LIMIT",
           "ITER_LIMIT", "NUMERICAL_ERROR", "INTERRUPTED"]
 
@@ -559,7 +560,8 @@ class _HnOpts(C.Structure):
                 ("interrupt_user", C.c_void_p), ("solver", C.c_int), ("precision", C.c_int),
                 ("mu_switch", C.c_double), ("dense_max_m", C.c_int64), ("pcg_max_iter", C.c_int),
                 ("x0", _dp), ("y0", _dp), ("s0", _dp), ("verbose", C.c_int), ("pivot_rule", C.c_int),
-                ("mu_target", C.c_double), ("gap_tol", C.c_double), ("endgame_mu", C.c_double)]
+                ("mu_target", C.c_double), ("gap_tol", C.c_double), ("endgame_mu", C.c_double),
+                ("loop", C.c_int)]
 
 
 class _HnResult(C.Structure):
@@ -571,7 +573,7 @@ class _HnResult(C.Structure):
                 ("scaled_rel_primal", C.c_double), ("scaled_rel_dual", C.c_double),
                 ("scaled_rel_gap", C.c_double), ("ne_solves", C.c_int64), ("refine_sweeps", C.c_int64), ("cg_iterations", C.c_int64),
                 ("fp64_from_iter", C.c_int64), ("warm_started", C.c_int), ("projected", C.c_int),
-                ("sparse_l_nnz", C.c_int64)]
+                ("sparse_l_nnz", C.c_int64), ("device_steps", C.c_int64), ("host_redos", C.c_int64)]
 
 
 LIB.hn_ipm_default_opts.argtypes = [C.POINTER(_HnOpts)]
@@ -602,6 +604,7 @@ class InteriorPointOptions:
         self.pivot_rule = HN_PIVOT_CLAMP  # HN_PIVOT_SKIP: Wright's skipped pivots, r = regularization
         self.mu_target = 0.0            # > 0: return the first iterate with mu <= mu_target
         self.gap_tol = 0.0              # > 0: also stop only once the caller's LP has rel_gap <= gap_tol
+        self.loop = HN_LOOP_GRAPH       # HN_LOOP_DEVICE / HN_LOOP_HOST: see hn_ipm_opts.loop
         self.verbose = False
         for k, v in kw.items():
             if not hasattr(self, k):
@@ -627,6 +630,7 @@ class InteriorPointResult:
         self.warm_started = bool(r.warm_started)
         self.projected = bool(r.projected)
         self.sparse_l_nnz = int(r.sparse_l_nnz)
+        self.device_steps, self.host_redos = int(r.device_steps), int(r.host_redos)
 
     def __repr__(self):
         return (f"InteriorPointResult({self.status}, obj={self.objective_value!r}, it={self.iterations}, "
@@ -678,6 +682,7 @@ class InteriorPointSolver:
         opts.pivot_rule = o.pivot_rule
         opts.mu_target = o.mu_target
         opts.gap_tol = o.gap_tol
+        opts.loop = o.loop
         if o.warm_start is not None:
             ws = [np.ascontiguousarray(a, np.float64) for a in o.warm_start]
             keep["ws"] = ws
