@@ -330,126 +330,148 @@ __device__ __forceinline__ float read_lane(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// (1) LDL' of the diagonal block A[k0:k0+b, k0:k0+b] by one wave, the block
-// in registers (lane j owns column j): step k forms the multipliers l_i of
-// column k (lane i its own) and lane j > k applies a_ij -= l_i a_kj down its
-// column.  Writes the strictly lower part of L, dinv[k0:k0+b] (1/pivot;
-// 0 for a skipped pivot) and X = L_KK^{-1} as the full 64 x 64 diagonal
-// block of G = L^{-1} (zeros above).  Pivots p <= tol are clamped to tol
-// (skip_rel == 0) or, with skip_rel > 0, p <= skip_rel * diag0 is skipped.
+// (1) LDL' of the diagonal block A[k0:k0+b, k0:k0+b] by four waves (a
+// 64 x 64 block, rows past b the identity).  Writes the strictly lower part
+// of L, dinv[k0:k0+b] (1/pivot; 0 for a skipped pivot) and X = L_KK^{-1} as
+// the full 64 x 64 diagonal block of G = L^{-1} (zeros above).  Pivots
+// p <= tol are clamped to tol (skip_rel == 0) or, with skip_rel > 0,
+// p <= skip_rel * diag0 is skipped.
+//
+// Right-looking: step k forms d_k, the multipliers l_j = A(k, j) / d_k of
+// column k (the trailing block stays symmetric, so lane j's own entry of
+// row k gives its multiplier) and updates a_ij -= l_i a_kj below row k.
+// Wave w keeps rows 16w..16w+15 of the block in registers (lane j: column
+// j).  The 16 steps of block kb change only rows >= 16kb, and wave kb's rows
+// only through its own steps, so wave kb runs them alone: the next pivot
+// and its reciprocal first (from values known before the step's
+// reciprocal: lane k+1's multiplier and diagonal entry), then its other
+// rows.  It then publishes the 16 multiplier vectors and pivot rows in LDS,
+// and the waves below apply the 16 steps to their rows, each row in step
+// order.  Every entry receives the same multiply-adds in the same order as
+// in a one-step-at-a-time sweep, so the factor does not depend on the
+// split.  The pivot steps form one dependent chain (pivot, reciprocal,
+// multiplier, next pivot); the four-wave split keeps 48 rows' updates off
+// it.
 template <typename T>
-__global__ __launch_bounds__(kWave) void k_ldl_diag(T* __restrict__ A, int m, int k0, int b, T* __restrict__ dinv,
-                                                    T* __restrict__ G, double tol, const double* __restrict__ diag0,
-                                                    double skip_rel, int* __restrict__ clamped) {
+__global__ __launch_bounds__(256) void k_ldl_diag(T* __restrict__ A, int m, int k0, int b, T* __restrict__ dinv,
+                                                  T* __restrict__ G, double tol, const double* __restrict__ diag0,
+                                                  double skip_rel, int* __restrict__ clamped) {
   __shared__ T S[kNB][kNB + 1];
   __shared__ T Xs[kNB][kNB + 1];
   __shared__ T Ts[3 * 256];
-  __shared__ T Lk[2][2 * kNB + 16];  // the multipliers of the step, by parity; zeros past row 63
-  __shared__ T Dk[kNB];              // 1 / pivot of each step
-  const int j = threadIdx.x;
-  // The block's lower triangle, row by row (lane j reads column j of row i:
-  // coalesced), mirrored into the upper half; rows past b are the identity.
-  // The loads carry no branch (addresses are clamped into the block's lower
-  // triangle, unused values dropped), so 16 rows are in flight at once.
-  for (int i0 = 0; i0 < kNB; i0 += 16) {
+  __shared__ T Lk[2][16][kNB];  // multipliers of a block's 16 steps, by block parity
+  __shared__ T Ak[2][16][kNB];  // the 16 pivot rows A(k, j), by block parity
+  __shared__ T Dk[kNB];     // 1 / pivot of each step
+  const int tid = threadIdx.x;
+  const int j = tid & (kWave - 1);
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
+  {
     T v[16];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int ii = min(i0 + u, b - 1);
+    for (int r = 0; r < 16; ++r) {
+      const int ii = min(16 * w + r, b - 1);
       const int jj = min(j, ii);
-      v[u] = A[(long long)(k0 + ii) * m + k0 + jj];
+      v[r] = A[(long long)(k0 + ii) * m + k0 + jj];
     }
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int i = i0 + u;
+    for (int r = 0; r < 16; ++r) {
+      const int i = 16 * w + r;
       if (j <= i) {
-        const T w = (i < b) ? v[u] : T(i == j ? 1 : 0);
-        S[i][j] = w;
-        S[j][i] = w;
+        const T x = (i < b) ? v[r] : T(i == j ? 1 : 0);
+        S[i][j] = x;
+        S[j][i] = x;
       }
     }
   }
   __syncthreads();
-  // Elimination in registers, one step per trip of a rolled loop (unrolled,
-  // the 64 steps were ~80 KB of straight-line code).  Lane j holds what is
-  // left of column j shifted so that c[0] is the entry in the current pivot
-  // row: at step k, c[t] = A(k + t, j).  A step writes the updated column one
-  // place down (c[t-1] = c[t] - l_{k+t} a_kj), so every register index is
-  // static.  The trailing block stays symmetric, so lane i's own c[0] =
-  // A(k, i) = A(i, k) gives it its multiplier l_i = A(i, k) / d_k; the
-  // multipliers reach the other lanes through an LDS vector
-  // (double-buffered by step parity, zeros past row 63) and are kept as
-  // column k of L in S.  The steps form one dependent chain (pivot, its
-  // reciprocal, multipliers, LDS, next pivot row), so each step updates row
-  // k+1 first and takes the next pivot and its reciprocal from it before
-  // the other 62 rows: the division overlaps their multiply-adds.
-  T c[kNB];
+  T c[16];  // c[r] = A(16 w + r, j)
 #pragma unroll
-  for (int i = 0; i < kNB; ++i) c[i] = S[i][j];
-  Lk[0][kNB + j] = T(0);
-  Lk[1][kNB + j] = T(0);
-  if (j < 16) {
-    Lk[0][2 * kNB + j] = T(0);
-    Lk[1][2 * kNB + j] = T(0);
-  }
+  for (int r = 0; r < 16; ++r) c[r] = S[16 * w + r][j];
   const double d0j = (skip_rel > 0.0 && j < b) ? diag0[k0 + j] : 1.0;
   int ncl = 0;
-  // 1 / pivot of step k (lane k's c[0]) under the pivot rule; a clamped or
-  // skipped pivot is counted when the step is one of the block's (live).
-  // Selects, not branches: the step stays one basic block, so the
-  // scheduler can overlap the division with the multiply-adds.
-  auto pivot_inverse = [&](T cv, int k, bool live) -> T {
-    const T piv = read_lane(cv, k);
+  // 1 / pivot under the pivot rule, from the pivot's value (wave-uniform)
+  auto pivot_inverse_of = [&](T piv, int k, bool live) -> T {
     const bool skipped = skip_rel > 0.0 && !((double)piv > skip_rel * read_lane(d0j, k));
     const bool clamped = !(skip_rel > 0.0) && piv <= T(tol);
     const T r = T(1) / (clamped ? T(tol) : piv);
     ncl += (live && (skipped || clamped)) ? 1 : 0;
     return skipped ? T(0) : r;
   };
-  T rinv = pivot_inverse(c[0], 0, true);
-  __syncthreads();  // every lane has read its column of S before S takes L
-  for (int k = 0; k < b; ++k) {
-    const int par = k & 1;
-    const T akj = c[0];  // A(k, j)
-    Dk[k] = rinv;  // every lane stores the same value
-    const T l = (j > k) ? akj * rinv : T(0);
-    Lk[par][j] = l;
-    S[j][k] = l;  // row j of L
-    __syncthreads();
-    const T a = (j > k) ? akj : T(0);
-    const T* lv = &Lk[par][k];
-    c[0] = c[1] - lv[1] * a;  // row k + 1: the next pivot row
-    const T rnext = pivot_inverse(c[0], min(k + 1, kNB - 1), k + 1 < b);
-    // the other rows in four groups of 16, each group's multipliers read
-    // from LDS while the previous group's multiply-adds run
-    T lg[2][16];
+  auto pivot_inverse = [&](T cv, int k, bool live) -> T { return pivot_inverse_of(read_lane(cv, k), k, live); };
+
+  T rinv = T(0);
+  if (w == 0) rinv = pivot_inverse(c[0], 0, true);
+  __syncthreads();  // every row of S is in registers before S takes L
+  for (int kb = 0; kb < 4; ++kb) {
+    if (16 * kb >= b) break;
+    const int buf = kb & 1;
+    if (w == kb) {
 #pragma unroll
-    for (int u = 0; u < 16; ++u) lg[0][u] = lv[2 + u];
+      for (int kk = 0; kk < 16; ++kk) {
+        const int k = 16 * kb + kk;
+        if (k >= b) continue;  // (not break: the loop must unroll fully)
+        const T akj = c[kk];  // A(k, j)
+        Dk[k] = rinv;  // every lane stores the same value
+        const T l = (j > k) ? akj * rinv : T(0);
+        Lk[buf][kk][j] = l;
+        Ak[buf][kk][j] = akj;
+        const T a = (j > k) ? akj : T(0);
+        if (kk < 15) {
+          // the next pivot from values known before this step's reciprocal:
+          // row k+1's multiplier A(k, k+1) / d_k and its diagonal entry
+          const T ak1 = read_lane(akj, k + 1);
+          const T lk1 = ak1 * rinv;  // lane k+1's l
+          const T piv = read_lane(c[kk + 1], k + 1) - lk1 * ak1;
+          c[kk + 1] = c[kk + 1] - lk1 * a;
+          rinv = pivot_inverse_of(piv, k + 1, k + 1 < b);
+        }
 #pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      if (g < 3) {
-#pragma unroll
-        for (int u = 0; u < 16; ++u) lg[(g + 1) & 1][u] = lv[18 + 16 * g + u];
-      }
-#pragma unroll
-      for (int u = 0; u < 16; ++u) {
-        const int t = 2 + 16 * g + u;
-        if (t < kNB) c[t - 1] = c[t] - lg[g & 1][u] * a;
+        for (int r = kk + 2; r < 16; ++r) c[r] = c[r] - Lk[buf][kk][16 * kb + r] * a;
       }
     }
-    c[kNB - 1] = T(0);
-    rinv = rnext;
+    __syncthreads();
+    const int ke = min(16, b - 16 * kb);
+    // columns 16kb.. of L into S (row j of L), by all waves
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int kk = 4 * w + q;
+      if (kk < ke) S[j][16 * kb + kk] = Lk[buf][kk][j];
+    }
+    if (w > kb) {
+      // the block's steps on this wave's rows, each row in step order (a
+      // full block unrolled, row 0 first: for wave kb+1 it is the next
+      // pivot row)
+      if (ke == 16) {
+        T a[16];
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) a[kk] = (j > 16 * kb + kk) ? Ak[buf][kk][j] : T(0);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          T x = c[r];
+#pragma unroll
+          for (int kk = 0; kk < 16; ++kk) x = x - Lk[buf][kk][16 * w + r] * a[kk];
+          c[r] = x;
+        }
+      } else {
+        for (int kk = 0; kk < ke; ++kk) {
+          const T a = (j > 16 * kb + kk) ? Ak[buf][kk][j] : T(0);
+#pragma unroll
+          for (int r = 0; r < 16; ++r) c[r] = c[r] - Lk[buf][kk][16 * w + r] * a;
+        }
+      }
+      if (w == kb + 1) rinv = pivot_inverse(c[0], 16 * w, 16 * w < b);
+    }
   }
   __syncthreads();
-  if (j < b) dinv[k0 + j] = Dk[j];
-  for (int i = 1; i < b; ++i)  // L below the diagonal, row by row (coalesced)
+  if (tid < b) dinv[k0 + tid] = Dk[tid];
+  for (int i = 1 + w; i < b; i += 4)  // L below the diagonal, row by row (coalesced)
     if (j < i) A[(long long)(k0 + i) * m + k0 + j] = S[i][j];
   if (j == 0 && ncl) atomicAdd(clamped, ncl);
   // X = L_KK^{-1} by 16 x 16 blocks: the four diagonal blocks by forward
-  // substitution (a column per lane, in registers), then block rows
-  // I = 1..3: X_IJ = -X_II sum_{J<=K<I} L_IK X_KJ.  Only the strictly lower
-  // part of S (= L) is read; entries of X above the diagonal are zero.
-  {
+  // substitution (a column per lane of wave 0), then block rows I = 1..3:
+  // X_IJ = -X_II sum_{J<=K<I} L_IK X_KJ, the entries spread over the four
+  // waves.  Only the strictly lower part of S (= L) is read.
+  if (w == 0) {
     const int o = 16 * (j >> 4), cc = j & 15;
     T x[16];
 #pragma unroll
@@ -463,27 +485,21 @@ __global__ __launch_bounds__(kWave) void k_ldl_diag(T* __restrict__ A, int m, in
     for (int i = 0; i < 16; ++i) Xs[o + i][o + cc] = x[i];
   }
   __syncthreads();
-  // every loop below has a static trip count and is unrolled, so a
-  // thread's 4I independent sums are in flight together instead of one
-  // dependent chain of LDS reads and multiply-adds after another
+  const int r = (tid >> 4) & 15, cc = tid & 15;
 #pragma unroll
   for (int I = 1; I < 4; ++I) {
 #pragma unroll
-    for (int q = 0; q < 4 * I; ++q) {
-      const int e = j + kWave * q;
-      const int J = q >> 2, r = (e >> 4) & 15, cc = e & 15;
+    for (int J = 0; J < I; ++J) {
       T sx = T(0);
 #pragma unroll
       for (int K = J; K < I; ++K)
 #pragma unroll
         for (int k = 0; k < 16; ++k) sx += S[16 * I + r][16 * K + k] * Xs[16 * K + k][16 * J + cc];
-      Ts[e] = sx;
+      Ts[(J << 8) + tid] = sx;
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 4 * I; ++q) {
-      const int e = j + kWave * q;
-      const int J = q >> 2, r = (e >> 4) & 15, cc = e & 15;
+    for (int J = 0; J < I; ++J) {
       T sx = T(0);
 #pragma unroll
       for (int k = 0; k < 16; ++k) sx += Xs[16 * I + r][16 * I + k] * Ts[(J << 8) + (k << 4) + cc];
@@ -491,7 +507,7 @@ __global__ __launch_bounds__(kWave) void k_ldl_diag(T* __restrict__ A, int m, in
     }
     __syncthreads();
   }
-  for (int i = 0; i < b; ++i)
+  for (int i = w; i < b; i += 4)
     if (j < b) G[(long long)(k0 + i) * m + k0 + j] = (j <= i) ? Xs[i][j] : T(0);
 }
 
@@ -629,7 +645,7 @@ static int dense_invert_typed(DenseNE& d, int m, double tol, double skip_rel, hi
   for (int K = 0; K < nb; ++K) {
     const int k0 = K * kNB;
     const int b = std::min(kNB, m - k0);
-    hipLaunchKernelGGL((k_ldl_diag<T>), dim3(1), dim3(kWave), 0, s, A, m, k0, b, dinv, G, tol, d.diag0.as<double>(),
+    hipLaunchKernelGGL((k_ldl_diag<T>), dim3(1), dim3(256), 0, s, A, m, k0, b, dinv, G, tol, d.diag0.as<double>(),
                        skip_rel, d.clamped.as<int>());
     const int rest = m - k0 - b;
     const int nt = (rest + kNB - 1) / kNB;
