@@ -748,94 +748,138 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
       mu = wg_sum(s, red) / (double)n;
     };
     // Optimal-face projection (not in the reference; opt-in by endgame_mu).
-    // B = {j : x_j >= z_j} estimates the support of the optimal primal face.
-    // With D_B = x_B (0 off B) and N_B = M D_B M' + r I factored once:
+    // Primal: over B = {j : x_j >= theta z_j} and with D_B = x_B (0 off B),
+    // N_B = M D_B M' + r I:
     //   x' = x_B + D_B M' w,   N_B w = b - M x_B        (x'_N = 0, M x' = b)
-    //   y' = lam + d,           N_B d = M D_B (c - M'lam)   (weighted least
-    //        squares for z'_B = c_B - M_B'y' = 0)
+    // tried with theta = 1, then 0.1, then 0.01 until x' is primal feasible
+    // to the tolerance: on a degenerate LP a basic column can still have
+    // x_j < z_j when the iterates stall (a pair that has not separated), and
+    // without it M_B cannot reach b.  The supports are nested, so a theta
+    // that adds no column is skipped.
+    // Dual: a weighted least squares for z'_B = 0,
+    //   y' = lam + d,   (M W M' + r I) d = M W (c - M'lam)
+    // first with the IPM's own scaling W = x/z over {j : x_j >= z_j}, in
+    // which the strongly basic columns (x/z ~ 1e5 and up) outweigh a weakly
+    // basic one that would otherwise leave its share of the misfit as
+    // negative reduced costs on the others; W is kept unclamped (up to
+    // 1e200), since capping it at the IPM's 1e10 flattens the strongly basic
+    // columns and brings those negative reduced costs back.  If z' is not
+    // dual feasible, W = x over the same support, then W = x/z over
+    // {x_j >= 100 z_j}.
     //   z' = c - M'y'
     // x' and z' are clamped at 0 (each by at most conv_tol) and the point is
     // accepted when it meets the reference's stop test (||M x' - b||,
     // ||rd|| = ||max(-z', 0)|| and mu below their tolerances) and, with
     // gap_tol, the relative gap; the solve then ends there.  Otherwise the
-    // iterates are left unchanged.
+    // iterates are left unchanged.  N_B is singular when M_B has fewer than
+    // m independent columns; both right-hand sides lie in its range, and
+    // the reference's regularisation (clamped pivots, r >= 1e-12 max diag)
+    // keeps the solutions' null-space parts at rounding size - a skipped
+    // pivot would instead fix them at an arbitrary value.
     auto project = [&]() -> bool {
-      for (int j = tid; j < n; j += NT) {
-        D[j] = (x[j] >= z[j]) ? x[j] : 0.0;
-        t[j] = (x[j] >= z[j]) ? x[j] : 0.0;
-      }
-      __syncthreads();
-      // N_B is singular when M_B has fewer than m independent columns
-      // (degenerate LPs); both right-hand sides lie in its range, and the
-      // reference's regularisation (clamped pivots, r >= 1e-12 max diag)
-      // keeps the solutions' null-space parts at rounding size - a skipped
-      // pivot would instead fix them at an arbitrary value
-      double reg = 0.0;
-      wg_factor<double>(c, A64, W64, sh64, red, prm.reg, false, &reg);
-      wg_spmv(c.mrp, c.mci, c.mv, t, tm, m);
-      __syncthreads();
-      for (int i = tid; i < m; i += NT) rhs[i] = b[i] - tm[i];
-      __syncthreads();
-      (void)wg_solve<double>(c, A64, rhs, dlam, reg, red, &refine);
-      wg_spmv(c.trp, c.tci, c.tv, dlam, t, n);
-      __syncthreads();
-      // x'_B below zero by no more than the tolerance (degenerate basic
-      // components) is clamped to 0; the residual below is that of the
-      // clamped point
-      double bad = 0.0, xneg = 0.0;
-      for (int j = tid; j < n; j += NT) {
-        const double v = (D[j] > 0.0) ? x[j] + D[j] * t[j] : 0.0;
-        dx[j] = fmax(v, 0.0);
-        xneg = nan_ignoring_max(xneg, -v);
-        if (!isfinite(v)) bad = 1.0;
-      }
-      wg_spmv(c.trp, c.tci, c.tv, lam, t, n);
-      __syncthreads();
-      for (int j = tid; j < n; j += NT) zs[j] = D[j] * (cc[j] - t[j]);
-      __syncthreads();
-      wg_spmv(c.mrp, c.mci, c.mv, zs, rhs2, m);
-      __syncthreads();
-      (void)wg_solve<double>(c, A64, rhs2, mdrd, reg, red, &refine);
-      for (int i = tid; i < m; i += NT) mdrd[i] = lam[i] + mdrd[i];  // y'
-      __syncthreads();
-      wg_spmv(c.trp, c.tci, c.tv, mdrd, t, n);
-      wg_spmv(c.mrp, c.mci, c.mv, dx, tm, m);
-      __syncthreads();
-      double zneg = 0.0, cx = 0.0, by = 0.0, rp2 = 0.0, xz = 0.0;
-      for (int j = tid; j < n; j += NT) {
-        const double v = cc[j] - t[j];
-        dz[j] = v;
-        if (!isfinite(v)) bad = 1.0;
-        zneg = nan_ignoring_max(zneg, -v);
-        cx += cc[j] * dx[j];
-        xz += dx[j] * fmax(v, 0.0);
-      }
-      for (int i = tid; i < m; i += NT) {
-        rp2 = nan_ignoring_max(rp2, fabs(tm[i] - b[i]));
-        by += b[i] * mdrd[i];
-        if (!isfinite(mdrd[i])) bad = 1.0;
-      }
-      bad = wg_max(bad, red);
-      xneg = wg_max(xneg, red);
-      zneg = wg_max(zneg, red);
-      rp2 = wg_max(rp2, red);
-      cx = wg_sum(cx, red);
-      by = wg_sum(by, red);
-      xz = wg_sum(xz, red) / (double)n;
-      // the reference's stop test on the projected point (rd = max(-z', 0))
-      bool ok = bad == 0.0 && xneg < prm.conv_tol && zneg < prm.conv_tol && rp2 < prm.conv_tol &&
-                xz < prm.comp_tol;
-      if (ok && prm.gap_tol > 0.0) ok = fabs(cx - by) <= prm.gap_tol * (1.0 + fabs(cx + I.off));
-      if (ok) {
+      double reg = 0.0, bad = 0.0, xneg = 0.0, rp2 = 0.0, prev_nb = -1.0;
+      bool primal_ok = false;
+      for (int ti = 0; ti < 3 && !primal_ok; ++ti) {
+        const double th = (ti == 0) ? 1.0 : (ti == 1 ? 0.1 : 0.01);
+        double nb = 0.0;
         for (int j = tid; j < n; j += NT) {
-          x[j] = dx[j];
-          z[j] = fmax(dz[j], 0.0);
+          const bool in = x[j] >= th * z[j];
+          D[j] = in ? x[j] : 0.0;
+          t[j] = in ? x[j] : 0.0;
+          nb += in ? 1.0 : 0.0;
         }
-        for (int i = tid; i < m; i += NT) lam[i] = mdrd[i];
+        nb = wg_sum(nb, red);
+        if (nb == prev_nb) continue;
+        prev_nb = nb;
+        wg_factor<double>(c, A64, W64, sh64, red, prm.reg, false, &reg);
+        wg_spmv(c.mrp, c.mci, c.mv, t, tm, m);
         __syncthreads();
+        for (int i = tid; i < m; i += NT) rhs[i] = b[i] - tm[i];
+        __syncthreads();
+        (void)wg_solve<double>(c, A64, rhs, dlam, reg, red, &refine);
+        wg_spmv(c.trp, c.tci, c.tv, dlam, t, n);
+        __syncthreads();
+        // x'_B below zero by no more than the tolerance (degenerate basic
+        // components) is clamped to 0; the residual is that of the clamped
+        // point
+        double b1 = 0.0, xn = 0.0;
+        for (int j = tid; j < n; j += NT) {
+          const double v = (D[j] > 0.0) ? x[j] + D[j] * t[j] : 0.0;
+          dx[j] = fmax(v, 0.0);
+          xn = nan_ignoring_max(xn, -v);
+          if (!isfinite(v)) b1 = 1.0;
+        }
+        __syncthreads();
+        wg_spmv(c.mrp, c.mci, c.mv, dx, tm, m);
+        __syncthreads();
+        double r2 = 0.0;
+        for (int i = tid; i < m; i += NT) r2 = nan_ignoring_max(r2, fabs(tm[i] - b[i]));
+        bad = wg_max(b1, red);
+        xneg = wg_max(xn, red);
+        rp2 = wg_max(r2, red);
+        primal_ok = bad == 0.0 && xneg < prm.conv_tol && rp2 < prm.conv_tol;
       }
       ++ne_solves;
-      return ok;
+      if (!primal_ok) return false;
+      for (int di = 0; di < 3; ++di) {
+        // weights x/z over {x >= z}, then x over {x >= z}, then x/z over
+        // {x >= 100 z}: the last drops columns whose pair has not
+        // separated, for when {x >= z} holds more columns than M has rank
+        // and z'_B = 0 cannot hold on all of them
+        const double th = (di == 2) ? 100.0 : 1.0;
+        for (int j = tid; j < n; j += NT) {
+          double w = 0.0;
+          if (x[j] >= th * z[j]) {
+            w = (di == 1) ? x[j] : x[j] / z[j];
+            if (!(w <= 1e200)) w = 1e200;  // also catches a NaN quotient
+          }
+          D[j] = w;
+        }
+        __syncthreads();
+        wg_factor<double>(c, A64, W64, sh64, red, prm.reg, false, &reg);
+        wg_spmv(c.trp, c.tci, c.tv, lam, t, n);
+        __syncthreads();
+        for (int j = tid; j < n; j += NT) zs[j] = D[j] * (cc[j] - t[j]);
+        __syncthreads();
+        wg_spmv(c.mrp, c.mci, c.mv, zs, rhs2, m);
+        __syncthreads();
+        (void)wg_solve<double>(c, A64, rhs2, mdrd, reg, red, &refine);
+        for (int i = tid; i < m; i += NT) mdrd[i] = lam[i] + mdrd[i];  // y'
+        __syncthreads();
+        wg_spmv(c.trp, c.tci, c.tv, mdrd, t, n);
+        __syncthreads();
+        double zneg = 0.0, cx = 0.0, by = 0.0, xz = 0.0, b2 = 0.0;
+        for (int j = tid; j < n; j += NT) {
+          const double v = cc[j] - t[j];
+          dz[j] = v;
+          if (!isfinite(v)) b2 = 1.0;
+          zneg = nan_ignoring_max(zneg, -v);
+          cx += cc[j] * dx[j];
+          xz += dx[j] * fmax(v, 0.0);
+        }
+        for (int i = tid; i < m; i += NT) {
+          by += b[i] * mdrd[i];
+          if (!isfinite(mdrd[i])) b2 = 1.0;
+        }
+        b2 = wg_max(b2, red);
+        zneg = wg_max(zneg, red);
+        cx = wg_sum(cx, red);
+        by = wg_sum(by, red);
+        xz = wg_sum(xz, red) / (double)n;
+        // the reference's stop test on the projected point (rd = max(-z', 0))
+        bool ok = b2 == 0.0 && zneg < prm.conv_tol && xz < prm.comp_tol;
+        if (ok && prm.gap_tol > 0.0) ok = fabs(cx - by) <= prm.gap_tol * (1.0 + fabs(cx + I.off));
+        if (ok) {
+          for (int j = tid; j < n; j += NT) {
+            x[j] = dx[j];
+            z[j] = fmax(dz[j], 0.0);
+          }
+          for (int i = tid; i < m; i += NT) lam[i] = mdrd[i];
+          __syncthreads();
+          return true;
+        }
+      }
+      return false;
     };
     double endgame_next = prm.endgame_mu;
     int projected = 0, endgame_iter = 0;

@@ -7,7 +7,11 @@ Cases:
     whole range of m, the largest ones near m = 2,000;
   * the large power-law LP's generator at reduced sizes (m x 2.5m, column
     counts on [2, 200], seed 3);
-  * the first 12 held-out 200 x 300 transportation LPs (seeds 1,000,000+).
+  * the first 12 held-out 200 x 300 transportation LPs (seeds 1,000,000+),
+    and ten more held-out ones whose iterates stall near a relative gap
+    of 1e-7 with the support of the optimal face still ambiguous (a basic
+    column with x_j < s_j, or a weakly basic one carrying a negative
+    reduced cost after the projection).
 Run:  python tests/golden/make_highs_fixtures.py
 """
 import json
@@ -66,10 +70,14 @@ def main():
         json.dump(data, open(OUT, "w"), indent=1)
 
 
+TRANSPORT_HARD = [1_000_101, 1_000_136, 1_000_150, 1_000_174, 1_000_179, 1_000_200, 1_000_207, 1_000_813, 1_000_963,
+                  1_000_984]
+
+
 def transport_main():
     """HiGHS optima of the first held-out transportation LPs (200 x 300)."""
     data = json.load(open(OUT))
-    for seed in range(1_000_000, 1_000_012):
+    for seed in list(range(1_000_000, 1_000_012)) + TRANSPORT_HARD:
         key = f"transport_{seed}"
         if key in data:
             continue
