@@ -235,15 +235,22 @@ __global__ __launch_bounds__(256) void k_gnn_embed(const float* __restrict__ F, 
 // with v_k = (float) val of the entry (through src for M', whose entry q is
 // M's entry src[q]); self weights multiply by the row's float reciprocal,
 // other weights divide by the neighbour's degree.
+// Each entry's weight depends on its own row and column only, so a row's
+// entries are independent: rows run one per thread (RPT = 1: short rows,
+// such as M' of a transportation LP with 2 entries per row) or one per wave
+// (RPT = 64: long rows, such as its M with 200-300; a thread looping over
+// such a row serialises its dependent loads).  Both give the same bits.
+template <int RPT>
 __global__ __launch_bounds__(256) void k_gnn_weights(const int* __restrict__ rp, const int* __restrict__ col,
                                                      const int* __restrict__ src, const double* __restrict__ val,
                                                      const int* __restrict__ orp, int self_deg, int rows,
                                                      float* __restrict__ w) {
-  const int r = blockIdx.x * 256 + threadIdx.x;
+  const int r = (int)(((long long)blockIdx.x * 256 + threadIdx.x) / RPT);
+  const int lane = (RPT == 1) ? 0 : (int)(threadIdx.x & (RPT - 1));
   if (r >= rows) return;
   const int k0 = rp[r], k1 = rp[r + 1];
   const float inv_self = k1 > k0 ? 1.f / (float)(k1 - k0) : 0.f;
-  for (int k = k0; k < k1; ++k) {
+  for (int k = k0 + lane; k < k1; k += RPT) {
     const int cc = col[k];
     const float v = (float)val[src ? src[k] : k];
     w[k] = self_deg ? v * inv_self : v / (float)(orp[cc + 1] - orp[cc]);
@@ -708,10 +715,19 @@ void weights(hn_gnn_graph* gr, const double* val, bool backward_pair, hipStream_
   float* w = gr->wgt.as<float>() + (backward_pair ? 2 * nnz : 0);
   const int* rp = gr->rp.as<int>();
   const int* trp = gr->trp.as<int>();
-  hipLaunchKernelGGL(k_gnn_weights, dim3(tgrid(m)), dim3(256), 0, s, rp, gr->ci.as<int>(), (const int*)nullptr, val,
-                     trp, backward_pair ? 0 : 1, m, w);
-  hipLaunchKernelGGL(k_gnn_weights, dim3(tgrid(n)), dim3(256), 0, s, trp, gr->tci.as<int>(), gr->tsrc.as<int>(), val,
-                     rp, backward_pair ? 0 : 1, n, w + nnz);
+  // a wave per row where the mean row holds 8 or more entries
+  if (nnz >= 8LL * m)
+    hipLaunchKernelGGL(k_gnn_weights<kWave>, dim3(wgrid(m)), dim3(256), 0, s, rp, gr->ci.as<int>(),
+                       (const int*)nullptr, val, trp, backward_pair ? 0 : 1, m, w);
+  else
+    hipLaunchKernelGGL(k_gnn_weights<1>, dim3(tgrid(m)), dim3(256), 0, s, rp, gr->ci.as<int>(), (const int*)nullptr,
+                       val, trp, backward_pair ? 0 : 1, m, w);
+  if (nnz >= 8LL * n)
+    hipLaunchKernelGGL(k_gnn_weights<kWave>, dim3(wgrid(n)), dim3(256), 0, s, trp, gr->tci.as<int>(),
+                       gr->tsrc.as<int>(), val, rp, backward_pair ? 0 : 1, n, w + nnz);
+  else
+    hipLaunchKernelGGL(k_gnn_weights<1>, dim3(tgrid(n)), dim3(256), 0, s, trp, gr->tci.as<int>(), gr->tsrc.as<int>(),
+                       val, rp, backward_pair ? 0 : 1, n, w + nnz);
 }
 
 int forward(hn_gnn* g, hn_gnn_graph* gr, const double* val, const double* b, const double* c, hipStream_t s) {
