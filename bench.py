@@ -622,7 +622,9 @@ def run_batch_bucket(args) -> dict:
     """The batched solver on the instances of the 4,096-LP workload whose m
     lies in [lo, hi): wall time, iterations, and the multiply-adds of the
     LDL' factorisations it ran (sum over instances of iterations * m^3 / 3,
-    one factorisation per iteration), for counter runs per size bucket."""
+    one factorisation per iteration), for counter runs per size bucket.
+    With --phases the timed call also prints the library's per-phase
+    workgroup time (hn_ipm_opts.verbose = 2) on stderr."""
     import hypernova_hip as H
     import lpgen
     lo, hi = (int(t) for t in args.batch_range.split(","))
@@ -630,7 +632,7 @@ def run_batch_bucket(args) -> dict:
     lps = [lp for lp in (lpgen.batch_lp(i) for i in range(args.batch)) if lo <= lp.m < hi]
     H.ipm_solve_batch(lps[:8], pivot_rule=H.HN_PIVOT_SKIP, gap_tol=KKT_TOL)
     t0 = time.perf_counter()
-    res = H.ipm_solve_batch(lps, pivot_rule=H.HN_PIVOT_SKIP, gap_tol=KKT_TOL)
+    res = H.ipm_solve_batch(lps, pivot_rule=H.HN_PIVOT_SKIP, gap_tol=KKT_TOL, verbose=2 if args.phases else False)
     wall = time.perf_counter() - t0
     fma = sum(r.iterations * lp.m ** 3 / 3.0 for lp, r in zip(lps, res))
     return {"batch_bucket": f"m in [{lo}, {hi})", "lps": len(lps), "wall_s": round(wall, 4),
@@ -748,6 +750,8 @@ def main() -> int:
     ap.add_argument("--chunks-At", default="", help="fixed chunked layout of A': items,span_log2,threads")
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--batch-range", default="", help="lo,hi: only the batched solver on instances with lo <= m < hi")
+    ap.add_argument("--phases", action="store_true",
+                    help="with --batch-range: print the per-phase workgroup time of the timed batched call")
     ap.add_argument("--seq-sample", type=int, default=64)
     ap.add_argument("--cpu-lp-limit", type=float, default=8.0)
     ap.add_argument("--ws-test", type=int, default=1000, help="held-out transportation LPs of the warm-start leg")

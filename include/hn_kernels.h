@@ -337,7 +337,9 @@ typedef struct hn_ipm_opts {
   const double* y0;            /* x0, s0 of length sf_n, y0 of length sf_m (all    */
   const double* s0;            /* three or none; else the LP's own x0/y0/s0); used */
                                /* only if it passes the test at hn_ipm_solve_batch */
-  int verbose;                 /* nonzero: per-iteration trace on stderr          */
+  int verbose;                 /* nonzero: per-iteration trace on stderr; batch:  */
+                               /* phase times of the call, and with 2 or more the */
+                               /* kernel's per-phase workgroup time as well       */
   int pivot_rule;              /* HN_PIVOT_CLAMP (reference, default) or HN_PIVOT_SKIP */
   double mu_target;            /* > 0: stop at the first iterate with mu <= mu_target and
                                   return it (status HN_STATUS_ITER_LIMIT unless the 1e-8

@@ -64,9 +64,43 @@ constexpr int kVmIo = 2, kVnIo = 3;  // leading slots held in the host-visible a
 
 constexpr int kStallIters = 3;
 
+// Phases of a solve, timed per workgroup when Params::timing is set (the
+// verbose >= 2 diagnostic): wall-clock ticks of the workgroup's first thread,
+// taken at phase boundaries.
+enum Phase {
+  PH_START = 0,  // start point, warm-start acceptance
+  PH_RESID,      // residuals, mu and the stop tests
+  PH_PROJECT,    // optimal-face projection attempts, their factorisations included
+  PH_RHS,        // D, M (D rd) and the right-hand side
+  PH_ASSEMBLE,   // dense N: zeroing and the column products
+  PH_FACTOR,     // blocked LDL': the rest (block bookkeeping)
+  PH_DIAG,       // blocked LDL': 64 x 64 diagonal blocks
+  PH_PANEL,      // blocked LDL': inverse of L11 and the panel W, L21
+  PH_UPDATE,     // blocked LDL': trailing updates on the matrix cores
+  PH_TRSV,       // triangular solves with the factor
+  PH_REFINE,     // refinement products N v and residuals
+  PH_STEP,       // directions, ratio tests, sigma, corrector right-hand side, update
+  kPhases
+};
+
 struct Out {
   int status, iterations, fp64_from, ne_solves, refine, restarted, accepted, projected;
   double rp, rd, mu;
+  unsigned long long ph[kPhases];  // phase ticks (Params::timing), else 0
+};
+
+// Phase clock: ph points at kPhases accumulators in LDS followed by the last
+// timestamp; only thread 0 reads the clock and writes them.
+struct Clock {
+  unsigned long long* ph;
+  int on;
+  __device__ __forceinline__ void mark(int p) const {
+    if (on && threadIdx.x == 0) {
+      const unsigned long long now = (unsigned long long)wall_clock64();
+      ph[p] += now - ph[kPhases];
+      ph[kPhases] = now;
+    }
+  }
 };
 
 struct Params {
@@ -79,6 +113,7 @@ struct Params {
                      // caller's objective (the gap of the caller's LP when no column has a bound shift)
   double endgame_mu; // > 0: optimal-face projection once mu <= endgame_mu (then after each halving of
                      // mu or every 4 iterations)
+  int timing;        // 1: per-phase wall-clock ticks into Out::ph
 };
 
 // A skipped pivot: its entry of D becomes "infinite", so its column of L and
@@ -153,7 +188,7 @@ struct SharedLDL {
 // dg (the diagonal of N before elimination) are skipped.
 template <typename T>
 __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W, SharedLDL<T>& sh, double tol,
-                                    const double* __restrict__ dg, double skip_rel) {
+                                    const double* __restrict__ dg, double skip_rel, Clock ck) {
   const int tid = threadIdx.x;
   const int ti = tid >> 4, tj = tid & 15;
   int nclamp = 0;
@@ -215,6 +250,7 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
         }
       }
     }
+    ck.mark(PH_DIAG);
     // write back L (unit, below the diagonal) and D; keep L11, D in LDS
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -273,6 +309,7 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
       }
     }
     __syncthreads();
+    ck.mark(PH_PANEL);
     // (3) trailing update of the lower triangle: A_ij -= sum_c W_ic L_jc,
     //     64 x 64 tiles (I, J <= I) in steps of 32 columns c.  A tile row's
     //     W stays in LDS for all its tiles; each step's L slice (columns
@@ -419,6 +456,7 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
       cbi = nc;
     }
     __syncthreads();  // the next block reads what the update wrote
+    ck.mark(PH_UPDATE);
   }
   return nclamp;
 }
@@ -527,10 +565,11 @@ __device__ __noinline__ void wg_apply_N(const Ctx& c, const double* __restrict__
 // dlam = N^{-1} rhs with refinement; returns true when refined to 1e-10.
 template <typename T>
 __device__ bool wg_solve(const Ctx& c, const T* A, const double* __restrict__ rhs, double* __restrict__ dlam,
-                         double reg, double* red, int* refine) {
+                         double reg, double* red, int* refine, Clock ck) {
   for (int i = threadIdx.x; i < c.m; i += NT) dlam[i] = rhs[i];
   __syncthreads();
   wg_ldlt_solve<T>(A, c.m, dlam);
+  ck.mark(PH_TRSV);
   double* r = c.vm[VM_R];
   double* nd = c.vm[VM_ND];
   double nb = 0.0;
@@ -546,11 +585,13 @@ __device__ bool wg_solve(const Ctx& c, const T* A, const double* __restrict__ rh
       a = nan_ignoring_max(a, fabs(ri));
     }
     a = wg_max(a, red);
+    ck.mark(PH_REFINE);
     if (a / nb <= 1e-10) return true;
     if (it == 5) break;
     wg_ldlt_solve<T>(A, c.m, r);
     for (int i = threadIdx.x; i < c.m; i += NT) dlam[i] = dlam[i] + r[i];
     __syncthreads();
+    ck.mark(PH_TRSV);
     if (threadIdx.x == 0) ++*refine;
   }
   return false;
@@ -597,10 +638,12 @@ __device__ __noinline__ double wg_assemble(const Ctx& c, T* __restrict__ A, doub
 // Assembles and factors N; *reg_out receives its regularisation.
 template <typename T>
 __device__ void wg_factor(const Ctx& c, T* A, T* W, SharedLDL<T>& sh, double* red, double reg_min, bool skip,
-                          double* reg_out) {
+                          double* reg_out, Clock ck) {
   *reg_out = wg_assemble<T>(c, A, reg_min, skip ? 0.0 : 1e-12, red);
-  wg_ldlt<T>(A, c.m, W, sh, 1e-10, c.vm[VM_DG], skip ? (sizeof(T) == 4 ? 1e-6 : 1e-13) : 0.0);
+  ck.mark(PH_ASSEMBLE);
+  wg_ldlt<T>(A, c.m, W, sh, 1e-10, c.vm[VM_DG], skip ? (sizeof(T) == 4 ? 1e-6 : 1e-13) : 0.0, ck);
   __syncthreads();
+  ck.mark(PH_FACTOR);
 }
 
 __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst, const int* __restrict__ order,
@@ -615,8 +658,10 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
   __shared__ double red[NT / kWave];
   __shared__ int s_next;
   __shared__ SharedLDL<double> sh64;
+  __shared__ unsigned long long s_ph[kPhases + 1];
   SharedLDL<float>& sh32 = *reinterpret_cast<SharedLDL<float>*>(&sh64);
   const int tid = threadIdx.x;
+  const Clock ck{s_ph, prm.timing}, ck_off{s_ph, 0};
   for (;;) {
     if (tid == 0) s_next = atomicAdd(queue, 1);
     __syncthreads();
@@ -625,6 +670,10 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
     if (q >= count) return;
     const int id = order[q];
     const Inst I = inst[id];
+    if (prm.timing && tid == 0) {
+      for (int p = 0; p < kPhases; ++p) s_ph[p] = 0;
+      s_ph[kPhases] = (unsigned long long)wall_clock64();
+    }
     Ctx c;
     c.m = I.m;
     c.n = I.n;
@@ -722,6 +771,7 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
     if (I.warm == 2) accepted = warm_acceptable();
     if (!accepted) cold_start();
     __syncthreads();
+    ck.mark(PH_START);
     bool done = false, numerr = false, stopped = false;
     bool warm_live = accepted, restarted = false;
     int stuck = 0;
@@ -791,12 +841,12 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
         nb = wg_sum(nb, red);
         if (nb == prev_nb) continue;
         prev_nb = nb;
-        wg_factor<double>(c, A64, W64, sh64, red, prm.reg, false, &reg);
+        wg_factor<double>(c, A64, W64, sh64, red, prm.reg, false, &reg, ck_off);
         wg_spmv(c.mrp, c.mci, c.mv, t, tm, m);
         __syncthreads();
         for (int i = tid; i < m; i += NT) rhs[i] = b[i] - tm[i];
         __syncthreads();
-        (void)wg_solve<double>(c, A64, rhs, dlam, reg, red, &refine);
+        (void)wg_solve<double>(c, A64, rhs, dlam, reg, red, &refine, ck_off);
         wg_spmv(c.trp, c.tci, c.tv, dlam, t, n);
         __syncthreads();
         // x'_B below zero by no more than the tolerance (degenerate basic
@@ -836,14 +886,14 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
           D[j] = w;
         }
         __syncthreads();
-        wg_factor<double>(c, A64, W64, sh64, red, prm.reg, false, &reg);
+        wg_factor<double>(c, A64, W64, sh64, red, prm.reg, false, &reg, ck_off);
         wg_spmv(c.trp, c.tci, c.tv, lam, t, n);
         __syncthreads();
         for (int j = tid; j < n; j += NT) zs[j] = D[j] * (cc[j] - t[j]);
         __syncthreads();
         wg_spmv(c.mrp, c.mci, c.mv, zs, rhs2, m);
         __syncthreads();
-        (void)wg_solve<double>(c, A64, rhs2, mdrd, reg, red, &refine);
+        (void)wg_solve<double>(c, A64, rhs2, mdrd, reg, red, &refine, ck_off);
         for (int i = tid; i < m; i += NT) mdrd[i] = lam[i] + mdrd[i];  // y'
         __syncthreads();
         wg_spmv(c.trp, c.tci, c.tv, mdrd, t, n);
@@ -885,6 +935,7 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
     int projected = 0, endgame_iter = 0;
     for (int iter = 0; iter < prm.max_iterations; ++iter) {
       residuals();
+      ck.mark(PH_RESID);
       if (rpn < prm.conv_tol && rdn < prm.conv_tol && mu < prm.comp_tol) {
         bool gap_ok = true;
         if (prm.gap_tol > 0.0) {  // optional relative-gap bar (not in the reference)
@@ -895,6 +946,7 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
           by = wg_sum(by, red);
           gap_ok = fabs(cx - by) <= prm.gap_tol * (1.0 + fabs(cx + I.off));
         }
+        ck.mark(PH_RESID);
         if (gap_ok) {
           done = true;
           break;
@@ -908,6 +960,7 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
           (mu <= endgame_next || (mu <= prm.endgame_mu && iter >= endgame_iter + 4))) {
         if (project()) {  // passed the stop test: the solve ends at the projected point
           residuals();
+          ck.mark(PH_PROJECT);
           projected = 1;
           done = true;
           break;
@@ -915,6 +968,7 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
         endgame_next = 0.5 * mu;  // next try once mu has halved, or after 4 iterations
         endgame_iter = iter;
         residuals();  // the attempt overwrote M x (tm) and M'lam (t), which the step reads
+        ck.mark(PH_PROJECT);
       }
       // a warm start that lets mu fall below its tolerance while the
       // residuals stay above theirs for kStallIters iterations has stalled:
@@ -942,19 +996,20 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
       wg_spmv(c.mrp, c.mci, c.mv, t, mdrd, m);
       __syncthreads();
       for (int i = tid; i < m; i += NT) rhs[i] = -rp[i] - mdrd[i] + tm[i];
+      ck.mark(PH_RHS);
       double reg = 0.0;
       bool ok;
       if (fp64) {
-        wg_factor<double>(c, A64, W64, sh64, red, prm.reg, prm.skip != 0, &reg);
-        ok = wg_solve<double>(c, A64, rhs, dlam, reg, red, &refine);
+        wg_factor<double>(c, A64, W64, sh64, red, prm.reg, prm.skip != 0, &reg, ck);
+        ok = wg_solve<double>(c, A64, rhs, dlam, reg, red, &refine, ck);
       } else {
-        wg_factor<float>(c, A32, W32, sh32, red, prm.reg, prm.skip != 0, &reg);
-        ok = wg_solve<float>(c, A32, rhs, dlam, reg, red, &refine);
+        wg_factor<float>(c, A32, W32, sh32, red, prm.reg, prm.skip != 0, &reg, ck);
+        ok = wg_solve<float>(c, A32, rhs, dlam, reg, red, &refine, ck);
         if (!ok) {  // FP32 factor not refinable: FP64 from here on
           fp64 = true;
           if (fp64_from < 0) fp64_from = iter;
-          wg_factor<double>(c, A64, W64, sh64, red, prm.reg, prm.skip != 0, &reg);
-          ok = wg_solve<double>(c, A64, rhs, dlam, reg, red, &refine);
+          wg_factor<double>(c, A64, W64, sh64, red, prm.reg, prm.skip != 0, &reg, ck);
+          ok = wg_solve<double>(c, A64, rhs, dlam, reg, red, &refine, ck);
         }
       }
       ++ne_solves;
@@ -997,10 +1052,11 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
       __syncthreads();
       for (int i = tid; i < m; i += NT) rhs2[i] = rhs[i] - mz[i];
       __syncthreads();
+      ck.mark(PH_STEP);
       if (fp64)
-        ok = wg_solve<double>(c, A64, rhs2, dlam, reg, red, &refine);
+        ok = wg_solve<double>(c, A64, rhs2, dlam, reg, red, &refine, ck);
       else
-        ok = wg_solve<float>(c, A32, rhs2, dlam, reg, red, &refine);
+        ok = wg_solve<float>(c, A32, rhs2, dlam, reg, red, &refine, ck);
       ++ne_solves;
       // corrector direction (:666-685)
       wg_spmv(c.trp, c.tci, c.tv, dlam, t, n);
@@ -1035,10 +1091,12 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
         break;
       }
       iterations = iter + 1;
+      ck.mark(PH_STEP);
     }
     if (!done && !stopped) {  // :728-753
       residuals();
       if (rpn < 1e-4 && rdn < 1e-4 && mu < 1e-4) done = true;
+      ck.mark(PH_RESID);
     }
     if (tid == 0) {
       Out o;
@@ -1057,6 +1115,7 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
       o.rp = rpn;
       o.rd = rdn;
       o.mu = mu;
+      for (int p = 0; p < kPhases; ++p) o.ph[p] = prm.timing ? s_ph[p] : 0ULL;
       out[id] = o;
     }
     __syncthreads();
@@ -1311,6 +1370,7 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
   prm.mu_target = o.mu_target;
   prm.gap_tol = o.gap_tol;
   prm.endgame_mu = o.endgame_mu;
+  prm.timing = o.verbose >= 2 ? 1 : 0;
   const int grid = (int)std::min<uint32_t>(count, 2u * (uint32_t)std::max(1, ctx().info.compute_units));
   HN_HIP(hipEventRecord(w.ev[0], s));
   hipLaunchKernelGGL(k_batch_ipm, dim3(grid), dim3(NT), 0, s, w.inst.as<Inst>(), w.order.as<int>(), (int)count,
@@ -1387,6 +1447,24 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
                  "(kernel %.1f ms), results %.1f ms\n",
                  count, span(t0, t_prep), span(t_prep, t_pack), span(t_pack, t_dev), (double)kms,
                  span(t_dev, std::chrono::steady_clock::now()));
+    if (prm.timing) {  // workgroup time per phase, summed over the LPs
+      int khz = 0;
+      HN_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx().device));
+      static const char* names[kPhases] = {"start", "residuals", "projection", "rhs",  "assembly", "factor",
+                                           "diag",  "panel",     "update",     "trsv", "refine",   "step"};
+      double tot[kPhases] = {0.0}, all = 0.0;
+      long long its = 0;
+      for (uint32_t k = 0; k < count; ++k) {
+        its += out[k].iterations;
+        for (int p = 0; p < kPhases; ++p) tot[p] += (double)out[k].ph[p];
+      }
+      for (int p = 0; p < kPhases; ++p) all += tot[p];
+      std::fprintf(stderr, "[hn batch] workgroup time %.1f ms over %lld iterations (clock %d kHz):", all / khz, its,
+                   khz);
+      for (int p = 0; p < kPhases; ++p)
+        std::fprintf(stderr, " %s %.1f%%", names[p], all > 0.0 ? 100.0 * tot[p] / all : 0.0);
+      std::fprintf(stderr, "\n");
+    }
   }
   return HN_OK;
 }

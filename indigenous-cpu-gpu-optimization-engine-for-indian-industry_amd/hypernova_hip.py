@@ -715,7 +715,11 @@ def ipm_solve_batch(lps, precision: int = HN_PREC_F64, max_iterations: int = 100
     """Solves many LPs in one persistent launch (hn_ipm_solve_batch).
     warm: optional list (one entry per LP, or None) of (x0, y0, s0) warm
     starts of the scaled standard forms.  verbose: the library prints the
-    time of each phase of the call on stderr.  gap_tol > 0: an instance
+    time of each phase of the call on stderr; verbose=2 also times the
+    phases inside the kernel (start, residuals, projection, right-hand
+    side, assembly, diagonal blocks, panels, trailing updates, triangular
+    solves, refinement, step) and prints each one's share of the
+    workgroups' time.  gap_tol > 0: an instance
     stops only once its relative gap |c'x - b'y| / (1 + |c'x|) is also at
     most gap_tol (besides the reference's absolute test).  gnn: optional
     (network handle, graph handle) - LPs without a warm start whose standard
