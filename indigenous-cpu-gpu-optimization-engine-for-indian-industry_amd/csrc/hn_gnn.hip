@@ -97,14 +97,34 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// Batched inference: the forward kernels below run gridDim.y instances of
+// one graph at once.  Instance y finds its workspace buffers at y * istride
+// bytes past the given pointers, and its LP's values, b, c and start-point
+// outputs in jobs[y] (jobs == nullptr: the given pointers, one instance).
+template <typename V>
+__device__ __forceinline__ V* at(V* p, long long off) {
+  return p ? reinterpret_cast<V*>(reinterpret_cast<char*>(p) + off) : p;
+}
+template <typename V>
+__device__ __forceinline__ const V* at(const V* p, long long off) {
+  return p ? reinterpret_cast<const V*>(reinterpret_cast<const char*>(p) + off) : p;
+}
+__device__ __forceinline__ long long inst_off(long long istride) { return (long long)blockIdx.y * istride; }
+
 // ---- features -----------------------------------------------------------------
 // S[0] = mean |c| (1 if 0), S[1] = mean |b| (1 if 0): per-block partial
 // sums, then one block adds them in a fixed order.
 constexpr int kScaleBlocks = 64;
 __global__ __launch_bounds__(256) void k_gnn_scale_part(const double* __restrict__ c, int n,
                                                         const double* __restrict__ b, int m,
-                                                        double* __restrict__ part) {
+                                                        double* __restrict__ part, const GnnJob* __restrict__ jobs,
+                                                        long long istride) {
   __shared__ double red[4];
+  if (jobs) {
+    c = jobs[blockIdx.y].c;
+    b = jobs[blockIdx.y].b;
+  }
+  part = at(part, inst_off(istride));
   double sc = 0.0, sb = 0.0;
   for (int j = blockIdx.x * 256 + threadIdx.x; j < n; j += kScaleBlocks * 256) sc += fabs(c[j]);
   for (int i = blockIdx.x * 256 + threadIdx.x; i < m; i += kScaleBlocks * 256) sb += fabs(b[i]);
@@ -116,7 +136,10 @@ __global__ __launch_bounds__(256) void k_gnn_scale_part(const double* __restrict
   }
 }
 
-__global__ void k_gnn_scale_fin(const double* __restrict__ part, int n, int m, float* __restrict__ S) {
+__global__ void k_gnn_scale_fin(const double* __restrict__ part, int n, int m, float* __restrict__ S,
+                                long long istride) {
+  part = at(part, inst_off(istride));
+  S = at(S, inst_off(istride));
   if (threadIdx.x == 0) {
     double sc = 0.0, sb = 0.0;
     for (int q = 0; q < kScaleBlocks; ++q) {
@@ -135,7 +158,12 @@ constexpr int kRankMax = 4096;  // longer rows get the neutral rank 0.5
 // (ties broken by column index) and its gap to the row minimum.
 __global__ __launch_bounds__(256) void k_gnn_row_rank(const int* __restrict__ rp, const int* __restrict__ ci,
                                                       const double* __restrict__ c, const float* __restrict__ S,
-                                                      float* __restrict__ rank, float* __restrict__ gap) {
+                                                      float* __restrict__ rank, float* __restrict__ gap,
+                                                      const GnnJob* __restrict__ jobs, long long istride) {
+  if (jobs) c = jobs[blockIdx.y].c;
+  S = at(S, inst_off(istride));
+  rank = at(rank, inst_off(istride));
+  gap = at(gap, inst_off(istride));
   __shared__ double cs[kRankMax];
   __shared__ int js[kRankMax];
   __shared__ double red[4];
@@ -171,7 +199,12 @@ __global__ __launch_bounds__(256) void k_gnn_row_rank(const int* __restrict__ rp
 __global__ void k_gnn_feat_v(const int* __restrict__ trp, const int* __restrict__ tsrc,
                              const float* __restrict__ rank, const float* __restrict__ gap,
                              const double* __restrict__ c, const float* __restrict__ S, int n,
-                             float* __restrict__ F) {
+                             float* __restrict__ F, const GnnJob* __restrict__ jobs, long long istride) {
+  if (jobs) c = jobs[blockIdx.y].c;
+  rank = at(rank, inst_off(istride));
+  gap = at(gap, inst_off(istride));
+  S = at(S, inst_off(istride));
+  F = at(F, inst_off(istride));
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const int q0 = trp[j], q1 = trp[j + 1];
@@ -201,7 +234,11 @@ __global__ void k_gnn_feat_v(const int* __restrict__ trp, const int* __restrict_
 // Constraint features, one thread per row (slack: the row holds a column
 // of degree 1, a property of the pattern computed once per graph).
 __global__ void k_gnn_feat_c(const int* __restrict__ slackrow, const double* __restrict__ b,
-                             const float* __restrict__ S, int m, float* __restrict__ F) {
+                             const float* __restrict__ S, int m, float* __restrict__ F,
+                             const GnnJob* __restrict__ jobs, long long istride) {
+  if (jobs) b = jobs[blockIdx.y].b;
+  S = at(S, inst_off(istride));
+  F = at(F, inst_off(istride));
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
   float* f = F + (long long)i * 4;
@@ -214,7 +251,9 @@ __global__ void k_gnn_feat_c(const int* __restrict__ slackrow, const double* __r
 // h[r][f] = relu(sum_t F[r][t] W[t][f] + bias[f]); one wave per row.
 __global__ __launch_bounds__(256) void k_gnn_embed(const float* __restrict__ F, int fstride, int nf,
                                                    const float* __restrict__ W, const float* __restrict__ bias,
-                                                   int rows, float* __restrict__ h) {
+                                                   int rows, float* __restrict__ h, long long istride) {
+  F = at(F, inst_off(istride));
+  h = at(h, inst_off(istride));
   const int r = (blockIdx.x * 256 + threadIdx.x) / kWave;
   const int f = threadIdx.x & (kWave - 1);
   if (r >= rows) return;
@@ -244,7 +283,10 @@ template <int RPT>
 __global__ __launch_bounds__(256) void k_gnn_weights(const int* __restrict__ rp, const int* __restrict__ col,
                                                      const int* __restrict__ src, const double* __restrict__ val,
                                                      const int* __restrict__ orp, int self_deg, int rows,
-                                                     float* __restrict__ w) {
+                                                     float* __restrict__ w, const GnnJob* __restrict__ jobs,
+                                                     long long istride) {
+  if (jobs) val = jobs[blockIdx.y].val;
+  w = at(w, inst_off(istride));
   const int r = (int)(((long long)blockIdx.x * 256 + threadIdx.x) / RPT);
   const int lane = (RPT == 1) ? 0 : (int)(threadIdx.x & (RPT - 1));
   if (r >= rows) return;
@@ -262,7 +304,10 @@ __global__ __launch_bounds__(256) void k_gnn_weights(const int* __restrict__ rp,
 // 16 per wave; per lane the A operand is row (l & 15), k = 4 (l >> 4) + t.
 __global__ __launch_bounds__(256) void k_gnn_gemm_fwd(const float* __restrict__ X1, const float* __restrict__ X2,
                                                       const float* __restrict__ W, const float* __restrict__ bias,
-                                                      int rows, float* __restrict__ Y) {
+                                                      int rows, float* __restrict__ Y, long long istride) {
+  X1 = at(X1, inst_off(istride));
+  X2 = at(X2, inst_off(istride));
+  Y = at(Y, inst_off(istride));
   __shared__ short Wt[H][2 * H + 4];  // Wt[j][k] = bf16(W[k][j])
   for (int e = threadIdx.x; e < 2 * H * H; e += 256) {
     const int k = e / H, j = e % H;
@@ -483,7 +528,12 @@ __global__ __launch_bounds__(256) void k_gnn_embed_dw(const float* __restrict__ 
 __global__ __launch_bounds__(256) void k_gnn_heads(const float* __restrict__ h, int rows, const float* __restrict__ w1,
                                                    const float* __restrict__ b1, const float* __restrict__ w2,
                                                    const float* __restrict__ b2, const float* __restrict__ S,
-                                                   int scale_by_cbar, float* __restrict__ o1, float* __restrict__ o2) {
+                                                   int scale_by_cbar, float* __restrict__ o1, float* __restrict__ o2,
+                                                   long long istride) {
+  h = at(h, inst_off(istride));
+  S = at(S, inst_off(istride));
+  o1 = at(o1, inst_off(istride));
+  o2 = at(o2, inst_off(istride));
   const int r = (blockIdx.x * 256 + threadIdx.x) / kWave;
   const int f = threadIdx.x & (kWave - 1);
   if (r >= rows) return;
@@ -624,7 +674,19 @@ __global__ void k_gnn_warm_point(const int* __restrict__ trp, const int* __restr
                                  const int* __restrict__ tsrc, const double* __restrict__ val,
                                  const double* __restrict__ c, const float* __restrict__ lx,
                                  const float* __restrict__ ls, const float* __restrict__ ly, int n, int m,
-                                 double* __restrict__ x0, double* __restrict__ y0, double* __restrict__ s0) {
+                                 double* __restrict__ x0, double* __restrict__ y0, double* __restrict__ s0,
+                                 const GnnJob* __restrict__ jobs, long long istride) {
+  if (jobs) {
+    const GnnJob J = jobs[blockIdx.y];
+    val = J.val;
+    c = J.c;
+    x0 = J.x0;
+    y0 = J.y0;
+    s0 = J.s0;
+  }
+  lx = at(lx, inst_off(istride));
+  ls = at(ls, inst_off(istride));
+  ly = at(ly, inst_off(istride));
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j < m) y0[j] = (double)ly[j];
   if (j >= n) return;
@@ -659,16 +721,10 @@ struct hn_gnn_graph {
   hn::DevBuf wgt;  // aggregation weights, 4 x nnz floats
   // host copy of the pattern, to recognise standard forms of this family
   std::vector<int64_t> h_rp, h_ci;
-  // inference-only workspaces and streams for concurrent warm starts
-  // (created on first use by gnn_warm_start_many, kept until freed)
-  std::vector<hn_gnn_graph*> clones;
-  std::vector<hipStream_t> streams;
-  std::vector<hipEvent_t> events;
-  ~hn_gnn_graph() {
-    for (hn_gnn_graph* c : clones) delete c;
-    for (hipStream_t st : streams) (void)hipStreamDestroy(st);
-    for (hipEvent_t e : events) (void)hipEventDestroy(e);
-  }
+  // batched inference (gnn_warm_start_many): one workspace block per
+  // instance of a chunk, and the chunk's jobs; grown on first use, kept
+  // until the graph is freed
+  hn::DevBuf bws, bjobs;
 };
 
 namespace hn {
@@ -709,25 +765,32 @@ std::vector<float> init_params(uint64_t seed) {
 // M, [nnz, 2 nnz) self weights over the rows of M' (forward), [2 nnz, 3 nnz)
 // and [3 nnz, 4 nnz) the other-degree weights of the same two patterns
 // (backward).
-void weights(hn_gnn_graph* gr, const double* val, bool backward_pair, hipStream_t s) {
+// w: 2 nnz floats per instance (the pair's two halves); `batch` instances
+// at istride bytes apart, their values in jobs[y].val when jobs is given.
+void weights_into(const hn_gnn_graph* gr, const double* val, float* w, bool backward_pair, const GnnJob* jobs,
+                  int batch, long long istride, hipStream_t s) {
   const int m = gr->m, n = gr->n;
   const long long nnz = gr->nnz;
-  float* w = gr->wgt.as<float>() + (backward_pair ? 2 * nnz : 0);
   const int* rp = gr->rp.as<int>();
   const int* trp = gr->trp.as<int>();
+  const unsigned B = (unsigned)batch;
   // a wave per row where the mean row holds 8 or more entries
   if (nnz >= 8LL * m)
-    hipLaunchKernelGGL(k_gnn_weights<kWave>, dim3(wgrid(m)), dim3(256), 0, s, rp, gr->ci.as<int>(),
-                       (const int*)nullptr, val, trp, backward_pair ? 0 : 1, m, w);
+    hipLaunchKernelGGL(k_gnn_weights<kWave>, dim3(wgrid(m), B), dim3(256), 0, s, rp, gr->ci.as<int>(),
+                       (const int*)nullptr, val, trp, backward_pair ? 0 : 1, m, w, jobs, istride);
   else
-    hipLaunchKernelGGL(k_gnn_weights<1>, dim3(tgrid(m)), dim3(256), 0, s, rp, gr->ci.as<int>(), (const int*)nullptr,
-                       val, trp, backward_pair ? 0 : 1, m, w);
+    hipLaunchKernelGGL(k_gnn_weights<1>, dim3(tgrid(m), B), dim3(256), 0, s, rp, gr->ci.as<int>(),
+                       (const int*)nullptr, val, trp, backward_pair ? 0 : 1, m, w, jobs, istride);
   if (nnz >= 8LL * n)
-    hipLaunchKernelGGL(k_gnn_weights<kWave>, dim3(wgrid(n)), dim3(256), 0, s, trp, gr->tci.as<int>(),
-                       gr->tsrc.as<int>(), val, rp, backward_pair ? 0 : 1, n, w + nnz);
+    hipLaunchKernelGGL(k_gnn_weights<kWave>, dim3(wgrid(n), B), dim3(256), 0, s, trp, gr->tci.as<int>(),
+                       gr->tsrc.as<int>(), val, rp, backward_pair ? 0 : 1, n, w + nnz, jobs, istride);
   else
-    hipLaunchKernelGGL(k_gnn_weights<1>, dim3(tgrid(n)), dim3(256), 0, s, trp, gr->tci.as<int>(), gr->tsrc.as<int>(),
-                       val, rp, backward_pair ? 0 : 1, n, w + nnz);
+    hipLaunchKernelGGL(k_gnn_weights<1>, dim3(tgrid(n), B), dim3(256), 0, s, trp, gr->tci.as<int>(),
+                       gr->tsrc.as<int>(), val, rp, backward_pair ? 0 : 1, n, w + nnz, jobs, istride);
+}
+
+void weights(hn_gnn_graph* gr, const double* val, bool backward_pair, hipStream_t s) {
+  weights_into(gr, val, gr->wgt.as<float>() + (backward_pair ? 2 * gr->nnz : 0), backward_pair, nullptr, 1, 0, s);
 }
 
 int forward(hn_gnn* g, hn_gnn_graph* gr, const double* val, const double* b, const double* c, hipStream_t s) {
@@ -740,34 +803,36 @@ int forward(hn_gnn* g, hn_gnn_graph* gr, const double* val, const double* b, con
   const int* tci = gr->tci.as<int>();
   const int* tsrc = gr->tsrc.as<int>();
   float* S = gr->S.as<float>();
-  hipLaunchKernelGGL(k_gnn_scale_part, dim3(kScaleBlocks), dim3(256), 0, s, c, n, b, m, gr->spart.as<double>());
-  hipLaunchKernelGGL(k_gnn_scale_fin, dim3(1), dim3(64), 0, s, gr->spart.as<double>(), n, m, S);
+  const GnnJob* nojob = nullptr;
+  hipLaunchKernelGGL(k_gnn_scale_part, dim3(kScaleBlocks), dim3(256), 0, s, c, n, b, m, gr->spart.as<double>(), nojob,
+                     0LL);
+  hipLaunchKernelGGL(k_gnn_scale_fin, dim3(1), dim3(64), 0, s, gr->spart.as<double>(), n, m, S, 0LL);
   hipLaunchKernelGGL(k_gnn_row_rank, dim3(m), dim3(256), 0, s, rp, ci, c, S, gr->rank.as<float>(),
-                     gr->gap.as<float>());
+                     gr->gap.as<float>(), nojob, 0LL);
   hipLaunchKernelGGL(k_gnn_feat_v, dim3(tgrid(n)), dim3(256), 0, s, trp, tsrc, gr->rank.as<float>(),
-                     gr->gap.as<float>(), c, S, n, gr->Fv.as<float>());
+                     gr->gap.as<float>(), c, S, n, gr->Fv.as<float>(), nojob, 0LL);
   hipLaunchKernelGGL(k_gnn_feat_c, dim3(tgrid(m)), dim3(256), 0, s, gr->slackrow.as<int>(), b, S, m,
-                     gr->Fc.as<float>());
+                     gr->Fc.as<float>(), nojob, 0LL);
   hipLaunchKernelGGL(k_gnn_embed, dim3(wgrid(n)), dim3(256), 0, s, gr->Fv.as<float>(), 8, FV, P + Lo.wev,
-                     P + Lo.bev, n, gr->hv[0].as<float>());
+                     P + Lo.bev, n, gr->hv[0].as<float>(), 0LL);
   hipLaunchKernelGGL(k_gnn_embed, dim3(wgrid(m)), dim3(256), 0, s, gr->Fc.as<float>(), 4, FC, P + Lo.wec,
-                     P + Lo.bec, m, gr->hc[0].as<float>());
+                     P + Lo.bec, m, gr->hc[0].as<float>(), 0LL);
   weights(gr, val, false, s);
   const float* wr = gr->wgt.as<float>();
   const float* wc = wr + gr->nnz;
   for (int l = 0; l < NL; ++l) {
     HN_TRY(spmm_rows_launch<float>(rp, ci, wr, m, gr->nnz, gr->hv[l].as<float>(), gr->ac[l].as<float>(), H, false, s));
     hipLaunchKernelGGL(k_gnn_gemm_fwd, dim3((m + 63) / 64), dim3(256), 0, s, gr->hc[l].as<float>(),
-                       gr->ac[l].as<float>(), P + Lo.wc[l], P + Lo.bc[l], m, gr->hc[l + 1].as<float>());
+                       gr->ac[l].as<float>(), P + Lo.wc[l], P + Lo.bc[l], m, gr->hc[l + 1].as<float>(), 0LL);
     HN_TRY(spmm_rows_launch<float>(trp, tci, wc, n, gr->nnz, gr->hc[l + 1].as<float>(), gr->av[l].as<float>(), H,
                                    false, s));
     hipLaunchKernelGGL(k_gnn_gemm_fwd, dim3((n + 63) / 64), dim3(256), 0, s, gr->hv[l].as<float>(),
-                       gr->av[l].as<float>(), P + Lo.wv[l], P + Lo.bv[l], n, gr->hv[l + 1].as<float>());
+                       gr->av[l].as<float>(), P + Lo.wv[l], P + Lo.bv[l], n, gr->hv[l + 1].as<float>(), 0LL);
   }
   hipLaunchKernelGGL(k_gnn_heads, dim3(wgrid(n)), dim3(256), 0, s, gr->hv[NL].as<float>(), n, P + Lo.wx, P + Lo.bx,
-                     P + Lo.ws, P + Lo.bs, S, 0, gr->lx.as<float>(), gr->ls.as<float>());
+                     P + Lo.ws, P + Lo.bs, S, 0, gr->lx.as<float>(), gr->ls.as<float>(), 0LL);
   hipLaunchKernelGGL(k_gnn_heads, dim3(wgrid(m)), dim3(256), 0, s, gr->hc[NL].as<float>(), m, P + Lo.wy, P + Lo.by,
-                     (const float*)nullptr, (const float*)nullptr, S, 1, gr->ly.as<float>(), (float*)nullptr);
+                     (const float*)nullptr, (const float*)nullptr, S, 1, gr->ly.as<float>(), (float*)nullptr, 0LL);
   count_launch(10 + 2 * NL);  // the aggregations count themselves
   return check_launch("gnn forward");
 }
@@ -1061,51 +1126,98 @@ namespace hn {
 
 namespace {
 
-constexpr int kInferStreams = 4;  // concurrent warm starts (hardware queues per process)
+// Batched inference workspace: the forward pass's buffers for one instance,
+// each at a 256-byte aligned offset of a block of `stride` bytes; a chunk of
+// instances uses consecutive blocks.  The layers ping-pong between two
+// embedding buffers per side (inference keeps no activations for a backward
+// pass).
+struct InferLayout {
+  long long S, spart, rank, gap, wgt, Fv, Fc, hv[2], hc[2], av, ac, lx, ls, ly, stride;
+  explicit InferLayout(const hn_gnn_graph* g) {
+    const long long m = g->m, n = g->n, nnz = std::max<long long>(g->nnz, 1);
+    long long o = 0;
+    auto take = [&](long long bytes) {
+      const long long r = o;
+      o += (bytes + 255) / 256 * 256;
+      return r;
+    };
+    S = take(4 * sizeof(float));
+    spart = take(2 * kScaleBlocks * sizeof(double));
+    rank = take(nnz * sizeof(float));
+    gap = take(nnz * sizeof(float));
+    wgt = take(2 * nnz * sizeof(float));
+    Fv = take(n * 8 * sizeof(float));
+    Fc = take(m * 4 * sizeof(float));
+    for (int k = 0; k < 2; ++k) {
+      hv[k] = take(n * H * sizeof(float));
+      hc[k] = take(m * H * sizeof(float));
+    }
+    av = take(n * H * sizeof(float));
+    ac = take(m * H * sizeof(float));
+    lx = take(n * sizeof(float));
+    ls = take(n * sizeof(float));
+    ly = take(m * sizeof(float));
+    stride = o;
+  }
+};
 
-// A workspace for inference only: the structure of `src` and the forward
-// pass's buffers (no training buffers).
-int clone_for_inference(const hn_gnn_graph* src, hn_gnn_graph** out) {
-  auto* gr = new hn_gnn_graph();
-  gr->m = src->m;
-  gr->n = src->n;
-  gr->nnz = src->nnz;
-  const int m = src->m, n = src->n;
-  const long long nnz = src->nnz;
-  auto dup = [&](DevBuf& d, const DevBuf& from, size_t bytes) -> int {
-    HN_TRY(d.reserve(std::max<size_t>(bytes, 4)));
-    if (bytes) HN_HIP(hipMemcpy(d.p, from.p, bytes, hipMemcpyDeviceToDevice));
-    return HN_OK;
-  };
-  const size_t fm = sizeof(float) * (size_t)m * H, fn = sizeof(float) * (size_t)n * H;
-  int st = dup(gr->rp, src->rp, sizeof(int) * (m + 1));
-  if (st == HN_OK) st = dup(gr->ci, src->ci, sizeof(int) * nnz);
-  if (st == HN_OK) st = dup(gr->trp, src->trp, sizeof(int) * (n + 1));
-  if (st == HN_OK) st = dup(gr->tci, src->tci, sizeof(int) * nnz);
-  if (st == HN_OK) st = dup(gr->tsrc, src->tsrc, sizeof(int) * nnz);
-  if (st == HN_OK) st = dup(gr->slackrow, src->slackrow, sizeof(int) * m);
-  if (st == HN_OK) st = gr->S.reserve(sizeof(float) * 4);
-  if (st == HN_OK) st = gr->spart.reserve(sizeof(double) * 2 * kScaleBlocks);
-  if (st == HN_OK) st = gr->rank.reserve(sizeof(float) * std::max<long long>(nnz, 1));
-  if (st == HN_OK) st = gr->gap.reserve(sizeof(float) * std::max<long long>(nnz, 1));
-  if (st == HN_OK) st = gr->wgt.reserve(sizeof(float) * 4 * std::max<long long>(nnz, 1));
-  if (st == HN_OK) st = gr->Fv.reserve(sizeof(float) * (size_t)n * 8);
-  if (st == HN_OK) st = gr->Fc.reserve(sizeof(float) * (size_t)m * 4);
-  for (int l = 0; l <= NL && st == HN_OK; ++l) {
-    st = gr->hv[l].reserve(fn);
-    if (st == HN_OK) st = gr->hc[l].reserve(fm);
-    if (st == HN_OK && l < NL) st = gr->av[l].reserve(fn);
-    if (st == HN_OK && l < NL) st = gr->ac[l].reserve(fm);
+constexpr long long kInferBytes = 2LL << 30;  // workspace of one chunk
+constexpr int kInferMaxChunk = 256;             // instances per chunk
+
+// The network's forward pass and start points for B instances at once
+// (jobs: B device records); the same kernels and arithmetic as forward() and
+// warm_point(), so every instance's start is the one a single call gives.
+int forward_batch(hn_gnn* g, hn_gnn_graph* gr, const GnnJob* jobs, int B, hipStream_t s) {
+  const Layout& Lo = L();
+  const InferLayout IL(gr);
+  const float* P = g->p.as<float>();
+  const int m = gr->m, n = gr->n;
+  const long long nnz = gr->nnz, st = IL.stride;
+  char* base = gr->bws.as<char>();
+  auto F = [&](long long off) { return reinterpret_cast<float*>(base + off); };
+  const int* rp = gr->rp.as<int>();
+  const int* ci = gr->ci.as<int>();
+  const int* trp = gr->trp.as<int>();
+  const int* tci = gr->tci.as<int>();
+  const int* tsrc = gr->tsrc.as<int>();
+  const unsigned U = (unsigned)B;
+  const double* none = nullptr;
+  float* S = F(IL.S);
+  double* spart = reinterpret_cast<double*>(base + IL.spart);
+  hipLaunchKernelGGL(k_gnn_scale_part, dim3(kScaleBlocks, U), dim3(256), 0, s, none, n, none, m, spart, jobs, st);
+  hipLaunchKernelGGL(k_gnn_scale_fin, dim3(1, U), dim3(64), 0, s, spart, n, m, S, st);
+  hipLaunchKernelGGL(k_gnn_row_rank, dim3(m, U), dim3(256), 0, s, rp, ci, none, S, F(IL.rank), F(IL.gap), jobs, st);
+  hipLaunchKernelGGL(k_gnn_feat_v, dim3(tgrid(n), U), dim3(256), 0, s, trp, tsrc, F(IL.rank), F(IL.gap), none, S, n,
+                     F(IL.Fv), jobs, st);
+  hipLaunchKernelGGL(k_gnn_feat_c, dim3(tgrid(m), U), dim3(256), 0, s, gr->slackrow.as<int>(), none, S, m, F(IL.Fc),
+                     jobs, st);
+  hipLaunchKernelGGL(k_gnn_embed, dim3(wgrid(n), U), dim3(256), 0, s, F(IL.Fv), 8, FV, P + Lo.wev, P + Lo.bev, n,
+                     F(IL.hv[0]), st);
+  hipLaunchKernelGGL(k_gnn_embed, dim3(wgrid(m), U), dim3(256), 0, s, F(IL.Fc), 4, FC, P + Lo.wec, P + Lo.bec, m,
+                     F(IL.hc[0]), st);
+  weights_into(gr, none, F(IL.wgt), false, jobs, B, st, s);
+  const float* wr = F(IL.wgt);
+  const float* wc = wr + nnz;
+  for (int l = 0; l < NL; ++l) {
+    const int cur = l & 1, nxt = cur ^ 1;
+    HN_TRY(spmm_rows_launch_batch<float>(rp, ci, wr, m, nnz, F(IL.hv[cur]), F(IL.ac), H, false, B, st, s));
+    hipLaunchKernelGGL(k_gnn_gemm_fwd, dim3((m + 63) / 64, U), dim3(256), 0, s, F(IL.hc[cur]), F(IL.ac),
+                       P + Lo.wc[l], P + Lo.bc[l], m, F(IL.hc[nxt]), st);
+    HN_TRY(spmm_rows_launch_batch<float>(trp, tci, wc, n, nnz, F(IL.hc[nxt]), F(IL.av), H, false, B, st, s));
+    hipLaunchKernelGGL(k_gnn_gemm_fwd, dim3((n + 63) / 64, U), dim3(256), 0, s, F(IL.hv[cur]), F(IL.av),
+                       P + Lo.wv[l], P + Lo.bv[l], n, F(IL.hv[nxt]), st);
   }
-  if (st == HN_OK) st = gr->lx.reserve(sizeof(float) * n);
-  if (st == HN_OK) st = gr->ls.reserve(sizeof(float) * n);
-  if (st == HN_OK) st = gr->ly.reserve(sizeof(float) * m);
-  if (st != HN_OK) {
-    delete gr;
-    return st;
-  }
-  *out = gr;
-  return HN_OK;
+  const int fin = NL & 1;  // the buffers the last layer wrote
+  hipLaunchKernelGGL(k_gnn_heads, dim3(wgrid(n), U), dim3(256), 0, s, F(IL.hv[fin]), n, P + Lo.wx, P + Lo.bx,
+                     P + Lo.ws, P + Lo.bs, S, 0, F(IL.lx), F(IL.ls), st);
+  hipLaunchKernelGGL(k_gnn_heads, dim3(wgrid(m), U), dim3(256), 0, s, F(IL.hc[fin]), m, P + Lo.wy, P + Lo.by,
+                     (const float*)nullptr, (const float*)nullptr, S, 1, F(IL.ly), (float*)nullptr, st);
+  const int nm = n > m ? n : m;
+  double* nod = nullptr;
+  hipLaunchKernelGGL(k_gnn_warm_point, dim3(tgrid(nm), U), dim3(256), 0, s, trp, tci, tsrc, none, none, F(IL.lx),
+                     F(IL.ls), F(IL.ly), n, m, nod, nod, nod, jobs, st);
+  count_launch(15 + 2 * NL);  // the aggregations count themselves
+  return check_launch("gnn batched forward");
 }
 
 int warm_point(hn_gnn_graph* gr, const double* val, const double* c, double* x0, double* y0, double* s0,
@@ -1113,7 +1225,7 @@ int warm_point(hn_gnn_graph* gr, const double* val, const double* c, double* x0,
   const int nm = gr->n > gr->m ? gr->n : gr->m;
   hipLaunchKernelGGL(k_gnn_warm_point, dim3(tgrid(nm)), dim3(256), 0, s, gr->trp.as<int>(), gr->tci.as<int>(),
                      gr->tsrc.as<int>(), val, c, gr->lx.as<float>(), gr->ls.as<float>(), gr->ly.as<float>(), gr->n,
-                     gr->m, x0, y0, s0);
+                     gr->m, x0, y0, s0, (const GnnJob*)nullptr, 0LL);
   count_launch();
   return check_launch("gnn warm start");
 }
@@ -1127,36 +1239,19 @@ bool gnn_pattern_matches(const hn_gnn_graph* g, int64_t m, int64_t n, const std:
 
 int gnn_warm_start_many(hn_gnn* net, hn_gnn_graph* g, const std::vector<GnnJob>& jobs, hipStream_t s) {
   if (jobs.empty()) return HN_OK;
-  const int K = (int)std::min<size_t>(kInferStreams, jobs.size());
-  while ((int)g->clones.size() < K - 1) {
-    hn_gnn_graph* c = nullptr;
-    HN_TRY(clone_for_inference(g, &c));
-    g->clones.push_back(c);
-  }
-  while ((int)g->streams.size() < K) {
-    hipStream_t st;
-    HN_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    g->streams.push_back(st);
-  }
-  while ((int)g->events.size() < K + 1) {
-    hipEvent_t e;
-    HN_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    g->events.push_back(e);
-  }
-  // fork from s (the inputs were uploaded on it), one workspace per stream,
-  // join back into s
-  HN_HIP(hipEventRecord(g->events[K], s));
-  for (int q = 0; q < K; ++q) HN_HIP(hipStreamWaitEvent(g->streams[q], g->events[K], 0));
-  for (size_t t = 0; t < jobs.size(); ++t) {
-    const int q = (int)(t % K);
-    hn_gnn_graph* ws = q == 0 ? g : g->clones[q - 1];
-    const GnnJob& j = jobs[t];
-    HN_TRY(forward(net, ws, j.val, j.b, j.c, g->streams[q]));
-    HN_TRY(warm_point(ws, j.val, j.c, j.x0, j.y0, j.s0, g->streams[q]));
-  }
-  for (int q = 0; q < K; ++q) {
-    HN_HIP(hipEventRecord(g->events[q], g->streams[q]));
-    HN_HIP(hipStreamWaitEvent(s, g->events[q], 0));
+  const InferLayout IL(g);
+  const long long fit = std::max<long long>(1, kInferBytes / IL.stride);
+  const int B = (int)std::min<long long>({fit, (long long)kInferMaxChunk, (long long)jobs.size()});
+  HN_TRY(g->bws.reserve((size_t)IL.stride * B));
+  HN_TRY(g->bjobs.reserve(sizeof(GnnJob) * jobs.size()));
+  // the records are read by the kernels queued on s after this copy; the
+  // caller's vector may go away on return, so the copy completes here
+  HN_HIP(hipMemcpyAsync(g->bjobs.p, jobs.data(), sizeof(GnnJob) * jobs.size(), hipMemcpyHostToDevice, s));
+  HN_HIP(hipStreamSynchronize(s));
+  const GnnJob* dj = g->bjobs.as<GnnJob>();
+  for (size_t t = 0; t < jobs.size(); t += B) {
+    const int b = (int)std::min<size_t>(B, jobs.size() - t);
+    HN_TRY(forward_batch(net, g, dj + t, b, s));
   }
   return HN_OK;
 }

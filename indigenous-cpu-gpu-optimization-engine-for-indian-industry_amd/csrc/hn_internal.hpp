@@ -170,6 +170,11 @@ int exclusive_scan(const int* in, int64_t n, int* out, DevBuf& tmp, hipStream_t 
 template <typename V>
 int spmm_rows_launch(const int* rowptr, const int* col, const V* val, int64_t nrows, int64_t nnz, const V* B, V* C,
                      int k, bool accumulate, hipStream_t s);
+// `batch` products over the same pattern in one launch: product y reads
+// val, B and writes C at y * istride bytes past the given pointers.
+template <typename V>
+int spmm_rows_launch_batch(const int* rowptr, const int* col, const V* val, int64_t nrows, int64_t nnz, const V* B,
+                           V* C, int k, bool accumulate, int batch, long long istride, hipStream_t s);
 
 // Two-pass deterministic FP64 reductions into a device scalar.
 int dot_launch(const double* a, const double* b, int64_t n, double* out, hipStream_t s);
@@ -190,8 +195,9 @@ struct GnnJob {
 // True when (m, n, rp, ci) is exactly the graph's pattern.
 bool gnn_pattern_matches(const hn_gnn_graph* g, int64_t m, int64_t n, const std::vector<int64_t>& rp,
                          const std::vector<int64_t>& ci);
-// Runs the jobs on a few streams forked from s (one workspace each) and joins
-// them back into s: work queued on s afterwards sees every start point.
+// Runs the network on every job, in chunks of instances that share one
+// launch per kernel (gridDim.y = instances), on stream s: work queued on s
+// afterwards sees every start point.
 int gnn_warm_start_many(hn_gnn* net, hn_gnn_graph* g, const std::vector<GnnJob>& jobs, hipStream_t s);
 
 inline unsigned grid_for(int64_t n, int block, int64_t cap = 1 << 20) {

@@ -260,13 +260,28 @@ int fill_launch(double* p, double v, int64_t n, hipStream_t s) {
 // ---- SpMM: C (+)= A * B, B row-major (ncols x k) -----------------------------
 // One wave per row; lanes cover 64 consecutive columns of B/C, so every
 // nonzero reads one contiguous strip of a B row.  Each entry of C is summed
-// over the row's nonzeros in order.
+// over the row's nonzeros in order.  A batch of products over the same
+// pattern runs as gridDim.y products: product y reads val, B and writes C
+// at y * istride bytes past the given pointers.
+template <typename V>
+__device__ __forceinline__ const V* shift(const V* p, long long bytes) {
+  return reinterpret_cast<const V*>(reinterpret_cast<const char*>(p) + bytes);
+}
+template <typename V>
+__device__ __forceinline__ V* shift(V* p, long long bytes) {
+  return reinterpret_cast<V*>(reinterpret_cast<char*>(p) + bytes);
+}
+
 template <typename V>
 __global__ __launch_bounds__(256) void k_spmm_rows(const int* __restrict__ rowptr,
                                                    const int* __restrict__ col,
-                                                   const V* __restrict__ val,
-                                                   const V* __restrict__ B, V* __restrict__ C,
-                                                   int nrows, int k, int accumulate) {
+                                                   const V* __restrict__ val_,
+                                                   const V* __restrict__ B_, V* __restrict__ C_,
+                                                   int nrows, int k, int accumulate, long long istride) {
+  const long long off = (long long)blockIdx.y * istride;
+  const V* __restrict__ val = shift(val_, off);
+  const V* __restrict__ B = shift(B_, off);
+  V* __restrict__ C = shift(C_, off);
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   if (wave >= nrows) return;
@@ -289,9 +304,13 @@ __global__ __launch_bounds__(256) void k_spmm_rows(const int* __restrict__ rowpt
 template <typename V>
 __global__ __launch_bounds__(256) void k_spmm_rows_long(const int* __restrict__ rowptr,
                                                         const int* __restrict__ col,
-                                                        const V* __restrict__ val,
-                                                        const V* __restrict__ B, V* __restrict__ C, int k,
-                                                        int accumulate) {
+                                                        const V* __restrict__ val_,
+                                                        const V* __restrict__ B_, V* __restrict__ C_, int k,
+                                                        int accumulate, long long istride) {
+  const long long off = (long long)blockIdx.y * istride;
+  const V* __restrict__ val = shift(val_, off);
+  const V* __restrict__ B = shift(B_, off);
+  V* __restrict__ C = shift(C_, off);
   __shared__ V part[4][kWave];
   const int r = blockIdx.x;
   const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
@@ -320,27 +339,36 @@ __global__ __launch_bounds__(256) void k_spmm_rows_long(const int* __restrict__ 
 }
 
 template <typename V>
-int spmm_rows_launch(const int* rowptr, const int* col, const V* val, int64_t nrows, int64_t nnz, const V* B, V* C,
-                     int k, bool accumulate, hipStream_t s) {
-  if (nrows <= 0 || k <= 0) return HN_OK;
+int spmm_rows_launch_batch(const int* rowptr, const int* col, const V* val, int64_t nrows, int64_t nnz, const V* B,
+                           V* C, int k, bool accumulate, int batch, long long istride, hipStream_t s) {
+  if (nrows <= 0 || k <= 0 || batch <= 0) return HN_OK;
   HN_CHECK(nrows * (int64_t)kWave < (int64_t)INT_MAX * 4, HN_ERR_UNSUPPORTED, "spmm: too many rows");
+  HN_CHECK(batch <= 65535, HN_ERR_UNSUPPORTED, "spmm: too many products in one batch");
   if (nnz >= 32 * nrows) {
     HN_CHECK(nrows < INT_MAX, HN_ERR_UNSUPPORTED, "spmm: too many rows");
-    hipLaunchKernelGGL((k_spmm_rows_long<V>), dim3((unsigned)nrows), dim3(256), 0, s, rowptr, col, val, B, C, k,
-                       accumulate ? 1 : 0);
+    hipLaunchKernelGGL((k_spmm_rows_long<V>), dim3((unsigned)nrows, (unsigned)batch), dim3(256), 0, s, rowptr, col,
+                       val, B, C, k, accumulate ? 1 : 0, istride);
   } else {
     const int64_t threads = nrows * (int64_t)kWave;
-    hipLaunchKernelGGL((k_spmm_rows<V>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, rowptr, col, val,
-                       B, C, (int)nrows, k, accumulate ? 1 : 0);
+    hipLaunchKernelGGL((k_spmm_rows<V>), dim3((unsigned)((threads + 255) / 256), (unsigned)batch), dim3(256), 0, s,
+                       rowptr, col, val, B, C, (int)nrows, k, accumulate ? 1 : 0, istride);
   }
   count_launch();
   return check_launch("spmm");
+}
+
+template <typename V>
+int spmm_rows_launch(const int* rowptr, const int* col, const V* val, int64_t nrows, int64_t nnz, const V* B, V* C,
+                     int k, bool accumulate, hipStream_t s) {
+  return spmm_rows_launch_batch<V>(rowptr, col, val, nrows, nnz, B, C, k, accumulate, 1, 0, s);
 }
 
 template int spmm_rows_launch<double>(const int*, const int*, const double*, int64_t, int64_t, const double*,
                                       double*, int, bool, hipStream_t);
 template int spmm_rows_launch<float>(const int*, const int*, const float*, int64_t, int64_t, const float*, float*,
                                      int, bool, hipStream_t);
+template int spmm_rows_launch_batch<float>(const int*, const int*, const float*, int64_t, int64_t, const float*,
+                                           float*, int, bool, int, long long, hipStream_t);
 
 }  // namespace hn
 
