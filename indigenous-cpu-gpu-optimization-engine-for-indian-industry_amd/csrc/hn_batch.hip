@@ -164,7 +164,12 @@ __device__ double wg_step(const double* __restrict__ v, const double* __restrict
   return (s < 1.0) ? s : 1.0;
 }
 
-// ---- dense LDL' in place (row-major, lower triangle) ------------------------
+// ---- dense LDL' in place (row-major) -------------------------------------------
+// The factor keeps L (unit, strictly below the diagonal) and D (on it) in
+// place of N's lower triangle and a copy of L' above the diagonal, where
+// N's upper half (never read after the assembly) was: the forward solve
+// then reads L's block columns as rows of L', contiguous across the threads
+// that handle consecutive rows, as the backward solve reads L itself.
 // LDS of the factorisation (FP64: 67 KB; gfx950 has 160 KB per workgroup).
 // L11 (factor and inverse phases) and the update tiles share storage.
 template <typename T>
@@ -251,14 +256,16 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
       }
     }
     ck.mark(PH_DIAG);
-    // write back L (unit, below the diagonal) and D; keep L11, D in LDS
+    // write back L (unit, below the diagonal), D, and above the diagonal
+    // L' (the elimination leaves L[j][i] at (i, j) for j > i); keep L11, D
+    // in LDS
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         const int i = 4 * ti + a, j = 4 * tj + c;
         if (i < b && j < b) {
-          if (j <= i) A[(long long)(k0 + i) * m + k0 + j] = P[a][c];
+          A[(long long)(k0 + i) * m + k0 + j] = P[a][c];
           sh.f.L11[i][j] = (j < i) ? P[a][c] : T(0);
           if (i == j) sh.f.D[i] = P[a][c];
         }
@@ -302,8 +309,10 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
         for (int q = 0; q < 16; ++q) {
           const int c = 16 * g + q;
           if (c < b) {
+            const T l = acc[q] / sh.f.D[c];
             wr[c] = acc[q];
-            row[c] = acc[q] / sh.f.D[c];
+            row[c] = l;
+            A[(long long)(k0 + c) * m + i] = l;  // L' above the diagonal
           }
         }
       }
@@ -464,6 +473,39 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
 // x = N^{-1} r with the LDL' factor in A (in-place on x, which holds r).
 // Every row segment is loaded in full before it is used, so each thread keeps
 // up to 64 loads in flight; the current block of x is shared through LDS.
+// x[i] -= sum_c A[k0 + c][i] xb[c] over the block's b columns, for the rows
+// i = i0, i0 + NT, ... below i1 (one per thread; the block's entries
+// A[k0 + c][i] of consecutive rows are contiguous), summed in column order.
+// A full block (b = 64, every block but the last) takes a path without the
+// per-column test: written with it, the compiler turns each of the 64 reads
+// into a branch of its own and spills their addresses.  xb is re-read from
+// LDS for every row: kept in registers across the rows, its 64 values push
+// the row's own 64 out to scratch memory.
+template <typename T>
+__device__ __forceinline__ void solve_rows(const T* __restrict__ A, int m, int k0, int b, int i0, int i1,
+                                           double* __restrict__ x, const double* xb) {
+  if (b == BK) {
+    for (int i = i0; i < i1; i += NT) {
+      asm volatile("" ::: "memory");
+      const T* col = A + (long long)k0 * m + i;
+      T lv[BK];
+#pragma unroll
+      for (int c = 0; c < BK; ++c) lv[c] = col[(long long)c * m];
+      double s = x[i];
+#pragma unroll
+      for (int c = 0; c < BK; ++c) s -= (double)lv[c] * xb[c];
+      x[i] = s;
+    }
+  } else {
+    for (int i = i0; i < i1; i += NT) {
+      asm volatile("" ::: "memory");
+      double s = x[i];
+      for (int c = 0; c < b; ++c) s -= (double)A[(long long)(k0 + c) * m + i] * xb[c];
+      x[i] = s;
+    }
+  }
+}
+
 template <typename T>
 __device__ __noinline__ void wg_ldlt_solve(const T* __restrict__ A, int m, double* __restrict__ x) {
   __shared__ double xb[BK];
@@ -475,7 +517,7 @@ __device__ __noinline__ void wg_ldlt_solve(const T* __restrict__ A, int m, doubl
     if (tid < 64) {
       T lr[BK];
 #pragma unroll
-      for (int c = 0; c < BK; ++c) lr[c] = (lane < b && c < lane) ? A[(long long)(k0 + lane) * m + k0 + c] : T(0);
+      for (int c = 0; c < BK; ++c) lr[c] = (lane < b && c < lane) ? A[(long long)(k0 + c) * m + k0 + lane] : T(0);
       double y = (lane < b) ? x[k0 + lane] : 0.0;
 #pragma unroll
       for (int c = 0; c < BK; ++c) {
@@ -488,17 +530,7 @@ __device__ __noinline__ void wg_ldlt_solve(const T* __restrict__ A, int m, doubl
       }
     }
     __syncthreads();
-    for (int i = k0 + b + tid; i < m; i += NT) {
-      const T* row = A + (long long)i * m + k0;
-      T lv[BK];
-#pragma unroll
-      for (int c = 0; c < BK; ++c) lv[c] = (c < b) ? row[c] : T(0);
-      double s = x[i];
-#pragma unroll
-      for (int c = 0; c < BK; ++c)
-        if (c < b) s -= (double)lv[c] * xb[c];
-      x[i] = s;
-    }
+    solve_rows<T>(A, m, k0, b, k0 + b + tid, m, x, xb);  // L[i][k0 + c] read from L' above the diagonal
     __syncthreads();
   }
   for (int i = tid; i < m; i += NT) {
@@ -527,16 +559,7 @@ __device__ __noinline__ void wg_ldlt_solve(const T* __restrict__ A, int m, doubl
       }
     }
     __syncthreads();
-    for (int i = tid; i < k0; i += NT) {
-      T lv[BK];
-#pragma unroll
-      for (int c = 0; c < BK; ++c) lv[c] = (c < b) ? A[(long long)(k0 + c) * m + i] : T(0);
-      double s = x[i];
-#pragma unroll
-      for (int c = 0; c < BK; ++c)
-        if (c < b) s -= (double)lv[c] * xb[c];
-      x[i] = s;
-    }
+    solve_rows<T>(A, m, k0, b, tid, k0, x, xb);
     __syncthreads();
   }
 }
