@@ -201,14 +201,25 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
     const int b = min(BK, m - k0);
     // (1) diagonal block, symmetric copy in registers
     T P[4][4];
+    if (b == BK) {  // a full block: no per-entry test around the reads
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < 4; ++a)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const int i = 4 * ti + a, j = 4 * tj + c;
-        const int hi = max(i, j), lo = min(i, j);
-        P[a][c] = (i < b && j < b) ? A[(long long)(k0 + hi) * m + k0 + lo] : T(i == j ? 1 : 0);
-      }
+        for (int c = 0; c < 4; ++c) {
+          const int i = 4 * ti + a, j = 4 * tj + c;
+          const int hi = max(i, j), lo = min(i, j);
+          P[a][c] = A[(long long)(k0 + hi) * m + k0 + lo];
+        }
+    } else {
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int i = 4 * ti + a, j = 4 * tj + c;
+          const int hi = max(i, j), lo = min(i, j);
+          P[a][c] = (i < b && j < b) ? A[(long long)(k0 + hi) * m + k0 + lo] : T(i == j ? 1 : 0);
+        }
+    }
     for (int k = 0; k < b; ++k) {
       const int par = k & 1, kk = k & 3;
       if (tj == (k >> 2)) {
@@ -294,8 +305,15 @@ __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W,
       if (i < m) {
         const T* row = A + (long long)i * m + k0;
         T av[BK];
+        // a full block reads its row without per-column tests (each would
+        // become a branch of its own around one read)
+        if (b == BK) {
 #pragma unroll
-        for (int k = 0; k < BK; ++k) av[k] = (k < b) ? row[k] : T(0);
+          for (int k = 0; k < BK; ++k) av[k] = row[k];
+        } else {
+#pragma unroll
+          for (int k = 0; k < BK; ++k) av[k] = (k < b) ? row[k] : T(0);
+        }
 #pragma unroll
         for (int k = 0; k < BK; ++k)
 #pragma unroll
