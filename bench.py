@@ -367,6 +367,27 @@ def run_lp(args) -> dict:
         log(f"batch {rname}: {count} LPs in {wall:.2f}s -> {count / wall:.1f} LPs/s {status}")
     batch["mixed_speedup_vs_f64_batch"] = round(batch["skip_rule_mixed"]["lps_per_s"] /
                                                 batch["skip_rule"]["lps_per_s"], 3)
+    # autotuned precision: per m bucket, FP64 or the FP32 factor, whichever
+    # solved a sample of the bucket faster (a one-time cost, timed apart)
+    sample = lps[::max(1, count // 512)]
+    t0 = time.perf_counter()
+    table = H.batch_autotune(sample, per_bucket=args.tune_per_bucket, pivot_rule=H.HN_PIVOT_SKIP, gap_tol=KKT_TOL)
+    tune_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    res = H.ipm_solve_batch(lps, precision=H.HN_PREC_AUTO, pivot_rule=H.HN_PIVOT_SKIP, gap_tol=KKT_TOL)
+    wall = time.perf_counter() - t0
+    status = {}
+    for r in res:
+        status[r.status] = status.get(r.status, 0) + 1
+    b = {"wall_s": round(wall, 3), "lps_per_s": round(count / wall, 2), "status": status,
+         "iterations_mean": round(float(np.mean([r.iterations for r in res])), 2),
+         "precision": "per m bucket from hn_batch_autotune", "autotune_s": round(tune_s, 2),
+         "autotune_sample": f"{len(sample)} LPs, at most {args.tune_per_bucket} per bucket",
+         "table": table, "at_iteration_limit": sum(r.iterations >= 100 for r in res)}
+    b.update(_kkt_max(res))
+    batch["skip_rule_autotuned"] = b
+    log(f"batch skip_rule_autotuned: {count} LPs in {wall:.2f}s -> {count / wall:.1f} LPs/s {status} "
+        f"(autotune {tune_s:.1f}s: {[(t['m_hi'], t['precision']) for t in table]})")
     res = runs["reference_rule"]
     # plain FP64 single-LP GPU solver, sequential, on a strided sample
     stride = max(1, count // args.seq_sample)
@@ -753,6 +774,7 @@ def main() -> int:
     ap.add_argument("--phases", action="store_true",
                     help="with --batch-range: print the per-phase workgroup time of the timed batched call")
     ap.add_argument("--seq-sample", type=int, default=64)
+    ap.add_argument("--tune-per-bucket", type=int, default=64, help="sample LPs per m bucket of the batch autotune")
     ap.add_argument("--cpu-lp-limit", type=float, default=8.0)
     ap.add_argument("--ws-test", type=int, default=1000, help="held-out transportation LPs of the warm-start leg")
     ap.add_argument("--cpu-single-limit", type=float, default=20.0, help="time limit of the reference CPU solve of the 2000x5000 LP")

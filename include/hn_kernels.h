@@ -298,6 +298,9 @@ typedef struct hn_lp {
 /* Arithmetic of the normal-equation solve. */
 #define HN_PREC_F64 1   /* plain FP64 everywhere                              */
 #define HN_PREC_MIXED 2 /* FP32 bulk, FP64 residuals/refinement, FP64 below mu_switch */
+#define HN_PREC_AUTO 3  /* hn_ipm_solve_batch only: per LP, the precision      */
+                        /* hn_batch_autotune measured faster for its m bucket  */
+                        /* (FP64 where nothing was measured)                   */
 
 /* Pivot rule of the normal-equation factorisation.
  * HN_PIVOT_CLAMP is the reference: N gets r = max(regularization,
@@ -451,7 +454,8 @@ int hn_ipm_solve(const hn_lp* lp, const hn_ipm_opts* opts, hn_ipm_result* res);
  * LP (largest first from a device work queue), each with the same algorithm
  * as hn_ipm_solve and a dense LDL' of its normal equations (standard-form
  * rows <= 20000 per LP).  opts->precision selects FP64 or FP32-factor +
- * FP64-refinement, opts->pivot_rule the factorisation's pivot rule and
+ * FP64-refinement, or per LP by the autotuned table (HN_PREC_AUTO,
+ * hn_batch_autotune), opts->pivot_rule the factorisation's pivot rule and
  * opts->mu_target an early stop; solver, time limit, interrupt and the
  * opts->x0/y0/s0 warm start are not used.  An LP's own lps[k].x0/y0/s0 warm
  * start is used when it is finite, x0 > 0, s0 > 0, its mu0 = x0's0/n is below
@@ -461,6 +465,27 @@ int hn_ipm_solve(const hn_lp* lp, const hn_ipm_opts* opts, hn_ipm_result* res);
  * No reference counterpart: the reference solves one LP per call
  * (InteriorPointSolver::solve, interior_point.cpp:402). */
 int hn_ipm_solve_batch(const hn_lp* lps, uint32_t count, const hn_ipm_opts* opts, hn_ipm_result* results);
+
+/* The batched solver's choices per size of LP, measured on the device. */
+#define HN_TUNE_BUCKETS 6
+typedef struct {
+  int m_hi[HN_TUNE_BUCKETS];       /* bucket b holds standard-form m <= m_hi[b] (and > m_hi[b-1]) */
+  int lps[HN_TUNE_BUCKETS];        /* sample LPs timed in the bucket (0: none, FP64 kept)         */
+  int precision[HN_TUNE_BUCKETS];  /* HN_PREC_F64 or HN_PREC_MIXED: the faster one               */
+  double ms_f64[HN_TUNE_BUCKETS];  /* batched wall time of the bucket's sample, FP64              */
+  double ms_mixed[HN_TUNE_BUCKETS];/* ... with the FP32 factor and FP64 refinement                */
+} hn_batch_tuning;
+/* Autotuner of the batched solver.  Splits the sample `lps` into m buckets
+ * (standard-form rows <= 128, 256, 512, 1024, 2048, larger), solves each
+ * bucket's LPs (at most per_bucket of them) as one batch in each precision
+ * under *opts (pivot rule, gap and end-game settings as the caller will use
+ * them; precision ignored), after one untimed warm-up batch of the whole
+ * sample, and keeps the faster precision per bucket.  The table is stored
+ * for the device (hn_ipm_solve_batch with HN_PREC_AUTO uses it until the
+ * next call or hn_finalize) and copied to *out when out is not null.
+ * No reference counterpart: the reference has no batched solver. */
+int hn_batch_autotune(const hn_lp* lps, uint32_t count, const hn_ipm_opts* opts, uint32_t per_bucket,
+                      hn_batch_tuning* out);
 
 /* ---- learned warm start ------------------------------------------------- */
 /* A bipartite constraint-variable graph network (4 message-passing layers,

@@ -52,6 +52,7 @@ struct Inst {
   long long dn, w;    // dense N (m*m elements), panel workspace (m*64 elements)
   int warm;           // 1: x, z, lam hold a warm start accepted on the host; 2: one predicted on the
                       // device, tested by the kernel before use
+  int mixed;          // 1: FP32 factor with FP64 refinement until mu < mu_switch; 0: FP64
   double off;         // objective constant of the standard form: c'x of the caller's LP = c_s'x_s + off
 };
 
@@ -106,7 +107,6 @@ struct Clock {
 struct Params {
   int max_iterations;
   double conv_tol, comp_tol, reg, mu_switch;
-  int mixed;
   int skip;  // HN_PIVOT_SKIP: r = reg only, tiny pivots skipped
   double mu_target;  // > 0: stop at the first iterate with mu <= mu_target
   double gap_tol;    // > 0: the stop test also needs |c_s'x_s - b_s'lam| <= gap_tol (1 + |c'x|), c'x the
@@ -755,7 +755,7 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
     double* mdrd = c.vm[VM_MDRD];
     double* mz = c.vm[VM_MZ];
     int fp64_from = -1, ne_solves = 0, refine = 0;
-    bool fp64 = !prm.mixed;
+    bool fp64 = !I.mixed;
     // start point (interior_point.cpp:443-453), unless the host accepted a
     // warm start and packed it into x, z, lam
     auto cold_start = [&]() {
@@ -1244,6 +1244,41 @@ int upload_pinned(DevBuf& d, const PinnedBuf& h, long long n, hipStream_t s) {
 
 }  // namespace
 
+namespace {
+
+// The autotuned table of the device the library runs on (hn_batch_autotune).
+struct BatchTuning {
+  std::mutex mtx;
+  bool valid = false;
+  hn_batch_tuning t{};
+};
+BatchTuning& batch_tuning() {
+  static BatchTuning* b = new BatchTuning();  // reset by hn_finalize
+  return *b;
+}
+constexpr int kTuneEdges[HN_TUNE_BUCKETS] = {128, 256, 512, 1024, 2048, 20000};
+int tune_bucket(int64_t m) {
+  int b = 0;
+  while (b + 1 < HN_TUNE_BUCKETS && m > kTuneEdges[b]) ++b;
+  return b;
+}
+
+// 1 when LP k (standard-form m rows) runs with the FP32 factor.
+int instance_mixed(int precision, int64_t m, const hn_batch_tuning* table) {
+  if (precision == HN_PREC_MIXED) return 1;
+  if (precision != HN_PREC_AUTO || !table) return 0;
+  return table->precision[tune_bucket(m)] == HN_PREC_MIXED ? 1 : 0;
+}
+
+}  // namespace
+
+void reset_batch_tuning() {
+  BatchTuning& bt = batch_tuning();
+  std::lock_guard<std::mutex> lk(bt.mtx);
+  bt.valid = false;
+}
+
+
 int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o, hn_ipm_result* res,
                          hn_gnn* net = nullptr, hn_gnn_graph* graph = nullptr) {
   const auto t0 = std::chrono::steady_clock::now();
@@ -1272,6 +1307,15 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
     if (f[k].m == 0 || f[k].n == 0 || lps[k].n == 0)
       return fail(HN_ERR_UNSUPPORTED, "batch LP " + std::to_string(k) + " has an empty standard form");
     if (f[k].m > 20000) return fail(HN_ERR_UNSUPPORTED, "batch LPs are limited to 20000 standard-form rows");
+  }
+  // the autotuned precision per m bucket, when asked for
+  hn_batch_tuning table{};
+  bool have_table = false;
+  if (o.precision == HN_PREC_AUTO) {
+    BatchTuning& bt = batch_tuning();
+    std::lock_guard<std::mutex> lk(bt.mtx);
+    have_table = bt.valid;
+    table = bt.t;
   }
   // pack: offsets first, then every LP's slices filled in parallel straight
   // into page-locked staging
@@ -1303,6 +1347,7 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
     I.w = wk;
     wk += (long long)F.m * BK;
     I.warm = warm_ok[k] ? 1 : (predict[k] ? 2 : 0);
+    I.mixed = instance_mixed(o.precision, F.m, have_table ? &table : nullptr);
     I.off = F.offset;
     any_warm = any_warm || warm_ok[k];
   }
@@ -1406,7 +1451,6 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
   prm.comp_tol = o.complementarity_tol;
   prm.reg = o.regularization;
   prm.mu_switch = o.mu_switch;
-  prm.mixed = o.precision == HN_PREC_MIXED;
   prm.skip = o.pivot_rule == HN_PIVOT_SKIP;
   prm.mu_target = o.mu_target;
   prm.gap_tol = o.gap_tol;
@@ -1519,8 +1563,8 @@ extern "C" int hn_ipm_solve_batch(const hn_lp* lps, uint32_t count, const hn_ipm
     o = *opts;
   else
     hn_ipm_default_opts(&o);
-  HN_CHECK(o.precision == HN_PREC_F64 || o.precision == HN_PREC_MIXED, HN_ERR_INVALID,
-           "hn_ipm_solve_batch: bad precision");
+  HN_CHECK(o.precision == HN_PREC_F64 || o.precision == HN_PREC_MIXED || o.precision == HN_PREC_AUTO,
+           HN_ERR_INVALID, "hn_ipm_solve_batch: bad precision");
   HN_CHECK(o.pivot_rule == HN_PIVOT_CLAMP || o.pivot_rule == HN_PIVOT_SKIP, HN_ERR_INVALID,
            "hn_ipm_solve_batch: bad pivot_rule");
   return hn::ipm_solve_batch_impl(lps, count, o, results);
@@ -1535,9 +1579,71 @@ extern "C" int hn_ipm_solve_batch_gnn(const hn_lp* lps, uint32_t count, const hn
     o = *opts;
   else
     hn_ipm_default_opts(&o);
-  HN_CHECK(o.precision == HN_PREC_F64 || o.precision == HN_PREC_MIXED, HN_ERR_INVALID,
-           "hn_ipm_solve_batch_gnn: bad precision");
+  HN_CHECK(o.precision == HN_PREC_F64 || o.precision == HN_PREC_MIXED || o.precision == HN_PREC_AUTO,
+           HN_ERR_INVALID, "hn_ipm_solve_batch_gnn: bad precision");
   HN_CHECK(o.pivot_rule == HN_PIVOT_CLAMP || o.pivot_rule == HN_PIVOT_SKIP, HN_ERR_INVALID,
            "hn_ipm_solve_batch_gnn: bad pivot_rule");
   return hn::ipm_solve_batch_impl(lps, count, o, results, net, graph);
+}
+
+extern "C" int hn_batch_autotune(const hn_lp* lps, uint32_t count, const hn_ipm_opts* opts, uint32_t per_bucket,
+                                 hn_batch_tuning* out) {
+  using namespace hn;
+  HN_CHECK(count > 0 && lps, HN_ERR_INVALID, "hn_batch_autotune: empty sample");
+  HN_CHECK(per_bucket > 0, HN_ERR_INVALID, "hn_batch_autotune: per_bucket must be positive");
+  hn_ipm_opts o;
+  if (opts)
+    o = *opts;
+  else
+    hn_ipm_default_opts(&o);
+  HN_CHECK(o.pivot_rule == HN_PIVOT_CLAMP || o.pivot_rule == HN_PIVOT_SKIP, HN_ERR_INVALID,
+           "hn_batch_autotune: bad pivot_rule");
+  o.verbose = 0;
+  // bucket the sample by standard-form rows
+  std::vector<std::vector<hn_lp>> bucket(HN_TUNE_BUCKETS);
+  for (uint32_t k = 0; k < count; ++k) {
+    StandardForm f;
+    HN_TRY(build_standard_form(lps[k], f));
+    std::vector<hn_lp>& b = bucket[tune_bucket(f.m)];
+    if (b.size() < per_bucket) b.push_back(lps[k]);
+  }
+  std::vector<hn_ipm_result> res(count);
+  std::vector<std::vector<double>> primal(count);
+  auto solve = [&](const std::vector<hn_lp>& set, int precision, double* ms) -> int {
+    for (size_t k = 0; k < set.size(); ++k) {
+      primal[k].assign(std::max<int64_t>(set[k].n, 1), 0.0);
+      std::memset(&res[k], 0, sizeof(hn_ipm_result));
+      res[k].primal = primal[k].data();
+    }
+    hn_ipm_opts oo = o;
+    oo.precision = precision;
+    const auto t0 = std::chrono::steady_clock::now();
+    HN_TRY(ipm_solve_batch_impl(set.data(), (uint32_t)set.size(), oo, res.data()));
+    *ms = 1e3 * std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return HN_OK;
+  };
+  {  // warm-up: grows the arenas and staging to the whole sample's size
+    std::vector<hn_lp> all;
+    for (const auto& b : bucket) all.insert(all.end(), b.begin(), b.end());
+    double ms = 0.0;
+    HN_TRY(solve(all, HN_PREC_F64, &ms));
+  }
+  hn_batch_tuning t{};
+  for (int b = 0; b < HN_TUNE_BUCKETS; ++b) {
+    t.m_hi[b] = kTuneEdges[b];
+    t.lps[b] = (int)bucket[b].size();
+    t.precision[b] = HN_PREC_F64;
+    if (bucket[b].empty()) continue;
+    HN_TRY(solve(bucket[b], HN_PREC_F64, &t.ms_f64[b]));
+    HN_TRY(solve(bucket[b], HN_PREC_MIXED, &t.ms_mixed[b]));
+    if (t.ms_mixed[b] < t.ms_f64[b]) t.precision[b] = HN_PREC_MIXED;
+  }
+  {
+    BatchTuning& bt = batch_tuning();
+    std::lock_guard<std::mutex> lk(bt.mtx);
+    bt.t = t;
+    bt.valid = true;
+  }
+  if (out) *out = t;
+  return HN_OK;
 }

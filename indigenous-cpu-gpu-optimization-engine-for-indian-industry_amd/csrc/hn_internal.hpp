@@ -79,6 +79,8 @@ struct HostOpScratch {
 HostOpScratch& host_scratch();
 // Frees the verifier's cached device buffers (hn_finalize).
 void release_verify_scratch();
+// Forgets the batched solver's autotuned table (hn_finalize).
+void reset_batch_tuning();
 
 // ---- device CSR -------------------------------------------------------------
 constexpr int kSpmvBlock = 256;

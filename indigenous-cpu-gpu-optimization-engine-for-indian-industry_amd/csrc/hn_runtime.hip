@@ -152,6 +152,7 @@ extern "C" int hn_finalize(void) {
     s.red.release();
   }
   release_verify_scratch();
+  reset_batch_tuning();
   (void)hipStreamDestroy(c.stream);
   c.stream = nullptr;
   c.initialized = false;

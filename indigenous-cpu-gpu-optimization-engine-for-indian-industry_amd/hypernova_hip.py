@@ -536,7 +536,7 @@ def gpu_visible() -> bool:
 # Interior-point solver (hn_ipm_solve)
 # ---------------------------------------------------------------------------
 HN_NE_AUTO, HN_NE_DENSE, HN_NE_PCG, HN_NE_SPARSE = 0, 1, 2, 3
-HN_PREC_F64, HN_PREC_MIXED = 1, 2
+HN_PREC_F64, HN_PREC_MIXED, HN_PREC_AUTO = 1, 2, 3
 HN_PIVOT_CLAMP, HN_PIVOT_SKIP = 0, 1
 HN_LOOP_GRAPH, HN_LOOP_DEVICE, HN_LOOP_HOST = 0, 1, 2
 STATUS = ["UNKNOWN", "OPTIMAL", "INFEASIBLE", "UNBOUNDED", "SUBOPTIMAL", "TIME_LIMIT",
@@ -768,6 +768,52 @@ def ipm_solve_batch(lps, precision: int = HN_PREC_F64, max_iterations: int = 100
         check(LIB.hn_ipm_solve_batch(arr, count, C.byref(opts), res))
     return [InteriorPointResult(res[k], bufs[k][0][:lps[k].n].copy(), bufs[k][1], bufs[k][2], bufs[k][3])
             for k in range(count)]
+
+
+HN_TUNE_BUCKETS = 6
+
+
+class _HnTuning(C.Structure):
+    _fields_ = [("m_hi", C.c_int * HN_TUNE_BUCKETS), ("lps", C.c_int * HN_TUNE_BUCKETS),
+                ("precision", C.c_int * HN_TUNE_BUCKETS), ("ms_f64", C.c_double * HN_TUNE_BUCKETS),
+                ("ms_mixed", C.c_double * HN_TUNE_BUCKETS)]
+
+
+LIB.hn_batch_autotune.argtypes = [C.POINTER(_HnLp), C.c_uint32, C.POINTER(_HnOpts), C.c_uint32, C.POINTER(_HnTuning)]
+LIB.hn_batch_autotune.restype = C.c_int
+
+
+def batch_autotune(lps, per_bucket: int = 128, pivot_rule: int = HN_PIVOT_CLAMP, gap_tol: float = 0.0,
+                   endgame_mu: float = 0.0, max_iterations: int = 100, mu_switch: float = 1e-4) -> list:
+    """Autotunes the batched solver on a sample (hn_batch_autotune): per m
+    bucket of the standard form, the batch of at most per_bucket sample LPs
+    is solved in FP64 and with the FP32 factor, and the faster precision is
+    stored for the device (ipm_solve_batch(precision=HN_PREC_AUTO) uses it).
+    Returns one dict per bucket: m_hi, lps, precision, ms_f64, ms_mixed."""
+    count = len(lps)
+    arr = (_HnLp * count)()
+    keep = []
+    for k, lp in enumerate(lps):
+        d = dict(rp=np.ascontiguousarray(lp.row_ptr, np.int64), ci=np.ascontiguousarray(lp.col, np.int64),
+                 v=np.ascontiguousarray(lp.val, np.float64), sense=np.ascontiguousarray(lp.sense, np.int8),
+                 rhs=np.ascontiguousarray(lp.rhs, np.float64), lb=np.ascontiguousarray(lp.lb, np.float64),
+                 ub=np.ascontiguousarray(lp.ub, np.float64), obj=np.ascontiguousarray(lp.obj, np.float64))
+        keep.append(d)
+        arr[k] = _HnLp(lp.m, lp.n, d["rp"].ctypes.data_as(_i64p), d["ci"].ctypes.data_as(_i64p), _dptr(d["v"]),
+                       d["sense"].ctypes.data_as(_i8p), _dptr(d["rhs"]), _dptr(d["lb"]), _dptr(d["ub"]),
+                       _dptr(d["obj"]), int(lp.maximize), float(getattr(lp, "obj_offset", 0.0)))
+    opts = _HnOpts()
+    check(LIB.hn_ipm_default_opts(C.byref(opts)))
+    opts.pivot_rule = pivot_rule
+    opts.gap_tol = gap_tol
+    opts.endgame_mu = endgame_mu
+    opts.max_iterations = max_iterations
+    opts.mu_switch = mu_switch
+    t = _HnTuning()
+    check(LIB.hn_batch_autotune(arr, count, C.byref(opts), per_bucket, C.byref(t)))
+    return [{"m_hi": t.m_hi[b], "lps": t.lps[b],
+             "precision": {HN_PREC_F64: "f64", HN_PREC_MIXED: "mixed"}[t.precision[b]],
+             "ms_f64": round(t.ms_f64[b], 2), "ms_mixed": round(t.ms_mixed[b], 2)} for b in range(HN_TUNE_BUCKETS)]
 
 
 LIB.hn_lp_standard_form.argtypes = [C.POINTER(_HnLp), C.POINTER(C.c_int64), C.POINTER(C.c_int64),

@@ -169,3 +169,39 @@ def test_batch_endgame_projection_transport(hip):
             assert rel(r.objective_value, h) <= 1e-8
             assert r.iterations <= p.iterations
     assert ended == len(lps)
+
+
+def test_batch_autotune_and_auto_precision(hip):
+    """hn_batch_autotune (not in the reference) times every m bucket of a
+    sample in FP64 and with the FP32 factor and keeps the faster;
+    HN_PREC_AUTO then solves each LP in its bucket's precision, with the
+    same bits as a batch in that precision alone (an LP's solve does not
+    depend on the others in its batch)."""
+    sample = [lpgen.batch_lp(i) for i in range(48)]
+    print(f"[case] autotune on {len(sample)} batch LPs", flush=True)
+    table = hip.batch_autotune(sample, per_bucket=8, pivot_rule=hip.HN_PIVOT_SKIP, gap_tol=1e-8)
+    print(f"  table {table}", flush=True)
+    assert [t["m_hi"] for t in table] == [128, 256, 512, 1024, 2048, 20000]
+    ms = [hip.lp_standard_form(lp)["m"] for lp in sample]
+    for b, t in enumerate(table):
+        lo = table[b - 1]["m_hi"] if b else 0
+        assert t["lps"] == min(8, sum(lo < m <= t["m_hi"] for m in ms))
+        if t["lps"]:
+            assert t["ms_f64"] > 0 and t["ms_mixed"] > 0
+            assert t["precision"] == ("mixed" if t["ms_mixed"] < t["ms_f64"] else "f64")
+        else:
+            assert t["precision"] == "f64"
+    lps = sample[:12]
+    kw = dict(pivot_rule=hip.HN_PIVOT_SKIP, gap_tol=1e-8)
+    auto = hip.ipm_solve_batch(lps, precision=hip.HN_PREC_AUTO, **kw)
+    runs = {"f64": hip.ipm_solve_batch(lps, precision=hip.HN_PREC_F64, **kw),
+            "mixed": hip.ipm_solve_batch(lps, precision=hip.HN_PREC_MIXED, **kw)}
+    for k, lp in enumerate(lps):
+        m = ms[k]
+        b = next(i for i, t in enumerate(table) if m <= t["m_hi"])
+        want = runs[table[b]["precision"]][k]
+        assert auto[k].status == want.status == "OPTIMAL"
+        assert auto[k].iterations == want.iterations
+        assert auto[k].objective_value == want.objective_value
+        assert np.array_equal(auto[k].primal, want.primal)
+        assert (auto[k].fp64_from_iter >= 0) == (table[b]["precision"] == "mixed")
