@@ -170,6 +170,27 @@ __device__ double wg_step(const double* __restrict__ v, const double* __restrict
 // N's upper half (never read after the assembly) was: the forward solve
 // then reads L's block columns as rows of L', contiguous across the threads
 // that handle consecutive rows, as the backward solve reads L itself.
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+// v_mfma_f64_16x16x4 / v_mfma_f32_16x16x4f32: lane l holds row l%16 of A
+// and column l%16 of B at k = l/16.
+__device__ __forceinline__ d4_t mfma_16x16x4(double a, double b, d4_t c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4_t mfma_16x16x4(float a, float b, f4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+// Tile row and column of output element q of block a (a: row half a >> 1,
+// column half a & 1 of the wave's 32 x 32 quarter at (rq, cq)): FP64 lane
+// l, element q is row 4q + l/16, column l%16 of a block; FP32 row
+// 4(l/16) + q (both read back from the device).
+template <typename T>
+__device__ __forceinline__ void mfma_entry(int ln, int rq, int cq, int a, int q, int& r, int& c) {
+  constexpr bool kF64 = std::is_same<T, double>::value;
+  r = rq + 16 * (a >> 1) + (kF64 ? 4 * q + (ln >> 4) : 4 * (ln >> 4) + q);
+  c = cq + 16 * (a & 1) + (ln & 15);
+}
+
 // LDS of the factorisation (FP64: 67 KB; gfx950 has 160 KB per workgroup).
 // L11 (factor and inverse phases) and the update tiles share storage.
 template <typename T>
@@ -177,10 +198,14 @@ struct SharedLDL {
   struct {
     T rk[2][BK];
     T D[BK];
+    double dgb[BK];  // dg of the block (skip rule)
     T L11[BK][BK + 1];
   } f;
   union {
-    T X[BK][BK];
+    struct {
+      T X[BK][65];   // L11^{-1}
+      T Ts[64][65];  // a 64-row tile of A21, then of W
+    } p;
     struct {
       T Wr[BK][65];      // W of the current tile row, column-major (k, row)
       T Ls[BK / 2][65];  // L slice of the current step, (k, row)
@@ -188,302 +213,370 @@ struct SharedLDL {
   };
 };
 
-// Returns the number of clamped (or skipped) pivots.  skip_rel == 0: the
-// reference's clamp of pivots <= tol; skip_rel > 0: pivots <= skip_rel times
-// dg (the diagonal of N before elimination) are skipped.
+// The blocked LDL' runs its three parts per 64-column block as separate
+// functions, so that each gets the registers of its own working set (values
+// live in one part would otherwise hold registers through the others, and
+// the loads of a part wait one at a time).
+//
+// (1) The diagonal block: returns the number of clamped (or skipped)
+// pivots.  skip_rel == 0: the reference's clamp of pivots <= tol;
+// skip_rel > 0: pivots <= skip_rel times dg (the diagonal of N before
+// elimination, staged in LDS for the block) are skipped.  Leaves L11 (unit)
+// and D in LDS.
 template <typename T>
-__device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W, SharedLDL<T>& sh, double tol,
-                                    const double* __restrict__ dg, double skip_rel, Clock ck) {
+__device__ __noinline__ int ldl_diag(T* __restrict__ A, int m, int k0, int b, SharedLDL<T>& sh, double tol,
+                                     const double* __restrict__ dg, double skip_rel) {
   const int tid = threadIdx.x;
   const int ti = tid >> 4, tj = tid & 15;
   int nclamp = 0;
-  for (int k0 = 0; k0 < m; k0 += BK) {
-    const int b = min(BK, m - k0);
-    // (1) diagonal block, symmetric copy in registers
-    T P[4][4];
-    if (b == BK) {  // a full block: no per-entry test around the reads
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const int i = 4 * ti + a, j = 4 * tj + c;
-          const int hi = max(i, j), lo = min(i, j);
-          P[a][c] = A[(long long)(k0 + hi) * m + k0 + lo];
-        }
-    } else {
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const int i = 4 * ti + a, j = 4 * tj + c;
-          const int hi = max(i, j), lo = min(i, j);
-          P[a][c] = (i < b && j < b) ? A[(long long)(k0 + hi) * m + k0 + lo] : T(i == j ? 1 : 0);
-        }
-    }
-    for (int k = 0; k < b; ++k) {
-      const int par = k & 1, kk = k & 3;
-      if (tj == (k >> 2)) {
-#pragma unroll
-        for (int a = 0; a < 4; ++a) sh.f.rk[par][4 * ti + a] = P[a][kk];
-      }
-      __syncthreads();
-      T piv = sh.f.rk[par][k];
-      T rinv;
-      if (skip_rel > 0.0) {
-        if (!((double)piv > skip_rel * dg[k0 + k])) {
-          piv = skipped_pivot<T>();
-          rinv = T(0);
-          ++nclamp;
-        } else {
-          rinv = T(1) / piv;
-        }
-      } else {
-        if (piv <= T(tol)) {
-          piv = T(tol);
-          ++nclamp;
-        }
-        rinv = T(1) / piv;
-      }
-      T ci[4], cj[4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) ci[a] = sh.f.rk[par][4 * ti + a];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) cj[c] = sh.f.rk[par][4 * tj + c];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        const int i = 4 * ti + a;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const int j = 4 * tj + c;
-          if (i == k && j == k)
-            P[a][c] = piv;
-          else if (j == k && i > k)
-            P[a][c] = ci[a] * rinv;
-          else if (i == k && j > k)
-            P[a][c] = cj[c] * rinv;
-          else if (i > k && j > k)
-            P[a][c] = P[a][c] - ci[a] * (cj[c] * rinv);
-        }
-      }
-    }
-    ck.mark(PH_DIAG);
-    // write back L (unit, below the diagonal), D, and above the diagonal
-    // L' (the elimination leaves L[j][i] at (i, j) for j > i); keep L11, D
-    // in LDS
+  if (skip_rel > 0.0 && tid < b) sh.f.dgb[tid] = dg[k0 + tid];
+  // symmetric copy of the block in registers
+  T P[4][4];
+  if (b == BK) {  // a full block: no per-entry test around the reads
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         const int i = 4 * ti + a, j = 4 * tj + c;
-        if (i < b && j < b) {
-          A[(long long)(k0 + i) * m + k0 + j] = P[a][c];
-          sh.f.L11[i][j] = (j < i) ? P[a][c] : T(0);
-          if (i == j) sh.f.D[i] = P[a][c];
-        }
+        const int hi = max(i, j), lo = min(i, j);
+        P[a][c] = A[(long long)(k0 + hi) * m + k0 + lo];
       }
-    __syncthreads();
-    // (2) panel.  X = L11^{-1} (unit lower), one column
-    //     per thread; then W = A21 L11^{-T} = A21 X' and L21 = W / D, 64 rows
-    //     at a time: thread (r = tid % 64, g = tid / 64) forms 16 columns.
-    T(*X)[BK] = sh.X;
-    if (tid < 64) {
-      const int j = tid;
-      for (int i = 0; i < BK; ++i) {
-        T acc = 0;
-        for (int k = 0; k < i; ++k)
-          if (k >= j) acc += sh.f.L11[i][k] * X[k][j];
-        X[i][j] = (i > j) ? -acc : T(i == j ? 1 : 0);
-      }
-    }
-    __syncthreads();
-    for (int i0 = k0 + b; i0 < m; i0 += 64) {
-      const int r = tid & 63, g = tid >> 6;
-      const int i = i0 + r;
-      T acc[16];
-#pragma unroll
-      for (int q = 0; q < 16; ++q) acc[q] = 0;
-      if (i < m) {
-        const T* row = A + (long long)i * m + k0;
-        T av[BK];
-        // a full block reads its row without per-column tests (each would
-        // become a branch of its own around one read)
-        if (b == BK) {
-#pragma unroll
-          for (int k = 0; k < BK; ++k) av[k] = row[k];
-        } else {
-#pragma unroll
-          for (int k = 0; k < BK; ++k) av[k] = (k < b) ? row[k] : T(0);
-        }
-#pragma unroll
-        for (int k = 0; k < BK; ++k)
-#pragma unroll
-          for (int q = 0; q < 16; ++q) acc[q] += av[k] * X[16 * g + q][k];
-      }
-      __syncthreads();  // every read of the A21 rows precedes the writes below
-      if (i < m) {
-        T* row = A + (long long)i * m + k0;
-        T* wr = W + (long long)i * BK;
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-          const int c = 16 * g + q;
-          if (c < b) {
-            const T l = acc[q] / sh.f.D[c];
-            wr[c] = acc[q];
-            row[c] = l;
-            A[(long long)(k0 + c) * m + i] = l;  // L' above the diagonal
-          }
-        }
-      }
-    }
-    __syncthreads();
-    ck.mark(PH_PANEL);
-    // (3) trailing update of the lower triangle: A_ij -= sum_c W_ic L_jc,
-    //     64 x 64 tiles (I, J <= I) in steps of 32 columns c.  A tile row's
-    //     W stays in LDS for all its tiles; each step's L slice (columns
-    //     < k0 + b of A, which the update does not write) and the next tile
-    //     row's W are loaded from memory into registers during the step
-    //     before, then stored to LDS rows padded to 65, so a store's lanes
-    //     of one row fall in different banks.
-    const int r0 = k0 + b;
-    const int nt = (m - r0 + 63) / 64;
-    const int ncb = (b + BK / 2 - 1) / (BK / 2);
-    const int nsteps = nt * (nt + 1) / 2 * ncb;
-    constexpr int kSlots = 64 * (BK / 2) / NT;  // a thread's (row, column) slots of an L slice
-    constexpr int kRowSlots = 64 * BK / NT;     // ... of a tile row's W
-    T pl[kSlots], pw[kRowSlots];
-    auto fetch_l = [&](int tJ, int cb) {
-      const int j0 = r0 + 64 * tJ, cw = min(BK / 2, b - cb);
-#pragma unroll
-      for (int u = 0; u < kSlots; ++u) {
-        const int e = tid + NT * u;
-        const int r = e / (BK / 2), c = e % (BK / 2);
-        const int j = j0 + r;
-        pl[u] = (j < m && c < cw) ? A[(long long)j * m + k0 + cb + c] : T(0);
-      }
-    };
-    auto fetch_w = [&](int tI) {  // W rows of tile row tI, all b columns
-      const int i0 = r0 + 64 * tI;
-#pragma unroll
-      for (int u = 0; u < kRowSlots; ++u) {
-        const int e = tid + NT * u;
-        const int r = e / BK, c = e % BK;
-        pw[u] = (i0 + r < m && c < b) ? W[(long long)(i0 + r) * BK + c] : T(0);
-      }
-    };
-    // Each thread owns 16 entries of a tile: the four 16 x 16 blocks of its
-    // wave's 32 x 32 quarter in the output layout of v_mfma_f64_16x16x4
-    // (lane l, element e: row 4e + l/16, column l%16 of a block) or of
-    // v_mfma_f32_16x16x4f32 (row 4(l/16) + e, column l%16); both layouts
-    // were read back from the device.  Operands: lane l holds row l%16 of
-    // A and column l%16 of B at k = l/16.
-    constexpr bool kF64 = std::is_same<T, double>::value;
-    const int ln = tid & (kWave - 1), wq = tid / kWave;
-    const int rq = 32 * (wq >> 1), cq = 32 * (wq & 1);
-    auto entry = [&](int a, int q, int& r, int& c) {  // tile row / column of entry (a, q)
-      // a: block (row half, column half), q: element
-      r = rq + 16 * (a >> 1) + (kF64 ? 4 * q + (ln >> 4) : 4 * (ln >> 4) + q);
-      c = cq + 16 * (a & 1) + (ln & 15);
-    };
-    T acc[4][4];
+  } else {
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) acc[a][c] = 0;
-    int tI = 0, tJ = 0, cbi = 0;
-    if (nsteps > 0) {
-      fetch_w(0);
-      fetch_l(0, 0);
+      for (int c = 0; c < 4; ++c) {
+        const int i = 4 * ti + a, j = 4 * tj + c;
+        const int hi = max(i, j), lo = min(i, j);
+        P[a][c] = (i < b && j < b) ? A[(long long)(k0 + hi) * m + k0 + lo] : T(i == j ? 1 : 0);
+      }
+  }
+  // four steps per pass, so the pivot column's register index is a
+  // constant (a variable index puts P in scratch memory)
+  for (int kb = 0; kb < b; kb += 4)
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) {
+    const int k = kb + kk, par = kk & 1;
+    if (k >= b) break;
+    if (tj == (kb >> 2)) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a) sh.f.rk[par][4 * ti + a] = P[a][kk];
     }
-    T aold[4][4];  // the tile's entries of A, read when its first step starts
-    for (int st = 0; st < nsteps; ++st) {
-      if (cbi == 0) {
-        const int i0 = r0 + 64 * tI, j0 = r0 + 64 * tJ;
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            int r, c;
-            entry(a, q, r, c);
-            const int i = i0 + r, j = j0 + c;
-            aold[a][q] = (i < m && j < m && j <= i) ? A[(long long)i * m + j] : T(0);
-          }
+    __syncthreads();
+    T piv = sh.f.rk[par][k];
+    T rinv;
+    if (skip_rel > 0.0) {
+      if (!((double)piv > skip_rel * sh.f.dgb[k])) {
+        piv = skipped_pivot<T>();
+        rinv = T(0);
+        ++nclamp;
+      } else {
+        rinv = T(1) / piv;
       }
-      if (tJ == 0 && cbi == 0) {  // a new tile row: its W into LDS
+    } else {
+      if (piv <= T(tol)) {
+        piv = T(tol);
+        ++nclamp;
+      }
+      rinv = T(1) / piv;
+    }
+    T ci[4], cj[4];
 #pragma unroll
-        for (int u = 0; u < kRowSlots; ++u) {
-          const int e = tid + NT * u;
-          sh.u.Wr[e % BK][e / BK] = pw[u];
+    for (int a = 0; a < 4; ++a) ci[a] = sh.f.rk[par][4 * ti + a];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) cj[c] = sh.f.rk[par][4 * tj + c];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int i = 4 * ti + a;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int j = 4 * tj + c;
+        if (i == k && j == k)
+          P[a][c] = piv;
+        else if (j == k && i > k)
+          P[a][c] = ci[a] * rinv;
+        else if (i == k && j > k)
+          P[a][c] = cj[c] * rinv;
+        else if (i > k && j > k)
+          P[a][c] = P[a][c] - ci[a] * (cj[c] * rinv);
+      }
+    }
+  }
+  // write back L (unit, below the diagonal), D, and above the diagonal
+  // L' (the elimination leaves L[j][i] at (i, j) for j > i); keep L11, D
+  // in LDS
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int i = 4 * ti + a, j = 4 * tj + c;
+      if (i < b && j < b) {
+        A[(long long)(k0 + i) * m + k0 + j] = P[a][c];
+        sh.f.L11[i][j] = (j < i) ? P[a][c] : T(0);
+        if (i == j) sh.f.D[i] = P[a][c];
+      }
+    }
+  __syncthreads();
+  return nclamp;
+}
+
+// (2) The panel, for the rows below the block (k0 + b < m).  X = L11^{-1}
+// (unit lower), one column per thread of the first wave, held in registers;
+// then W = A21 L11^{-T} = A21 X' and L21 = W / D for 64 rows at a time: the
+// rows are staged through LDS by coalesced loads, thread (r = tid % 64,
+// g = tid / 64) forms 16 columns of row r in column order, and W, L21 and
+// L21' leave through LDS by coalesced stores.
+template <typename T>
+__device__ __noinline__ void ldl_panel(T* __restrict__ A, int m, int k0, int b, T* __restrict__ W,
+                                       SharedLDL<T>& sh) {
+  const int tid = threadIdx.x;
+  typedef T t4 __attribute__((ext_vector_type(4)));
+  T(*X)[65] = sh.p.X;
+  T(*Ts)[65] = sh.p.Ts;
+  const int ln = tid & (kWave - 1), wq = tid / kWave;
+  const int rq = 32 * (wq >> 1), cq = 32 * (wq & 1);
+  if (tid < 64) {
+    const int j = tid;
+    T x[BK];
+#pragma unroll
+    for (int i = 0; i < BK; ++i) {
+      // x[k] = 0 for k < j: those terms add exact zeros
+      T acc = 0;
+#pragma unroll
+      for (int k = 0; k < i; ++k) acc += sh.f.L11[i][k] * x[k];
+      x[i] = (i > j) ? -acc : T(i == j ? 1 : 0);
+    }
+#pragma unroll
+    for (int i = 0; i < BK; ++i) X[i][j] = x[i];
+  }
+  for (int i0 = k0 + b; i0 < m; i0 += 64) {
+    const int rows = min(64, m - i0);
+    if (b == BK && rows == 64) {  // no per-entry test around the loads
+#pragma unroll
+      for (int u = 0; u < 64 * BK / NT; ++u) {
+        const int e = tid + NT * u, rr = e / BK, cc = e % BK;
+        Ts[rr][cc] = A[(long long)(i0 + rr) * m + k0 + cc];
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < 64 * BK / NT; ++u) {
+        const int e = tid + NT * u, rr = e / BK, cc = e % BK;
+        Ts[rr][cc] = (rr < rows && cc < b) ? A[(long long)(i0 + rr) * m + k0 + cc] : T(0);
+      }
+    }
+    __syncthreads();
+    // W tile (r, c) = sum_k Ts[r][k] X[c][k] on the matrix cores, in the
+    // operand and output layouts of the trailing update below: each wave
+    // forms a 32 x 32 quarter as four 16 x 16 blocks, k in steps of 4
+    t4 dd[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) dd[a] = t4{0, 0, 0, 0};
+#pragma unroll
+    for (int c0 = 0; c0 < BK; c0 += 4) {
+      const int kk = c0 + (ln >> 4);
+      const T a0 = Ts[rq + (ln & 15)][kk];
+      const T a1 = Ts[rq + 16 + (ln & 15)][kk];
+      const T b0 = X[cq + (ln & 15)][kk];
+      const T b1 = X[cq + 16 + (ln & 15)][kk];
+      dd[0] = mfma_16x16x4(a0, b0, dd[0]);
+      dd[1] = mfma_16x16x4(a0, b1, dd[1]);
+      dd[2] = mfma_16x16x4(a1, b0, dd[2]);
+      dd[3] = mfma_16x16x4(a1, b1, dd[3]);
+    }
+    __syncthreads();  // every read of the staged rows precedes the writes below
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int rr, cc;
+        mfma_entry<T>(ln, rq, cq, a, q, rr, cc);
+        Ts[rr][cc] = dd[a][q];
+      }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 64 * BK / NT; ++u) {
+      const int e = tid + NT * u, rr = e / BK, cc = e % BK;
+      if (rr < rows && cc < b) {
+        const T w = Ts[rr][cc];
+        W[(long long)(i0 + rr) * BK + cc] = w;
+        A[(long long)(i0 + rr) * m + k0 + cc] = w / sh.f.D[cc];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 64 * BK / NT; ++u) {
+      const int e = tid + NT * u, cc = e / 64, rr = e % 64;
+      if (rr < rows && cc < b) A[(long long)(k0 + cc) * m + i0 + rr] = Ts[rr][cc] / sh.f.D[cc];  // L' above the diagonal
+    }
+    __syncthreads();  // the next rows are staged into Ts
+  }
+  __syncthreads();
+}
+
+// (3) The trailing update of the lower triangle: A_ij -= sum_c W_ic L_jc,
+// 64 x 64 tiles (I, J <= I) in steps of 32 columns c.  A tile row's W stays
+// in LDS for all its tiles; each step's L slice (columns < k0 + b of A,
+// which the update does not write) and the next tile row's W are loaded
+// from memory into registers during the step before, then stored to LDS
+// rows padded to 65, so a store's lanes of one row fall in different banks.
+template <typename T>
+__device__ __noinline__ void ldl_update(T* __restrict__ A, int m, int k0, int b, const T* __restrict__ W,
+                                        SharedLDL<T>& sh) {
+  const int tid = threadIdx.x;
+  const int r0 = k0 + b;
+  const int nt = (m - r0 + 63) / 64;
+  const int ncb = (b + BK / 2 - 1) / (BK / 2);
+  const int nsteps = nt * (nt + 1) / 2 * ncb;
+  constexpr int kSlots = 64 * (BK / 2) / NT;  // a thread's (row, column) slots of an L slice
+  constexpr int kRowSlots = 64 * BK / NT;     // ... of a tile row's W
+  T pl[kSlots], pw[kRowSlots];
+  auto fetch_l = [&](int tJ, int cb) {
+    const int j0 = r0 + 64 * tJ, cw = min(BK / 2, b - cb);
+#pragma unroll
+    for (int u = 0; u < kSlots; ++u) {
+      const int e = tid + NT * u;
+      const int r = e / (BK / 2), c = e % (BK / 2);
+      const int j = j0 + r;
+      pl[u] = (j < m && c < cw) ? A[(long long)j * m + k0 + cb + c] : T(0);
+    }
+  };
+  auto fetch_w = [&](int tI) {  // W rows of tile row tI, all b columns
+    const int i0 = r0 + 64 * tI;
+#pragma unroll
+    for (int u = 0; u < kRowSlots; ++u) {
+      const int e = tid + NT * u;
+      const int r = e / BK, c = e % BK;
+      pw[u] = (i0 + r < m && c < b) ? W[(long long)(i0 + r) * BK + c] : T(0);
+    }
+  };
+  // Each thread owns 16 entries of a tile: the four 16 x 16 blocks of its
+  // wave's 32 x 32 quarter in the output layout of v_mfma_f64_16x16x4
+  // (lane l, element e: row 4e + l/16, column l%16 of a block) or of
+  // v_mfma_f32_16x16x4f32 (row 4(l/16) + e, column l%16); both layouts
+  // were read back from the device.  Operands: lane l holds row l%16 of
+  // A and column l%16 of B at k = l/16.
+  constexpr bool kF64 = std::is_same<T, double>::value;
+  const int ln = tid & (kWave - 1), wq = tid / kWave;
+  const int rq = 32 * (wq >> 1), cq = 32 * (wq & 1);
+  auto entry = [&](int a, int q, int& r, int& c) {  // tile row / column of entry (a, q)
+    // a: block (row half, column half), q: element
+    r = rq + 16 * (a >> 1) + (kF64 ? 4 * q + (ln >> 4) : 4 * (ln >> 4) + q);
+    c = cq + 16 * (a & 1) + (ln & 15);
+  };
+  T acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[a][c] = 0;
+  int tI = 0, tJ = 0, cbi = 0;
+  if (nsteps > 0) {
+    fetch_w(0);
+    fetch_l(0, 0);
+  }
+  T aold[4][4];  // the tile's entries of A, read when its first step starts
+  for (int st = 0; st < nsteps; ++st) {
+    if (cbi == 0) {
+      const int i0 = r0 + 64 * tI, j0 = r0 + 64 * tJ;
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          int r, c;
+          entry(a, q, r, c);
+          const int i = i0 + r, j = j0 + c;
+          aold[a][q] = (i < m && j < m && j <= i) ? A[(long long)i * m + j] : T(0);
         }
-      }
+    }
+    if (tJ == 0 && cbi == 0) {  // a new tile row: its W into LDS
 #pragma unroll
-      for (int u = 0; u < kSlots; ++u) {
+      for (int u = 0; u < kRowSlots; ++u) {
         const int e = tid + NT * u;
-        const int r = e / (BK / 2), c = e % (BK / 2);
-        sh.u.Ls[c][r] = pl[u];
+        sh.u.Wr[e % BK][e / BK] = pw[u];
       }
-      __syncthreads();
-      int nI = tI, nJ = tJ, nc = cbi + 1;  // the next step
-      if (nc == ncb) {
-        nc = 0;
-        if (++nJ > nI) {
-          nJ = 0;
-          ++nI;
-        }
-      }
-      if (st + 1 < nsteps) {
-        fetch_l(nJ, nc * (BK / 2));
-        if (nJ == 0 && nc == 0) fetch_w(nI);
-      }
-      {
-        // 32 columns in steps of 4 (columns past the block's width are zero)
-        typedef T t4 __attribute__((ext_vector_type(4)));
-        t4 dd[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) dd[a] = t4{acc[a][0], acc[a][1], acc[a][2], acc[a][3]};
-#pragma unroll
-        for (int c0 = 0; c0 < BK / 2; c0 += 4) {
-          const int kk = c0 + (ln >> 4);
-          const T a0 = sh.u.Wr[cbi * (BK / 2) + kk][rq + (ln & 15)];
-          const T a1 = sh.u.Wr[cbi * (BK / 2) + kk][rq + 16 + (ln & 15)];
-          const T b0 = sh.u.Ls[kk][cq + (ln & 15)];
-          const T b1 = sh.u.Ls[kk][cq + 16 + (ln & 15)];
-          if constexpr (kF64) {
-            dd[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, dd[0], 0, 0, 0);
-            dd[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, dd[1], 0, 0, 0);
-            dd[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, dd[2], 0, 0, 0);
-            dd[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, dd[3], 0, 0, 0);
-          } else {
-            dd[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, dd[0], 0, 0, 0);
-            dd[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, dd[1], 0, 0, 0);
-            dd[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, dd[2], 0, 0, 0);
-            dd[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, dd[3], 0, 0, 0);
-          }
-        }
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) acc[a][q] = dd[a][q];
-      }
-      __syncthreads();
-      if (cbi == ncb - 1) {  // the tile's last step: subtract it from A
-        const int i0 = r0 + 64 * tI, j0 = r0 + 64 * tJ;
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            int r, c;
-            entry(a, q, r, c);
-            const int i = i0 + r, j = j0 + c;
-            if (i < m && j < m && j <= i) A[(long long)i * m + j] = aold[a][q] - acc[a][q];
-            acc[a][q] = 0;
-          }
-      }
-      tI = nI;
-      tJ = nJ;
-      cbi = nc;
     }
-    __syncthreads();  // the next block reads what the update wrote
-    ck.mark(PH_UPDATE);
+#pragma unroll
+    for (int u = 0; u < kSlots; ++u) {
+      const int e = tid + NT * u;
+      const int r = e / (BK / 2), c = e % (BK / 2);
+      sh.u.Ls[c][r] = pl[u];
+    }
+    __syncthreads();
+    int nI = tI, nJ = tJ, nc = cbi + 1;  // the next step
+    if (nc == ncb) {
+      nc = 0;
+      if (++nJ > nI) {
+        nJ = 0;
+        ++nI;
+      }
+    }
+    if (st + 1 < nsteps) {
+      fetch_l(nJ, nc * (BK / 2));
+      if (nJ == 0 && nc == 0) fetch_w(nI);
+    }
+    {
+      // 32 columns in steps of 4 (columns past the block's width are zero)
+      typedef T t4 __attribute__((ext_vector_type(4)));
+      t4 dd[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) dd[a] = t4{acc[a][0], acc[a][1], acc[a][2], acc[a][3]};
+#pragma unroll
+      for (int c0 = 0; c0 < BK / 2; c0 += 4) {
+        const int kk = c0 + (ln >> 4);
+        const T a0 = sh.u.Wr[cbi * (BK / 2) + kk][rq + (ln & 15)];
+        const T a1 = sh.u.Wr[cbi * (BK / 2) + kk][rq + 16 + (ln & 15)];
+        const T b0 = sh.u.Ls[kk][cq + (ln & 15)];
+        const T b1 = sh.u.Ls[kk][cq + 16 + (ln & 15)];
+        if constexpr (kF64) {
+          dd[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, dd[0], 0, 0, 0);
+          dd[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, dd[1], 0, 0, 0);
+          dd[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, dd[2], 0, 0, 0);
+          dd[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, dd[3], 0, 0, 0);
+        } else {
+          dd[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, dd[0], 0, 0, 0);
+          dd[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, dd[1], 0, 0, 0);
+          dd[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, dd[2], 0, 0, 0);
+          dd[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, dd[3], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[a][q] = dd[a][q];
+    }
+    __syncthreads();
+    if (cbi == ncb - 1) {  // the tile's last step: subtract it from A
+      const int i0 = r0 + 64 * tI, j0 = r0 + 64 * tJ;
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          int r, c;
+          entry(a, q, r, c);
+          const int i = i0 + r, j = j0 + c;
+          if (i < m && j < m && j <= i) A[(long long)i * m + j] = aold[a][q] - acc[a][q];
+          acc[a][q] = 0;
+        }
+    }
+    tI = nI;
+    tJ = nJ;
+    cbi = nc;
+  }
+  __syncthreads();  // the next block reads what the update wrote
+}
+
+template <typename T>
+__device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W, SharedLDL<T>& sh, double tol,
+                                    const double* __restrict__ dg, double skip_rel, Clock ck) {
+  int nclamp = 0;
+  for (int k0 = 0; k0 < m; k0 += BK) {
+    const int b = min(BK, m - k0);
+    nclamp += ldl_diag<T>(A, m, k0, b, sh, tol, dg, skip_rel);
+    ck.mark(PH_DIAG);
+    if (k0 + b < m) {
+      ldl_panel<T>(A, m, k0, b, W, sh);
+      ck.mark(PH_PANEL);
+      ldl_update<T>(A, m, k0, b, W, sh);
+      ck.mark(PH_UPDATE);
+    }
   }
   return nclamp;
 }
