@@ -207,8 +207,8 @@ struct SharedLDL {
       T Ts[64][65];  // a 64-row tile of A21, then of W
     } p;
     struct {
-      T Wr[BK][65];      // W of the current tile row, column-major (k, row)
-      T Ls[BK / 2][65];  // L slice of the current step, (k, row)
+      T Wr[BK][65];  // W of the current tile row, column-major (k, row)
+      T Ls[BK][65];  // L slice of the current tile, (k, row)
     } u;
   };
 };
@@ -412,83 +412,98 @@ __device__ __noinline__ void ldl_panel(T* __restrict__ A, int m, int k0, int b, 
   __syncthreads();
 }
 
-// (3) The trailing update of the lower triangle: A_ij -= sum_c W_ic L_jc,
-// 64 x 64 tiles (I, J <= I) in steps of 32 columns c.  A tile row's W stays
-// in LDS for all its tiles; each step's L slice (columns < k0 + b of A,
-// which the update does not write) and the next tile row's W are loaded
-// from memory into registers during the step before, then stored to LDS
-// rows padded to 65, so a store's lanes of one row fall in different banks.
+// (3) The trailing update of the lower triangle: A_ij -= sum_c W_ic L_jc
+// over the block's b columns c, by 64 x 64 tiles (I, J <= I), row by row.
+// A tile row's W stays in LDS for all its tiles.  While a tile runs on the
+// matrix cores, the next tile's L slice (columns < k0 + b of A, which the
+// update does not write), its entries of A and, at a new tile row, its W
+// rows are loaded into registers; they are stored to LDS rows padded to 65
+// (a store's lanes of one row fall in different banks) when that tile
+// starts.
 template <typename T>
 __device__ __noinline__ void ldl_update(T* __restrict__ A, int m, int k0, int b, const T* __restrict__ W,
                                         SharedLDL<T>& sh) {
+  typedef T t4 __attribute__((ext_vector_type(4)));
   const int tid = threadIdx.x;
   const int r0 = k0 + b;
   const int nt = (m - r0 + 63) / 64;
-  const int ncb = (b + BK / 2 - 1) / (BK / 2);
-  const int nsteps = nt * (nt + 1) / 2 * ncb;
-  constexpr int kSlots = 64 * (BK / 2) / NT;  // a thread's (row, column) slots of an L slice
-  constexpr int kRowSlots = 64 * BK / NT;     // ... of a tile row's W
-  T pl[kSlots], pw[kRowSlots];
-  auto fetch_l = [&](int tJ, int cb) {
-    const int j0 = r0 + 64 * tJ, cw = min(BK / 2, b - cb);
-#pragma unroll
-    for (int u = 0; u < kSlots; ++u) {
-      const int e = tid + NT * u;
-      const int r = e / (BK / 2), c = e % (BK / 2);
-      const int j = j0 + r;
-      pl[u] = (j < m && c < cw) ? A[(long long)j * m + k0 + cb + c] : T(0);
-    }
-  };
-  auto fetch_w = [&](int tI) {  // W rows of tile row tI, all b columns
-    const int i0 = r0 + 64 * tI;
-#pragma unroll
-    for (int u = 0; u < kRowSlots; ++u) {
-      const int e = tid + NT * u;
-      const int r = e / BK, c = e % BK;
-      pw[u] = (i0 + r < m && c < b) ? W[(long long)(i0 + r) * BK + c] : T(0);
-    }
-  };
-  // Each thread owns 16 entries of a tile: the four 16 x 16 blocks of its
-  // wave's 32 x 32 quarter in the output layout of v_mfma_f64_16x16x4
-  // (lane l, element e: row 4e + l/16, column l%16 of a block) or of
-  // v_mfma_f32_16x16x4f32 (row 4(l/16) + e, column l%16); both layouts
-  // were read back from the device.  Operands: lane l holds row l%16 of
-  // A and column l%16 of B at k = l/16.
-  constexpr bool kF64 = std::is_same<T, double>::value;
+  const int ntiles = nt * (nt + 1) / 2;
+  constexpr int kSlots = 64 * BK / NT;  // a thread's (row, column) slots of a 64 x BK slice
   const int ln = tid & (kWave - 1), wq = tid / kWave;
   const int rq = 32 * (wq >> 1), cq = 32 * (wq & 1);
-  auto entry = [&](int a, int q, int& r, int& c) {  // tile row / column of entry (a, q)
-    // a: block (row half, column half), q: element
-    r = rq + 16 * (a >> 1) + (kF64 ? 4 * q + (ln >> 4) : 4 * (ln >> 4) + q);
-    c = cq + 16 * (a & 1) + (ln & 15);
+  T pl[kSlots], pw[kSlots], pa[16];
+  // Tiles that lie wholly inside the matrix (all but the last tile row and
+  // column) load without per-entry tests: written with them, every load
+  // becomes a branch of its own.  Entries above the diagonal of a diagonal
+  // tile are loaded (they are inside the matrix) but not written back.
+  const bool full_b = b == BK;
+  auto fetch_l = [&](int tJ) {  // L rows of tile column tJ
+    const int j0 = r0 + 64 * tJ;
+    if (full_b && j0 + 64 <= m) {
+#pragma unroll
+      for (int u = 0; u < kSlots; ++u) {
+        const int e = tid + NT * u;
+        pl[u] = A[(long long)(j0 + e / BK) * m + k0 + e % BK];
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < kSlots; ++u) {
+        const int e = tid + NT * u;
+        const int r = e / BK, c = e % BK;
+        pl[u] = (j0 + r < m && c < b) ? A[(long long)(j0 + r) * m + k0 + c] : T(0);
+      }
+    }
   };
-  T acc[4][4];
+  auto fetch_w = [&](int tI) {  // W rows of tile row tI
+    const int i0 = r0 + 64 * tI;
+    if (full_b && i0 + 64 <= m) {
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+      for (int u = 0; u < kSlots; ++u) {
+        const int e = tid + NT * u;
+        pw[u] = W[(long long)(i0 + e / BK) * BK + e % BK];
+      }
+    } else {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) acc[a][c] = 0;
-  int tI = 0, tJ = 0, cbi = 0;
-  if (nsteps > 0) {
-    fetch_w(0);
-    fetch_l(0, 0);
-  }
-  T aold[4][4];  // the tile's entries of A, read when its first step starts
-  for (int st = 0; st < nsteps; ++st) {
-    if (cbi == 0) {
-      const int i0 = r0 + 64 * tI, j0 = r0 + 64 * tJ;
+      for (int u = 0; u < kSlots; ++u) {
+        const int e = tid + NT * u;
+        const int r = e / BK, c = e % BK;
+        pw[u] = (i0 + r < m && c < b) ? W[(long long)(i0 + r) * BK + c] : T(0);
+      }
+    }
+  };
+  auto fetch_a = [&](int tI, int tJ) {  // the thread's 16 entries of tile (tI, tJ)
+    const int i0 = r0 + 64 * tI, j0 = r0 + 64 * tJ;
+    if (i0 + 64 <= m) {  // j0 <= i0
 #pragma unroll
       for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           int r, c;
-          entry(a, q, r, c);
+          mfma_entry<T>(ln, rq, cq, a, q, r, c);
+          pa[4 * a + q] = A[(long long)(i0 + r) * m + j0 + c];
+        }
+    } else {
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          int r, c;
+          mfma_entry<T>(ln, rq, cq, a, q, r, c);
           const int i = i0 + r, j = j0 + c;
-          aold[a][q] = (i < m && j < m && j <= i) ? A[(long long)i * m + j] : T(0);
+          pa[4 * a + q] = (i < m && j < m && j <= i) ? A[(long long)i * m + j] : T(0);
         }
     }
-    if (tJ == 0 && cbi == 0) {  // a new tile row: its W into LDS
+  };
+  int tI = 0, tJ = 0;
+  if (ntiles > 0) {
+    fetch_w(0);
+    fetch_l(0);
+    fetch_a(0, 0);
+  }
+  for (int t = 0; t < ntiles; ++t) {
+    if (tJ == 0) {  // a new tile row: its W into LDS
 #pragma unroll
-      for (int u = 0; u < kRowSlots; ++u) {
+      for (int u = 0; u < kSlots; ++u) {
         const int e = tid + NT * u;
         sh.u.Wr[e % BK][e / BK] = pw[u];
       }
@@ -496,69 +511,63 @@ __device__ __noinline__ void ldl_update(T* __restrict__ A, int m, int k0, int b,
 #pragma unroll
     for (int u = 0; u < kSlots; ++u) {
       const int e = tid + NT * u;
-      const int r = e / (BK / 2), c = e % (BK / 2);
-      sh.u.Ls[c][r] = pl[u];
+      sh.u.Ls[e % BK][e / BK] = pl[u];
     }
+    T aold[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) aold[u] = pa[u];
     __syncthreads();
-    int nI = tI, nJ = tJ, nc = cbi + 1;  // the next step
-    if (nc == ncb) {
-      nc = 0;
-      if (++nJ > nI) {
-        nJ = 0;
-        ++nI;
-      }
+    int nI = tI, nJ = tJ + 1;  // the next tile
+    if (nJ > nI) {
+      nJ = 0;
+      ++nI;
     }
-    if (st + 1 < nsteps) {
-      fetch_l(nJ, nc * (BK / 2));
-      if (nJ == 0 && nc == 0) fetch_w(nI);
+    if (t + 1 < ntiles) {
+      fetch_l(nJ);
+      if (nJ == 0) fetch_w(nI);
+      fetch_a(nI, nJ);
     }
+    t4 dd[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) dd[a] = t4{0, 0, 0, 0};
+#pragma unroll
+    for (int c0 = 0; c0 < BK; c0 += 4) {
+      const int kk = c0 + (ln >> 4);
+      const T a0 = sh.u.Wr[kk][rq + (ln & 15)];
+      const T a1 = sh.u.Wr[kk][rq + 16 + (ln & 15)];
+      const T b0 = sh.u.Ls[kk][cq + (ln & 15)];
+      const T b1 = sh.u.Ls[kk][cq + 16 + (ln & 15)];
+      dd[0] = mfma_16x16x4(a0, b0, dd[0]);
+      dd[1] = mfma_16x16x4(a0, b1, dd[1]);
+      dd[2] = mfma_16x16x4(a1, b0, dd[2]);
+      dd[3] = mfma_16x16x4(a1, b1, dd[3]);
+    }
+    __syncthreads();  // every read of Ls and Wr precedes the next tile's stores
     {
-      // 32 columns in steps of 4 (columns past the block's width are zero)
-      typedef T t4 __attribute__((ext_vector_type(4)));
-      t4 dd[4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) dd[a] = t4{acc[a][0], acc[a][1], acc[a][2], acc[a][3]};
-#pragma unroll
-      for (int c0 = 0; c0 < BK / 2; c0 += 4) {
-        const int kk = c0 + (ln >> 4);
-        const T a0 = sh.u.Wr[cbi * (BK / 2) + kk][rq + (ln & 15)];
-        const T a1 = sh.u.Wr[cbi * (BK / 2) + kk][rq + 16 + (ln & 15)];
-        const T b0 = sh.u.Ls[kk][cq + (ln & 15)];
-        const T b1 = sh.u.Ls[kk][cq + 16 + (ln & 15)];
-        if constexpr (kF64) {
-          dd[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, dd[0], 0, 0, 0);
-          dd[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, dd[1], 0, 0, 0);
-          dd[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, dd[2], 0, 0, 0);
-          dd[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, dd[3], 0, 0, 0);
-        } else {
-          dd[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, dd[0], 0, 0, 0);
-          dd[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, dd[1], 0, 0, 0);
-          dd[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, dd[2], 0, 0, 0);
-          dd[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, dd[3], 0, 0, 0);
-        }
-      }
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) acc[a][q] = dd[a][q];
-    }
-    __syncthreads();
-    if (cbi == ncb - 1) {  // the tile's last step: subtract it from A
       const int i0 = r0 + 64 * tI, j0 = r0 + 64 * tJ;
+      if (i0 + 64 <= m && tJ < tI) {
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          int r, c;
-          entry(a, q, r, c);
-          const int i = i0 + r, j = j0 + c;
-          if (i < m && j < m && j <= i) A[(long long)i * m + j] = aold[a][q] - acc[a][q];
-          acc[a][q] = 0;
-        }
+          for (int q = 0; q < 4; ++q) {
+            int r, c;
+            mfma_entry<T>(ln, rq, cq, a, q, r, c);
+            A[(long long)(i0 + r) * m + j0 + c] = aold[4 * a + q] - dd[a][q];
+          }
+      } else {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            int r, c;
+            mfma_entry<T>(ln, rq, cq, a, q, r, c);
+            const int i = i0 + r, j = j0 + c;
+            if (i < m && j < m && j <= i) A[(long long)i * m + j] = aold[4 * a + q] - dd[a][q];
+          }
+      }
     }
     tI = nI;
     tJ = nJ;
-    cbi = nc;
   }
   __syncthreads();  // the next block reads what the update wrote
 }
