@@ -42,6 +42,7 @@ namespace {
 
 constexpr int NT = 256;
 constexpr int BK = 64;  // LDL' block width
+constexpr int kLdw = 2 * BK;  // row stride of W: the two blocks of a trailing update
 
 struct Inst {
   int m, n;
@@ -207,8 +208,8 @@ struct SharedLDL {
       T Ts[64][65];  // a 64-row tile of A21, then of W
     } p;
     struct {
-      T Wr[BK][65];  // W of the current tile row, column-major (k, row)
-      T Ls[BK][65];  // L slice of the current tile, (k, row)
+      T Wr[kLdw][65];  // W of the current tile row, column-major (k, row)
+      T Ls[BK][65];    // L slice of the current step, (k, row)
     } u;
   };
 };
@@ -327,7 +328,7 @@ __device__ __noinline__ int ldl_diag(T* __restrict__ A, int m, int k0, int b, Sh
 // g = tid / 64) forms 16 columns of row r in column order, and W, L21 and
 // L21' leave through LDS by coalesced stores.
 template <typename T>
-__device__ __noinline__ void ldl_panel(T* __restrict__ A, int m, int k0, int b, T* __restrict__ W,
+__device__ __noinline__ void ldl_panel(T* __restrict__ A, int m, int k0, int b, T* __restrict__ W, int wc,
                                        SharedLDL<T>& sh) {
   const int tid = threadIdx.x;
   typedef T t4 __attribute__((ext_vector_type(4)));
@@ -398,7 +399,7 @@ __device__ __noinline__ void ldl_panel(T* __restrict__ A, int m, int k0, int b, 
       const int e = tid + NT * u, rr = e / BK, cc = e % BK;
       if (rr < rows && cc < b) {
         const T w = Ts[rr][cc];
-        W[(long long)(i0 + rr) * BK + cc] = w;
+        W[(long long)(i0 + rr) * kLdw + wc + cc] = w;
         A[(long long)(i0 + rr) * m + k0 + cc] = w / sh.f.D[cc];
       }
     }
@@ -412,62 +413,66 @@ __device__ __noinline__ void ldl_panel(T* __restrict__ A, int m, int k0, int b, 
   __syncthreads();
 }
 
-// (3) The trailing update of the lower triangle: A_ij -= sum_c W_ic L_jc
-// over the block's b columns c, by 64 x 64 tiles (I, J <= I), row by row.
-// A tile row's W stays in LDS for all its tiles.  While a tile runs on the
-// matrix cores, the next tile's L slice (columns < k0 + b of A, which the
-// update does not write), its entries of A and, at a new tile row, its W
-// rows are loaded into registers; they are stored to LDS rows padded to 65
-// (a store's lanes of one row fall in different banks) when that tile
-// starts.
+// (3) A trailing update of the lower triangle: A_ij -= sum_c W_ic L_jc over
+// kw <= 128 columns c (L_jc = A[j][kc + c], W_ic = W[i][c]), for the rows
+// i >= r0 and the first ntc tile columns from r0, by 64 x 64 tiles
+// (I, J <= I), row by row, in steps of 64 columns c.  Folding two blocks'
+// updates into one pass over the trailing matrix halves its reads and
+// writes per multiply-add.  A tile row's W stays in LDS for all its tiles.
+// While a step runs on the matrix cores, the next step's L slice (columns
+// < r0 of A, which the update does not write), at a new tile the entries of
+// A it updates and at a new tile row its W rows are loaded into registers;
+// they are stored to LDS rows padded to 65 (a store's lanes of one row fall
+// in different banks) when that step starts.
 template <typename T>
-__device__ __noinline__ void ldl_update(T* __restrict__ A, int m, int k0, int b, const T* __restrict__ W,
-                                        SharedLDL<T>& sh) {
+__device__ __noinline__ void ldl_update(T* __restrict__ A, int m, int kc, int kw, int r0, int ntc,
+                                        const T* __restrict__ W, SharedLDL<T>& sh) {
   typedef T t4 __attribute__((ext_vector_type(4)));
   const int tid = threadIdx.x;
-  const int r0 = k0 + b;
   const int nt = (m - r0 + 63) / 64;
-  const int ntiles = nt * (nt + 1) / 2;
-  constexpr int kSlots = 64 * BK / NT;  // a thread's (row, column) slots of a 64 x BK slice
+  const int nch = (kw + BK - 1) / BK;  // 64-column steps per tile
+  // tiles: tile row tI has min(tI + 1, ntc) of them
+  const int ntiles = ntc >= nt ? nt * (nt + 1) / 2 : ntc * (ntc + 1) / 2 + (nt - ntc) * ntc;
+  const int nsteps = ntiles * nch;
+  constexpr int kSlots = 64 * BK / NT;  // a thread's (row, column) slots of a 64 x 64 slice
   const int ln = tid & (kWave - 1), wq = tid / kWave;
   const int rq = 32 * (wq >> 1), cq = 32 * (wq & 1);
   T pl[kSlots], pw[kSlots], pa[16];
-  // Tiles that lie wholly inside the matrix (all but the last tile row and
+  // Slices that lie wholly inside the matrix (all but the last tile row and
   // column) load without per-entry tests: written with them, every load
   // becomes a branch of its own.  Entries above the diagonal of a diagonal
   // tile are loaded (they are inside the matrix) but not written back.
-  const bool full_b = b == BK;
-  auto fetch_l = [&](int tJ) {  // L rows of tile column tJ
-    const int j0 = r0 + 64 * tJ;
-    if (full_b && j0 + 64 <= m) {
+  auto fetch_l = [&](int tJ, int ch) {  // L rows of tile column tJ, step ch
+    const int j0 = r0 + 64 * tJ, cw = min(BK, kw - BK * ch), c0 = kc + BK * ch;
+    if (cw == BK && j0 + 64 <= m) {
 #pragma unroll
       for (int u = 0; u < kSlots; ++u) {
         const int e = tid + NT * u;
-        pl[u] = A[(long long)(j0 + e / BK) * m + k0 + e % BK];
+        pl[u] = A[(long long)(j0 + e / BK) * m + c0 + e % BK];
       }
     } else {
 #pragma unroll
       for (int u = 0; u < kSlots; ++u) {
         const int e = tid + NT * u;
         const int r = e / BK, c = e % BK;
-        pl[u] = (j0 + r < m && c < b) ? A[(long long)(j0 + r) * m + k0 + c] : T(0);
+        pl[u] = (j0 + r < m && c < cw) ? A[(long long)(j0 + r) * m + c0 + c] : T(0);
       }
     }
   };
-  auto fetch_w = [&](int tI) {  // W rows of tile row tI
-    const int i0 = r0 + 64 * tI;
-    if (full_b && i0 + 64 <= m) {
+  auto fetch_w = [&](int tI, int ch) {  // W rows of tile row tI, step ch
+    const int i0 = r0 + 64 * tI, cw = min(BK, kw - BK * ch), c0 = BK * ch;
+    if (cw == BK && i0 + 64 <= m) {
 #pragma unroll
       for (int u = 0; u < kSlots; ++u) {
         const int e = tid + NT * u;
-        pw[u] = W[(long long)(i0 + e / BK) * BK + e % BK];
+        pw[u] = W[(long long)(i0 + e / BK) * kLdw + c0 + e % BK];
       }
     } else {
 #pragma unroll
       for (int u = 0; u < kSlots; ++u) {
         const int e = tid + NT * u;
         const int r = e / BK, c = e % BK;
-        pw[u] = (i0 + r < m && c < b) ? W[(long long)(i0 + r) * BK + c] : T(0);
+        pw[u] = (i0 + r < m && c < cw) ? W[(long long)(i0 + r) * kLdw + c0 + c] : T(0);
       }
     }
   };
@@ -494,18 +499,26 @@ __device__ __noinline__ void ldl_update(T* __restrict__ A, int m, int k0, int b,
         }
     }
   };
-  int tI = 0, tJ = 0;
-  if (ntiles > 0) {
-    fetch_w(0);
-    fetch_l(0);
+  int tI = 0, tJ = 0, ch = 0;
+  if (nsteps > 0) {
+    fetch_w(0, 0);
+    fetch_l(0, 0);
     fetch_a(0, 0);
   }
-  for (int t = 0; t < ntiles; ++t) {
+  T aold[16];
+  t4 dd[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) dd[a] = t4{0, 0, 0, 0};
+  for (int st = 0; st < nsteps; ++st) {
+    if (ch == 0) {
+#pragma unroll
+      for (int u = 0; u < 16; ++u) aold[u] = pa[u];
+    }
     if (tJ == 0) {  // a new tile row: its W into LDS
 #pragma unroll
       for (int u = 0; u < kSlots; ++u) {
         const int e = tid + NT * u;
-        sh.u.Wr[e % BK][e / BK] = pw[u];
+        sh.u.Wr[BK * ch + e % BK][e / BK] = pw[u];
       }
     }
 #pragma unroll
@@ -513,28 +526,25 @@ __device__ __noinline__ void ldl_update(T* __restrict__ A, int m, int k0, int b,
       const int e = tid + NT * u;
       sh.u.Ls[e % BK][e / BK] = pl[u];
     }
-    T aold[16];
-#pragma unroll
-    for (int u = 0; u < 16; ++u) aold[u] = pa[u];
     __syncthreads();
-    int nI = tI, nJ = tJ + 1;  // the next tile
-    if (nJ > nI) {
-      nJ = 0;
-      ++nI;
+    int nI = tI, nJ = tJ, nc = ch + 1;  // the next step
+    if (nc == nch) {
+      nc = 0;
+      if (++nJ > nI || nJ >= ntc) {
+        nJ = 0;
+        ++nI;
+      }
     }
-    if (t + 1 < ntiles) {
-      fetch_l(nJ);
-      if (nJ == 0) fetch_w(nI);
-      fetch_a(nI, nJ);
+    if (st + 1 < nsteps) {
+      fetch_l(nJ, nc);
+      if (nJ == 0) fetch_w(nI, nc);
+      if (nc == 0) fetch_a(nI, nJ);
     }
-    t4 dd[4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) dd[a] = t4{0, 0, 0, 0};
 #pragma unroll
     for (int c0 = 0; c0 < BK; c0 += 4) {
       const int kk = c0 + (ln >> 4);
-      const T a0 = sh.u.Wr[kk][rq + (ln & 15)];
-      const T a1 = sh.u.Wr[kk][rq + 16 + (ln & 15)];
+      const T a0 = sh.u.Wr[BK * ch + kk][rq + (ln & 15)];
+      const T a1 = sh.u.Wr[BK * ch + kk][rq + 16 + (ln & 15)];
       const T b0 = sh.u.Ls[kk][cq + (ln & 15)];
       const T b1 = sh.u.Ls[kk][cq + 16 + (ln & 15)];
       dd[0] = mfma_16x16x4(a0, b0, dd[0]);
@@ -542,8 +552,8 @@ __device__ __noinline__ void ldl_update(T* __restrict__ A, int m, int k0, int b,
       dd[2] = mfma_16x16x4(a1, b0, dd[2]);
       dd[3] = mfma_16x16x4(a1, b1, dd[3]);
     }
-    __syncthreads();  // every read of Ls and Wr precedes the next tile's stores
-    {
+    __syncthreads();  // every read of Ls and Wr precedes the next step's stores
+    if (ch == nch - 1) {  // the tile's last step: subtract it from A
       const int i0 = r0 + 64 * tI, j0 = r0 + 64 * tJ;
       if (i0 + 64 <= m && tJ < tI) {
 #pragma unroll
@@ -565,27 +575,41 @@ __device__ __noinline__ void ldl_update(T* __restrict__ A, int m, int k0, int b,
             if (i < m && j < m && j <= i) A[(long long)i * m + j] = aold[4 * a + q] - dd[a][q];
           }
       }
+#pragma unroll
+      for (int a = 0; a < 4; ++a) dd[a] = t4{0, 0, 0, 0};
     }
     tI = nI;
     tJ = nJ;
+    ch = nc;
   }
   __syncthreads();  // the next block reads what the update wrote
 }
 
+// Blocked LDL' by pairs of 64-column blocks a = [K, K + 64) and b = [K + 64,
+// K + 128): factor a, update block column b by a (one tile column), factor
+// b, then update the trailing matrix from K + 128 by a and b together.
 template <typename T>
 __device__ __noinline__ int wg_ldlt(T* __restrict__ A, int m, T* __restrict__ W, SharedLDL<T>& sh, double tol,
                                     const double* __restrict__ dg, double skip_rel, Clock ck) {
   int nclamp = 0;
-  for (int k0 = 0; k0 < m; k0 += BK) {
-    const int b = min(BK, m - k0);
-    nclamp += ldl_diag<T>(A, m, k0, b, sh, tol, dg, skip_rel);
+  for (int K = 0; K < m; K += 2 * BK) {
+    const int ba = min(BK, m - K);
+    nclamp += ldl_diag<T>(A, m, K, ba, sh, tol, dg, skip_rel);
     ck.mark(PH_DIAG);
-    if (k0 + b < m) {
-      ldl_panel<T>(A, m, k0, b, W, sh);
-      ck.mark(PH_PANEL);
-      ldl_update<T>(A, m, k0, b, W, sh);
-      ck.mark(PH_UPDATE);
-    }
+    if (K + ba >= m) break;
+    ldl_panel<T>(A, m, K, ba, W, 0, sh);
+    ck.mark(PH_PANEL);
+    const int kb = K + BK, bb = min(BK, m - kb);
+    ldl_update<T>(A, m, K, ba, kb, 1, W, sh);
+    ck.mark(PH_UPDATE);
+    nclamp += ldl_diag<T>(A, m, kb, bb, sh, tol, dg, skip_rel);
+    ck.mark(PH_DIAG);
+    if (kb + bb >= m) break;
+    ldl_panel<T>(A, m, kb, bb, W, BK, sh);
+    ck.mark(PH_PANEL);
+    const int r0 = kb + bb;
+    ldl_update<T>(A, m, K, ba + bb, r0, (m - r0 + 63) / 64, W, sh);
+    ck.mark(PH_UPDATE);
   }
   return nclamp;
 }
@@ -1447,7 +1471,7 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
     I.dn = dn;
     dn += (long long)F.m * F.m;
     I.w = wk;
-    wk += (long long)F.m * BK;
+    wk += (long long)F.m * kLdw;
     I.warm = warm_ok[k] ? 1 : (predict[k] ? 2 : 0);
     I.mixed = instance_mixed(o.precision, F.m, have_table ? &table : nullptr);
     I.off = F.offset;
@@ -1463,7 +1487,7 @@ int ipm_solve_batch_impl(const hn_lp* lps, uint32_t count, const hn_ipm_opts& o,
                         I.vm + (long long)(VM_COUNT - kVmIo) * m <= n_vm && I.vn >= 0 &&
                         I.vn + (long long)(VN_COUNT - kVnIo) * n <= n_vn && I.om >= 0 && I.om + m <= n_om &&
                         I.on >= 0 && I.on + n <= n_on && I.dn >= 0 && I.dn + m * m <= dn && I.w >= 0 &&
-                        I.w + m * BK <= wk;
+                        I.w + m * kLdw <= wk;
     HN_CHECK(inside, HN_ERR_INVALID, "batch LP " + std::to_string(k) + ": a slice lies outside its arena");
   }
   BatchWorkspace& w = bws();
