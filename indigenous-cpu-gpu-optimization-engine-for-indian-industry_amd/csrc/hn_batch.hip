@@ -127,6 +127,13 @@ template <>
 __device__ __forceinline__ float skipped_pivot<float>() { return 1e30f; }
 
 // ---- workgroup helpers -----------------------------------------------------
+// A workgroup runs one wave per SIMD, so a loop that waits for each of its
+// loads before the next is bound by memory latency.  The helpers below keep
+// several independent loads in flight per thread while every row and every
+// thread-local sum still accumulates in its original order (results are
+// bit-identical to the one-at-a-time loops).
+
+// y = A x, one row per thread, summed left to right.
 __device__ __forceinline__ void wg_spmv(const int* __restrict__ rp, const int* __restrict__ ci,
                                         const double* __restrict__ v, const double* __restrict__ x,
                                         double* __restrict__ y, int rows) {
@@ -148,6 +155,23 @@ __device__ __forceinline__ void wg_spmv(const int* __restrict__ rp, const int* _
   }
 }
 
+// for j = tid, tid + NT, ... < n: fetch(j, u) loads element j's operands
+// into slot u, apply(j, u) computes and stores.  The loads of kEachU
+// consecutive elements of the thread are issued before any of their
+// stores; elements are applied in the plain loop's order.
+constexpr int kEachU = 4;
+template <class Fetch, class Apply>
+__device__ __forceinline__ void wg_each(int n, Fetch&& fetch, Apply&& apply) {
+  for (int j0 = threadIdx.x; j0 < n; j0 += kEachU * NT) {
+#pragma unroll
+    for (int u = 0; u < kEachU; ++u)
+      if (j0 + u * NT < n) fetch(j0 + u * NT, u);
+#pragma unroll
+    for (int u = 0; u < kEachU; ++u)
+      if (j0 + u * NT < n) apply(j0 + u * NT, u);
+  }
+}
+
 __device__ __forceinline__ double wg_sum(double v, double* red) { return block_sum<double, NT>(v, red); }
 __device__ __forceinline__ double wg_max(double v, double* red) { return block_max<double, NT>(v, red); }
 __device__ __forceinline__ double wg_min(double v, double* red) { return block_min<double, NT>(v, red); }
@@ -155,11 +179,19 @@ __device__ __forceinline__ double wg_min(double v, double* red) { return block_m
 // min over d < 0 of -v/d, then the reference's min(1, 0.99 alpha)
 __device__ double wg_step(const double* __restrict__ v, const double* __restrict__ d, int n, double* red) {
   double a = 1.0;
-  for (int i = threadIdx.x; i < n; i += NT)
-    if (d[i] < 0.0) {
-      const double q = -v[i] / d[i];
-      a = (q < a) ? q : a;
-    }
+  double vv[kEachU], dd[kEachU];
+  wg_each(
+      n,
+      [&](int i, int u) {
+        vv[u] = v[i];
+        dd[u] = d[i];
+      },
+      [&](int, int u) {
+        if (dd[u] < 0.0) {
+          const double q = -vv[u] / dd[u];
+          a = (q < a) ? q : a;
+        }
+      });
   a = wg_min(a, red);
   const double s = 0.99 * a;
   return (s < 1.0) ? s : 1.0;
@@ -721,7 +753,15 @@ __device__ __noinline__ void wg_apply_N(const Ctx& c, const double* __restrict__
   double* t = c.vn[VN_T];
   wg_spmv(c.trp, c.tci, c.tv, v, t, c.n);
   __syncthreads();
-  for (int j = threadIdx.x; j < c.n; j += NT) t[j] = c.vn[VN_D][j] * t[j];
+  const double* D = c.vn[VN_D];
+  double dq[kEachU], tq[kEachU];
+  wg_each(
+      c.n,
+      [&](int j, int u) {
+        dq[u] = D[j];
+        tq[u] = t[j];
+      },
+      [&](int j, int u) { t[j] = dq[u] * tq[u]; });
   __syncthreads();
   wg_spmv(c.mrp, c.mci, c.mv, t, nd, c.m);
   __syncthreads();
@@ -774,21 +814,66 @@ __device__ __noinline__ double wg_assemble(const Ctx& c, T* __restrict__ A, doub
   __syncthreads();
   const double* D = c.vn[VN_D];
   double mxd = 0.0;
+  // Row i of N, one thread per row: for each column j of row i (in order)
+  // with d_j > 0 and each entry (rr, j) of that column (in order),
+  // N[i][rr] += (a_ij a_rrj) d_j.  Every such column holds row i itself,
+  // so the diagonal is summed in registers, in the same order and type as
+  // its read-modify-writes would have, and stored once; only the other
+  // rows' entries are updated in memory.  The columns' indices, scales and
+  // first two entries are loaded kAsmG columns at a time ahead of the
+  // updates, which stay in order.
+  constexpr int kAsmG = 4;
   for (int i = threadIdx.x; i < m; i += NT) {
     double diag = 0.0;
+    T diag_t = T(0);
     T* row = A + (long long)i * m;
-    for (int k = c.mrp[i]; k < c.mrp[i + 1]; ++k) {
-      const int j = c.mci[k];
-      const double d = D[j];
-      if (d <= 0.0) continue;
-      const double a = c.mv[k];
-      for (int q = c.trp[j]; q < c.trp[j + 1]; ++q) {
-        const int rr = c.tci[q];
-        const double val = (a * c.tv[q]) * d;
-        if (rr == i) diag += val;
-        row[rr] = row[rr] + (T)val;
+    const int kb = c.mrp[i], ke = c.mrp[i + 1];
+    for (int k0 = kb; k0 < ke; k0 += kAsmG) {
+      int jq[kAsmG], q0[kAsmG], q1[kAsmG], r0[kAsmG], r1[kAsmG];
+      double aq[kAsmG], dq[kAsmG], t0[kAsmG], t1[kAsmG];
+#pragma unroll
+      for (int u = 0; u < kAsmG; ++u)
+        if (k0 + u < ke) {
+          jq[u] = c.mci[k0 + u];
+          aq[u] = c.mv[k0 + u];
+        }
+#pragma unroll
+      for (int u = 0; u < kAsmG; ++u)
+        if (k0 + u < ke) {
+          dq[u] = D[jq[u]];
+          q0[u] = c.trp[jq[u]];
+          q1[u] = c.trp[jq[u] + 1];
+        }
+#pragma unroll
+      for (int u = 0; u < kAsmG; ++u)
+        if (k0 + u < ke) {
+          if (q0[u] < q1[u]) {
+            r0[u] = c.tci[q0[u]];
+            t0[u] = c.tv[q0[u]];
+          }
+          if (q0[u] + 1 < q1[u]) {
+            r1[u] = c.tci[q0[u] + 1];
+            t1[u] = c.tv[q0[u] + 1];
+          }
+        }
+#pragma unroll
+      for (int u = 0; u < kAsmG; ++u) {
+        if (k0 + u >= ke || dq[u] <= 0.0) continue;
+        const double a = aq[u], d = dq[u];
+        for (int q = q0[u]; q < q1[u]; ++q) {
+          const int rr = (q == q0[u]) ? r0[u] : (q == q0[u] + 1) ? r1[u] : c.tci[q];
+          const double tq = (q == q0[u]) ? t0[u] : (q == q0[u] + 1) ? t1[u] : c.tv[q];
+          const double val = (a * tq) * d;
+          if (rr == i) {
+            diag += val;
+            diag_t = diag_t + (T)val;
+          } else {
+            row[rr] = row[rr] + (T)val;
+          }
+        }
       }
     }
+    row[i] = diag_t;
     mxd = nan_ignoring_max(mxd, diag);
     c.vm[VM_DG][i] = diag;
   }
@@ -954,11 +1039,22 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
         rp[i] = v;
         a = nan_ignoring_max(a, fabs(v));
       }
-      for (int j = tid; j < n; j += NT) {
-        const double v = t[j] + z[j] - cc[j];
-        rd[j] = v;
-        bb = nan_ignoring_max(bb, fabs(v));
-        s += x[j] * z[j];
+      {
+        double tq[kEachU], zq[kEachU], cq[kEachU], xq[kEachU];
+        wg_each(
+            n,
+            [&](int j, int u) {
+              tq[u] = t[j];
+              zq[u] = z[j];
+              cq[u] = cc[j];
+              xq[u] = x[j];
+            },
+            [&](int j, int u) {
+              const double v = tq[u] + zq[u] - cq[u];
+              rd[j] = v;
+              bb = nan_ignoring_max(bb, fabs(v));
+              s += xq[u] * zq[u];
+            });
       }
       rpn = wg_max(a, red);
       rdn = wg_max(bb, red);
@@ -1153,11 +1249,21 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
         fp64 = true;
         if (fp64_from < 0) fp64_from = iter;
       }
-      for (int j = tid; j < n; j += NT) {
-        double d = x[j] / z[j];
-        if (!(d >= 0.0) || !isfinite(d) || d > 1e10) d = 1e10;
-        D[j] = d;
-        t[j] = d * rd[j];
+      {
+        double xq[kEachU], zq[kEachU], rq[kEachU];
+        wg_each(
+            n,
+            [&](int j, int u) {
+              xq[u] = x[j];
+              zq[u] = z[j];
+              rq[u] = rd[j];
+            },
+            [&](int j, int u) {
+              double d = xq[u] / zq[u];
+              if (!(d >= 0.0) || !isfinite(d) || d > 1e10) d = 1e10;
+              D[j] = d;
+              t[j] = d * rq[u];
+            });
       }
       __syncthreads();
       wg_spmv(c.mrp, c.mci, c.mv, t, mdrd, m);
@@ -1184,12 +1290,23 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
       wg_spmv(c.trp, c.tci, c.tv, dlam, t, n);
       __syncthreads();
       int bad = 0;
-      for (int j = tid; j < n; j += NT) {
-        const double s = t[j] + rd[j];
-        const double a = D[j] * s - x[j];
-        dx[j] = a;
-        dz[j] = -s;
-        if (!isfinite(a) || !isfinite(s)) bad = 1;
+      {
+        double tq[kEachU], rq[kEachU], dq[kEachU], xq[kEachU];
+        wg_each(
+            n,
+            [&](int j, int u) {
+              tq[u] = t[j];
+              rq[u] = rd[j];
+              dq[u] = D[j];
+              xq[u] = x[j];
+            },
+            [&](int j, int u) {
+              const double s = tq[u] + rq[u];
+              const double a = dq[u] * s - xq[u];
+              dx[j] = a;
+              dz[j] = -s;
+              if (!isfinite(a) || !isfinite(s)) bad = 1;
+            });
       }
       for (int i = tid; i < m; i += NT)
         if (!isfinite(dlam[i])) bad = 1;
@@ -1200,7 +1317,18 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
       const double ap_a = wg_step(x, dx, n, red);
       const double ad_a = wg_step(z, dz, n, red);
       double s = 0.0;
-      for (int j = tid; j < n; j += NT) s += (x[j] + ap_a * dx[j]) * (z[j] + ad_a * dz[j]);
+      {
+        double xq[kEachU], dxq[kEachU], zq[kEachU], dzq[kEachU];
+        wg_each(
+            n,
+            [&](int j, int u) {
+              xq[u] = x[j];
+              dxq[u] = dx[j];
+              zq[u] = z[j];
+              dzq[u] = dz[j];
+            },
+            [&](int, int u) { s += (xq[u] + ap_a * dxq[u]) * (zq[u] + ad_a * dzq[u]); });
+      }
       const double mu_aff = wg_sum(s, red) / (double)n;
       double sigma;
       if (!isfinite(mu_aff) || mu_aff < 0.0 || !(mu > 0.0)) {
@@ -1210,9 +1338,19 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
         sigma = sigma < 0.1 ? 0.1 : (sigma > 0.9 ? 0.9 : sigma);
       }
       const double smu = sigma * mu;
-      for (int j = tid; j < n; j += NT) {
-        const double zinv = 1.0 / z[j];
-        zs[j] = (smu - dx[j] * dz[j]) * zinv;
+      {
+        double zq[kEachU], dxq[kEachU], dzq[kEachU];
+        wg_each(
+            n,
+            [&](int j, int u) {
+              zq[u] = z[j];
+              dxq[u] = dx[j];
+              dzq[u] = dz[j];
+            },
+            [&](int j, int u) {
+              const double zinv = 1.0 / zq[u];
+              zs[j] = (smu - dxq[u] * dzq[u]) * zinv;
+            });
       }
       __syncthreads();
       wg_spmv(c.mrp, c.mci, c.mv, zs, mz, m);
@@ -1229,12 +1367,24 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
       wg_spmv(c.trp, c.tci, c.tv, dlam, t, n);
       __syncthreads();
       bad = 0;
-      for (int j = tid; j < n; j += NT) {
-        const double s2 = t[j] + rd[j];
-        const double a = D[j] * s2 - x[j] + zs[j];
-        dx[j] = a;
-        dz[j] = -s2;
-        if (!isfinite(a) || !isfinite(s2)) bad = 1;
+      {
+        double tq[kEachU], rq[kEachU], dq[kEachU], xq[kEachU], sq[kEachU];
+        wg_each(
+            n,
+            [&](int j, int u) {
+              tq[u] = t[j];
+              rq[u] = rd[j];
+              dq[u] = D[j];
+              xq[u] = x[j];
+              sq[u] = zs[j];
+            },
+            [&](int j, int u) {
+              const double s2 = tq[u] + rq[u];
+              const double a = dq[u] * s2 - xq[u] + sq[u];
+              dx[j] = a;
+              dz[j] = -s2;
+              if (!isfinite(a) || !isfinite(s2)) bad = 1;
+            });
       }
       for (int i = tid; i < m; i += NT)
         if (!isfinite(dlam[i])) bad = 1;
@@ -1245,12 +1395,23 @@ __global__ __launch_bounds__(NT) void k_batch_ipm(const Inst* __restrict__ inst,
       const double ap = wg_step(x, dx, n, red);
       const double ad = wg_step(z, dz, n, red);
       bad = 0;
-      for (int j = tid; j < n; j += NT) {
-        const double xi = x[j] + ap * dx[j];
-        const double zi = z[j] + ad * dz[j];
-        x[j] = xi;
-        z[j] = zi;
-        if (xi < 0.0 || zi < 0.0 || !isfinite(xi) || !isfinite(zi)) bad = 1;
+      {
+        double xq[kEachU], dxq[kEachU], zq[kEachU], dzq[kEachU];
+        wg_each(
+            n,
+            [&](int j, int u) {
+              xq[u] = x[j];
+              dxq[u] = dx[j];
+              zq[u] = z[j];
+              dzq[u] = dz[j];
+            },
+            [&](int j, int u) {
+              const double xi = xq[u] + ap * dxq[u];
+              const double zi = zq[u] + ad * dzq[u];
+              x[j] = xi;
+              z[j] = zi;
+              if (xi < 0.0 || zi < 0.0 || !isfinite(xi) || !isfinite(zi)) bad = 1;
+            });
       }
       for (int i = tid; i < m; i += NT) lam[i] = lam[i] + ad * dlam[i];
       if (wg_max((double)bad, red) > 0.0) {
