@@ -155,7 +155,10 @@ __global__ void k_gnn_scale_fin(const double* __restrict__ part, int n, int m, f
 constexpr int kRankMax = 4096;  // longer rows get the neutral rank 0.5
 
 // One workgroup per row: rank of each entry's cost among the row's costs
-// (ties broken by column index) and its gap to the row minimum.
+// (ties broken by column index) and its gap to the row minimum.  The row's
+// costs and columns are staged in dynamic LDS sized for the graph's longest
+// row (rank_len entries of 12 bytes), so short-row graphs keep many
+// workgroups resident per CU.
 __global__ __launch_bounds__(256) void k_gnn_row_rank(const int* __restrict__ rp, const int* __restrict__ ci,
                                                       const double* __restrict__ c, const float* __restrict__ S,
                                                       float* __restrict__ rank, float* __restrict__ gap,
@@ -164,8 +167,7 @@ __global__ __launch_bounds__(256) void k_gnn_row_rank(const int* __restrict__ rp
   S = at(S, inst_off(istride));
   rank = at(rank, inst_off(istride));
   gap = at(gap, inst_off(istride));
-  __shared__ double cs[kRankMax];
-  __shared__ int js[kRankMax];
+  extern __shared__ __align__(8) unsigned char hn_rank_lds[];
   __shared__ double red[4];
   const int i = blockIdx.x;
   const int k0 = rp[i], len = rp[i + 1] - k0;
@@ -180,6 +182,8 @@ __global__ __launch_bounds__(256) void k_gnn_row_rank(const int* __restrict__ rp
     }
     return;
   }
+  double* cs = reinterpret_cast<double*>(hn_rank_lds);  // len <= kRankMax <= the launch's rank_len
+  int* js = reinterpret_cast<int*>(cs + len);
   for (int t = threadIdx.x; t < len; t += 256) {
     js[t] = ci[k0 + t];
     cs[t] = c[js[t]];
@@ -248,18 +252,29 @@ __global__ void k_gnn_feat_c(const int* __restrict__ slackrow, const double* __r
   f[3] = 0.f;
 }
 
-// h[r][f] = relu(sum_t F[r][t] W[t][f] + bias[f]); one wave per row.
+// h[r][f] = relu(sum_t F[r][t] W[t][f] + bias[f]).
 __global__ __launch_bounds__(256) void k_gnn_embed(const float* __restrict__ F, int fstride, int nf,
                                                    const float* __restrict__ W, const float* __restrict__ bias,
                                                    int rows, float* __restrict__ h, long long istride) {
   F = at(F, inst_off(istride));
   h = at(h, inst_off(istride));
-  const int r = (blockIdx.x * 256 + threadIdx.x) / kWave;
-  const int f = threadIdx.x & (kWave - 1);
+  // 16 lanes per row, 4 consecutive features per lane (one 16-byte store):
+  // 4 rows per wave; each output is summed as bias + F[t] W[t] in t order
+  const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long r = g / (H / 4);
+  const int f0 = 4 * (int)(g % (H / 4));
   if (r >= rows) return;
-  float s = bias[f];
-  for (int t = 0; t < nf; ++t) s += F[(long long)r * fstride + t] * W[t * H + f];
-  h[(long long)r * H + f] = fmaxf(s, 0.f);
+  float4 s = make_float4(bias[f0], bias[f0 + 1], bias[f0 + 2], bias[f0 + 3]);
+  for (int t = 0; t < nf; ++t) {
+    const float x = F[r * fstride + t];
+    const float4 w = *reinterpret_cast<const float4*>(W + t * H + f0);
+    s.x += x * w.x;
+    s.y += x * w.y;
+    s.z += x * w.z;
+    s.w += x * w.w;
+  }
+  *reinterpret_cast<float4*>(h + r * H + f0) =
+      make_float4(fmaxf(s.x, 0.f), fmaxf(s.y, 0.f), fmaxf(s.z, 0.f), fmaxf(s.w, 0.f));
 }
 
 // ---- message passing --------------------------------------------------------
@@ -302,6 +317,7 @@ __global__ __launch_bounds__(256) void k_gnn_weights(const int* __restrict__ rp,
 // ---- dense layers on MFMA -------------------------------------------------------
 // Y = relu([X1, X2] W + b), W: 128 x 64 row-major.  64 rows per workgroup,
 // 16 per wave; per lane the A operand is row (l & 15), k = 4 (l >> 4) + t.
+constexpr int kGemmRows = 64;  // rows per workgroup
 __global__ __launch_bounds__(256) void k_gnn_gemm_fwd(const float* __restrict__ X1, const float* __restrict__ X2,
                                                       const float* __restrict__ W, const float* __restrict__ bias,
                                                       int rows, float* __restrict__ Y, long long istride) {
@@ -697,6 +713,8 @@ __global__ void k_gnn_warm_point(const int* __restrict__ trp, const int* __restr
 }
 
 unsigned wgrid(long long rows) { return (unsigned)((rows * kWave + 255) / 256); }
+// k_gnn_embed: 16 threads per row
+unsigned egrid(long long rows) { return (unsigned)((rows * (H / 4) + 255) / 256); }
 unsigned tgrid(long long n) { return (unsigned)((n + 255) / 256); }
 unsigned cgrid(long long rows) { return (unsigned)((rows + kChunk - 1) / kChunk); }
 
@@ -721,6 +739,7 @@ struct hn_gnn_graph {
   hn::DevBuf wgt;  // aggregation weights, 4 x nnz floats
   // host copy of the pattern, to recognise standard forms of this family
   std::vector<int64_t> h_rp, h_ci;
+  int rank_len = 1;  // longest row up to kRankMax: the LDS the row ranks stage
   // batched inference (gnn_warm_start_many): one workspace block per
   // instance of a chunk, and the chunk's jobs; grown on first use, kept
   // until the graph is freed
@@ -793,6 +812,10 @@ void weights(hn_gnn_graph* gr, const double* val, bool backward_pair, hipStream_
   weights_into(gr, val, gr->wgt.as<float>() + (backward_pair ? 2 * gr->nnz : 0), backward_pair, nullptr, 1, 0, s);
 }
 
+// Dynamic LDS of k_gnn_row_rank on this graph: costs and columns of its
+// longest row (at most kRankMax entries).
+size_t rank_lds(const hn_gnn_graph* gr) { return (size_t)gr->rank_len * (sizeof(double) + sizeof(int)); }
+
 int forward(hn_gnn* g, hn_gnn_graph* gr, const double* val, const double* b, const double* c, hipStream_t s) {
   const Layout& Lo = L();
   const float* P = g->p.as<float>();
@@ -807,26 +830,26 @@ int forward(hn_gnn* g, hn_gnn_graph* gr, const double* val, const double* b, con
   hipLaunchKernelGGL(k_gnn_scale_part, dim3(kScaleBlocks), dim3(256), 0, s, c, n, b, m, gr->spart.as<double>(), nojob,
                      0LL);
   hipLaunchKernelGGL(k_gnn_scale_fin, dim3(1), dim3(64), 0, s, gr->spart.as<double>(), n, m, S, 0LL);
-  hipLaunchKernelGGL(k_gnn_row_rank, dim3(m), dim3(256), 0, s, rp, ci, c, S, gr->rank.as<float>(),
+  hipLaunchKernelGGL(k_gnn_row_rank, dim3(m), dim3(256), rank_lds(gr), s, rp, ci, c, S, gr->rank.as<float>(),
                      gr->gap.as<float>(), nojob, 0LL);
   hipLaunchKernelGGL(k_gnn_feat_v, dim3(tgrid(n)), dim3(256), 0, s, trp, tsrc, gr->rank.as<float>(),
                      gr->gap.as<float>(), c, S, n, gr->Fv.as<float>(), nojob, 0LL);
   hipLaunchKernelGGL(k_gnn_feat_c, dim3(tgrid(m)), dim3(256), 0, s, gr->slackrow.as<int>(), b, S, m,
                      gr->Fc.as<float>(), nojob, 0LL);
-  hipLaunchKernelGGL(k_gnn_embed, dim3(wgrid(n)), dim3(256), 0, s, gr->Fv.as<float>(), 8, FV, P + Lo.wev,
+  hipLaunchKernelGGL(k_gnn_embed, dim3(egrid(n)), dim3(256), 0, s, gr->Fv.as<float>(), 8, FV, P + Lo.wev,
                      P + Lo.bev, n, gr->hv[0].as<float>(), 0LL);
-  hipLaunchKernelGGL(k_gnn_embed, dim3(wgrid(m)), dim3(256), 0, s, gr->Fc.as<float>(), 4, FC, P + Lo.wec,
+  hipLaunchKernelGGL(k_gnn_embed, dim3(egrid(m)), dim3(256), 0, s, gr->Fc.as<float>(), 4, FC, P + Lo.wec,
                      P + Lo.bec, m, gr->hc[0].as<float>(), 0LL);
   weights(gr, val, false, s);
   const float* wr = gr->wgt.as<float>();
   const float* wc = wr + gr->nnz;
   for (int l = 0; l < NL; ++l) {
     HN_TRY(spmm_rows_launch<float>(rp, ci, wr, m, gr->nnz, gr->hv[l].as<float>(), gr->ac[l].as<float>(), H, false, s));
-    hipLaunchKernelGGL(k_gnn_gemm_fwd, dim3((m + 63) / 64), dim3(256), 0, s, gr->hc[l].as<float>(),
+    hipLaunchKernelGGL(k_gnn_gemm_fwd, dim3((m + kGemmRows - 1) / kGemmRows), dim3(256), 0, s, gr->hc[l].as<float>(),
                        gr->ac[l].as<float>(), P + Lo.wc[l], P + Lo.bc[l], m, gr->hc[l + 1].as<float>(), 0LL);
     HN_TRY(spmm_rows_launch<float>(trp, tci, wc, n, gr->nnz, gr->hc[l + 1].as<float>(), gr->av[l].as<float>(), H,
                                    false, s));
-    hipLaunchKernelGGL(k_gnn_gemm_fwd, dim3((n + 63) / 64), dim3(256), 0, s, gr->hv[l].as<float>(),
+    hipLaunchKernelGGL(k_gnn_gemm_fwd, dim3((n + kGemmRows - 1) / kGemmRows), dim3(256), 0, s, gr->hv[l].as<float>(),
                        gr->av[l].as<float>(), P + Lo.wv[l], P + Lo.bv[l], n, gr->hv[l + 1].as<float>(), 0LL);
   }
   hipLaunchKernelGGL(k_gnn_heads, dim3(wgrid(n)), dim3(256), 0, s, gr->hv[NL].as<float>(), n, P + Lo.wx, P + Lo.bx,
@@ -1014,6 +1037,8 @@ extern "C" int hn_gnn_graph_create(int64_t m, int64_t n, const int64_t* row_ptr,
   gr->n = (int)n;
   gr->nnz = nnz;
   gr->h_rp.assign(row_ptr, row_ptr + m + 1);
+  for (int64_t i = 0; i < m; ++i)
+    gr->rank_len = (int)std::max<int64_t>(gr->rank_len, std::min<int64_t>(row_ptr[i + 1] - row_ptr[i], kRankMax));
   gr->h_ci.assign(col, col + nnz);
   auto up = [&](DevBuf& d, const std::vector<int>& h) -> int {
     HN_TRY(d.reserve(sizeof(int) * std::max<size_t>(h.size(), 1)));
@@ -1186,14 +1211,14 @@ int forward_batch(hn_gnn* g, hn_gnn_graph* gr, const GnnJob* jobs, int B, hipStr
   double* spart = reinterpret_cast<double*>(base + IL.spart);
   hipLaunchKernelGGL(k_gnn_scale_part, dim3(kScaleBlocks, U), dim3(256), 0, s, none, n, none, m, spart, jobs, st);
   hipLaunchKernelGGL(k_gnn_scale_fin, dim3(1, U), dim3(64), 0, s, spart, n, m, S, st);
-  hipLaunchKernelGGL(k_gnn_row_rank, dim3(m, U), dim3(256), 0, s, rp, ci, none, S, F(IL.rank), F(IL.gap), jobs, st);
+  hipLaunchKernelGGL(k_gnn_row_rank, dim3(m, U), dim3(256), rank_lds(gr), s, rp, ci, none, S, F(IL.rank), F(IL.gap), jobs, st);
   hipLaunchKernelGGL(k_gnn_feat_v, dim3(tgrid(n), U), dim3(256), 0, s, trp, tsrc, F(IL.rank), F(IL.gap), none, S, n,
                      F(IL.Fv), jobs, st);
   hipLaunchKernelGGL(k_gnn_feat_c, dim3(tgrid(m), U), dim3(256), 0, s, gr->slackrow.as<int>(), none, S, m, F(IL.Fc),
                      jobs, st);
-  hipLaunchKernelGGL(k_gnn_embed, dim3(wgrid(n), U), dim3(256), 0, s, F(IL.Fv), 8, FV, P + Lo.wev, P + Lo.bev, n,
+  hipLaunchKernelGGL(k_gnn_embed, dim3(egrid(n), U), dim3(256), 0, s, F(IL.Fv), 8, FV, P + Lo.wev, P + Lo.bev, n,
                      F(IL.hv[0]), st);
-  hipLaunchKernelGGL(k_gnn_embed, dim3(wgrid(m), U), dim3(256), 0, s, F(IL.Fc), 4, FC, P + Lo.wec, P + Lo.bec, m,
+  hipLaunchKernelGGL(k_gnn_embed, dim3(egrid(m), U), dim3(256), 0, s, F(IL.Fc), 4, FC, P + Lo.wec, P + Lo.bec, m,
                      F(IL.hc[0]), st);
   weights_into(gr, none, F(IL.wgt), false, jobs, B, st, s);
   const float* wr = F(IL.wgt);
@@ -1201,10 +1226,10 @@ int forward_batch(hn_gnn* g, hn_gnn_graph* gr, const GnnJob* jobs, int B, hipStr
   for (int l = 0; l < NL; ++l) {
     const int cur = l & 1, nxt = cur ^ 1;
     HN_TRY(spmm_rows_launch_batch<float>(rp, ci, wr, m, nnz, F(IL.hv[cur]), F(IL.ac), H, false, B, st, s));
-    hipLaunchKernelGGL(k_gnn_gemm_fwd, dim3((m + 63) / 64, U), dim3(256), 0, s, F(IL.hc[cur]), F(IL.ac),
+    hipLaunchKernelGGL(k_gnn_gemm_fwd, dim3((m + kGemmRows - 1) / kGemmRows, U), dim3(256), 0, s, F(IL.hc[cur]), F(IL.ac),
                        P + Lo.wc[l], P + Lo.bc[l], m, F(IL.hc[nxt]), st);
     HN_TRY(spmm_rows_launch_batch<float>(trp, tci, wc, n, nnz, F(IL.hc[nxt]), F(IL.av), H, false, B, st, s));
-    hipLaunchKernelGGL(k_gnn_gemm_fwd, dim3((n + 63) / 64, U), dim3(256), 0, s, F(IL.hv[cur]), F(IL.av),
+    hipLaunchKernelGGL(k_gnn_gemm_fwd, dim3((n + kGemmRows - 1) / kGemmRows, U), dim3(256), 0, s, F(IL.hv[cur]), F(IL.av),
                        P + Lo.wv[l], P + Lo.bv[l], n, F(IL.hv[nxt]), st);
   }
   const int fin = NL & 1;  // the buffers the last layer wrote

@@ -338,12 +338,151 @@ __global__ __launch_bounds__(256) void k_spmm_rows_long(const int* __restrict__ 
   }
 }
 
+// Rows of exactly 64 columns (the GNN's hidden width), 16-byte lanes: a
+// row of B or C is 256 B (float) or 512 B (double), so kW64 = 16 (float)
+// or 32 (double) lanes cover one row with one vector load each.
+//
+// Short rows: each group of kW64 lanes owns a row (4 or 2 rows per wave),
+// every output element summed over the row's nonzeros in order - the
+// arithmetic of k_spmm_rows, with a quarter (or half) of its waves and
+// each group's next two gathers issued together.
+template <typename V>
+struct Vec4Of;
+template <>
+struct Vec4Of<float> {
+  typedef float4 T;
+  static constexpr int kPer = 4;
+};
+template <>
+struct Vec4Of<double> {
+  typedef double2 T;
+  static constexpr int kPer = 2;
+};
+
+template <typename V>
+__device__ __forceinline__ void axpy_vec(float4& a, float w, const float4& b) {
+  a.x += w * b.x;
+  a.y += w * b.y;
+  a.z += w * b.z;
+  a.w += w * b.w;
+}
+template <typename V>
+__device__ __forceinline__ void axpy_vec(double2& a, double w, const double2& b) {
+  a.x += w * b.x;
+  a.y += w * b.y;
+}
+__device__ __forceinline__ float4 add_vec(const float4& a, const float4& b) {
+  return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+__device__ __forceinline__ double2 add_vec(const double2& a, const double2& b) {
+  return make_double2(a.x + b.x, a.y + b.y);
+}
+
+template <typename V>
+__global__ __launch_bounds__(256) void k_spmm_rows64(const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                     const V* __restrict__ val_, const V* __restrict__ B_,
+                                                     V* __restrict__ C_, int nrows, int accumulate,
+                                                     long long istride) {
+  typedef typename Vec4Of<V>::T VT;
+  constexpr int kW64 = 64 / Vec4Of<V>::kPer;  // lanes per row
+  const long long off = (long long)blockIdx.y * istride;
+  const V* __restrict__ val = shift(val_, off);
+  const VT* __restrict__ B = reinterpret_cast<const VT*>(shift(B_, off));
+  VT* __restrict__ C = reinterpret_cast<VT*>(shift(C_, off));
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long r = t / kW64;
+  const int li = (int)(t % kW64);
+  if (r >= nrows) return;
+  const int beg = rowptr[r], end = rowptr[r + 1];
+  VT acc;
+  acc = VT{};
+  int e = beg;
+  for (; e + 1 < end; e += 2) {
+    const int j0 = col[e], j1 = col[e + 1];
+    const V w0 = val[e], w1 = val[e + 1];
+    const VT b0 = B[(long long)j0 * kW64 + li];
+    const VT b1 = B[(long long)j1 * kW64 + li];
+    axpy_vec<V>(acc, w0, b0);
+    axpy_vec<V>(acc, w1, b1);
+  }
+  if (e < end) axpy_vec<V>(acc, val[e], B[(long long)col[e] * kW64 + li]);
+  VT* o = C + r * kW64 + li;
+  *o = accumulate ? add_vec(*o, acc) : acc;
+}
+
+// Long rows: one wave per row (4 rows per workgroup).  Lane group g of the
+// wave takes the nonzeros g, g + 4, g + 8, ... into two running sums
+// (g + 8t and g + 8t + 4) and the groups' partial rows are added in the
+// order ((p0 + p1) + p2) + p3 - the split and order of k_spmm_rows_long's
+// four waves, so the result is the same bit for bit, with 16-byte gathers
+// and eight of them in flight per wave.
+template <typename V>
+__global__ __launch_bounds__(256) void k_spmm_rows64_long(const int* __restrict__ rowptr,
+                                                          const int* __restrict__ col,
+                                                          const V* __restrict__ val_, const V* __restrict__ B_,
+                                                          V* __restrict__ C_, int nrows, int accumulate,
+                                                          long long istride) {
+  typedef typename Vec4Of<V>::T VT;
+  constexpr int kW64 = 64 / Vec4Of<V>::kPer;  // lanes per row slice
+  constexpr int kG = kWave / kW64;            // lane groups per wave (4 float, 2 double)
+  static_assert(kG == 4 || kG == 2, "lane groups per wave");
+  const long long off = (long long)blockIdx.y * istride;
+  const V* __restrict__ val = shift(val_, off);
+  const VT* __restrict__ B = reinterpret_cast<const VT*>(shift(B_, off));
+  VT* __restrict__ C = reinterpret_cast<VT*>(shift(C_, off));
+  __shared__ VT part[4][4][kW64];  // [wave][split][lane]
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const int g = lane / kW64, li = lane % kW64;
+  const long long r = (long long)blockIdx.x * 4 + w;
+  const bool live = r < nrows;
+  const int k0 = live ? rowptr[r] : 0, k1 = live ? rowptr[r + 1] : 0;
+  // with 2 groups (double) each group plays two of the four splits
+#pragma unroll
+  for (int h = 0; h < 4 / kG; ++h) {
+    const int sp = g + h * kG;  // the split this pass plays: nonzeros sp, sp + 4, ...
+    VT a0, a1;
+    a0 = VT{};
+    a1 = VT{};
+    int e = k0 + sp;
+    for (; e + 4 < k1; e += 8) {
+      const int j0 = col[e], j1 = col[e + 4];
+      const V w0 = val[e], w1 = val[e + 4];
+      const VT b0 = B[(long long)j0 * kW64 + li];
+      const VT b1 = B[(long long)j1 * kW64 + li];
+      axpy_vec<V>(a0, w0, b0);
+      axpy_vec<V>(a1, w1, b1);
+    }
+    for (; e < k1; e += 4) axpy_vec<V>(a0, val[e], B[(long long)col[e] * kW64 + li]);
+    part[w][sp][li] = add_vec(a0, a1);
+  }
+  __syncthreads();
+  if (live && g == 0) {
+    const VT acc = add_vec(add_vec(add_vec(part[w][0][li], part[w][1][li]), part[w][2][li]), part[w][3][li]);
+    VT* o = C + r * kW64 + li;
+    *o = accumulate ? add_vec(*o, acc) : acc;
+  }
+}
+
 template <typename V>
 int spmm_rows_launch_batch(const int* rowptr, const int* col, const V* val, int64_t nrows, int64_t nnz, const V* B,
                            V* C, int k, bool accumulate, int batch, long long istride, hipStream_t s) {
   if (nrows <= 0 || k <= 0 || batch <= 0) return HN_OK;
   HN_CHECK(nrows * (int64_t)kWave < (int64_t)INT_MAX * 4, HN_ERR_UNSUPPORTED, "spmm: too many rows");
   HN_CHECK(batch <= 65535, HN_ERR_UNSUPPORTED, "spmm: too many products in one batch");
+  const bool vec_ok = k == 64 && reinterpret_cast<uintptr_t>(B) % 16 == 0 &&
+                      reinterpret_cast<uintptr_t>(C) % 16 == 0 && istride % 16 == 0;
+  if (vec_ok) {
+    if (nnz >= 32 * nrows) {
+      hipLaunchKernelGGL((k_spmm_rows64_long<V>), dim3((unsigned)((nrows + 3) / 4), (unsigned)batch), dim3(256), 0,
+                         s, rowptr, col, val, B, C, (int)nrows, accumulate ? 1 : 0, istride);
+    } else {
+      const int64_t threads = nrows * (int64_t)(64 / Vec4Of<V>::kPer);
+      hipLaunchKernelGGL((k_spmm_rows64<V>), dim3((unsigned)((threads + 255) / 256), (unsigned)batch), dim3(256), 0,
+                         s, rowptr, col, val, B, C, (int)nrows, accumulate ? 1 : 0, istride);
+    }
+    count_launch();
+    return check_launch("spmm");
+  }
   if (nnz >= 32 * nrows) {
     HN_CHECK(nrows < INT_MAX, HN_ERR_UNSUPPORTED, "spmm: too many rows");
     hipLaunchKernelGGL((k_spmm_rows_long<V>), dim3((unsigned)nrows, (unsigned)batch), dim3(256), 0, s, rowptr, col,
