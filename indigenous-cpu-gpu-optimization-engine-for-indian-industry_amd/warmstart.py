@@ -372,12 +372,15 @@ if __name__ == "__main__":
     ap.add_argument("--out-params", default=None)
     ap.add_argument("--report", default=None)
     ap.add_argument("--params", default=None, help="evaluate these trained parameters (.npy) without training")
+    ap.add_argument("--gap-tol", type=float, default=0.0, help="with --params: relative-gap bar of both solves")
+    ap.add_argument("--endgame-mu", type=float, default=0.0, help="with --params: optimal-face projection from this mu")
     a = ap.parse_args()
     if a.params:
         HH.init(-1)
         net = Network(0)
         net.set_params(np.load(a.params))
-        r = evaluate(net, transport_family(), range(1_000_000, 1_000_000 + a.test), log=lambda s: print(s, flush=True))
+        r = evaluate(net, transport_family(), range(1_000_000, 1_000_000 + a.test), log=lambda s: print(s, flush=True),
+                     gap_tol=a.gap_tol, endgame_mu=a.endgame_mu)
     else:
         r = run_experiment(a.train, a.test, a.epochs, lr=a.lr, out_params=a.out_params,
                            log=lambda s: print(s, flush=True))
