@@ -318,12 +318,17 @@ __global__ __launch_bounds__(256) void k_gnn_weights(const int* __restrict__ rp,
 // Y = relu([X1, X2] W + b), W: 128 x 64 row-major.  64 rows per workgroup,
 // 16 per wave; per lane the A operand is row (l & 15), k = 4 (l >> 4) + t.
 constexpr int kGemmRows = 64;  // rows per workgroup
-__global__ __launch_bounds__(256) void k_gnn_gemm_fwd(const float* __restrict__ X1, const float* __restrict__ X2,
-                                                      const float* __restrict__ W, const float* __restrict__ bias,
-                                                      int rows, float* __restrict__ Y, long long istride) {
-  X1 = at(X1, inst_off(istride));
-  X2 = at(X2, inst_off(istride));
-  Y = at(Y, inst_off(istride));
+// AGG: the second input is not read from X2 but formed on the fly as the
+// row's weighted sum of Hs rows over the pattern (trp, tci) with weights
+// wgt - for each output element from zero, the row's entries in order, the
+// arithmetic of the library's SpMM on rows of 64 - so the aggregate of a
+// short-row side is never written to and read back from memory.
+template <bool AGG>
+__device__ __forceinline__ void gemm_fwd_rows(const float* __restrict__ X1, const float* __restrict__ X2,
+                                              const int* __restrict__ trp, const int* __restrict__ tci,
+                                              const float* __restrict__ wgt, const float* __restrict__ Hs,
+                                              const float* __restrict__ W, const float* __restrict__ bias, int rows,
+                                              float* __restrict__ Y) {
   __shared__ short Wt[H][2 * H + 4];  // Wt[j][k] = bf16(W[k][j])
   for (int e = threadIdx.x; e < 2 * H * H; e += 256) {
     const int k = e / H, j = e % H;
@@ -334,16 +339,34 @@ __global__ __launch_bounds__(256) void k_gnn_gemm_fwd(const float* __restrict__ 
   const int row0 = blockIdx.x * 64 + wave * 16;
   const int li = l & 15, kq = l >> 4;
   const int ra = row0 + li;
+  int e0 = 0, e1 = 0;
+  if (AGG && ra < rows) {
+    e0 = trp[ra];
+    e1 = trp[ra + 1];
+  }
   f32x4 acc[4];
 #pragma unroll
   for (int jb = 0; jb < 4; ++jb) acc[jb] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int ks = 0; ks < 8; ++ks) {
-    const float* X = ks < 4 ? X1 : X2;
     const int cofs = (ks & 3) * 16 + 4 * kq;
     bf16x4 a;
     if (ra < rows) {
-      const float4 v = *reinterpret_cast<const float4*>(X + (long long)ra * H + cofs);
+      float4 v;
+      if (AGG && ks >= 4) {
+        v = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int e = e0; e < e1; ++e) {
+          const float w = wgt[e];
+          const float4 b = *reinterpret_cast<const float4*>(Hs + (long long)tci[e] * H + cofs);
+          v.x += w * b.x;
+          v.y += w * b.y;
+          v.z += w * b.z;
+          v.w += w * b.w;
+        }
+      } else {
+        const float* X = ks < 4 ? X1 : X2;
+        v = *reinterpret_cast<const float4*>(X + (long long)ra * H + cofs);
+      }
       a = bf16x4{f2bf(v.x), f2bf(v.y), f2bf(v.z), f2bf(v.w)};
     } else {
       a = bf16x4{0, 0, 0, 0};
@@ -365,6 +388,24 @@ __global__ __launch_bounds__(256) void k_gnn_gemm_fwd(const float* __restrict__ 
       if (row < rows) Y[(long long)row * H + j] = fmaxf(acc[jb][r] + bj, 0.f);
     }
   }
+}
+
+__global__ __launch_bounds__(256) void k_gnn_gemm_fwd(const float* __restrict__ X1, const float* __restrict__ X2,
+                                                      const float* __restrict__ W, const float* __restrict__ bias,
+                                                      int rows, float* __restrict__ Y, long long istride) {
+  gemm_fwd_rows<false>(at(X1, inst_off(istride)), at(X2, inst_off(istride)), nullptr, nullptr, nullptr, nullptr, W,
+                       bias, rows, at(Y, inst_off(istride)));
+}
+
+// The same layer with its aggregate formed in place (inference: the
+// aggregate is not kept for a backward pass).
+__global__ __launch_bounds__(256) void k_gnn_gemm_fwd_agg(const float* __restrict__ X1, const int* __restrict__ trp,
+                                                          const int* __restrict__ tci, const float* __restrict__ wgt,
+                                                          const float* __restrict__ Hs, const float* __restrict__ W,
+                                                          const float* __restrict__ bias, int rows,
+                                                          float* __restrict__ Y, long long istride) {
+  gemm_fwd_rows<true>(at(X1, inst_off(istride)), nullptr, trp, tci, at(wgt, inst_off(istride)),
+                      at(Hs, inst_off(istride)), W, bias, rows, at(Y, inst_off(istride)));
 }
 
 // dX = dZ W' split into dX1 (self input) and dX2 (aggregate input), with
@@ -1228,9 +1269,14 @@ int forward_batch(hn_gnn* g, hn_gnn_graph* gr, const GnnJob* jobs, int B, hipStr
     HN_TRY(spmm_rows_launch_batch<float>(rp, ci, wr, m, nnz, F(IL.hv[cur]), F(IL.ac), H, false, B, st, s));
     hipLaunchKernelGGL(k_gnn_gemm_fwd, dim3((m + kGemmRows - 1) / kGemmRows, U), dim3(256), 0, s, F(IL.hc[cur]), F(IL.ac),
                        P + Lo.wc[l], P + Lo.bc[l], m, F(IL.hc[nxt]), st);
-    HN_TRY(spmm_rows_launch_batch<float>(trp, tci, wc, n, nnz, F(IL.hc[nxt]), F(IL.av), H, false, B, st, s));
-    hipLaunchKernelGGL(k_gnn_gemm_fwd, dim3((n + kGemmRows - 1) / kGemmRows, U), dim3(256), 0, s, F(IL.hv[cur]), F(IL.av),
-                       P + Lo.wv[l], P + Lo.bv[l], n, F(IL.hv[nxt]), st);
+    if (nnz < 32LL * n) {  // short rows: the aggregate over M' formed inside the layer
+      hipLaunchKernelGGL(k_gnn_gemm_fwd_agg, dim3((n + kGemmRows - 1) / kGemmRows, U), dim3(256), 0, s,
+                         F(IL.hv[cur]), trp, tci, wc, F(IL.hc[nxt]), P + Lo.wv[l], P + Lo.bv[l], n, F(IL.hv[nxt]), st);
+    } else {
+      HN_TRY(spmm_rows_launch_batch<float>(trp, tci, wc, n, nnz, F(IL.hc[nxt]), F(IL.av), H, false, B, st, s));
+      hipLaunchKernelGGL(k_gnn_gemm_fwd, dim3((n + kGemmRows - 1) / kGemmRows, U), dim3(256), 0, s, F(IL.hv[cur]),
+                         F(IL.av), P + Lo.wv[l], P + Lo.bv[l], n, F(IL.hv[nxt]), st);
+    }
   }
   const int fin = NL & 1;  // the buffers the last layer wrote
   hipLaunchKernelGGL(k_gnn_heads, dim3(wgrid(n), U), dim3(256), 0, s, F(IL.hv[fin]), n, P + Lo.wx, P + Lo.bx,
